@@ -1,0 +1,173 @@
+/*
+ * symla_hip.h -- C-ABI of the MI355X numeric phase of symla's multifrontal
+ * LDL^T solver (libsymla_hip.so).
+ *
+ * Plain pointers and sizes only.  Status codes: 0 = ok; otherwise one of the
+ * SYMLA_ERR_* values, with a message from symla_last_error() (thread-local)
+ * or symla_plan_last_error(plan).  The C++ facade in include/symla/solver.hpp
+ * turns these codes into the exceptions the reference throws.
+ *
+ * Every entry point below cites the reference interface (file:line in the
+ * symla repository) whose role it takes.
+ */
+#ifndef SYMLA_HIP_H
+#define SYMLA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SYMLA_OK = 0,
+  SYMLA_ERR_INVALID = 1, /* std::invalid_argument in the reference */
+  SYMLA_ERR_LOGIC = 2,   /* std::logic_error */
+  SYMLA_ERR_RUNTIME = 3, /* std::runtime_error */
+  SYMLA_ERR_DEVICE = 4   /* HIP runtime failure */
+};
+
+/* OrderingType (include/symla/ordering.hpp:31-35) plus a caller permutation. */
+enum { SYMLA_ORDER_AMD = 0, SYMLA_ORDER_METIS = 1, SYMLA_ORDER_NATURAL = 2, SYMLA_ORDER_USER = 3 };
+
+/* Mode (include/symla/solver.hpp:20-23). */
+enum { SYMLA_MODE_THRESHOLD = 0, SYMLA_MODE_STATIC = 1 };
+
+typedef struct symla_symbolic symla_symbolic;
+typedef struct symla_plan symla_plan;
+
+const char* symla_last_error(void);
+
+/* ---- Symbolic analysis (CPU) ----
+ * Replaces SymbolicFactor::analyze (include/symla/symbolic.hpp:195-399) as
+ * called by SymLDLT::analyzePattern (include/symla/solver.hpp:133-137).
+ * A: n x n CSC (colptr[n+1], rowidx); either triangle or both may be stored.
+ * user_perm (perm[k] = original index of pivot k) is read when ordering is
+ * SYMLA_ORDER_USER.  max_relax_size / max_relax_fill: SymbolicFactorOptions
+ * (symbolic.hpp:161-184; defaults 64 and 0.25). */
+int symla_analyze(int n, const int64_t* colptr, const int* rowidx, int ordering, const int* user_perm,
+                  int max_relax_size, double max_relax_fill, symla_symbolic** out);
+
+typedef struct {
+  int n;
+  int nsn;               /* supernodes */
+  int64_t nnz_l;         /* sum of column counts */
+  int64_t pattern_total; /* sum of supernode row-pattern lengths */
+  int max_front;         /* largest row pattern */
+} symla_symbolic_info_t;
+int symla_symbolic_info(const symla_symbolic* s, symla_symbolic_info_t* info);
+/* Copies SymbolicFactor's fields: perm[n], etree_parent[n] (etree.parent),
+ * sn_first[nsn], sn_ncols[nsn], sn_rowptr[nsn+1], sn_rows[pattern_total]
+ * (Supernode, symbolic.hpp:138-159).  Any pointer may be NULL. */
+int symla_symbolic_get(const symla_symbolic* s, int* perm, int* etree_parent, int* sn_first, int* sn_ncols,
+                       int64_t* sn_rowptr, int* sn_rows);
+void symla_symbolic_free(symla_symbolic* s);
+
+/* ---- Numeric plan ----
+ * Takes the setup part of MultifrontalFactorizer::factorize
+ * (include/symla/multifrontal.hpp:299-429): supernode tree, value
+ * permutation (PermutedLower, :199-242) as a precomputed scatter map, and the
+ * device pool that holds every front, its contribution block and its L/D
+ * factors.  A's pattern must be the one given to symla_analyze. */
+int symla_plan_create(const symla_symbolic* sym, const int64_t* a_colptr, const int* a_rowidx, symla_plan** out);
+void symla_plan_destroy(symla_plan* p);
+const char* symla_plan_last_error(const symla_plan* p);
+
+typedef struct {
+  int mode;                     /* SYMLA_MODE_* */
+  double pivot_threshold;       /* carried like the reference; no effect on decisions (dense_kernel.hpp:57-69) */
+  double zero_tolerance;        /* DenseLDLTOptions::zero_tolerance (1e-300) */
+  double relative_pivot_floor;  /* DenseLDLTOptions::relative_pivot_floor (1e-12) */
+  /* Mode::StaticRegularized (solver.hpp:54-111, :328-417) */
+  const int* expected_sign;     /* original index space, size n, values -1/0/+1; NULL = no expectation */
+  double initial_delta;         /* <= 0: sqrt(eps) * max|A| */
+  int max_retries;              /* 10 */
+  double growth;                /* 10 */
+  double static_rel_floor;      /* 1e-8 */
+  double static_abs_floor;      /* 1e-300 */
+} symla_factor_opts;
+
+typedef struct {
+  int n_pos, n_neg, n_zero;     /* Inertia (inertia.hpp:14-18) */
+  int singular;                 /* NumericFactor::singular */
+  int n_singular;
+  double total_perturbation;    /* NumericFactor::totalPerturbation */
+  int num_perturbed;
+  int retries_used;             /* regularizationRetriesUsed() */
+  double last_delta;            /* lastRegularizationDelta() */
+  int delayed_fronts;           /* fronts that handed delayed columns to their parent */
+  int repaired;                 /* 1 when the delayed-pivot repair pass ran */
+} symla_factor_info;
+
+/* SymLDLT::factorize / refactorize (solver.hpp:140-172) through
+ * MultifrontalFactorizer::factorize (multifrontal.hpp:299-805); in static
+ * mode the inertia-controlled retry loop of factorizeStaticRegularized
+ * (solver.hpp:328-417).  a_vals: values in the CSC order of plan_create's
+ * pattern; host memory (symla_factorize) or device memory
+ * (symla_factorize_device).  Returns SYMLA_ERR_RUNTIME when the static retry
+ * budget is exhausted (the plan is then not factorized). */
+int symla_factorize(symla_plan* p, const double* a_vals, const symla_factor_opts* opts, symla_factor_info* info);
+int symla_factorize_device(symla_plan* p, const double* d_a_vals, const symla_factor_opts* opts,
+                           symla_factor_info* info);
+
+/* SymLDLT::singularColumns (solver.hpp:251-257): original indices. */
+int symla_singular_cols(const symla_plan* p, int* out, int cap);
+
+/* SymLDLT::solve (solver.hpp:198-218) / MultifrontalSolver::solve
+ * (solve.hpp:159-431).  B, X: n x nrhs column-major.  Host pointers
+ * (symla_solve) or device pointers (symla_solve_device).  SYMLA_ERR_LOGIC
+ * before a factorization or on a singular one. */
+int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx);
+int symla_solve_device(symla_plan* p, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+
+/* SymLDLT::solveWithRefinement (solver.hpp:234-240) with iterativeRefine
+ * (refine.hpp:97-144).  a_vals must be the values last factorized. */
+typedef struct {
+  int max_iterations; /* 10 */
+  double tolerance;   /* 1e-14 */
+} symla_refine_opts;
+typedef struct {
+  int iterations;
+  double initial_relative_residual; /* refine.hpp's residual (its own |A| row-sum norm) */
+  double final_relative_residual;
+  double backward_error;            /* max_c ||r_c||inf / (||A||inf ||x_c||inf + ||b_c||inf), symmetric ||A||inf */
+} symla_refine_info;
+int symla_solve_refined(symla_plan* p, const double* a_vals, const double* B, int64_t ldb, int nrhs, double* X,
+                        int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info);
+int symla_solve_refined_device(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
+                               double* dX, int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info);
+
+/* NumericFactor::fronts (multifrontal.hpp:118-186), copied to the host on
+ * request: one SupernodeFactor.  rows[m] (final-order rowIndices), L
+ * (m x npiv column-major, ld m), d_diag/d_off[npiv] (BlockDiagonalD),
+ * pivot_start/pivot_kind[npiv] (PivotBlock; kind 1 or 2), *nblocks = pivot
+ * blocks.  Pass NULL arrays to query m and npiv first. */
+int symla_front_count(const symla_plan* p);
+int symla_front_view(symla_plan* p, int si, int* m, int* npiv, int* rows, double* L, double* d_diag,
+                     double* d_off, int* pivot_start, int* pivot_kind, int* nblocks);
+
+/* DenseLDLT::factor / factorStatic (include/symla/dense_kernel.hpp:393-837,
+ * :919-1059) on one dense n x n column-major matrix, run by the same device
+ * front kernels (test entry).  A is factored in place (lower triangle
+ * meaningful).  Outputs have capacity n; inertia3 = {pos, neg, zero}. */
+typedef struct {
+  double zero_tolerance;
+  double relative_pivot_floor;
+  int static_mode;
+  double static_delta;
+  double static_rel_floor;
+  double static_abs_floor;
+  const int* expected_sign; /* static mode, size n or NULL */
+} symla_dense_opts;
+int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_dense_opts* opts, double* d_diag,
+                       double* d_off, int* perm, int* pivot_start, int* pivot_kind, int* n_pivot_blocks,
+                       int* n_factored, int* inertia3, double* total_perturbation, int* num_perturbed);
+
+/* Device-side timing of the last factorize / solve (milliseconds, HIP
+ * events on the plan's stream). */
+int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,421 @@
+// capi.cpp -- extern "C" entry points of libsymla_hip.so (include/symla_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/symla_hip.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "symbolic.hpp"
+
+using namespace symla_hip;
+
+struct symla_symbolic {
+  SymbolicResult r;
+};
+
+struct symla_plan {
+  std::unique_ptr<Plan> plan;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg) {
+  g_lastError = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_lastError.clear();
+    return SYMLA_OK;
+  } catch (const std::invalid_argument& e) {
+    return fail(SYMLA_ERR_INVALID, e.what());
+  } catch (const std::logic_error& e) {
+    return fail(SYMLA_ERR_LOGIC, e.what());
+  } catch (const DeviceError& e) {
+    return fail(SYMLA_ERR_DEVICE, e.what());
+  } catch (const std::runtime_error& e) {
+    return fail(SYMLA_ERR_RUNTIME, e.what());
+  } catch (const std::exception& e) {
+    return fail(SYMLA_ERR_RUNTIME, e.what());
+  }
+}
+
+int planGuarded(symla_plan* p, const std::function<void()>& f) {
+  const int rc = guarded(f);
+  if (p) p->err = (rc == SYMLA_OK) ? std::string() : g_lastError;
+  return rc;
+}
+
+FactorOptions toFactorOptions(const symla_factor_opts* o, int n) {
+  FactorOptions f;
+  if (!o) return f;
+  f.mode = o->mode;
+  f.zeroTol = o->zero_tolerance;
+  f.relFloor = o->relative_pivot_floor;
+  if (o->expected_sign) {
+    f.expectedSignOrig.assign(o->expected_sign, o->expected_sign + n);
+    f.haveExpectation = true;
+  }
+  f.initialDelta = o->initial_delta;
+  f.maxRetries = o->max_retries;
+  f.growth = o->growth;
+  f.staticRelFloor = o->static_rel_floor;
+  f.staticAbsFloor = o->static_abs_floor;
+  return f;
+}
+
+void fillInfo(const FactorInfo& fi, symla_factor_info* info) {
+  if (!info) return;
+  info->n_pos = fi.pos;
+  info->n_neg = fi.neg;
+  info->n_zero = fi.zero;
+  info->singular = fi.singular ? 1 : 0;
+  info->n_singular = fi.nSingular;
+  info->total_perturbation = fi.totalPerturbation;
+  info->num_perturbed = fi.numPerturbed;
+  info->retries_used = fi.retriesUsed;
+  info->last_delta = fi.lastDelta;
+  info->delayed_fronts = fi.delayedFronts;
+  info->repaired = fi.repaired ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* symla_last_error(void) { return g_lastError.c_str(); }
+
+int symla_analyze(int n, const int64_t* colptr, const int* rowidx, int ordering, const int* user_perm,
+                  int max_relax_size, double max_relax_fill, symla_symbolic** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("symla_analyze: out is null");
+    *out = nullptr;
+    if (n < 0) throw std::invalid_argument("symla_analyze: n must be non-negative");
+    if (n > 0 && (!colptr || (colptr[n] > 0 && !rowidx))) throw std::invalid_argument("symla_analyze: null pattern");
+    if (ordering < 0 || ordering > 3) throw std::invalid_argument("symla::computeOrdering: unknown OrderingType");
+    SymbolicOptions so;
+    so.max_relax_size = max_relax_size;
+    so.max_relax_fill_fraction = max_relax_fill;
+    auto s = std::make_unique<symla_symbolic>();
+    s->r = analyze(n, colptr, rowidx, static_cast<Ordering>(ordering), so, user_perm);
+    *out = s.release();
+  });
+}
+
+int symla_symbolic_info(const symla_symbolic* s, symla_symbolic_info_t* info) {
+  return guarded([&] {
+    if (!s || !info) throw std::invalid_argument("symla_symbolic_info: null argument");
+    info->n = s->r.n;
+    info->nsn = static_cast<int>(s->r.supernodes.size());
+    info->nnz_l = s->r.nnzL;
+    int64_t tot = 0;
+    int mx = 0;
+    for (const auto& sn : s->r.supernodes) {
+      tot += static_cast<int64_t>(sn.rowPattern.size());
+      mx = std::max(mx, static_cast<int>(sn.rowPattern.size()));
+    }
+    info->pattern_total = tot;
+    info->max_front = mx;
+  });
+}
+
+int symla_symbolic_get(const symla_symbolic* s, int* perm, int* etree_parent, int* sn_first, int* sn_ncols,
+                       int64_t* sn_rowptr, int* sn_rows) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("symla_symbolic_get: null symbolic");
+    const auto& r = s->r;
+    if (perm) std::copy(r.perm.begin(), r.perm.end(), perm);
+    if (etree_parent) std::copy(r.etreeParent.begin(), r.etreeParent.end(), etree_parent);
+    int64_t off = 0;
+    for (size_t i = 0; i < r.supernodes.size(); ++i) {
+      const auto& sn = r.supernodes[i];
+      if (sn_first) sn_first[i] = sn.firstCol;
+      if (sn_ncols) sn_ncols[i] = sn.ncols;
+      if (sn_rowptr) sn_rowptr[i] = off;
+      if (sn_rows) std::copy(sn.rowPattern.begin(), sn.rowPattern.end(), sn_rows + off);
+      off += static_cast<int64_t>(sn.rowPattern.size());
+    }
+    if (sn_rowptr) sn_rowptr[r.supernodes.size()] = off;
+  });
+}
+
+void symla_symbolic_free(symla_symbolic* s) { delete s; }
+
+int symla_plan_create(const symla_symbolic* sym, const int64_t* a_colptr, const int* a_rowidx, symla_plan** out) {
+  return guarded([&] {
+    if (!sym || !out) throw std::invalid_argument("symla_plan_create: null argument");
+    *out = nullptr;
+    if (sym->r.n > 0 && !a_colptr) throw std::invalid_argument("symla_plan_create: null pattern");
+    auto p = std::make_unique<symla_plan>();
+    p->plan = std::make_unique<Plan>(sym->r, a_colptr, a_rowidx);
+    *out = p.release();
+  });
+}
+
+void symla_plan_destroy(symla_plan* p) { delete p; }
+
+const char* symla_plan_last_error(const symla_plan* p) { return p ? p->err.c_str() : g_lastError.c_str(); }
+
+static int factorizeImpl(symla_plan* p, const double* host, const double* dev, const symla_factor_opts* opts,
+                         symla_factor_info* info) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_factorize: null plan");
+    FactorOptions fo = toFactorOptions(opts, p->plan->n());
+    FactorInfo fi;
+    p->plan->factorize(host, dev, fo, fi);
+    fillInfo(fi, info);
+  });
+}
+
+int symla_factorize(symla_plan* p, const double* a_vals, const symla_factor_opts* opts, symla_factor_info* info) {
+  return factorizeImpl(p, a_vals, nullptr, opts, info);
+}
+
+int symla_factorize_device(symla_plan* p, const double* d_a_vals, const symla_factor_opts* opts,
+                           symla_factor_info* info) {
+  return factorizeImpl(p, nullptr, d_a_vals, opts, info);
+}
+
+int symla_singular_cols(const symla_plan* p, int* out, int cap) {
+  if (!p) return fail(SYMLA_ERR_INVALID, "symla_singular_cols: null plan"), -1;
+  const auto& sc = p->plan->singularColsOrig();
+  const int cnt = std::min<int>(cap, static_cast<int>(sc.size()));
+  for (int i = 0; i < cnt && out; ++i) out[i] = sc[i];
+  return static_cast<int>(sc.size());
+}
+
+int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_solve: null plan");
+    if (ldb < p->plan->n() || ldx < p->plan->n())
+      throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
+    p->plan->solve(B, ldb, nrhs, X, ldx, false);
+  });
+}
+
+int symla_solve_device(symla_plan* p, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_solve_device: null plan");
+    if (ldb < p->plan->n() || ldx < p->plan->n())
+      throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
+    p->plan->solve(dB, ldb, nrhs, dX, ldx, true);
+  });
+}
+
+static int refinedImpl(symla_plan* p, const double* aHost, const double* aDev, const double* B, int64_t ldb,
+                       int nrhs, double* X, int64_t ldx, bool dev, const symla_refine_opts* opts,
+                       symla_refine_info* info) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_solve_refined: null plan");
+    if (ldb < p->plan->n() || ldx < p->plan->n())
+      throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
+    const int maxIt = opts ? opts->max_iterations : 10;
+    const double tol = opts ? opts->tolerance : 1e-14;
+    RefineResult r = p->plan->solveRefined(aHost, aDev, B, ldb, nrhs, X, ldx, dev, maxIt, tol);
+    if (info) {
+      info->iterations = r.iterations;
+      info->initial_relative_residual = r.initialRel;
+      info->final_relative_residual = r.finalRel;
+      info->backward_error = r.backwardError;
+    }
+  });
+}
+
+int symla_solve_refined(symla_plan* p, const double* a_vals, const double* B, int64_t ldb, int nrhs, double* X,
+                        int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info) {
+  return refinedImpl(p, a_vals, nullptr, B, ldb, nrhs, X, ldx, false, opts, info);
+}
+
+int symla_solve_refined_device(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
+                               double* dX, int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info) {
+  return refinedImpl(p, nullptr, d_a_vals, dB, ldb, nrhs, dX, ldx, true, opts, info);
+}
+
+int symla_front_count(const symla_plan* p) { return p ? p->plan->numSN() : -1; }
+
+int symla_front_view(symla_plan* p, int si, int* m, int* npiv, int* rows, double* L, double* d_diag, double* d_off,
+                     int* pivot_start, int* pivot_kind, int* nblocks) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_front_view: null plan");
+    p->plan->frontView(si, m, npiv, rows, L, d_diag, d_off, pivot_start, pivot_kind, nblocks);
+  });
+}
+
+int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms) {
+  if (!p) return fail(SYMLA_ERR_INVALID, "symla_plan_timing: null plan");
+  if (factor_ms) *factor_ms = p->plan->lastFactorMs();
+  if (solve_ms) *solve_ms = p->plan->lastSolveMs();
+  return SYMLA_OK;
+}
+
+// One dense front through the device kernels (DenseLDLT::factor /
+// factorStatic test entry).
+int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_dense_opts* opts, double* d_diag,
+                       double* d_off, int* perm, int* pivot_start, int* pivot_kind, int* n_pivot_blocks,
+                       int* n_factored, int* inertia3, double* total_perturbation, int* num_perturbed) {
+  return guarded([&] {
+    if (n < 0 || lda < n) throw std::invalid_argument("DenseLDLT::factor: A must be square");
+    const int nE = (n_eligible < 0) ? n : n_eligible;
+    if (nE < 0 || nE > n) throw std::invalid_argument("DenseLDLT::factor: n_eligible out of range");
+    if (n == 0) {
+      if (n_factored) *n_factored = 0;
+      if (n_pivot_blocks) *n_pivot_blocks = 0;
+      if (inertia3) inertia3[0] = inertia3[1] = inertia3[2] = 0;
+      return;
+    }
+    hipStream_t st;
+    hipCheck(hipStreamCreate(&st), "stream");
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    auto upI = [&](DBuf<int>& b, const std::vector<int>& v) { b.upload(v, st); };
+    DBuf<double> dA;
+    dA.alloc(static_cast<size_t>(n) * n);
+    hipCheck(hipMemcpy2DAsync(dA.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                              hipMemcpyHostToDevice, st),
+             "A upload");
+    DBuf<int> dm, dnE, dnc, dfirst, dfi, drel, dcp, dcl, daRow, dperm, drow, dpk, dnF, dinert, dkcur, dpc, dstop,
+        dnpert, dlist, dsign;
+    DBuf<int64_t> dio, dacp, daSrc;
+    DBuf<double> ddd, ddo, dpert, dws;
+    DBuf<double*> dF;
+    DBuf<unsigned long long> dom, drm;
+    upI(dm, {n});
+    upI(dnE, {nE});
+    upI(dnc, {nE});
+    upI(dfirst, {0});
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; ++i) ident[i] = i;
+    upI(dfi, ident);
+    upI(drel, std::vector<int>(n, -1));
+    upI(dcp, {0, 0});
+    dcl.alloc(1);
+    daRow.alloc(1);
+    daSrc.alloc(1);
+    dio.upload(std::vector<int64_t>{0}, st);
+    dacp.upload(std::vector<int64_t>(n + 1, 0), st);
+    dperm.alloc(n);
+    drow.alloc(n);
+    dpk.alloc(n);
+    upI(dpk, std::vector<int>(n, 0));
+    ddd.alloc(n);
+    ddo.alloc(n);
+    dnF.alloc(1);
+    dinert.alloc(3);
+    dkcur.alloc(1);
+    dpc.alloc(1);
+    dstop.alloc(1);
+    dnpert.alloc(1);
+    dpert.alloc(1);
+    dom.alloc(1);
+    drm.alloc(1);
+    dws.alloc(2 * static_cast<size_t>(nE) + 2);
+    dF.upload(std::vector<double*>{dA.p}, st);
+    upI(dlist, {0});
+    const int* signPtr = nullptr;
+    if (opts && opts->static_mode && opts->expected_sign) {
+      upI(dsign, std::vector<int>(opts->expected_sign, opts->expected_sign + n));
+      signPtr = dsign.p;
+    }
+    DevFronts f;
+    f.m = dm.p;
+    f.nE = dnE.p;
+    f.ncols = dnc.p;
+    f.F = dF.p;
+    f.idxOff = dio.p;
+    f.frontIdx = dfi.p;
+    f.relmap = drel.p;
+    f.childPtr = dcp.p;
+    f.childList = dcl.p;
+    f.firstCol = dfirst.p;
+    f.acolPtr = dacp.p;
+    f.aRow = daRow.p;
+    f.aSrc = daSrc.p;
+    f.perm = dperm.p;
+    f.rowIdx = drow.p;
+    f.dDiag = ddd.p;
+    f.dOff = ddo.p;
+    f.pkind = dpk.p;
+    f.nF = dnF.p;
+    f.inertia = dinert.p;
+    f.origMaxBits = dom.p;
+    f.runMaxBits = drm.p;
+    f.kcur = dkcur.p;
+    f.pcount = dpc.p;
+    f.stopped = dstop.p;
+    f.pert = dpert.p;
+    f.npert = dnpert.p;
+    FactorParams prm;
+    prm.zeroTol = opts ? opts->zero_tolerance : 1e-300;
+    prm.relFloor = opts ? opts->relative_pivot_floor : 1e-12;
+    prm.staticMode = (opts && opts->static_mode) ? 1 : 0;
+    prm.staticDelta = opts ? opts->static_delta : 0.0;
+    prm.staticRelFloor = opts ? opts->static_rel_floor : 1e-8;
+    prm.staticAbsFloor = opts ? opts->static_abs_floor : 1e-300;
+    prm.expectedSign = signPtr;
+    launchResetFronts(1, f, st);
+    launchResetPerm(dlist.p, 1, f, st);
+    if (n <= kSmallFrontMax) {
+      launchFactorSmall(dlist.p, 1, n, f, prm, st);
+    } else {
+      launchFrontMaxAbs(0, n, f, st);
+      const int iters = (nE + kPanel - 1) / kPanel;
+      for (int it = 0; it < iters; ++it) {
+        launchPanel(0, f, prm, dws.p, st);
+        launchPanelRows(0, n - nE, f, st);
+        launchSyrk(0, n - it * kPanel, f, st);
+      }
+      launchFrontFinish(0, f, st);
+    }
+    hipCheck(hipMemcpy2DAsync(A, sizeof(double) * lda, dA.p, sizeof(double) * n, sizeof(double) * n, n,
+                              hipMemcpyDeviceToHost, st),
+             "A download");
+    std::vector<int> hperm(n), hpk(n);
+    std::vector<double> hdd(n), hdo(n);
+    int hnF = 0, hin[3] = {0, 0, 0}, hnp = 0;
+    double hpert = 0.0;
+    hipCheck(hipMemcpyAsync(hperm.data(), dperm.p, sizeof(int) * n, hipMemcpyDeviceToHost, st), "perm");
+    hipCheck(hipMemcpyAsync(hpk.data(), dpk.p, sizeof(int) * n, hipMemcpyDeviceToHost, st), "pkind");
+    hipCheck(hipMemcpyAsync(hdd.data(), ddd.p, sizeof(double) * n, hipMemcpyDeviceToHost, st), "d");
+    hipCheck(hipMemcpyAsync(hdo.data(), ddo.p, sizeof(double) * n, hipMemcpyDeviceToHost, st), "d");
+    hipCheck(hipMemcpyAsync(&hnF, dnF.p, sizeof(int), hipMemcpyDeviceToHost, st), "nF");
+    hipCheck(hipMemcpyAsync(hin, dinert.p, 3 * sizeof(int), hipMemcpyDeviceToHost, st), "inertia");
+    hipCheck(hipMemcpyAsync(&hpert, dpert.p, sizeof(double), hipMemcpyDeviceToHost, st), "pert");
+    hipCheck(hipMemcpyAsync(&hnp, dnpert.p, sizeof(int), hipMemcpyDeviceToHost, st), "npert");
+    hipCheck(hipStreamSynchronize(st), "dense factor");
+    int nb = 0;
+    for (int i = 0; i < n; ++i) {
+      if (perm) perm[i] = hperm[i];
+      if (d_diag) d_diag[i] = (i < hnF) ? hdd[i] : 0.0;
+      if (d_off) d_off[i] = (i < hnF && hpk[i] == 2) ? hdo[i] : 0.0;
+      if (i < hnF && (hpk[i] == 1 || hpk[i] == 2)) {
+        if (pivot_start) pivot_start[nb] = i;
+        if (pivot_kind) pivot_kind[nb] = hpk[i];
+        ++nb;
+      }
+    }
+    if (n_pivot_blocks) *n_pivot_blocks = nb;
+    if (n_factored) *n_factored = hnF;
+    if (inertia3) {
+      inertia3[0] = hin[0];
+      inertia3[1] = hin[1];
+      inertia3[2] = hin[2];
+    }
+    if (total_perturbation) *total_perturbation = hpert;
+    if (num_perturbed) *num_perturbed = hnp;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// device.hpp -- shared host/device definitions for the numeric-phase kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace symla_hip {
+
+// Front dimensions above this use the panel path (global memory); at or below
+// it the whole packed lower triangle lives in LDS.
+constexpr int kSmallFrontMax = 192;
+// Panel width of the large-front path.
+constexpr int kPanel = 64;
+// Threads of the small-front and panel kernels.
+constexpr int kSmallThreads = 256;
+constexpr int kPanelThreads = 512;
+
+// Per-front metadata and outputs, all device arrays indexed by supernode id.
+// Per-position arrays (frontIdx, perm, rowIdx, dDiag, dOff, pkind, relmap)
+// are indexed by idxOff[s] + local position, local position in [0, m[s]).
+struct DevFronts {
+  const int* m;          // front order
+  const int* nE;         // eligible (fully summed) columns
+  const int* ncols;      // own columns
+  double* const* F;      // front storage, column-major, ld = m
+  const int64_t* idxOff; // offset into per-position arrays
+  const int* frontIdx;   // final-order global index of pre-permutation local position
+  const int* relmap;     // child position -> parent pre-permutation position (-1: none)
+  const int* childPtr;   // children CSR (ordered as the reference: increasing id)
+  const int* childList;
+  const int* firstCol;   // first own column (final order)
+  const int64_t* acolPtr;  // A entries per final column (CSR over columns)
+  const int* aRow;       // local pre-permutation row in the owning front
+  const int64_t* aSrc;   // index into the caller's value array
+  // outputs / state
+  int* perm;             // local permutation: position -> pre-permutation position
+  int* rowIdx;           // frontIdx[perm[i]]
+  double* dDiag;
+  double* dOff;
+  int* pkind;            // 1: 1x1 at position, 2: first of 2x2, 0: second of 2x2 / none
+  int* nF;               // pivots factored
+  int* inertia;          // 3 per front: pos, neg, zero
+  unsigned long long* origMaxBits;  // max|F| at front start (bits of a non-negative double)
+  unsigned long long* runMaxBits;   // running max (bits)
+  int* kcur;             // next pivot position (panel path)
+  int* pcount;           // pivots in the last panel (panel path)
+  int* stopped;          // 1 when a front stopped on a delay
+  double* pert;          // static mode: sum |perturbation|
+  int* npert;            // static mode: perturbed pivots
+};
+
+struct FactorParams {
+  double zeroTol;        // DenseLDLTOptions::zero_tolerance
+  double relFloor;       // DenseLDLTOptions::relative_pivot_floor
+  int staticMode;        // 1: factorStatic semantics
+  double staticDelta;    // > 0: delta; <= 0: sqrt(eps) * orig_max per front
+  double staticRelFloor;
+  double staticAbsFloor;
+  const int* expectedSign;  // final-order expected signs (static mode) or null
+};
+
+}  // namespace symla_hip

@@ -1,0 +1,1012 @@
+// factor_kernels.hip -- gfx950 kernels of the multifrontal numeric
+// factorization: front reset, assembly (zero + value scatter + extend-add),
+// small-front Bunch-Kaufman in LDS, and the large-front panel path (panel
+// pivoting on the eligible rows, row-parallel multipliers for the remaining
+// rows, FP64 MFMA symmetric rank-p trailing update).
+//
+// Pivot decision logic follows the reference's DenseLDLT::factor
+// (include/symla/dense_kernel.hpp:585-792) and DenseLDLT::factorStatic
+// (dense_kernel.hpp:962-1035); see the comments at each step.
+#include "device.hpp"
+#include "kernels.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace symla_hip {
+
+namespace {
+
+constexpr double kAlpha = 0.64038820320220756872767623199676;  // (1 + sqrt(17)) / 8, dense_kernel.hpp:400
+
+__device__ inline double bitsToDouble(unsigned long long b) { return __longlong_as_double(static_cast<long long>(b)); }
+__device__ inline unsigned long long doubleToBits(double d) {
+  return static_cast<unsigned long long>(__double_as_longlong(d));
+}
+// max of non-negative doubles via the ordering of their bit patterns
+__device__ inline void atomicMaxNonNeg(unsigned long long* addr, double v) {
+  if (v > 0.0) atomicMax(addr, doubleToBits(v));
+}
+
+// Candidate combine for the lambda search: larger value wins; equal values
+// keep the smaller row (the reference scans rows upward with a strict '>').
+__device__ inline void argmaxCombine(double& v, int& i, double v2, int i2) {
+  if (i2 < 0) return;
+  if (i < 0 || v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+template <int NT>
+__device__ inline void blockArgmax(double& v, int& idx, double* sv, int* si) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double v2 = __shfl_xor(v, off, 64);
+    const int i2 = __shfl_xor(idx, off, 64);
+    argmaxCombine(v, idx, v2, i2);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[w] = v;
+    si[w] = idx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = sv[0];
+    int bi = si[0];
+    for (int t = 1; t < NT / 64; ++t) argmaxCombine(bv, bi, sv[t], si[t]);
+    sv[NT / 64] = bv;
+    si[NT / 64] = bi;
+  }
+  __syncthreads();
+  v = sv[NT / 64];
+  idx = si[NT / 64];
+  __syncthreads();
+}
+
+template <int NT>
+__device__ inline double blockMax(double v, double* sv) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sv[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sv[0];
+    for (int t = 1; t < NT / 64; ++t) b = fmax(b, sv[t]);
+    sv[NT / 64] = b;
+  }
+  __syncthreads();
+  v = sv[NT / 64];
+  __syncthreads();
+  return v;
+}
+
+__device__ inline void countInertia(double d, int& pos, int& neg, int& zero) {
+  if (d > 0)
+    ++pos;
+  else if (d < 0)
+    ++neg;
+  else
+    ++zero;
+}
+
+// Inertia of an accepted 2x2 block (dense_kernel.hpp:753-768).
+__device__ inline void countInertia2(double d11, double d22, double det, int& pos, int& neg) {
+  if (det < 0) {
+    ++pos;
+    ++neg;
+  } else {
+    const double tr = d11 + d22;
+    if (tr > 0)
+      pos += 2;
+    else if (tr < 0)
+      neg += 2;
+    else {
+      ++pos;
+      ++neg;
+    }
+  }
+}
+
+// Static pivot (dense_kernel.hpp:982-1011): returns the pivot to use and the
+// perturbation applied.
+__device__ inline double staticPivot(double a_kk, int sign, double delta, double floorv, double& applied) {
+  bool need;
+  if (sign > 0)
+    need = a_kk < floorv;
+  else if (sign < 0)
+    need = a_kk > -floorv;
+  else
+    need = fabs(a_kk) < floorv;
+  double d = a_kk;
+  applied = 0.0;
+  if (need) {
+    const double s = (sign != 0) ? static_cast<double>(sign) : (a_kk >= 0.0 ? 1.0 : -1.0);
+    d = a_kk + s * delta;
+    applied = d - a_kk;
+  }
+  return d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Reset per-front state for a factorization (no memset nodes: safe to record).
+// ---------------------------------------------------------------------------
+__global__ void k_reset_fronts(int numSN, DevFronts fd) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < numSN; s += gridDim.x * blockDim.x) {
+    fd.nF[s] = 0;
+    fd.inertia[3 * s] = 0;
+    fd.inertia[3 * s + 1] = 0;
+    fd.inertia[3 * s + 2] = 0;
+    fd.origMaxBits[s] = 0ull;
+    fd.runMaxBits[s] = 0ull;
+    fd.kcur[s] = 0;
+    fd.pcount[s] = 0;
+    fd.stopped[s] = 0;
+    fd.pert[s] = 0.0;
+    fd.npert[s] = 0;
+  }
+}
+
+__global__ void k_reset_perm(const int* list, int count, DevFronts fd) {
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int m = fd.m[s];
+  const int64_t io = fd.idxOff[s];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) fd.perm[io + i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of one column tile of one front (multifrontal.hpp:498-535):
+// zero the lower triangle, add A's own-column entries, then extend-add every
+// child's contribution block in child order.  One workgroup owns each
+// (front, column tile), so every target entry has exactly one writer and the
+// summation order is fixed.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks, int tileW, DevFronts fd,
+                                                   const double* __restrict__ aVals, int monotone) {
+  const int task = blockIdx.x;
+  if (task >= ntasks) return;
+  const int s = tasks[task].x;
+  const int c0 = tasks[task].y;
+  const int m = fd.m[s];
+  const int c1 = min(c0 + tileW, m);
+  double* F = fd.F[s];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+
+  for (int j = c0 + wv; j < c1; j += nw) {
+    double* col = F + static_cast<size_t>(j) * m;
+    for (int i = j + lane; i < m; i += 64) col[i] = 0.0;
+  }
+  __syncthreads();
+
+  const int nc = fd.ncols[s];
+  const int fc = fd.firstCol[s];
+  for (int j = c0 + threadIdx.x; j < min(c1, nc); j += blockDim.x) {
+    const int gcol = fc + j;
+    double* col = F + static_cast<size_t>(j) * m;
+    for (int64_t t = fd.acolPtr[gcol]; t < fd.acolPtr[gcol + 1]; ++t) col[fd.aRow[t]] += aVals[fd.aSrc[t]];
+  }
+  __syncthreads();
+
+  for (int ci = fd.childPtr[s]; ci < fd.childPtr[s + 1]; ++ci) {
+    const int c = fd.childList[ci];
+    const int mc = fd.m[c];
+    const int nfc = fd.nF[c];
+    const double* Fc = fd.F[c];
+    const int* rel = fd.relmap + fd.idxOff[c];
+    int bLo = nfc, bHi = mc;
+    if (monotone) {
+      // rel[] is non-decreasing over [nfc, mc): find the columns landing in [c0, c1)
+      int lo = nfc, hi = mc;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rel[mid] < c0)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      bLo = lo;
+      hi = mc;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rel[mid] < c1)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      bHi = lo;
+    }
+    for (int b = bLo + wv; b < bHi; b += nw) {
+      const int lb = rel[b];
+      if (lb < 0) continue;
+      const double* ccol = Fc + static_cast<size_t>(b) * mc;
+      for (int a = b + lane; a < mc; a += 64) {
+        const int la = rel[a];
+        if (la < 0) continue;
+        const int tr = la > lb ? la : lb;
+        const int tc = la > lb ? lb : la;
+        if (tc < c0 || tc >= c1) continue;
+        F[tr + static_cast<size_t>(tc) * m] += ccol[a];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small fronts: the packed lower triangle lives in LDS; right-looking
+// Bunch-Kaufman with the rank-1/rank-2 update applied eagerly after every
+// pivot, which is the reference's panel_size = 1 schedule for fronts with
+// fewer than 256 eligible columns (multifrontal.hpp:564-566).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSmallThreads) void k_factor_small(const int* list, int count, DevFronts fd,
+                                                                 FactorParams prm) {
+  constexpr int NT = kSmallThreads;
+  extern __shared__ double smem[];
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int m = fd.m[s];
+  const int nE = fd.nE[s];
+  double* F = fd.F[s];
+  const int64_t io = fd.idxOff[s];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = NT / 64;
+
+  double* redV = smem;                                    // NT/64 + 1
+  int* redI = reinterpret_cast<int*>(smem + 32);          // NT/64 + 1
+  double* A = smem + 64;                                  // packed lower, m(m+1)/2
+  const int npk = m * (m + 1) / 2;
+  int* perm = reinterpret_cast<int*>(A + npk);            // m
+
+  auto P = [m](int i, int j) { return j * m - (j * (j - 1)) / 2 + (i - j); };
+
+  double om = 0.0;
+  for (int j = wv; j < m; j += nw) {
+    const double* col = F + static_cast<size_t>(j) * m;
+    const int base = P(j, j) - j;
+    for (int i = j + lane; i < m; i += 64) {
+      const double v = col[i];
+      A[base + i] = v;
+      om = fmax(om, fabs(v));
+    }
+  }
+  for (int i = tid; i < m; i += NT) perm[i] = i;
+  __syncthreads();
+  om = blockMax<NT>(om, redV);
+
+  const bool stat = prm.staticMode != 0;
+  double origMax = om;
+  if (!stat && origMax < prm.zeroTol) origMax = 1.0;        // dense_kernel.hpp:414
+  if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;  // dense_kernel.hpp:935
+  double tmax = origMax;  // thread-local contribution to running_max
+
+  const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+  const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+
+  int npos = 0, nneg = 0, nzero = 0, npert = 0;
+  double pert = 0.0;
+  int k = 0;
+  bool stopped = false;
+
+  // symmetric interchange of positions a < b over the whole front
+  auto swapSym = [&](int a, int b) {
+    for (int t = tid; t < m; t += NT) {
+      int x = -1, y = -1;
+      if (t < a) {
+        x = P(a, t);
+        y = P(b, t);
+      } else if (t == a) {
+        x = P(a, a);
+        y = P(b, b);
+      } else if (t < b) {
+        x = P(t, a);
+        y = P(b, t);
+      } else if (t > b) {
+        x = P(t, a);
+        y = P(t, b);
+      }
+      if (x >= 0) {
+        const double tv = A[x];
+        A[x] = A[y];
+        A[y] = tv;
+      }
+    }
+    if (tid == 0) {
+      const int tp = perm[a];
+      perm[a] = perm[b];
+      perm[b] = tp;
+    }
+    __syncthreads();
+  };
+
+  // accept a 1x1 pivot at k: scale column, eager rank-1 update
+  auto accept1 = [&](double d) {
+    if (tid == 0) {
+      fd.dDiag[io + k] = d;
+      fd.dOff[io + k] = 0.0;
+      fd.pkind[io + k] = 1;
+    }
+    countInertia(d, npos, nneg, nzero);
+    tmax = fmax(tmax, fabs(d));
+    const int ck = P(k, k) - k;
+    for (int i = k + 1 + tid; i < m; i += NT) {
+      const double c = A[ck + i];
+      tmax = fmax(tmax, fabs(c));
+      A[ck + i] = c / d;
+    }
+    __syncthreads();
+    for (int j = k + 1 + wv; j < m; j += nw) {
+      const double lj = A[ck + j];
+      const int cj = P(j, j) - j;
+      for (int i = j + lane; i < m; i += 64) {
+        const double li = A[ck + i];
+        const double v = fma(-(li * d), lj, A[cj + i]);
+        A[cj + i] = v;
+        tmax = fmax(tmax, fabs(v));
+      }
+    }
+    __syncthreads();
+    ++k;
+  };
+
+  if (stat) {
+    const int* sg = prm.expectedSign;
+    while (k < nE) {
+      const double akk = A[P(k, k)];
+      const int sign = sg ? sg[fd.frontIdx[io + k]] : 0;
+      double applied;
+      const double d = staticPivot(akk, sign, delta, sfloor, applied);
+      if (applied != 0.0) {
+        pert += fabs(applied);
+        ++npert;
+      }
+      accept1(d);
+    }
+  } else {
+    while (k < nE) {
+      // lambda = max_{k<i<nE} |A(i,k)| at row r (dense_kernel.hpp:588-596)
+      double lv = 0.0;
+      int li = -1;
+      {
+        const int ck = P(k, k) - k;
+        for (int i = k + 1 + tid; i < nE; i += NT) {
+          const double v = fabs(A[ck + i]);
+          if (v > 0.0) argmaxCombine(lv, li, v, i);
+        }
+      }
+      blockArgmax<NT>(lv, li, redV, redI);
+      const double akk = A[P(k, k)];
+      int action;  // 0 stop, 1 1x1 at k, 2 1x1 after swap k<->r, 3 2x2 at (k, r)
+      const int r = li;
+      if (lv <= prm.zeroTol) {  // dense_kernel.hpp:601-616
+        const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
+        action = (fabs(akk) <= isolatedFloor) ? 0 : 1;
+      } else if (fabs(akk) >= kAlpha * lv) {  // :617
+        action = 1;
+      } else {
+        double sg = 0.0;  // sigma over the eligible rows of column r, i != r (:628-633)
+        for (int i = k + tid; i < nE; i += NT) {
+          if (i == r) continue;
+          const double v = fabs(i > r ? A[P(i, r)] : A[P(r, i)]);
+          sg = fmax(sg, v);
+        }
+        sg = blockMax<NT>(sg, redV);
+        if (fabs(akk) * sg >= kAlpha * lv * lv)  // :641-645
+          action = 1;
+        else if (fabs(A[P(r, r)]) >= kAlpha * sg)
+          action = 2;
+        else
+          action = 3;
+      }
+      if (action == 0) {
+        stopped = true;
+        break;
+      }
+      if (action == 2) swapSym(k, r);
+      if (action == 1 || action == 2) {
+        accept1(A[P(k, k)]);
+        continue;
+      }
+      // 2x2 pivot (dense_kernel.hpp:721-792)
+      if (r != k + 1) swapSym(k + 1, r);
+      const double d11 = A[P(k, k)];
+      const double d21 = A[P(k + 1, k)];
+      const double d22 = A[P(k + 1, k + 1)];
+      const double det = d11 * d22 - d21 * d21;
+      const double rmax = blockMax<NT>(tmax, redV);
+      if (fabs(det) <= prm.zeroTol * fmax(1.0, rmax) * fmax(1.0, rmax)) {
+        stopped = true;
+        break;
+      }
+      tmax = fmax(tmax, fmax(fabs(d11), fmax(fabs(d21), fabs(d22))));
+      if (tid == 0) {
+        fd.dDiag[io + k] = d11;
+        fd.dOff[io + k] = d21;
+        fd.dDiag[io + k + 1] = d22;
+        fd.dOff[io + k + 1] = 0.0;
+        fd.pkind[io + k] = 2;
+        fd.pkind[io + k + 1] = 0;
+      }
+      countInertia2(d11, d22, det, npos, nneg);
+      const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
+      const int c0 = P(k, k) - k, c1 = P(k + 1, k + 1) - (k + 1);
+      for (int i = k + 2 + tid; i < m; i += NT) {
+        const double a0 = A[c0 + i], a1 = A[c1 + i];
+        tmax = fmax(tmax, fmax(fabs(a0), fabs(a1)));
+        A[c0 + i] = a0 * inv11 + a1 * inv21;
+        A[c1 + i] = a0 * inv21 + a1 * inv22;
+      }
+      __syncthreads();
+      for (int j = k + 2 + wv; j < m; j += nw) {
+        const double lj0 = A[c0 + j], lj1 = A[c1 + j];
+        const int cj = P(j, j) - j;
+        for (int i = j + lane; i < m; i += 64) {
+          const double li0 = A[c0 + i], li1 = A[c1 + i];
+          const double w0 = li0 * d11 + li1 * d21;
+          const double w1 = li0 * d21 + li1 * d22;
+          const double v = A[cj + i] - (w0 * lj0 + w1 * lj1);
+          A[cj + i] = v;
+          tmax = fmax(tmax, fabs(v));
+        }
+      }
+      __syncthreads();
+      k += 2;
+    }
+  }
+  (void)stopped;
+
+  // write back
+  for (int j = wv; j < m; j += nw) {
+    double* col = F + static_cast<size_t>(j) * m;
+    const int base = P(j, j) - j;
+    for (int i = j + lane; i < m; i += 64) col[i] = A[base + i];
+  }
+  for (int i = tid; i < m; i += NT) fd.perm[io + i] = perm[i];
+  const double rm = blockMax<NT>(tmax, redV);
+  if (tid == 0) {
+    fd.nF[s] = k;
+    fd.inertia[3 * s] = npos;
+    fd.inertia[3 * s + 1] = nneg;
+    fd.inertia[3 * s + 2] = nzero;
+    fd.origMaxBits[s] = doubleToBits(origMax);
+    fd.runMaxBits[s] = doubleToBits(rm);
+    fd.pert[s] = pert;
+    fd.npert[s] = npert;
+    fd.stopped[s] = (k < nE) ? 1 : 0;
+    fd.kcur[s] = k;
+  }
+}
+
+size_t smallFrontSmemBytes(int m) {
+  return sizeof(double) * (64 + static_cast<size_t>(m) * (m + 1) / 2) + sizeof(int) * (m + 2);
+}
+
+// ---------------------------------------------------------------------------
+// Large fronts.
+// ---------------------------------------------------------------------------
+
+// max|F| over the lower triangle (dense_kernel.hpp:413-415).
+__global__ __launch_bounds__(256) void k_front_maxabs(int s, DevFronts fd) {
+  const int m = fd.m[s];
+  const double* F = fd.F[s];
+  double v = 0.0;
+  const int lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwTot = (gridDim.x * blockDim.x) >> 6;
+  for (int j = gw; j < m; j += nwTot) {
+    const double* col = F + static_cast<size_t>(j) * m;
+    for (int i = j + lane; i < m; i += 64) v = fmax(v, fabs(col[i]));
+  }
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if (lane == 0) atomicMaxNonNeg(fd.origMaxBits + s, v);
+}
+
+// Panel pivoting on the eligible rows [k, nE) of one front.  Columns not yet
+// pivoted are never modified in place: the current value of a candidate
+// column is recomputed from its stored (pre-panel) values minus the
+// correction of this panel's pivots, like the reference's setColumnRaw
+// (dense_kernel.hpp:516-545).  Accepted pivots write their multipliers for
+// the eligible rows; the remaining rows are solved by k_panel_rows.
+__global__ __launch_bounds__(kPanelThreads) void k_panel(int s, DevFronts fd, FactorParams prm, double* ws) {
+  constexpr int NT = kPanelThreads;
+  __shared__ double wvec[kPanel + 2];
+  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
+  __shared__ int pkd[kPanel + 2];
+  __shared__ double redV[NT / 64 + 1];
+  __shared__ int redI[NT / 64 + 1];
+
+  const int tid = threadIdx.x;
+  if (fd.stopped[s]) {
+    if (tid == 0) fd.pcount[s] = 0;
+    return;
+  }
+  const int m = fd.m[s];
+  const int nE = fd.nE[s];
+  double* F = fd.F[s];
+  const int64_t io = fd.idxOff[s];
+  const int k0 = fd.kcur[s];
+  if (k0 >= nE) {
+    if (tid == 0) fd.pcount[s] = 0;
+    return;
+  }
+  const int nbEnd = min(k0 + kPanel, nE);
+  double* curK = ws;
+  double* curR = ws + nE;
+  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+
+  const bool stat = prm.staticMode != 0;
+  double origMax = bitsToDouble(fd.origMaxBits[s]);
+  if (!stat && origMax < prm.zeroTol) origMax = 1.0;
+  if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
+  const double runMax0 = fmax(origMax, bitsToDouble(fd.runMaxBits[s]));
+  double tmax = 0.0;
+  const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+  const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+
+  int npos = 0, nneg = 0, nzero = 0, npert = 0;
+  double pert = 0.0;
+  int k = k0, p = 0;
+  bool stop = false;
+
+  // wvec[q] = (D L^T)(k0+q, row) for this panel's p pivots
+  auto computeW = [&](int row) {
+    for (int q = tid; q < p; q += NT) {
+      const int kind = pkd[q];
+      double w;
+      if (kind == 1) {
+        w = pdd[q] * Fat(row, k0 + q);
+      } else if (kind == 2) {
+        w = pdd[q] * Fat(row, k0 + q) + pdo[q] * Fat(row, k0 + q + 1);
+      } else {
+        w = pdo[q - 1] * Fat(row, k0 + q - 1) + pdd[q] * Fat(row, k0 + q);
+      }
+      wvec[q] = w;
+    }
+    __syncthreads();
+  };
+  // out[i] = stored column `col` (rows >= k) minus the panel correction
+  auto currentColumn = [&](int col, double* out) {
+    for (int i = k + tid; i < nE; i += NT) {
+      const double raw = (i >= col) ? Fat(i, col) : Fat(col, i);
+      double acc = 0.0;
+      for (int q = 0; q < p; ++q) acc = fma(Fat(i, k0 + q), wvec[q], acc);
+      out[i] = raw - acc;
+    }
+    __syncthreads();
+  };
+  // symmetric interchange of positions a < b in storage (all rows, incl. the
+  // factored part and the not-fully-summed rows) and in the column buffers
+  auto swapSym = [&](int a, int b) {
+    for (int t = tid; t < m; t += NT) {
+      if (t < a) {
+        const double x = Fat(a, t);
+        Fat(a, t) = Fat(b, t);
+        Fat(b, t) = x;
+      } else if (t == a) {
+        const double x = Fat(a, a);
+        Fat(a, a) = Fat(b, b);
+        Fat(b, b) = x;
+      } else if (t < b) {
+        const double x = Fat(t, a);
+        Fat(t, a) = Fat(b, t);
+        Fat(b, t) = x;
+      } else if (t > b) {
+        const double x = Fat(t, a);
+        Fat(t, a) = Fat(t, b);
+        Fat(t, b) = x;
+      }
+    }
+    if (tid == 0) {
+      const int tp = fd.perm[io + a];
+      fd.perm[io + a] = fd.perm[io + b];
+      fd.perm[io + b] = tp;
+      double x = curK[a];
+      curK[a] = curK[b];
+      curK[b] = x;
+      x = curR[a];
+      curR[a] = curR[b];
+      curR[b] = x;
+    }
+    __syncthreads();
+  };
+  auto accept1 = [&](const double* col, double d) {
+    if (tid == 0) {
+      fd.dDiag[io + k] = d;
+      fd.dOff[io + k] = 0.0;
+      fd.pkind[io + k] = 1;
+      pdd[p] = d;
+      pdo[p] = 0.0;
+      pkd[p] = 1;
+      Fat(k, k) = d;
+    }
+    countInertia(d, npos, nneg, nzero);
+    tmax = fmax(tmax, fabs(d));
+    for (int i = k + 1 + tid; i < nE; i += NT) {
+      const double c = col[i];
+      tmax = fmax(tmax, fabs(c));
+      Fat(i, k) = c / d;
+    }
+    __syncthreads();
+    ++k;
+    ++p;
+  };
+
+  while (k < nbEnd) {
+    computeW(k);
+    currentColumn(k, curK);
+    if (stat) {
+      const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k]] : 0;
+      double applied;
+      const double d = staticPivot(curK[k], sign, delta, sfloor, applied);
+      if (applied != 0.0) {
+        pert += fabs(applied);
+        ++npert;
+      }
+      accept1(curK, d);
+      continue;
+    }
+    double lv = 0.0;
+    int li = -1;
+    for (int i = k + 1 + tid; i < nE; i += NT) {
+      const double v = fabs(curK[i]);
+      if (v > 0.0) argmaxCombine(lv, li, v, i);
+    }
+    blockArgmax<NT>(lv, li, redV, redI);
+    const double akk = curK[k];
+    const int r = li;
+    int action;
+    if (lv <= prm.zeroTol) {
+      const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
+      action = (fabs(akk) <= isolatedFloor) ? 0 : 1;
+    } else if (fabs(akk) >= kAlpha * lv) {
+      action = 1;
+    } else {
+      computeW(r);
+      currentColumn(r, curR);
+      // the reference mirrors column r into row r, so column k's entry at
+      // row r becomes column r's value at row k (dense_kernel.hpp:543)
+      if (tid == 0) curK[r] = curR[k];
+      __syncthreads();
+      double sg = 0.0;
+      for (int i = k + tid; i < nE; i += NT)
+        if (i != r) sg = fmax(sg, fabs(curR[i]));
+      sg = blockMax<NT>(sg, redV);
+      if (fabs(akk) * sg >= kAlpha * lv * lv)
+        action = 1;
+      else if (fabs(curR[r]) >= kAlpha * sg)
+        action = 2;
+      else
+        action = 3;
+    }
+    if (action == 0) {
+      stop = true;
+      break;
+    }
+    if (action == 1) {
+      accept1(curK, curK[k]);
+      continue;
+    }
+    if (action == 2) {
+      swapSym(k, r);
+      accept1(curR, curR[k]);
+      continue;
+    }
+    // 2x2 at (k, r): bring r to k+1
+    if (r != k + 1) swapSym(k + 1, r);
+    const double d11 = curK[k];
+    const double d21 = curK[k + 1];
+    const double d22 = curR[k + 1];
+    const double det = d11 * d22 - d21 * d21;
+    const double rmax = fmax(runMax0, blockMax<NT>(tmax, redV));
+    if (fabs(det) <= prm.zeroTol * fmax(1.0, rmax) * fmax(1.0, rmax)) {
+      stop = true;
+      break;
+    }
+    tmax = fmax(tmax, fmax(fabs(d11), fmax(fabs(d21), fabs(d22))));
+    if (tid == 0) {
+      fd.dDiag[io + k] = d11;
+      fd.dOff[io + k] = d21;
+      fd.dDiag[io + k + 1] = d22;
+      fd.dOff[io + k + 1] = 0.0;
+      fd.pkind[io + k] = 2;
+      fd.pkind[io + k + 1] = 0;
+      pdd[p] = d11;
+      pdo[p] = d21;
+      pkd[p] = 2;
+      pdd[p + 1] = d22;
+      pdo[p + 1] = 0.0;
+      pkd[p + 1] = 0;
+      Fat(k, k) = d11;
+      Fat(k + 1, k) = d21;
+      Fat(k + 1, k + 1) = d22;
+    }
+    countInertia2(d11, d22, det, npos, nneg);
+    const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
+    for (int i = k + 2 + tid; i < nE; i += NT) {
+      const double a0 = curK[i], a1 = curR[i];
+      tmax = fmax(tmax, fmax(fabs(a0), fabs(a1)));
+      Fat(i, k) = a0 * inv11 + a1 * inv21;
+      Fat(i, k + 1) = a0 * inv21 + a1 * inv22;
+    }
+    __syncthreads();
+    k += 2;
+    p += 2;
+  }
+
+  const double bm = blockMax<NT>(tmax, redV);
+  if (tid == 0) {
+    fd.kcur[s] = k;
+    fd.pcount[s] = p;
+    fd.stopped[s] = stop ? 1 : 0;
+    fd.inertia[3 * s] += npos;
+    fd.inertia[3 * s + 1] += nneg;
+    fd.inertia[3 * s + 2] += nzero;
+    fd.pert[s] += pert;
+    fd.npert[s] += npert;
+    atomicMaxNonNeg(fd.runMaxBits + s, fmax(bm, origMax));
+  }
+}
+
+// Multipliers of the rows that are not fully summed ([nE, m)) for the last
+// panel: row-parallel forward substitution against U = D L^T of the panel.
+__global__ __launch_bounds__(256) void k_panel_rows(int s, DevFronts fd) {
+  __shared__ double U[(kPanel + 1) * (kPanel + 1)];
+  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
+  __shared__ int pkd[kPanel + 2];
+  const int p = fd.pcount[s];
+  if (p == 0) return;
+  const int m = fd.m[s];
+  const int nE = fd.nE[s];
+  const int k1 = fd.kcur[s];
+  const int k0 = k1 - p;
+  double* F = fd.F[s];
+  const int64_t io = fd.idxOff[s];
+  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+  const int P1 = kPanel + 1;
+  for (int q = threadIdx.x; q < p; q += blockDim.x) {
+    pdd[q] = fd.dDiag[io + k0 + q];
+    pdo[q] = fd.dOff[io + k0 + q];
+    pkd[q] = fd.pkind[io + k0 + q];
+  }
+  __syncthreads();
+  // U(q', q) for q' before the block containing q
+  for (int t = threadIdx.x; t < p * p; t += blockDim.x) {
+    const int qp = t % p, q = t / p;
+    double u = 0.0;
+    if (qp < q) {
+      const int kind = pkd[qp];
+      if (kind == 1)
+        u = pdd[qp] * Fat(k0 + q, k0 + qp);
+      else if (kind == 2)
+        u = (q > qp + 1) ? pdd[qp] * Fat(k0 + q, k0 + qp) + pdo[qp] * Fat(k0 + q, k0 + qp + 1) : 0.0;
+      else
+        u = pdo[qp - 1] * Fat(k0 + q, k0 + qp - 1) + pdd[qp] * Fat(k0 + q, k0 + qp);
+    }
+    U[qp + q * P1] = u;
+  }
+  __syncthreads();
+  const int i = nE + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double mx = 0.0;
+  int q = 0;
+  while (q < p) {
+    if (pkd[q] == 1) {
+      double x = Fat(i, k0 + q);
+      for (int qp = 0; qp < q; ++qp) x = fma(-Fat(i, k0 + qp), U[qp + q * P1], x);
+      mx = fmax(mx, fabs(x));
+      Fat(i, k0 + q) = x / pdd[q];
+      ++q;
+    } else {
+      double x0 = Fat(i, k0 + q), x1 = Fat(i, k0 + q + 1);
+      for (int qp = 0; qp < q; ++qp) {
+        const double l = Fat(i, k0 + qp);
+        x0 = fma(-l, U[qp + q * P1], x0);
+        x1 = fma(-l, U[qp + (q + 1) * P1], x1);
+      }
+      mx = fmax(mx, fmax(fabs(x0), fabs(x1)));
+      const double d11 = pdd[q], d21 = pdo[q], d22 = pdd[q + 1];
+      const double det = d11 * d22 - d21 * d21;
+      const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
+      Fat(i, k0 + q) = x0 * inv11 + x1 * inv21;
+      Fat(i, k0 + q + 1) = x0 * inv21 + x1 * inv22;
+      q += 2;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
+}
+
+// Trailing update T(k1:m, k1:m) -= (L D) L^T over the lower triangle, for
+// the last panel's p pivots, with v_mfma_f64_16x16x4f64.  64x64 output tile
+// per workgroup, four waves of 32x32.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
+  constexpr int TS = 64;
+  constexpr int KP = kPanel + 4;  // padded K
+  __shared__ double As[TS][KP + 1];
+  __shared__ double Bs[TS][KP + 1];
+  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
+  __shared__ int pkd[kPanel + 2];
+  const int p = fd.pcount[s];
+  if (p == 0) return;
+  const int m = fd.m[s];
+  const int k1 = fd.kcur[s];
+  const int k0 = k1 - p;
+  const int N = m - k1;
+  if (N <= 0) return;
+  const int T = (N + TS - 1) / TS;
+  // lower-triangular tile index
+  const int t = blockIdx.x;
+  int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  if (ti >= T) return;
+  const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
+  double* F = fd.F[s];
+  const int64_t io = fd.idxOff[s];
+  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+  for (int q = threadIdx.x; q < p; q += blockDim.x) {
+    pdd[q] = fd.dDiag[io + k0 + q];
+    pdo[q] = fd.dOff[io + k0 + q];
+    pkd[q] = fd.pkind[io + k0 + q];
+  }
+  __syncthreads();
+  const int pp = (p + 3) & ~3;
+  for (int e = threadIdx.x; e < TS * pp; e += blockDim.x) {
+    const int r = e % TS, q = e / TS;
+    double a = 0.0, b = 0.0;
+    if (q < p) {
+      const int ri = i0 + r, rj = j0 + r;
+      if (ri < m) {
+        const int kind = pkd[q];
+        if (kind == 1)
+          a = Fat(ri, k0 + q) * pdd[q];
+        else if (kind == 2)
+          a = Fat(ri, k0 + q) * pdd[q] + Fat(ri, k0 + q + 1) * pdo[q];
+        else
+          a = Fat(ri, k0 + q - 1) * pdo[q - 1] + Fat(ri, k0 + q) * pdd[q];
+      }
+      if (rj < m) b = Fat(rj, k0 + q);
+    }
+    As[r][q] = a;
+    Bs[r][q] = b;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  v4d acc[2][2];
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int kk = 0; kk < pp; kk += 4) {
+    const double a0 = As[wr + lr][kk + lk];
+    const double a1 = As[wr + 16 + lr][kk + lk];
+    const double b0 = Bs[wc + lr][kk + lk];
+    const double b1 = Bs[wc + 16 + lr][kk + lk];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+  double mx = 0.0;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = i0 + wr + bi * 16 + 4 * lk + rr;
+        const int col = j0 + wc + bj * 16 + lr;
+        if (row < m && col < m && (ti != tj || row >= col)) {
+          const double v = Fat(row, col) - acc[bi][bj][rr];
+          Fat(row, col) = v;
+          mx = fmax(mx, fabs(v));
+        }
+      }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
+}
+
+__global__ void k_front_finish(int s, DevFronts fd) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) fd.nF[s] = fd.kcur[s];
+}
+
+// rowIdx = frontIdx[perm[i]] for every position of the listed fronts.
+__global__ void k_make_rowidx(const int* list, int count, DevFronts fd) {
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int m = fd.m[s];
+  const int64_t io = fd.idxOff[s];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) fd.rowIdx[io + i] = fd.frontIdx[io + fd.perm[io + i]];
+}
+
+// Sum of the per-front statistics (multifrontal.hpp:795-801); one block.
+__global__ void k_reduce_stats(int numSN, DevFronts fd, long long* outI, double* outD) {
+  __shared__ long long sI[5][256];
+  __shared__ double sD[256];
+  long long pos = 0, neg = 0, zero = 0, delayedFronts = 0, npert = 0;
+  double pert = 0.0;
+  for (int s = threadIdx.x; s < numSN; s += blockDim.x) {
+    pos += fd.inertia[3 * s];
+    neg += fd.inertia[3 * s + 1];
+    zero += fd.inertia[3 * s + 2];
+    delayedFronts += (fd.nF[s] < fd.nE[s]) ? 1 : 0;
+    npert += fd.npert[s];
+  }
+  sI[0][threadIdx.x] = pos;
+  sI[1][threadIdx.x] = neg;
+  sI[2][threadIdx.x] = zero;
+  sI[3][threadIdx.x] = delayedFronts;
+  sI[4][threadIdx.x] = npert;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long r[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < static_cast<int>(blockDim.x); ++t)
+      for (int q = 0; q < 5; ++q) r[q] += sI[q][t];
+    for (int q = 0; q < 5; ++q) outI[q] = r[q];
+    // perturbation sum in front order (deterministic)
+    for (int s = 0; s < numSN; ++s) pert += fd.pert[s];
+    outD[0] = pert;
+  }
+  (void)sD;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launchResetFronts(int numSN, const DevFronts& fd, hipStream_t st) {
+  if (numSN <= 0) return;
+  const int blocks = (numSN + 255) / 256;
+  k_reset_fronts<<<blocks, 256, 0, st>>>(numSN, fd);
+}
+void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_t st) {
+  if (count <= 0) return;
+  k_reset_perm<<<count, 256, 0, st>>>(list, count, fd);
+}
+void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& fd, const double* aVals, int monotone,
+                    hipStream_t st) {
+  if (ntasks <= 0) return;
+  k_assemble<<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+}
+void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
+                       hipStream_t st) {
+  if (count <= 0) return;
+  const size_t smem = smallFrontSmemBytes(maxM);
+  static bool attrSet = false;
+  if (!attrSet) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_small),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attrSet = true;
+  }
+  k_factor_small<<<count, kSmallThreads, smem, st>>>(list, count, fd, prm);
+}
+void launchFrontMaxAbs(int s, int m, const DevFronts& fd, hipStream_t st) {
+  const int waves = m;
+  int blocks = (waves + 3) / 4;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  k_front_maxabs<<<blocks, 256, 0, st>>>(s, fd);
+}
+void launchPanel(int s, const DevFronts& fd, const FactorParams& prm, double* ws, hipStream_t st) {
+  k_panel<<<1, kPanelThreads, 0, st>>>(s, fd, prm, ws);
+}
+void launchPanelRows(int s, int rows, const DevFronts& fd, hipStream_t st) {
+  if (rows <= 0) return;
+  k_panel_rows<<<(rows + 255) / 256, 256, 0, st>>>(s, fd);
+}
+void launchSyrk(int s, int maxTrailing, const DevFronts& fd, hipStream_t st) {
+  if (maxTrailing <= 0) return;
+  const int T = (maxTrailing + 63) / 64;
+  const int tiles = T * (T + 1) / 2;
+  k_syrk_f64<<<tiles, 256, 0, st>>>(s, fd);
+}
+void launchFrontFinish(int s, const DevFronts& fd, hipStream_t st) { k_front_finish<<<1, 64, 0, st>>>(s, fd); }
+void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st) {
+  if (count <= 0) return;
+  k_make_rowidx<<<count, 256, 0, st>>>(list, count, fd);
+}
+void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st) {
+  k_reduce_stats<<<1, 256, 0, st>>>(numSN, fd, outI, outD);
+}
+
+}  // namespace symla_hip

@@ -1,0 +1,42 @@
+// kernels.hpp -- host-side launchers of the numeric-phase kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace symla_hip {
+
+// factor_kernels.hip
+size_t smallFrontSmemBytes(int m);
+void launchResetFronts(int numSN, const DevFronts& fd, hipStream_t st);
+void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_t st);
+void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& fd, const double* aVals, int monotone,
+                    hipStream_t st);
+void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
+                       hipStream_t st);
+void launchFrontMaxAbs(int s, int m, const DevFronts& fd, hipStream_t st);
+void launchPanel(int s, const DevFronts& fd, const FactorParams& prm, double* ws, hipStream_t st);
+void launchPanelRows(int s, int rows, const DevFronts& fd, hipStream_t st);
+void launchSyrk(int s, int maxTrailing, const DevFronts& fd, hipStream_t st);
+void launchFrontFinish(int s, const DevFronts& fd, hipStream_t st);
+void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st);
+void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st);
+
+// solve_kernels.hip
+void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st);
+void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st);
+void launchFwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* Z,
+               double* G, double* Dl, const int64_t* gOff, hipStream_t st);
+void launchBwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
+               double* Dl, const int64_t* gOff, hipStream_t st);
+void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
+                    const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
+void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);
+void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st);
+void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
+                     unsigned long long* outBits, hipStream_t st);
+void launchZeroU64(unsigned long long* p, int count, hipStream_t st);
+
+}  // namespace symla_hip

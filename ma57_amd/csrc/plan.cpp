@@ -1,0 +1,872 @@
+// plan.cpp -- device pool, level schedule, factorization driver, solve and
+// refinement of the MI355X numeric phase.
+//
+// Factorization runs level by level over the supernode tree (children before
+// parents; multifrontal.hpp:13-18 makes supernode index order a postorder).
+// The static schedule assumes no delayed pivots and runs every level without
+// a host round trip.  One readback at the end tells whether some front
+// delayed columns to its parent; only then a host-driven pass re-runs the
+// factorization level by level, sizing each front with the delayed columns it
+// actually receives (multifrontal.hpp:453-479).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace symla_hip {
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    std::ostringstream o;
+    o << "HIP error in " << what << ": " << hipGetErrorString(e);
+    throw DeviceError(o.str());
+  }
+}
+
+namespace {
+double bitsToDoubleHost(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+}  // namespace
+
+Plan::Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx) {
+  n_ = sym.n;
+  perm_ = sym.perm;
+  sn_ = sym.supernodes;
+  numSN_ = static_cast<int>(sn_.size());
+  hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
+  hipCheck(hipEventCreate(&ev0_), "hipEventCreate");
+  hipCheck(hipEventCreate(&ev1_), "hipEventCreate");
+
+  // supernode tree (multifrontal.hpp:312-332)
+  colToSN_.assign(n_, -1);
+  for (int s = 0; s < numSN_; ++s)
+    for (int c = sn_[s].firstCol; c < sn_[s].firstCol + sn_[s].ncols; ++c) colToSN_[c] = s;
+  parentSN_.assign(numSN_, -1);
+  children_.assign(numSN_, {});
+  for (int s = 0; s < numSN_; ++s) {
+    const int last = sn_[s].firstCol + sn_[s].ncols - 1;
+    const int p = sym.etreeParent[last];
+    if (p != -1) {
+      parentSN_[s] = colToSN_[p];
+      children_[colToSN_[p]].push_back(s);
+    }
+  }
+  for (int s = 0; s < numSN_; ++s)
+    if (parentSN_[s] == -1) roots_.push_back(s);
+  level_.assign(numSN_, 0);
+  int maxLevel = 0;
+  for (int s = 0; s < numSN_; ++s) {
+    int l = 0;
+    for (int c : children_[s]) l = std::max(l, level_[c] + 1);
+    level_[s] = l;
+    maxLevel = std::max(maxLevel, l);
+  }
+  levelFronts_.assign(numSN_ ? maxLevel + 1 : 0, {});
+  for (int s = 0; s < numSN_; ++s) levelFronts_[level_[s]].push_back(s);
+
+  // A pattern
+  aColptr_.assign(colptr, colptr + n_ + 1);
+  nnzA_ = aColptr_[n_];
+  aRowidx_.assign(rowidx, rowidx + nnzA_);
+  for (int64_t t = 0; t < nnzA_; ++t)
+    if (aRowidx_[t] < 0 || aRowidx_[t] >= n_) throw std::invalid_argument("symla: row index out of range");
+
+  buildStaticLayout();
+  // per-front state
+  dnF_.alloc(numSN_);
+  dinertia_.alloc(3 * static_cast<size_t>(numSN_));
+  dkcur_.alloc(numSN_);
+  dpcount_.alloc(numSN_);
+  dstopped_.alloc(numSN_);
+  dnpert_.alloc(numSN_);
+  dpert_.alloc(numSN_);
+  dorigMax_.alloc(numSN_);
+  drunMax_.alloc(numSN_);
+  dstatI_.alloc(8);
+  dstatD_.alloc(4);
+  dvals_.alloc(nnzA_);
+  uploadLayout();
+  buildSchedule();
+  buildRefineCsr();
+  {
+    std::vector<int> pv(perm_);
+    dpermVec_.upload(pv, stream_);
+  }
+  dnorms_.alloc(16);
+  hipCheck(hipStreamSynchronize(stream_), "plan setup");
+}
+
+Plan::~Plan() {
+  for (double* p : dynAllocs_) (void)hipFree(p);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Static layout: front s is [own columns | symbolic update rows] =
+// rowPattern (symbolic.hpp:138-147), F_s is m x m column-major in the pool.
+void Plan::buildStaticLayout() {
+  hm_.resize(numSN_);
+  hnE_.resize(numSN_);
+  hncols_.resize(numSN_);
+  hfirst_.resize(numSN_);
+  hidxOff_.assign(numSN_ + 1, 0);
+  for (int s = 0; s < numSN_; ++s) {
+    hm_[s] = static_cast<int>(sn_[s].rowPattern.size());
+    hnE_[s] = sn_[s].ncols;
+    hncols_[s] = sn_[s].ncols;
+    hfirst_[s] = sn_[s].firstCol;
+    hidxOff_[s + 1] = hidxOff_[s] + hm_[s];
+  }
+  idxUsed_ = hidxOff_[numSN_];
+  hfrontIdx_.assign(idxUsed_, 0);
+  hrelmap_.assign(idxUsed_, -1);
+  for (int s = 0; s < numSN_; ++s)
+    std::copy(sn_[s].rowPattern.begin(), sn_[s].rowPattern.end(), hfrontIdx_.begin() + hidxOff_[s]);
+  for (int c = 0; c < numSN_; ++c) {
+    const int p = parentSN_[c];
+    if (p < 0) continue;
+    const auto& pp = sn_[p].rowPattern;
+    for (int i = sn_[c].ncols; i < hm_[c]; ++i) {
+      const int g = sn_[c].rowPattern[i];
+      auto it = std::lower_bound(pp.begin(), pp.end(), g);
+      if (it == pp.end() || *it != g) throw std::runtime_error("symla: child row missing from parent pattern");
+      hrelmap_[hidxOff_[c] + i] = static_cast<int>(it - pp.begin());
+    }
+  }
+  hchildPtr_.assign(numSN_ + 1, 0);
+  hchildList_.clear();
+  for (int s = 0; s < numSN_; ++s) {
+    for (int c : children_[s]) hchildList_.push_back(c);
+    hchildPtr_[s + 1] = static_cast<int>(hchildList_.size());
+  }
+  // A scatter map (PermutedLower::build, multifrontal.hpp:199-242): lower
+  // entries of the original matrix, placed by their permuted column.
+  std::vector<int> inv(n_);
+  for (int k = 0; k < n_; ++k) inv[perm_[k]] = k;
+  std::vector<int64_t> cnt(n_ + 1, 0);
+  for (int c = 0; c < n_; ++c)
+    for (int64_t t = aColptr_[c]; t < aColptr_[c + 1]; ++t) {
+      const int r = aRowidx_[t];
+      if (r < c) continue;
+      ++cnt[std::min(inv[r], inv[c]) + 1];
+    }
+  hacolPtr_.assign(n_ + 1, 0);
+  for (int k = 0; k < n_; ++k) hacolPtr_[k + 1] = hacolPtr_[k] + cnt[k + 1];
+  std::vector<int64_t> pos(hacolPtr_.begin(), hacolPtr_.end() - 1);
+  std::vector<int> hiRow(hacolPtr_[n_]);
+  haSrc_.assign(hacolPtr_[n_], 0);
+  for (int c = 0; c < n_; ++c)
+    for (int64_t t = aColptr_[c]; t < aColptr_[c + 1]; ++t) {
+      const int r = aRowidx_[t];
+      if (r < c) continue;
+      const int lo = std::min(inv[r], inv[c]), hi = std::max(inv[r], inv[c]);
+      hiRow[pos[lo]] = hi;
+      haSrc_[pos[lo]++] = t;
+    }
+  haRow_.assign(hacolPtr_[n_], 0);
+  for (int k = 0; k < n_; ++k) {
+    const int64_t b = hacolPtr_[k], e = hacolPtr_[k + 1];
+    std::vector<int64_t> ord(e - b);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hiRow[b + x] < hiRow[b + y]; });
+    std::vector<int> hr(e - b);
+    std::vector<int64_t> sr(e - b);
+    for (size_t q = 0; q < ord.size(); ++q) {
+      hr[q] = hiRow[b + ord[q]];
+      sr[q] = haSrc_[b + ord[q]];
+    }
+    const int s = colToSN_[k];
+    const auto& pat = sn_[s].rowPattern;
+    for (size_t q = 0; q < ord.size(); ++q) {
+      auto it = std::lower_bound(pat.begin(), pat.end(), hr[q]);
+      if (it == pat.end() || *it != hr[q]) throw std::runtime_error("symla: A entry outside the front pattern");
+      haRow_[b + q] = static_cast<int>(it - pat.begin());
+      haSrc_[b + q] = sr[q];
+    }
+  }
+  // pool
+  size_t total = 0;
+  std::vector<size_t> off(numSN_);
+  for (int s = 0; s < numSN_; ++s) {
+    off[s] = total;
+    total += static_cast<size_t>(hm_[s]) * hm_[s];
+  }
+  if (pool_.n < total) pool_.alloc(total);
+  hF_.resize(numSN_);
+  for (int s = 0; s < numSN_; ++s) hF_[s] = pool_.p + off[s];
+  maxNE_ = 0;
+  for (int s = 0; s < numSN_; ++s) maxNE_ = std::max(maxNE_, hnE_[s]);
+  dws_.ensure(2 * static_cast<size_t>(std::max(maxNE_, 1)));
+  for (double* p : dynAllocs_) (void)hipFree(p);
+  dynAllocs_.clear();
+}
+
+void Plan::ensureIdxCapacity(int64_t need) {
+  const size_t cap = dperm_.n;
+  if (static_cast<size_t>(need) <= cap) return;
+  const size_t ncap = std::max<size_t>(need, 2 * cap);
+  auto grow = [&](auto& buf) {
+    using T = std::remove_pointer_t<decltype(buf.p)>;
+    T* np = nullptr;
+    hipCheck(hipMalloc(reinterpret_cast<void**>(&np), ncap * sizeof(T)), "hipMalloc grow");
+    if (buf.p && idxUsed_ > 0)
+      hipCheck(hipMemcpyAsync(np, buf.p, idxUsed_ * sizeof(T), hipMemcpyDeviceToDevice, stream_), "grow copy");
+    hipCheck(hipStreamSynchronize(stream_), "grow sync");
+    if (buf.p) (void)hipFree(buf.p);
+    buf.p = np;
+    buf.n = ncap;
+  };
+  grow(dperm_);
+  grow(drowIdx_);
+  grow(dpkind_);
+  grow(ddDiag_);
+  grow(ddOff_);
+}
+
+void Plan::uploadLayout() {
+  dm_.upload(hm_, stream_);
+  dnE_.upload(hnE_, stream_);
+  dncols_.upload(hncols_, stream_);
+  dfirst_.upload(hfirst_, stream_);
+  didxOff_.upload(hidxOff_, stream_);
+  dfrontIdx_.upload(hfrontIdx_, stream_);
+  drelmap_.upload(hrelmap_, stream_);
+  dchildPtr_.upload(hchildPtr_, stream_);
+  dchildList_.upload(hchildList_, stream_);
+  dacolPtr_.upload(hacolPtr_, stream_);
+  daRow_.upload(haRow_, stream_);
+  daSrc_.upload(haSrc_, stream_);
+  std::vector<double*> fp(hF_);
+  dF_.upload(fp, stream_);
+  const size_t cap = std::max<size_t>(static_cast<size_t>(idxUsed_), 1);
+  if (dperm_.n < cap) {
+    dperm_.alloc(cap);
+    drowIdx_.alloc(cap);
+    dpkind_.alloc(cap);
+    ddDiag_.alloc(cap);
+    ddOff_.alloc(cap);
+  }
+  // solve workspaces depend on the layout
+  gOffNrhs_ = -1;
+  hipCheck(hipStreamSynchronize(stream_), "uploadLayout");
+}
+
+void Plan::buildSchedule() {
+  std::vector<int> small, all;
+  std::vector<int2> tasks;
+  sched_.assign(levelFronts_.size(), {});
+  levelFrontOff_.assign(levelFronts_.size() + 1, 0);
+  for (size_t L = 0; L < levelFronts_.size(); ++L) {
+    auto& ls = sched_[L];
+    ls.smallOff = static_cast<int64_t>(small.size());
+    ls.taskOff = static_cast<int64_t>(tasks.size());
+    levelFrontOff_[L] = static_cast<int64_t>(all.size());
+    for (int s : levelFronts_[L]) {
+      all.push_back(s);
+      for (int c0 = 0; c0 < hm_[s]; c0 += 64) tasks.push_back(make_int2(s, c0));
+      if (hm_[s] <= kSmallFrontMax) {
+        small.push_back(s);
+        ls.smallMaxM = std::max(ls.smallMaxM, hm_[s]);
+      } else {
+        ls.large.push_back(s);
+      }
+    }
+    ls.smallCount = static_cast<int>(small.size() - ls.smallOff);
+    ls.taskCount = static_cast<int>(tasks.size() - ls.taskOff);
+  }
+  levelFrontOff_[levelFronts_.size()] = static_cast<int64_t>(all.size());
+  dsmall_.upload(small, stream_);
+  dtasks_.upload(tasks, stream_);
+  dallFronts_.upload(all, stream_);
+  hipCheck(hipStreamSynchronize(stream_), "buildSchedule");
+}
+
+// Symmetric CSR (both triangles from the stored lower entries, the
+// selfadjointView<Lower> of refine.hpp:116) and the stored entries grouped by
+// row (sparseMaxAbsRowSum, refine.hpp:69-81).
+void Plan::buildRefineCsr() {
+  std::vector<int64_t> symCnt(n_ + 1, 0), stCnt(n_ + 1, 0);
+  for (int c = 0; c < n_; ++c)
+    for (int64_t t = aColptr_[c]; t < aColptr_[c + 1]; ++t) {
+      const int r = aRowidx_[t];
+      ++stCnt[r + 1];
+      if (r < c) continue;
+      ++symCnt[r + 1];
+      if (r != c) ++symCnt[c + 1];
+    }
+  for (int i = 0; i < n_; ++i) {
+    symCnt[i + 1] += symCnt[i];
+    stCnt[i + 1] += stCnt[i];
+  }
+  std::vector<int64_t> symSrc(symCnt[n_]), stSrc(stCnt[n_]);
+  std::vector<int> symCol(symCnt[n_]);
+  std::vector<int64_t> ps(symCnt.begin(), symCnt.end() - 1), pt(stCnt.begin(), stCnt.end() - 1);
+  for (int c = 0; c < n_; ++c)
+    for (int64_t t = aColptr_[c]; t < aColptr_[c + 1]; ++t) {
+      const int r = aRowidx_[t];
+      stSrc[pt[r]++] = t;
+      if (r < c) continue;
+      symCol[ps[r]] = c;
+      symSrc[ps[r]++] = t;
+      if (r != c) {
+        symCol[ps[c]] = r;
+        symSrc[ps[c]++] = t;
+      }
+    }
+  dsymPtr_.upload(symCnt, stream_);
+  dsymCol_.upload(symCol, stream_);
+  dsymSrc_.upload(symSrc, stream_);
+  dstPtr_.upload(stCnt, stream_);
+  dstSrc_.upload(stSrc, stream_);
+}
+
+DevFronts Plan::devFronts() {
+  DevFronts f;
+  f.m = dm_.p;
+  f.nE = dnE_.p;
+  f.ncols = dncols_.p;
+  f.F = dF_.p;
+  f.idxOff = didxOff_.p;
+  f.frontIdx = dfrontIdx_.p;
+  f.relmap = drelmap_.p;
+  f.childPtr = dchildPtr_.p;
+  f.childList = dchildList_.p;
+  f.firstCol = dfirst_.p;
+  f.acolPtr = dacolPtr_.p;
+  f.aRow = daRow_.p;
+  f.aSrc = daSrc_.p;
+  f.perm = dperm_.p;
+  f.rowIdx = drowIdx_.p;
+  f.dDiag = ddDiag_.p;
+  f.dOff = ddOff_.p;
+  f.pkind = dpkind_.p;
+  f.nF = dnF_.p;
+  f.inertia = dinertia_.p;
+  f.origMaxBits = dorigMax_.p;
+  f.runMaxBits = drunMax_.p;
+  f.kcur = dkcur_.p;
+  f.pcount = dpcount_.p;
+  f.stopped = dstopped_.p;
+  f.pert = dpert_.p;
+  f.npert = dnpert_.p;
+  return f;
+}
+
+FactorParams Plan::params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const {
+  FactorParams p;
+  p.zeroTol = o.zeroTol;
+  p.relFloor = o.relFloor;
+  p.staticMode = staticMode ? 1 : 0;
+  p.staticDelta = delta;
+  p.staticRelFloor = o.staticRelFloor;
+  p.staticAbsFloor = o.staticAbsFloor;
+  p.expectedSign = signDev;
+  return p;
+}
+
+void Plan::factorLargeFront(int s, const FactorParams& prm) {
+  DevFronts fd = devFronts();
+  const int m = hm_[s], nE = hnE_[s];
+  dws_.ensure(2 * static_cast<size_t>(std::max(nE, 1)));
+  launchFrontMaxAbs(s, m, fd, stream_);
+  const int iters = (nE + kPanel - 1) / kPanel;
+  for (int it = 0; it < iters; ++it) {
+    launchPanel(s, fd, prm, dws_.p, stream_);
+    launchPanelRows(s, m - nE, fd, stream_);
+    launchSyrk(s, m - it * kPanel, fd, stream_);
+  }
+  launchFrontFinish(s, fd, stream_);
+}
+
+void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
+  if (layoutDirty_) {
+    buildStaticLayout();
+    uploadLayout();
+    layoutDirty_ = false;
+  }
+  DevFronts fd = devFronts();
+  launchResetFronts(numSN_, fd, stream_);
+  launchResetPerm(dallFronts_.p, numSN_, fd, stream_);
+  for (size_t L = 0; L < sched_.size(); ++L) {
+    const auto& ls = sched_[L];
+    launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, 64, fd, aValsDev, 1, stream_);
+    launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
+    for (int s : ls.large) factorLargeFront(s, prm);
+  }
+  launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
+  launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
+}
+
+void Plan::collectStats(FactorInfo& info, bool& anyDelay) {
+  long long si[8];
+  double sd[4];
+  hipCheck(hipMemcpyAsync(si, dstatI_.p, 5 * sizeof(long long), hipMemcpyDeviceToHost, stream_), "stats");
+  hipCheck(hipMemcpyAsync(sd, dstatD_.p, sizeof(double), hipMemcpyDeviceToHost, stream_), "stats");
+  hipCheck(hipStreamSynchronize(stream_), "factorize");
+  info.pos = static_cast<int>(si[0]);
+  info.neg = static_cast<int>(si[1]);
+  info.zero = static_cast<int>(si[2]);
+  info.delayedFronts = static_cast<int>(si[3]);
+  info.numPerturbed = static_cast<int>(si[4]);
+  info.totalPerturbation = sd[0];
+  anyDelay = si[3] > 0;
+}
+
+// Host-driven level-by-level factorization that sizes every front with the
+// delayed columns handed up by its children (multifrontal.hpp:453-479) and
+// forwards leftover rows by their actual post-pivoting identity
+// (multifrontal.hpp:612, :627-643).
+void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
+  buildStaticLayout();
+  layoutDirty_ = true;
+  // final-order row of every A entry, for re-deriving local rows
+  std::vector<int> inv(n_);
+  for (int k = 0; k < n_; ++k) inv[perm_[k]] = k;
+  std::vector<int> entryHi(haSrc_.size());
+  for (size_t t = 0; t < haSrc_.size(); ++t) {
+    const int64_t src = haSrc_[t];
+    // original column of src: search colptr
+    const int c = static_cast<int>(std::upper_bound(aColptr_.begin(), aColptr_.end(), src) - aColptr_.begin()) - 1;
+    const int r = aRowidx_[src];
+    entryHi[t] = std::max(inv[r], inv[c]);
+  }
+  uploadLayout();
+  hnF_.assign(numSN_, 0);
+  std::vector<std::vector<int>> rowIdxHost(numSN_);
+  DevFronts fd = devFronts();
+  launchResetFronts(numSN_, fd, stream_);
+  DBuf<int> dlist;
+  DBuf<int2> dtask;
+  for (size_t L = 0; L < levelFronts_.size(); ++L) {
+    const auto& fronts = levelFronts_[L];
+    for (int s : fronts) {
+      const auto& pat = sn_[s].rowPattern;
+      std::vector<int> extras;
+      for (int c : children_[s])
+        for (int i = hnF_[c]; i < hm_[c]; ++i) {
+          const int g = rowIdxHost[c][i];
+          if (!std::binary_search(pat.begin(), pat.end(), g)) extras.push_back(g);
+        }
+      std::sort(extras.begin(), extras.end());
+      extras.erase(std::unique(extras.begin(), extras.end()), extras.end());
+      const int nExtra = static_cast<int>(extras.size());
+      const int ncols = sn_[s].ncols;
+      if (nExtra > 0) {
+        const int m = static_cast<int>(pat.size()) + nExtra;
+        std::vector<int> fi(m);
+        for (int j = 0; j < ncols; ++j) fi[j] = sn_[s].firstCol + j;
+        for (int j = 0; j < nExtra; ++j) fi[ncols + j] = extras[j];
+        for (int j = ncols; j < static_cast<int>(pat.size()); ++j) fi[ncols + nExtra + (j - ncols)] = pat[j];
+        const int64_t off = idxUsed_;
+        ensureIdxCapacity(off + m);
+        idxUsed_ += m;
+        hfrontIdx_.resize(idxUsed_, 0);
+        hrelmap_.resize(idxUsed_, -1);
+        std::copy(fi.begin(), fi.end(), hfrontIdx_.begin() + off);
+        hidxOff_[s] = off;
+        hm_[s] = m;
+        hnE_[s] = ncols + nExtra;
+        double* np = nullptr;
+        hipCheck(hipMalloc(reinterpret_cast<void**>(&np), sizeof(double) * static_cast<size_t>(m) * m),
+                 "hipMalloc front");
+        dynAllocs_.push_back(np);
+        hF_[s] = np;
+        // own-column A entries: local rows shift past the extras
+        for (int j = 0; j < ncols; ++j) {
+          const int gcol = sn_[s].firstCol + j;
+          for (int64_t t = hacolPtr_[gcol]; t < hacolPtr_[gcol + 1]; ++t) {
+            const int hi = entryHi[t];
+            auto it = std::lower_bound(pat.begin(), pat.end(), hi);
+            const int pidx = static_cast<int>(it - pat.begin());
+            haRow_[t] = pidx < ncols ? pidx : pidx + nExtra;
+          }
+        }
+      }
+      // children's leftover rows -> this front's local positions
+      const int m = hm_[s];
+      const int64_t io = hidxOff_[s];
+      for (int c : children_[s]) {
+        const int64_t ioc = hidxOff_[c];
+        for (int i = 0; i < hm_[c]; ++i) {
+          int v = -1;
+          if (i >= hnF_[c]) {
+            const int g = rowIdxHost[c][i];
+            // local position of g in s's front index list
+            // front index list = [own columns | extras (sorted) | update rows (sorted)]
+            const int* beg = hfrontIdx_.data() + io;
+            const int* extrasEnd = beg + hnE_[s];
+            if (g >= sn_[s].firstCol && g < sn_[s].firstCol + ncols) {
+              v = g - sn_[s].firstCol;
+            } else {
+              const int* p1 = std::lower_bound(beg + ncols, extrasEnd, g);
+              if (p1 != extrasEnd && *p1 == g) {
+                v = static_cast<int>(p1 - beg);
+              } else {
+                const int* p2 = std::lower_bound(extrasEnd, beg + m, g);
+                if (p2 != beg + m && *p2 == g) v = static_cast<int>(p2 - beg);
+              }
+            }
+            if (v < 0) throw std::runtime_error("symla: forwarded row not found in parent front");
+          }
+          hrelmap_[ioc + i] = v;
+        }
+      }
+    }
+    uploadLayout();
+    fd = devFronts();
+    std::vector<int> small, all;
+    std::vector<int2> tasks;
+    int maxM = 0;
+    std::vector<int> large;
+    for (int s : fronts) {
+      all.push_back(s);
+      tasks.push_back(make_int2(s, 0));
+      if (hm_[s] <= kSmallFrontMax) {
+        small.push_back(s);
+        maxM = std::max(maxM, hm_[s]);
+      } else {
+        large.push_back(s);
+      }
+    }
+    std::vector<int> lists(all);
+    lists.insert(lists.end(), small.begin(), small.end());
+    dlist.upload(lists, stream_);
+    dtask.upload(tasks, stream_);
+    launchResetPerm(dlist.p, static_cast<int>(all.size()), fd, stream_);
+    launchAssemble(dtask.p, static_cast<int>(tasks.size()), 1 << 30, fd, aValsDev, 0, stream_);
+    launchFactorSmall(dlist.p + all.size(), static_cast<int>(small.size()), maxM, fd, prm, stream_);
+    for (int s : large) factorLargeFront(s, prm);
+    launchMakeRowIdx(dlist.p, static_cast<int>(all.size()), fd, stream_);
+    // read back pivot counts and row identities of this level
+    std::vector<int> nfAll(numSN_);
+    hipCheck(hipMemcpyAsync(nfAll.data(), dnF_.p, sizeof(int) * numSN_, hipMemcpyDeviceToHost, stream_), "nF");
+    for (int s : fronts) {
+      rowIdxHost[s].resize(hm_[s]);
+      hipCheck(hipMemcpyAsync(rowIdxHost[s].data(), drowIdx_.p + hidxOff_[s], sizeof(int) * hm_[s],
+                              hipMemcpyDeviceToHost, stream_),
+               "rowIdx");
+    }
+    hipCheck(hipStreamSynchronize(stream_), "dynamic level");
+    for (int s : fronts) hnF_[s] = nfAll[s];
+  }
+  // the relmap of every child was rewritten for its parent; upload the final layout
+  uploadLayout();
+  fd = devFronts();
+  launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
+  singular_ = false;
+  singularColsOrig_.clear();
+  std::vector<int> sing;
+  for (int r : roots_)
+    for (int i = hnF_[r]; i < hm_[r]; ++i) sing.push_back(rowIdxHost[r][i]);
+  std::sort(sing.begin(), sing.end());
+  for (int g : sing) singularColsOrig_.push_back(perm_[g]);
+  singular_ = !sing.empty();
+}
+
+void Plan::factorize(const double* aHost, const double* aDev, const FactorOptions& opts, FactorInfo& info) {
+  factorized_ = false;
+  const double* vals = aDev;
+  if (aHost) {
+    hipCheck(hipMemcpyAsync(dvals_.p, aHost, sizeof(double) * nnzA_, hipMemcpyHostToDevice, stream_), "values");
+    vals = dvals_.p;
+  }
+  info = FactorInfo{};
+  hipCheck(hipEventRecord(ev0_, stream_), "event");
+  if (opts.mode == 0) {
+    FactorParams prm = params(opts, false, 0.0, nullptr);
+    singular_ = false;
+    singularColsOrig_.clear();
+    runStatic(vals, prm);
+    bool anyDelay = false;
+    collectStats(info, anyDelay);
+    if (anyDelay) {
+      runDynamic(vals, prm);
+      collectStats(info, anyDelay);
+      info.repaired = true;
+    }
+    info.singular = singular_;
+    info.nSingular = static_cast<int>(singularColsOrig_.size());
+    hipCheck(hipEventRecord(ev1_, stream_), "event");
+    hipCheck(hipEventSynchronize(ev1_), "event");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+    factorMs_ = ms;
+    factorized_ = true;
+    return;
+  }
+  // Mode::StaticRegularized (solver.hpp:328-417)
+  const bool haveExp = opts.haveExpectation;
+  std::vector<int> signsOrig(n_, 0);
+  if (haveExp) {
+    if (static_cast<int>(opts.expectedSignOrig.size()) != n_)
+      throw std::invalid_argument("symla: expected sign pattern size does not match the matrix size");
+    signsOrig = opts.expectedSignOrig;
+  }
+  std::vector<int> signsFinal(n_);
+  for (int t = 0; t < n_; ++t) signsFinal[t] = signsOrig[perm_[t]];
+  dsignFinal_.upload(signsFinal, stream_);
+  int expectedPos = 0, expectedNeg = 0;
+  for (int v : signsOrig) {
+    if (v > 0)
+      ++expectedPos;
+    else if (v < 0)
+      ++expectedNeg;
+  }
+  double delta = opts.initialDelta;
+  if (delta <= 0.0) {
+    launchZeroU64(dnorms_.p, 1, stream_);
+    if (nnzA_ > 0) launchAbsMax(static_cast<int>(nnzA_), 1, vals, nnzA_, dnorms_.p, stream_);
+    unsigned long long b = 0;
+    hipCheck(hipMemcpyAsync(&b, dnorms_.p, sizeof(b), hipMemcpyDeviceToHost, stream_), "maxabs");
+    hipCheck(hipStreamSynchronize(stream_), "maxabs");
+    double maxAbs = bitsToDoubleHost(b);
+    if (maxAbs <= 0.0) maxAbs = 1.0;
+    delta = std::sqrt(std::numeric_limits<double>::epsilon()) * maxAbs;
+  }
+  const int maxAttempts = std::max(1, opts.maxRetries + 1);
+  for (int attempt = 0; attempt < maxAttempts; ++attempt) {
+    FactorParams prm = params(opts, true, delta, dsignFinal_.p);
+    singular_ = false;
+    singularColsOrig_.clear();
+    runStatic(vals, prm);
+    bool anyDelay = false;
+    collectStats(info, anyDelay);
+    info.lastDelta = delta;
+    info.retriesUsed = attempt;
+    const bool fullCoverage = (expectedPos + expectedNeg == n_);
+    const bool ok = !haveExp || (info.zero == 0 && !singular_ &&
+                                 (!fullCoverage || (info.pos == expectedPos && info.neg == expectedNeg)));
+    if (ok) {
+      hipCheck(hipEventRecord(ev1_, stream_), "event");
+      hipCheck(hipEventSynchronize(ev1_), "event");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+      factorMs_ = ms;
+      factorized_ = true;
+      return;
+    }
+    if (attempt + 1 >= maxAttempts) {
+      std::ostringstream oss;
+      oss << "symla::SymLDLT::factorize (Mode::StaticRegularized): inertia-controlled regularization retry budget ("
+          << opts.maxRetries << " retries) exhausted without matching the expected inertia. expected (n_pos="
+          << expectedPos << ", n_neg=" << expectedNeg << ", n_zero=0), got (n_pos=" << info.pos
+          << ", n_neg=" << info.neg << ", n_zero=" << info.zero << ", singular=" << (singular_ ? "true" : "false")
+          << ") at delta=" << delta << ". Refusing to report a wrong-inertia factorization as successful.";
+      throw std::runtime_error(oss.str());
+    }
+    delta *= opts.growth;
+  }
+}
+
+void Plan::ensureSolveWs(int nrhs) {
+  const size_t nn = static_cast<size_t>(n_) * nrhs;
+  dBp_.ensure(nn);
+  dZ_.ensure(nn);
+  dX_.ensure(nn);
+  if (gOffNrhs_ != nrhs) {
+    std::vector<int64_t> off(numSN_ + 1, 0);
+    for (int s = 0; s < numSN_; ++s) off[s + 1] = off[s] + static_cast<int64_t>(hm_[s]) * nrhs;
+    dgOff_.upload(off, stream_);
+    dG_.ensure(static_cast<size_t>(off[numSN_]));
+    dDl_.ensure(static_cast<size_t>(off[numSN_]));
+    gOffNrhs_ = nrhs;
+  }
+}
+
+void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  ensureSolveWs(nrhs);
+  DevFronts fd = devFronts();
+  launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, dBp_.p, stream_);
+  const int nl = static_cast<int>(levelFronts_.size());
+  for (int L = 0; L < nl; ++L) {
+    const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
+    launchFwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dBp_.p, dZ_.p, dG_.p, dDl_.p, dgOff_.p, stream_);
+  }
+  for (int L = nl - 1; L >= 0; --L) {
+    const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
+    launchBwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, stream_);
+  }
+  launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
+}
+
+void Plan::solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo) {
+  if (!factorized_) throw std::logic_error("symla::SymLDLT::solve: factorize() must be called before solve()");
+  if (singular_)
+    throw std::logic_error(
+        "symla::SymLDLT::solve: factorization is numerically singular (see isSingular()/singularColumns()); refusing "
+        "to solve");
+  if (nrhs < 0) throw std::invalid_argument("symla: nrhs must be non-negative");
+  if (nrhs == 0 || n_ == 0) return;
+  hipCheck(hipEventRecord(ev0_, stream_), "event");
+  if (deviceIo) {
+    solveCore(B, ldb, nrhs, X, ldx);
+  } else {
+    const size_t nn = static_cast<size_t>(n_) * nrhs;
+    dBin_.ensure(nn);
+    dXout_.ensure(nn);
+    hipCheck(hipMemcpy2DAsync(dBin_.p, sizeof(double) * n_, B, sizeof(double) * ldb, sizeof(double) * n_, nrhs,
+                              hipMemcpyHostToDevice, stream_),
+             "B upload");
+    solveCore(dBin_.p, n_, nrhs, dXout_.p, n_);
+    hipCheck(hipMemcpy2DAsync(X, sizeof(double) * ldx, dXout_.p, sizeof(double) * n_, sizeof(double) * n_, nrhs,
+                              hipMemcpyDeviceToHost, stream_),
+             "X download");
+  }
+  hipCheck(hipEventRecord(ev1_, stream_), "event");
+  hipCheck(hipEventSynchronize(ev1_), "solve");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  solveMs_ = ms;
+}
+
+RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const double* B, int64_t ldb, int nrhs,
+                                double* X, int64_t ldx, bool deviceIo, int maxIt, double tol) {
+  if (!factorized_) throw std::logic_error("symla::SymLDLT::solve: factorize() must be called before solve()");
+  if (singular_)
+    throw std::logic_error(
+        "symla::SymLDLT::solve: factorization is numerically singular (see isSingular()/singularColumns()); refusing "
+        "to solve");
+  RefineResult res;
+  if (nrhs <= 0 || n_ == 0) return res;
+  const double* vals = aDev;
+  if (aHost) {
+    hipCheck(hipMemcpyAsync(dvals_.p, aHost, sizeof(double) * nnzA_, hipMemcpyHostToDevice, stream_), "values");
+    vals = dvals_.p;
+  }
+  const size_t nn = static_cast<size_t>(n_) * nrhs;
+  dBr_.ensure(nn);
+  dR_.ensure(nn);
+  dDx_.ensure(nn);
+  dXn_.ensure(nn);
+  dRn_.ensure(nn);
+  dXout_.ensure(nn);
+  hipCheck(hipMemcpy2DAsync(dBr_.p, sizeof(double) * n_, B, sizeof(double) * ldb, sizeof(double) * n_, nrhs,
+                            deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_),
+           "B copy");
+  double* Xc = dXout_.p;
+  double* Rc = dR_.p;
+  double* Xn = dXn_.p;
+  double* Rn = dRn_.p;
+  dnorms_.ensure(8 + 3 * static_cast<size_t>(nrhs));
+  auto norm = [&](const double* v, int nc, int64_t ld, unsigned long long* slot) {
+    launchAbsMax(n_, nc, v, ld, slot, stream_);
+  };
+  // normA (refine.hpp:106) and the symmetric infinity norm
+  launchZeroU64(dnorms_.p, 8, stream_);
+  launchRowSumMax(n_, dstPtr_.p, dstSrc_.p, vals, dnorms_.p + 0, stream_);
+  launchRowSumMax(n_, dsymPtr_.p, dsymSrc_.p, vals, dnorms_.p + 1, stream_);
+  norm(dBr_.p, nrhs, n_, dnorms_.p + 2);
+  unsigned long long hb[8];
+  hipCheck(hipMemcpyAsync(hb, dnorms_.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "norms");
+  hipCheck(hipStreamSynchronize(stream_), "norms");
+  const double normA = bitsToDoubleHost(hb[0]);
+  const double normAsym = bitsToDoubleHost(hb[1]);
+  const double normB = bitsToDoubleHost(hb[2]);
+
+  solveCore(dBr_.p, n_, nrhs, Xc, n_);
+  auto relRes = [&](const double* R, const double* Xv) {
+    launchZeroU64(dnorms_.p + 3, 2, stream_);
+    norm(R, nrhs, n_, dnorms_.p + 3);
+    norm(Xv, nrhs, n_, dnorms_.p + 4);
+    unsigned long long h2[2];
+    hipCheck(hipMemcpyAsync(h2, dnorms_.p + 3, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_),
+             "norms");
+    hipCheck(hipStreamSynchronize(stream_), "norms");
+    const double nR = bitsToDoubleHost(h2[0]), nX = bitsToDoubleHost(h2[1]);
+    const double denom = normA * nX + normB;
+    return denom > 0.0 ? nR / denom : nR;
+  };
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymSrc_.p, vals, dBr_.p, Xc, Rc, stream_);
+  double r0 = relRes(Rc, Xc);
+  res.initialRel = r0;
+  res.finalRel = r0;
+  for (int it = 0; it < maxIt; ++it) {  // refine.hpp:121-141
+    if (r0 <= tol) break;
+    solveCore(Rc, n_, nrhs, dDx_.p, n_);
+    launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, Xn, stream_);
+    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymSrc_.p, vals, dBr_.p, Xn, Rn, stream_);
+    const double rNew = relRes(Rn, Xn);
+    if (!(rNew < r0)) break;
+    std::swap(Xc, Xn);
+    std::swap(Rc, Rn);
+    r0 = rNew;
+    res.iterations = it + 1;
+    res.finalRel = r0;
+  }
+  // backward error per column with the symmetric infinity norm
+  {
+    launchZeroU64(dnorms_.p + 8, 3 * nrhs, stream_);
+    for (int c = 0; c < nrhs; ++c) {
+      norm(Rc + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 8 + 3 * c);
+      norm(Xc + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 9 + 3 * c);
+      norm(dBr_.p + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 10 + 3 * c);
+    }
+    std::vector<unsigned long long> h(3 * static_cast<size_t>(nrhs));
+    hipCheck(hipMemcpyAsync(h.data(), dnorms_.p + 8, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            stream_),
+             "be");
+    hipCheck(hipStreamSynchronize(stream_), "be");
+    double be = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+      const double nr = bitsToDoubleHost(h[3 * c]), nx = bitsToDoubleHost(h[3 * c + 1]),
+                   nb = bitsToDoubleHost(h[3 * c + 2]);
+      const double den = normAsym * nx + nb;
+      be = std::max(be, den > 0.0 ? nr / den : nr);
+    }
+    res.backwardError = be;
+  }
+  hipCheck(hipMemcpy2DAsync(X, sizeof(double) * ldx, Xc, sizeof(double) * n_, sizeof(double) * n_, nrhs,
+                            deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_),
+           "X out");
+  hipCheck(hipStreamSynchronize(stream_), "refine");
+  return res;
+}
+
+void Plan::frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
+                     int* nblocks) {
+  if (!factorized_) throw std::logic_error("symla: no factorization");
+  if (si < 0 || si >= numSN_) throw std::invalid_argument("symla: front index out of range");
+  int nf = 0;
+  hipCheck(hipMemcpy(&nf, dnF_.p + si, sizeof(int), hipMemcpyDeviceToHost), "view");
+  const int mm = hm_[si];
+  if (m) *m = mm;
+  if (npiv) *npiv = nf;
+  const int64_t io = hidxOff_[si];
+  if (rows) hipCheck(hipMemcpy(rows, drowIdx_.p + io, sizeof(int) * mm, hipMemcpyDeviceToHost), "view rows");
+  if (L && nf > 0) {
+    hipCheck(hipMemcpy(L, hF_[si], sizeof(double) * static_cast<size_t>(mm) * nf, hipMemcpyDeviceToHost), "view L");
+    for (int j = 0; j < nf; ++j)
+      for (int i = 0; i < j && i < mm; ++i) L[i + static_cast<size_t>(j) * mm] = 0.0;  // upper part is not stored
+  }
+  std::vector<int> kinds(nf);
+  std::vector<double> dg(nf), dfo(nf);
+  if (nf > 0) {
+    hipCheck(hipMemcpy(kinds.data(), dpkind_.p + io, sizeof(int) * nf, hipMemcpyDeviceToHost), "view");
+    hipCheck(hipMemcpy(dg.data(), ddDiag_.p + io, sizeof(double) * nf, hipMemcpyDeviceToHost), "view");
+    hipCheck(hipMemcpy(dfo.data(), ddOff_.p + io, sizeof(double) * nf, hipMemcpyDeviceToHost), "view");
+  }
+  int nb = 0;
+  for (int i = 0; i < nf; ++i) {
+    if (dd) dd[i] = dg[i];
+    if (dof) dof[i] = (kinds[i] == 2) ? dfo[i] : 0.0;
+    if (kinds[i] == 1 || kinds[i] == 2) {
+      if (ps) ps[nb] = i;
+      if (pk) pk[nb] = kinds[i];
+      ++nb;
+    }
+  }
+  if (nblocks) *nblocks = nb;
+}
+
+}  // namespace symla_hip

@@ -1,0 +1,190 @@
+// plan.hpp -- host-side owner of the device pool and the level schedule of
+// the multifrontal numeric phase.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "symbolic.hpp"
+
+namespace symla_hip {
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hipCheck(hipError_t e, const char* what);
+
+// Device buffer with RAII free.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipCheck(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void upload(const std::vector<T>& v, hipStream_t st) {
+    ensure(v.size());
+    if (!v.empty())
+      hipCheck(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+  }
+};
+
+struct FactorOptions {
+  int mode = 0;
+  double zeroTol = 1e-300;
+  double relFloor = 1e-12;
+  std::vector<int> expectedSignOrig;  // empty = none
+  bool haveExpectation = false;
+  double initialDelta = 0.0;
+  int maxRetries = 10;
+  double growth = 10.0;
+  double staticRelFloor = 1e-8;
+  double staticAbsFloor = 1e-300;
+};
+
+struct FactorInfo {
+  int pos = 0, neg = 0, zero = 0;
+  bool singular = false;
+  int nSingular = 0;
+  double totalPerturbation = 0.0;
+  int numPerturbed = 0;
+  int retriesUsed = 0;
+  double lastDelta = 0.0;
+  int delayedFronts = 0;
+  bool repaired = false;
+};
+
+struct RefineResult {
+  int iterations = 0;
+  double initialRel = 0.0, finalRel = 0.0, backwardError = 0.0;
+};
+
+class Plan {
+ public:
+  Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx);
+  ~Plan();
+
+  // values: device pointer (already resident) after `setValuesDevice`.
+  void factorize(const double* aValsHostOrNull, const double* aValsDev, const FactorOptions& opts, FactorInfo& info);
+  void solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo);
+  RefineResult solveRefined(const double* aHost, const double* aDev, const double* B, int64_t ldb, int nrhs,
+                            double* X, int64_t ldx, bool deviceIo, int maxIt, double tol);
+
+  int n() const { return n_; }
+  int numSN() const { return numSN_; }
+  bool factorized() const { return factorized_; }
+  bool singular() const { return singular_; }
+  const std::vector<int>& singularColsOrig() const { return singularColsOrig_; }
+  void frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
+                 int* nblocks);
+  double lastFactorMs() const { return factorMs_; }
+  double lastSolveMs() const { return solveMs_; }
+  std::string lastError;
+
+ private:
+  // symbolic
+  int n_ = 0, numSN_ = 0;
+  std::vector<int> perm_;
+  std::vector<Supernode> sn_;
+  std::vector<int> parentSN_, roots_;
+  std::vector<std::vector<int>> children_;
+  std::vector<int> level_;
+  std::vector<std::vector<int>> levelFronts_;
+  std::vector<int> colToSN_;
+  // A pattern
+  int64_t nnzA_ = 0;
+  std::vector<int64_t> aColptr_;
+  std::vector<int> aRowidx_;
+  // host layout mirrors
+  std::vector<int> hm_, hnE_, hncols_, hfirst_, hchildPtr_, hchildList_;
+  std::vector<int64_t> hidxOff_, hacolPtr_, haSrc_;
+  std::vector<int> hfrontIdx_, hrelmap_, haRow_;
+  int64_t idxUsed_ = 0;
+  std::vector<int> hnF_, hrowIdx_;  // host copies after factorization (dynamic mode / views)
+  // device layout
+  hipStream_t stream_ = nullptr;
+  DBuf<double> pool_;
+  std::vector<double*> hF_;
+  std::vector<double*> dynAllocs_;
+  DBuf<double*> dF_;
+  DBuf<int> dm_, dnE_, dncols_, dfirst_, dchildPtr_, dchildList_, dfrontIdx_, drelmap_, daRow_;
+  DBuf<int64_t> didxOff_, dacolPtr_, daSrc_;
+  DBuf<int> dperm_, drowIdx_, dpkind_, dnF_, dinertia_, dkcur_, dpcount_, dstopped_, dnpert_;
+  DBuf<double> ddDiag_, ddOff_, dpert_;
+  DBuf<unsigned long long> dorigMax_, drunMax_;
+  DBuf<long long> dstatI_;
+  DBuf<double> dstatD_;
+  DBuf<double> dvals_;
+  DBuf<double> dws_;
+  DBuf<int> dsignFinal_;
+  // static schedule
+  struct LevelSched {
+    int64_t smallOff = 0;
+    int smallCount = 0;
+    int smallMaxM = 0;
+    int64_t taskOff = 0;
+    int taskCount = 0;
+    std::vector<int> large;
+  };
+  std::vector<LevelSched> sched_;
+  DBuf<int> dsmall_;
+  DBuf<int2> dtasks_;
+  DBuf<int> dallFronts_;
+  DBuf<int> dlevelFronts_;
+  std::vector<int64_t> levelFrontOff_;
+  int maxNE_ = 0;
+  // solve
+  DBuf<int> dpermVec_;
+  DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
+  DBuf<int64_t> dgOff_;
+  int gOffNrhs_ = -1;
+  // refinement CSR
+  DBuf<int64_t> dsymPtr_, dsymSrc_, dstPtr_, dstSrc_;
+  DBuf<int> dsymCol_;
+  DBuf<double> dR_, dDx_, dXn_, dRn_, dBr_;
+  DBuf<unsigned long long> dnorms_;
+  // state
+  bool factorized_ = false;
+  bool singular_ = false;
+  std::vector<int> singularColsOrig_;
+  bool layoutDirty_ = false;  // dynamic mode changed the layout
+  double factorMs_ = 0.0, solveMs_ = 0.0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+  void buildStaticLayout();
+  void uploadLayout();
+  void buildSchedule();
+  void buildRefineCsr();
+  DevFronts devFronts();
+  FactorParams params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const;
+  void runStatic(const double* aValsDev, const FactorParams& prm);
+  void runDynamic(const double* aValsDev, const FactorParams& prm);
+  void factorLargeFront(int s, const FactorParams& prm);
+  void collectStats(FactorInfo& info, bool& anyDelay);
+  void finishSingular();
+  void ensureSolveWs(int nrhs);
+  void solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  void ensureIdxCapacity(int64_t need);
+};
+
+}  // namespace symla_hip

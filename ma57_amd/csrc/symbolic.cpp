@@ -1,0 +1,764 @@
+// symbolic.cpp -- CPU symbolic analysis (see symbolic.hpp).
+#include "symbolic.hpp"
+
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <stdexcept>
+#include <unordered_map>
+#include <vector>
+
+namespace symla_hip {
+
+Graph buildSymmetricGraph(int n, const int64_t* colptr, const int* rowidx) {
+  // Pattern of A + A^T without the diagonal (ordering.hpp:44-75 semantics):
+  // count, fill, then sort + unique per vertex.
+  Graph g;
+  g.n = n;
+  std::vector<int64_t> cnt(n + 1, 0);
+  for (int c = 0; c < n; ++c)
+    for (int64_t t = colptr[c]; t < colptr[c + 1]; ++t) {
+      const int r = rowidx[t];
+      if (r == c) continue;
+      if (r < 0 || r >= n) throw std::invalid_argument("symla: row index out of range");
+      ++cnt[c + 1];
+      ++cnt[r + 1];
+    }
+  std::vector<int64_t> ptr(n + 1, 0);
+  for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
+  std::vector<int> tmp(ptr[n]);
+  std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+  for (int c = 0; c < n; ++c)
+    for (int64_t t = colptr[c]; t < colptr[c + 1]; ++t) {
+      const int r = rowidx[t];
+      if (r == c) continue;
+      tmp[pos[c]++] = r;
+      tmp[pos[r]++] = c;
+    }
+  g.ptr.assign(n + 1, 0);
+  g.adj.reserve(tmp.size());
+  for (int i = 0; i < n; ++i) {
+    auto b = tmp.begin() + ptr[i], e = tmp.begin() + ptr[i + 1];
+    std::sort(b, e);
+    auto u = std::unique(b, e);
+    g.adj.insert(g.adj.end(), b, u);
+    g.ptr[i + 1] = static_cast<int64_t>(g.adj.size());
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Minimum degree on the quotient graph.
+//
+// Variables carry an approximate external degree; eliminating the variable p
+// of least degree turns it into an element whose variable list is the union
+// of its variable neighbours and the lists of the elements it absorbs.  The
+// degree of each variable i in that list is bounded by
+//   |A_i| + |L_p \ i| + sum_{e in E_i, e != p} |L_e \ L_p|
+// (the bound of Amestoy, Davis and Duff, SIMAX 1996).  Variables of the new
+// element with identical element and variable adjacency are merged into
+// supervariables and leave together.  Written from the published method.
+// ---------------------------------------------------------------------------
+std::vector<int> minimumDegreeOrder(const Graph& g) {
+  const int n = g.n;
+  std::vector<int> order;
+  order.reserve(n);
+  if (n == 0) return order;
+
+  enum : char { VAR = 0, ELEM = 1, DEAD = 2 };
+  std::vector<char> status(n, VAR);
+  std::vector<int> nv(n, 1);
+  std::vector<std::vector<int>> varAdj(n), elemAdj(n), Le(n);
+  std::vector<int> degree(n, 0);
+  std::vector<int> memberNext(n, -1), memberTail(n);
+  for (int i = 0; i < n; ++i) {
+    varAdj[i].assign(g.adj.begin() + g.ptr[i], g.adj.begin() + g.ptr[i + 1]);
+    degree[i] = static_cast<int>(varAdj[i].size());
+    memberTail[i] = i;
+  }
+  // degree buckets (doubly linked)
+  std::vector<int> head(n + 1, -1), nxt(n, -1), prv(n, -1);
+  auto bucketInsert = [&](int i) {
+    int d = std::min(std::max(degree[i], 0), n);
+    nxt[i] = head[d];
+    prv[i] = -1;
+    if (head[d] != -1) prv[head[d]] = i;
+    head[d] = i;
+  };
+  auto bucketRemove = [&](int i) {
+    int d = std::min(std::max(degree[i], 0), n);
+    if (prv[i] != -1)
+      nxt[prv[i]] = nxt[i];
+    else
+      head[d] = nxt[i];
+    if (nxt[i] != -1) prv[nxt[i]] = prv[i];
+    nxt[i] = prv[i] = -1;
+  };
+  for (int i = 0; i < n; ++i) bucketInsert(i);
+
+  std::vector<int> mark(n, 0);
+  int tag = 0;
+  std::vector<int> wval(n, 0), wtag(n, 0);
+  int wt = 0;
+  int nel = 0;
+  int mindeg = 0;
+  std::vector<int> Lp;
+
+  while (nel < n) {
+    while (mindeg <= n && head[mindeg] == -1) ++mindeg;
+    if (mindeg > n) break;  // should not happen
+    const int p = head[mindeg];
+    bucketRemove(p);
+
+    // --- form element p ---
+    ++tag;
+    mark[p] = tag;
+    Lp.clear();
+    for (int i : varAdj[p])
+      if (status[i] == VAR && nv[i] > 0 && mark[i] != tag) {
+        mark[i] = tag;
+        Lp.push_back(i);
+      }
+    for (int e : elemAdj[p]) {
+      if (status[e] != ELEM) continue;
+      for (int i : Le[e])
+        if (status[i] == VAR && nv[i] > 0 && mark[i] != tag) {
+          mark[i] = tag;
+          Lp.push_back(i);
+        }
+      status[e] = DEAD;  // absorbed into p
+      std::vector<int>().swap(Le[e]);
+    }
+    status[p] = ELEM;
+    std::vector<int>().swap(varAdj[p]);
+    std::vector<int>().swap(elemAdj[p]);
+    for (int v = p; v != -1; v = memberNext[v]) order.push_back(v);
+    nel += nv[p];
+
+    int degLp = 0;
+    for (int i : Lp) degLp += nv[i];
+
+    // --- update adjacency of the variables in Lp ---
+    for (int i : Lp) {
+      bucketRemove(i);
+      auto& E = elemAdj[i];
+      size_t w = 0;
+      for (size_t t = 0; t < E.size(); ++t)
+        if (status[E[t]] == ELEM && E[t] != p) E[w++] = E[t];
+      E.resize(w);
+      E.push_back(p);
+      auto& A = varAdj[i];
+      w = 0;
+      for (size_t t = 0; t < A.size(); ++t) {
+        const int j = A[t];
+        if (status[j] == VAR && nv[j] > 0 && mark[j] != tag) A[w++] = j;
+      }
+      A.resize(w);
+    }
+
+    // --- |L_e \ L_p| for elements adjacent to Lp ---
+    ++wt;
+    for (int i : Lp) {
+      for (int e : elemAdj[i]) {
+        if (e == p || wtag[e] == wt) continue;
+        wtag[e] = wt;
+        auto& L = Le[e];
+        int s = 0;
+        size_t w = 0;
+        for (size_t t = 0; t < L.size(); ++t) {
+          const int j = L[t];
+          if (status[j] != VAR || nv[j] == 0) continue;
+          L[w++] = j;
+          if (mark[j] != tag) s += nv[j];
+        }
+        L.resize(w);
+        wval[e] = s;
+      }
+    }
+
+    // --- approximate degrees ---
+    const int remaining = n - nel;
+    for (int i : Lp) {
+      int dA = 0;
+      for (int j : varAdj[i]) dA += nv[j];
+      long long dE = 0;
+      auto& E = elemAdj[i];
+      size_t w = 0;
+      for (size_t t = 0; t < E.size(); ++t) {
+        const int e = E[t];
+        if (e == p) {
+          E[w++] = e;
+          continue;
+        }
+        if (wval[e] == 0) {
+          // L_e is contained in L_p: element e is absorbed into p.
+          status[e] = DEAD;
+          std::vector<int>().swap(Le[e]);
+          continue;
+        }
+        dE += wval[e];
+        E[w++] = e;
+      }
+      E.resize(w);
+      long long d = static_cast<long long>(dA) + (degLp - nv[i]) + dE;
+      d = std::min<long long>(d, static_cast<long long>(degree[i]) + degLp - nv[i]);
+      d = std::min<long long>(d, remaining - nv[i]);
+      degree[i] = static_cast<int>(std::max<long long>(d, 0));
+    }
+
+    // --- supervariable detection among Lp ---
+    if (Lp.size() > 1) {
+      std::unordered_map<uint64_t, std::vector<int>> buckets;
+      buckets.reserve(Lp.size() * 2);
+      for (int i : Lp) {
+        auto& E = elemAdj[i];
+        auto& A = varAdj[i];
+        std::sort(E.begin(), E.end());
+        std::sort(A.begin(), A.end());
+        uint64_t h = E.size() * 1000003ull + A.size();
+        for (int e : E) h = h * 31 + static_cast<uint64_t>(e);
+        for (int j : A) h = h * 37 + static_cast<uint64_t>(j);
+        buckets[h].push_back(i);
+      }
+      for (auto& kv : buckets) {
+        auto& lst = kv.second;
+        for (size_t a = 0; a < lst.size(); ++a) {
+          const int i = lst[a];
+          if (nv[i] == 0) continue;
+          for (size_t b = a + 1; b < lst.size(); ++b) {
+            const int j = lst[b];
+            if (nv[j] == 0) continue;
+            if (elemAdj[i] == elemAdj[j] && varAdj[i] == varAdj[j]) {
+              // merge j into i
+              degree[i] -= nv[j];
+              nv[i] += nv[j];
+              nv[j] = 0;
+              status[j] = DEAD;
+              memberNext[memberTail[i]] = j;
+              memberTail[i] = memberTail[j];
+              std::vector<int>().swap(elemAdj[j]);
+              std::vector<int>().swap(varAdj[j]);
+            }
+          }
+        }
+      }
+    }
+
+    // --- element p's list keeps the surviving principal variables ---
+    auto& L = Le[p];
+    L.clear();
+    for (int i : Lp)
+      if (nv[i] > 0) {
+        L.push_back(i);
+        if (degree[i] < 0) degree[i] = 0;
+        bucketInsert(i);
+        mindeg = std::min(mindeg, std::min(degree[i], n));
+      }
+  }
+  if (static_cast<int>(order.size()) != n) throw std::runtime_error("symla: minimum degree ordering failed");
+  return order;
+}
+
+// ---------------------------------------------------------------------------
+// Nested dissection by recursive level-structure bisection.
+//
+// For each connected piece: find a pseudo-peripheral vertex by repeated BFS,
+// build the level structure, and take as separator the smallest level whose
+// two sides are both at most 65% of the piece (thinned to the vertices that
+// actually touch the next level).  Pieces at or below the leaf size are
+// ordered by minimum degree on the induced subgraph.  Order = [side A, side
+// B, separator].
+// ---------------------------------------------------------------------------
+namespace {
+
+struct NDState {
+  const Graph& g;
+  std::vector<int>& out;
+  std::vector<int> where;   // piece id of each vertex (-1 = ordered)
+  std::vector<int> level;   // BFS level scratch
+  std::vector<int> local;   // global -> local index scratch
+  int nextPiece = 1;
+  NDState(const Graph& gg, std::vector<int>& o)
+      : g(gg), out(o), where(gg.n, 0), level(gg.n, -1), local(gg.n, -1) {}
+
+  void orderLeaf(const std::vector<int>& V) {
+    Graph sub;
+    sub.n = static_cast<int>(V.size());
+    for (int t = 0; t < sub.n; ++t) local[V[t]] = t;
+    sub.ptr.assign(sub.n + 1, 0);
+    const int pid = where[V[0]];
+    for (int t = 0; t < sub.n; ++t) {
+      const int v = V[t];
+      for (int64_t q = g.ptr[v]; q < g.ptr[v + 1]; ++q) {
+        const int u = g.adj[q];
+        if (where[u] == pid) sub.adj.push_back(local[u]);
+      }
+      sub.ptr[t + 1] = static_cast<int64_t>(sub.adj.size());
+    }
+    std::vector<int> o = minimumDegreeOrder(sub);
+    for (int t : o) out.push_back(V[t]);
+    for (int v : V) {
+      local[v] = -1;
+      where[v] = -1;
+    }
+  }
+
+  // BFS inside piece `pid` from `src`; returns vertices in BFS order and sets level[].
+  int bfs(int src, int pid, std::vector<int>& q) {
+    q.clear();
+    q.push_back(src);
+    level[src] = 0;
+    int maxLev = 0;
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int v = q[h];
+      for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
+        const int u = g.adj[t];
+        if (where[u] != pid || level[u] >= 0) continue;
+        level[u] = level[v] + 1;
+        maxLev = std::max(maxLev, level[u]);
+        q.push_back(u);
+      }
+    }
+    return maxLev;
+  }
+
+  void dissect(std::vector<int> V) {
+    const int leafSize = 192;
+    if (V.empty()) return;
+    if (static_cast<int>(V.size()) <= leafSize) {
+      orderLeaf(V);
+      return;
+    }
+    const int pid = where[V[0]];
+    // connected components of the piece
+    std::vector<int> q;
+    std::vector<std::vector<int>> comps;
+    for (int v : V) level[v] = -1;
+    for (int v : V) {
+      if (level[v] >= 0) continue;
+      bfs(v, pid, q);
+      comps.push_back(q);
+    }
+    for (int v : V) level[v] = -1;
+    if (comps.size() > 1) {
+      for (auto& c : comps) {
+        const int np = nextPiece++;
+        for (int v : c) where[v] = np;
+        dissect(std::move(c));
+      }
+      return;
+    }
+    // pseudo-peripheral vertex
+    int src = V[0];
+    int ecc = bfs(src, pid, q);
+    for (int iter = 0; iter < 8; ++iter) {
+      // farthest vertex of minimum degree in the last level
+      int best = -1, bestDeg = 1 << 30;
+      for (size_t t = q.size(); t-- > 0;) {
+        const int v = q[t];
+        if (level[v] != ecc) break;
+        int d = 0;
+        for (int64_t s = g.ptr[v]; s < g.ptr[v + 1]; ++s) d += where[g.adj[s]] == pid;
+        if (d < bestDeg) {
+          bestDeg = d;
+          best = v;
+        }
+      }
+      for (int v : q) level[v] = -1;
+      const int e2 = bfs(best, pid, q);
+      if (e2 <= ecc) {
+        for (int v : q) level[v] = -1;
+        ecc = bfs(best, pid, q);
+        src = best;
+        break;
+      }
+      ecc = e2;
+      src = best;
+    }
+    const int total = static_cast<int>(q.size());
+    std::vector<int> levCount(ecc + 2, 0);
+    for (int v : q) ++levCount[level[v]];
+    // choose separator level
+    int bestL = -1;
+    double bestCost = 1e300;
+    int before = 0;
+    for (int L = 0; L <= ecc; ++L) {
+      const int a = before, s = levCount[L], b = total - before - s;
+      before += s;
+      if (a == 0 || b == 0) continue;
+      const double imb = static_cast<double>(std::max(a, b)) / total;
+      double cost = s;
+      if (imb > 0.65) cost *= 1.0 + 20.0 * (imb - 0.65);
+      if (cost < bestCost) {
+        bestCost = cost;
+        bestL = L;
+      }
+    }
+    if (bestL < 0) {
+      for (int v : q) level[v] = -1;
+      orderLeaf(V);
+      return;
+    }
+    std::vector<int> A, B, S;
+    for (int v : q) {
+      if (level[v] < bestL)
+        A.push_back(v);
+      else if (level[v] > bestL)
+        B.push_back(v);
+      else {
+        bool touchesNext = false;
+        for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
+          const int u = g.adj[t];
+          if (where[u] == pid && level[u] == bestL + 1) {
+            touchesNext = true;
+            break;
+          }
+        }
+        if (touchesNext)
+          S.push_back(v);
+        else
+          A.push_back(v);
+      }
+    }
+    for (int v : q) level[v] = -1;
+    const int pa = nextPiece++, pb = nextPiece++, ps = nextPiece++;
+    for (int v : A) where[v] = pa;
+    for (int v : B) where[v] = pb;
+    for (int v : S) where[v] = ps;
+    dissect(std::move(A));
+    dissect(std::move(B));
+    // separator last, ordered by minimum degree on its induced subgraph
+    if (!S.empty()) orderLeaf(S);
+  }
+};
+
+}  // namespace
+
+std::vector<int> nestedDissectionOrder(const Graph& g) {
+  std::vector<int> out;
+  out.reserve(g.n);
+  if (g.n == 0) return out;
+  NDState st(g, out);
+  std::vector<int> all(g.n);
+  std::iota(all.begin(), all.end(), 0);
+  st.dissect(std::move(all));
+  if (static_cast<int>(out.size()) != g.n) throw std::runtime_error("symla: nested dissection failed");
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Elimination tree and postorder.
+// ---------------------------------------------------------------------------
+std::vector<int> eliminationTree(const Graph& g, const std::vector<int>& perm, const std::vector<int>& inv) {
+  // Liu's algorithm with path compression (elimination_tree.hpp:78-96
+  // describes the same textbook method).
+  const int n = g.n;
+  std::vector<int> parent(n, -1), ancestor(n, -1);
+  for (int j = 0; j < n; ++j) {
+    const int v = perm[j];
+    for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
+      int r = inv[g.adj[t]];
+      if (r >= j) continue;
+      while (ancestor[r] != -1 && ancestor[r] != j) {
+        const int nx = ancestor[r];
+        ancestor[r] = j;
+        r = nx;
+      }
+      if (ancestor[r] == -1) {
+        ancestor[r] = j;
+        parent[r] = j;
+      }
+    }
+  }
+  return parent;
+}
+
+std::vector<int> postorderForest(const std::vector<int>& parent) {
+  const int n = static_cast<int>(parent.size());
+  // children lists in increasing index order (head insertion in reverse)
+  std::vector<int> firstChild(n, -1), nextSib(n, -1);
+  for (int j = n - 1; j >= 0; --j) {
+    const int p = parent[j];
+    if (p != -1) {
+      nextSib[j] = firstChild[p];
+      firstChild[p] = j;
+    }
+  }
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<int> stack;
+  std::vector<int> it(n, -2);
+  for (int r = 0; r < n; ++r) {
+    if (parent[r] != -1) continue;
+    stack.push_back(r);
+    it[r] = firstChild[r];
+    while (!stack.empty()) {
+      const int v = stack.back();
+      const int c = it[v];
+      if (c != -1) {
+        it[v] = nextSib[c];
+        stack.push_back(c);
+        it[c] = firstChild[c];
+      } else {
+        order.push_back(v);
+        stack.pop_back();
+      }
+    }
+  }
+  return order;
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline (symbolic.hpp:195-399 of the reference, with column counts by
+// row-subtree traversal and supernode patterns built supernode-wise instead
+// of materializing every column's pattern).
+// ---------------------------------------------------------------------------
+namespace {
+
+int64_t storageNnzOf(int ncols, size_t m) {  // symbolic.hpp:297-301
+  const int64_t k = ncols;
+  const int64_t mm = static_cast<int64_t>(m);
+  return k * mm - k * (k - 1) / 2;
+}
+
+}  // namespace
+
+SymbolicResult analyze(int n, const int64_t* colptr, const int* rowidx, Ordering ordering,
+                       const SymbolicOptions& opts, const int* userPerm) {
+  SymbolicResult sr;
+  sr.n = n;
+  if (n == 0) return sr;
+  Graph g = buildSymmetricGraph(n, colptr, rowidx);
+
+  // --- step 1: fill-reducing order ---
+  std::vector<int> perm0;
+  switch (ordering) {
+    case Ordering::Natural:
+      perm0.resize(n);
+      std::iota(perm0.begin(), perm0.end(), 0);
+      break;
+    case Ordering::AMD:
+      perm0 = minimumDegreeOrder(g);
+      break;
+    case Ordering::Metis:
+      perm0 = nestedDissectionOrder(g);
+      break;
+    case Ordering::User: {
+      if (!userPerm) throw std::invalid_argument("symla: user ordering requires a permutation");
+      perm0.assign(userPerm, userPerm + n);
+      std::vector<char> seen(n, 0);
+      for (int v : perm0) {
+        if (v < 0 || v >= n || seen[v]) throw std::invalid_argument("symla: user ordering is not a permutation");
+        seen[v] = 1;
+      }
+      break;
+    }
+  }
+  std::vector<int> inv0(n);
+  for (int k = 0; k < n; ++k) inv0[perm0[k]] = k;
+
+  // --- step 3: etree + postorder (symbolic.hpp:212-216) ---
+  std::vector<int> tree0 = eliminationTree(g, perm0, inv0);
+  std::vector<int> post = postorderForest(tree0);
+  std::vector<int> finalPerm(n);
+  for (int k = 0; k < n; ++k) finalPerm[k] = perm0[post[k]];
+  std::vector<int> inv1(n);
+  for (int k = 0; k < n; ++k) inv1[finalPerm[k]] = k;
+
+  // --- step 4: etree on the postordered pattern + column counts ---
+  std::vector<int> parent = eliminationTree(g, finalPerm, inv1);
+  std::vector<int> colCount(n, 1);
+  {
+    // row-subtree traversal: row i of L is the union of etree paths from
+    // each j in Adj(i), j < i, up to i.
+    std::vector<int> flag(n, -1);
+    for (int i = 0; i < n; ++i) {
+      flag[i] = i;
+      const int v = finalPerm[i];
+      for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
+        int j = inv1[g.adj[t]];
+        if (j >= i) continue;
+        while (flag[j] != i) {
+          flag[j] = i;
+          ++colCount[j];
+          j = parent[j];
+        }
+      }
+    }
+  }
+
+  // --- step 5: fundamental supernodes (symbolic.hpp:229-247) ---
+  struct FSN {
+    int first, ncols;
+    std::vector<int> pattern;  // stage-1 index space
+  };
+  std::vector<FSN> funda;
+  {
+    int j = 0;
+    while (j < n) {
+      const int start = j;
+      while (j + 1 < n && parent[j] == j + 1 && colCount[j] == colCount[j + 1] + 1) ++j;
+      funda.push_back({start, j - start + 1, {}});
+      ++j;
+    }
+  }
+  const int numFunda = static_cast<int>(funda.size());
+  std::vector<int> colToFunda(n);
+  for (int i = 0; i < numFunda; ++i)
+    for (int c = funda[i].first; c < funda[i].first + funda[i].ncols; ++c) colToFunda[c] = i;
+  std::vector<int> parentFunda(numFunda, -1);
+  std::vector<std::vector<int>> childrenFunda(numFunda);
+  for (int i = 0; i < numFunda; ++i) {
+    const int lastCol = funda[i].first + funda[i].ncols - 1;
+    const int p = parent[lastCol];
+    if (p != -1) {
+      parentFunda[i] = colToFunda[p];
+      childrenFunda[colToFunda[p]].push_back(i);
+    }
+  }
+  // Supernode patterns: own columns, A rows below, children's update rows.
+  {
+    std::vector<int> mark(n, -1);
+    for (int i = 0; i < numFunda; ++i) {
+      auto& pat = funda[i].pattern;
+      const int f = funda[i].first, nc = funda[i].ncols;
+      for (int c = f; c < f + nc; ++c) {
+        mark[c] = i;
+        pat.push_back(c);
+      }
+      for (int c = f; c < f + nc; ++c) {
+        const int v = finalPerm[c];
+        for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
+          const int r = inv1[g.adj[t]];
+          if (r > c && mark[r] != i) {
+            mark[r] = i;
+            pat.push_back(r);
+          }
+        }
+      }
+      for (int ch : childrenFunda[i]) {
+        const auto& cp = funda[ch].pattern;
+        for (size_t t = funda[ch].ncols; t < cp.size(); ++t) {
+          const int r = cp[t];
+          if (mark[r] != i) {
+            mark[r] = i;
+            pat.push_back(r);
+          }
+        }
+      }
+      std::sort(pat.begin(), pat.end());
+      if (static_cast<int>(pat.size()) != colCount[f])
+        throw std::runtime_error("symla: supernode pattern does not match column count");
+    }
+  }
+
+  // --- step 6b: relaxed amalgamation (symbolic.hpp:269-329) ---
+  std::vector<int> dsu(numFunda);
+  std::iota(dsu.begin(), dsu.end(), 0);
+  auto find = [&](int x) {
+    while (dsu[x] != x) {
+      dsu[x] = dsu[dsu[x]];
+      x = dsu[x];
+    }
+    return x;
+  };
+  struct Agg {
+    int ncols;
+    std::vector<int> rowPattern;
+    std::vector<int> segments;
+  };
+  std::vector<Agg> agg(numFunda);
+  for (int i = 0; i < numFunda; ++i) {
+    agg[i].ncols = funda[i].ncols;
+    agg[i].rowPattern = funda[i].pattern;
+    agg[i].segments = {i};
+  }
+  std::vector<int> merged;
+  for (int i = 0; i < numFunda; ++i) {
+    for (int c : childrenFunda[i]) {
+      const int cr = find(c);
+      if (cr == i) continue;
+      const int mergedNcols = agg[i].ncols + agg[cr].ncols;
+      if (mergedNcols > opts.max_relax_size) continue;
+      merged.clear();
+      merged.reserve(agg[i].rowPattern.size() + agg[cr].rowPattern.size());
+      std::set_union(agg[i].rowPattern.begin(), agg[i].rowPattern.end(), agg[cr].rowPattern.begin(),
+                     agg[cr].rowPattern.end(), std::back_inserter(merged));
+      const int64_t mergedStorage = storageNnzOf(mergedNcols, merged.size());
+      const int64_t origStorage = storageNnzOf(agg[i].ncols, agg[i].rowPattern.size()) +
+                                  storageNnzOf(agg[cr].ncols, agg[cr].rowPattern.size());
+      const double extraFrac = static_cast<double>(mergedStorage - origStorage) / static_cast<double>(mergedStorage);
+      if (extraFrac > opts.max_relax_fill_fraction) continue;
+      dsu[cr] = i;
+      agg[i].ncols = mergedNcols;
+      agg[i].rowPattern.swap(merged);
+      agg[i].segments.insert(agg[i].segments.end() - 1, agg[cr].segments.begin(), agg[cr].segments.end());
+      std::vector<int>().swap(agg[cr].rowPattern);
+    }
+  }
+
+  // --- step 6c/6d: group postorder and relabeling (symbolic.hpp:335-366) ---
+  std::vector<int> finalRoots;
+  for (int i = 0; i < numFunda; ++i)
+    if (find(i) == i) finalRoots.push_back(i);
+  const int numFinal = static_cast<int>(finalRoots.size());
+  std::vector<int> compactOf(numFunda, -1);
+  for (int k = 0; k < numFinal; ++k) compactOf[finalRoots[k]] = k;
+  std::vector<int> finalParentCompact(numFinal);
+  for (int k = 0; k < numFinal; ++k) {
+    const int pf = parentFunda[finalRoots[k]];
+    finalParentCompact[k] = (pf == -1) ? -1 : compactOf[find(pf)];
+  }
+  std::vector<int> groupPostorder = postorderForest(finalParentCompact);
+  std::vector<int> newOrder;
+  newOrder.reserve(n);
+  sr.supernodes.reserve(numFinal);
+  std::vector<int> groupOfSupernode;
+  for (int gk : groupPostorder) {
+    const int r = finalRoots[gk];
+    Supernode sn;
+    sn.firstCol = static_cast<int>(newOrder.size());
+    sn.ncols = agg[r].ncols;
+    for (int fi : agg[r].segments)
+      for (int c = funda[fi].first; c < funda[fi].first + funda[fi].ncols; ++c) newOrder.push_back(c);
+    sr.supernodes.push_back(std::move(sn));
+    groupOfSupernode.push_back(r);
+  }
+  std::vector<int> inv2(n);  // stage-1 index -> final index
+  for (int k = 0; k < n; ++k) inv2[newOrder[k]] = k;
+
+  // --- step 6e/6f: final perm, etree and patterns ---
+  sr.perm.resize(n);
+  for (int k = 0; k < n; ++k) sr.perm[k] = finalPerm[newOrder[k]];
+  sr.etreeParent.assign(n, -1);
+  sr.colCount.assign(n, 0);
+  for (int j = 0; j < n; ++j) {
+    sr.etreeParent[inv2[j]] = parent[j] == -1 ? -1 : inv2[parent[j]];
+    sr.colCount[inv2[j]] = colCount[j];
+  }
+  {
+    std::vector<int> mark(n, -1);
+    for (size_t s = 0; s < sr.supernodes.size(); ++s) {
+      auto& sn = sr.supernodes[s];
+      const int r = groupOfSupernode[s];
+      std::vector<int> pat;
+      for (int fi : agg[r].segments)
+        for (int x : funda[fi].pattern) {
+          const int y = inv2[x];
+          if (mark[y] != static_cast<int>(s)) {
+            mark[y] = static_cast<int>(s);
+            pat.push_back(y);
+          }
+        }
+      std::sort(pat.begin(), pat.end());
+      for (int t = 0; t < sn.ncols; ++t)
+        if (pat[t] != sn.firstCol + t) throw std::runtime_error("symla: supernode columns are not contiguous");
+      sn.rowPattern.swap(pat);
+    }
+  }
+  sr.nnzL = 0;
+  for (int c : colCount) sr.nnzL += c;
+  return sr;
+}
+
+}  // namespace symla_hip
