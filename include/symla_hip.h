@@ -167,6 +167,33 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
  * events on the plan's stream). */
 int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms);
 
+/* Per-kernel-class profile (no counterpart in the reference, which only
+ * has the compile-time SYMLA_PROFILE accumulators of multifrontal.hpp:56-78).
+ * When enabled, every launch of the listed kernel classes is bracketed by
+ * HIP events on the plan's stream; symla_plan_get_profile returns the sums
+ * since the previous call and resets them.  schur_flops is the algorithmic
+ * flop count p*r*(r+1) of the MFMA trailing updates (lower triangle, 2 flops
+ * per multiply-add). */
+typedef struct {
+  double syrk_ms;
+  long long syrk_launches;
+  double schur_flops;
+  double panel_ms;
+  long long panel_launches;
+  double panel_rows_ms;
+  long long panel_rows_launches;
+  double small_ms;
+  long long small_launches;
+  double assemble_ms;
+  long long assemble_launches;
+  double fwd_ms;
+  long long fwd_launches;
+  double bwd_ms;
+  long long bwd_launches;
+} symla_profile;
+int symla_plan_set_profiling(symla_plan* p, int enable);
+int symla_plan_get_profile(symla_plan* p, symla_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,6 +252,35 @@ int symla_front_view(symla_plan* p, int si, int* m, int* npiv, int* rows, double
   });
 }
 
+int symla_plan_set_profiling(symla_plan* p, int enable) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_plan_set_profiling: null plan");
+    p->plan->setProfiling(enable != 0);
+  });
+}
+
+int symla_plan_get_profile(symla_plan* p, symla_profile* out) {
+  return planGuarded(p, [&] {
+    if (!p || !out) throw std::invalid_argument("symla_plan_get_profile: null argument");
+    const auto r = p->plan->takeProfile();
+    out->syrk_ms = r.ms[Plan::kSyrk];
+    out->syrk_launches = r.launches[Plan::kSyrk];
+    out->schur_flops = r.schurFlops;
+    out->panel_ms = r.ms[Plan::kPanel];
+    out->panel_launches = r.launches[Plan::kPanel];
+    out->panel_rows_ms = r.ms[Plan::kPanelRows];
+    out->panel_rows_launches = r.launches[Plan::kPanelRows];
+    out->small_ms = r.ms[Plan::kSmall];
+    out->small_launches = r.launches[Plan::kSmall];
+    out->assemble_ms = r.ms[Plan::kAssemble];
+    out->assemble_launches = r.launches[Plan::kAssemble];
+    out->fwd_ms = r.ms[Plan::kFwd];
+    out->fwd_launches = r.launches[Plan::kFwd];
+    out->bwd_ms = r.ms[Plan::kBwd];
+    out->bwd_launches = r.launches[Plan::kBwd];
+  });
+}
+
 int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms) {
   if (!p) return fail(SYMLA_ERR_INVALID, "symla_plan_timing: null plan");
   if (factor_ms) *factor_ms = p->plan->lastFactorMs();
@@ -357,6 +386,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     f.stopped = dstop.p;
     f.pert = dpert.p;
     f.npert = dnpert.p;
+    f.schurFlops = nullptr;
     FactorParams prm;
     prm.zeroTol = opts ? opts->zero_tolerance : 1e-300;
     prm.relFloor = opts ? opts->relative_pivot_floor : 1e-12;

@@ -47,6 +47,7 @@ struct DevFronts {
   int* stopped;          // 1 when a front stopped on a delay
   double* pert;          // static mode: sum |perturbation|
   int* npert;            // static mode: perturbed pivots
+  unsigned long long* schurFlops;  // algorithmic flops of the MFMA trailing updates (p*r*(r+1) per update)
 };
 
 struct FactorParams {

@@ -147,6 +147,7 @@ __global__ void k_reset_fronts(int numSN, DevFronts fd) {
     fd.pert[s] = 0.0;
     fd.npert[s] = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && fd.schurFlops) *fd.schurFlops = 0ull;
 }
 
 __global__ void k_reset_perm(const int* list, int count, DevFronts fd) {
@@ -819,7 +820,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(int s, DevFronts fd) {
 
 // Trailing update T(k1:m, k1:m) -= (L D) L^T over the lower triangle, for
 // the last panel's p pivots, with v_mfma_f64_16x16x4f64.  64x64 output tile
-// per workgroup, four waves of 32x32.
+// per workgroup, four waves of 32x32.  Operand layout of the instruction on
+// gfx950 (measured): A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// result D[i][j] in lane j + 16*(i%4), register i/4.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
@@ -845,6 +848,8 @@ __global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
   const int tj = t - ti * (ti + 1) / 2;
   if (ti >= T) return;
   const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
+  if (t == 0 && threadIdx.x == 0 && fd.schurFlops)
+    atomicAdd(fd.schurFlops, static_cast<unsigned long long>(p) * N * (N + 1));
   double* F = fd.F[s];
   const int64_t io = fd.idxOff[s];
   auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
@@ -895,7 +900,7 @@ __global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + 4 * lk + rr;
+        const int row = i0 + wr + bi * 16 + 4 * rr + lk;
         const int col = j0 + wc + bj * 16 + lr;
         if (row < m && col < m && (ti != tj || row >= col)) {
           const double v = Fat(row, col) - acc[bi][bj][rr];

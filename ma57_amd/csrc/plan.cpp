@@ -103,10 +103,60 @@ Plan::Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx) 
     dpermVec_.upload(pv, stream_);
   }
   dnorms_.alloc(16);
+  dschur_.alloc(1);
+  hipCheck(hipMemset(dschur_.p, 0, sizeof(unsigned long long)), "memset");
   hipCheck(hipStreamSynchronize(stream_), "plan setup");
 }
 
+void Plan::setProfiling(bool on) {
+  profiling_ = on;
+  profAcc_ = Profile{};
+  for (auto& pc : prof_) pc.used = 0;
+}
+
+void Plan::profBegin(int cat) {
+  if (!profiling_) return;
+  auto& pc = prof_[cat];
+  while (pc.ev.size() < pc.used + 2) {
+    hipEvent_t e;
+    hipCheck(hipEventCreate(&e), "hipEventCreate");
+    pc.ev.push_back(e);
+  }
+  hipCheck(hipEventRecord(pc.ev[pc.used], stream_), "hipEventRecord");
+}
+
+void Plan::profEnd(int cat) {
+  if (!profiling_) return;
+  auto& pc = prof_[cat];
+  hipCheck(hipEventRecord(pc.ev[pc.used + 1], stream_), "hipEventRecord");
+  pc.used += 2;
+  profAcc_.launches[cat] += 1;
+}
+
+void Plan::profHarvest() {
+  if (!profiling_) return;
+  hipCheck(hipStreamSynchronize(stream_), "profile sync");
+  for (int c = 0; c < kNumCats; ++c) {
+    auto& pc = prof_[c];
+    for (size_t i = 0; i + 1 < pc.used; i += 2) {
+      float ms = 0.f;
+      hipCheck(hipEventElapsedTime(&ms, pc.ev[i], pc.ev[i + 1]), "hipEventElapsedTime");
+      profAcc_.ms[c] += ms;
+    }
+    pc.used = 0;
+  }
+}
+
+Plan::Profile Plan::takeProfile() {
+  profHarvest();
+  Profile out = profAcc_;
+  profAcc_ = Profile{};
+  return out;
+}
+
 Plan::~Plan() {
+  for (auto& pc : prof_)
+    for (hipEvent_t e : pc.ev) (void)hipEventDestroy(e);
   for (double* p : dynAllocs_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -360,6 +410,7 @@ DevFronts Plan::devFronts() {
   f.stopped = dstopped_.p;
   f.pert = dpert_.p;
   f.npert = dnpert_.p;
+  f.schurFlops = dschur_.p;
   return f;
 }
 
@@ -382,9 +433,17 @@ void Plan::factorLargeFront(int s, const FactorParams& prm) {
   launchFrontMaxAbs(s, m, fd, stream_);
   const int iters = (nE + kPanel - 1) / kPanel;
   for (int it = 0; it < iters; ++it) {
+    profBegin(kPanel);
     launchPanel(s, fd, prm, dws_.p, stream_);
-    launchPanelRows(s, m - nE, fd, stream_);
+    profEnd(kPanel);
+    if (m > nE) {
+      profBegin(kPanelRows);
+      launchPanelRows(s, m - nE, fd, stream_);
+      profEnd(kPanelRows);
+    }
+    profBegin(kSyrk);
     launchSyrk(s, m - it * kPanel, fd, stream_);
+    profEnd(kSyrk);
   }
   launchFrontFinish(s, fd, stream_);
 }
@@ -400,8 +459,14 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
   launchResetPerm(dallFronts_.p, numSN_, fd, stream_);
   for (size_t L = 0; L < sched_.size(); ++L) {
     const auto& ls = sched_[L];
+    profBegin(kAssemble);
     launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, 64, fd, aValsDev, 1, stream_);
-    launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
+    profEnd(kAssemble);
+    if (ls.smallCount > 0) {
+      profBegin(kSmall);
+      launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
+      profEnd(kSmall);
+    }
     for (int s : ls.large) factorLargeFront(s, prm);
   }
   launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
@@ -409,6 +474,12 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
 }
 
 void Plan::collectStats(FactorInfo& info, bool& anyDelay) {
+  if (profiling_) {
+    unsigned long long fl = 0;
+    hipCheck(hipMemcpyAsync(&fl, dschur_.p, sizeof(fl), hipMemcpyDeviceToHost, stream_), "schur flops");
+    hipCheck(hipStreamSynchronize(stream_), "schur flops");
+    profAcc_.schurFlops += static_cast<double>(fl);
+  }
   long long si[8];
   double sd[4];
   hipCheck(hipMemcpyAsync(si, dstatI_.p, 5 * sizeof(long long), hipMemcpyDeviceToHost, stream_), "stats");
@@ -691,11 +762,15 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   const int nl = static_cast<int>(levelFronts_.size());
   for (int L = 0; L < nl; ++L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
+    profBegin(kFwd);
     launchFwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dBp_.p, dZ_.p, dG_.p, dDl_.p, dgOff_.p, stream_);
+    profEnd(kFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
+    profBegin(kBwd);
     launchBwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, stream_);
+    profEnd(kBwd);
   }
   launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
 }

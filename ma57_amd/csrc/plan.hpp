@@ -101,6 +101,16 @@ class Plan {
   double lastSolveMs() const { return solveMs_; }
   std::string lastError;
 
+  // per-kernel-class event profile
+  enum ProfCat { kSyrk = 0, kPanel, kPanelRows, kSmall, kAssemble, kFwd, kBwd, kNumCats };
+  struct Profile {
+    double ms[kNumCats] = {};
+    long long launches[kNumCats] = {};
+    double schurFlops = 0.0;
+  };
+  void setProfiling(bool on);
+  Profile takeProfile();
+
  private:
   // symbolic
   int n_ = 0, numSN_ = 0;
@@ -170,6 +180,17 @@ class Plan {
   bool layoutDirty_ = false;  // dynamic mode changed the layout
   double factorMs_ = 0.0, solveMs_ = 0.0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  bool profiling_ = false;
+  struct ProfEvents {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+  };
+  ProfEvents prof_[kNumCats];
+  Profile profAcc_;
+  DBuf<unsigned long long> dschur_;
+  void profBegin(int cat);
+  void profEnd(int cat);
+  void profHarvest();  // after a stream sync: fold recorded event pairs into profAcc_
 
   void buildStaticLayout();
   void uploadLayout();

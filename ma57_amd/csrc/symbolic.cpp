@@ -378,22 +378,28 @@ struct NDState {
     const int total = static_cast<int>(q.size());
     std::vector<int> levCount(ecc + 2, 0);
     for (int v : q) ++levCount[level[v]];
-    // choose separator level
-    int bestL = -1;
-    double bestCost = 1e300;
+    // choose the separator level: the smallest level leaving both sides at
+    // most 70% of the piece; if none does, the most balanced level
+    int bestL = -1, bestS = 0, bestBig = 0;
+    int fallbackL = -1, fallbackBig = total + 1;
     int before = 0;
     for (int L = 0; L <= ecc; ++L) {
       const int a = before, s = levCount[L], b = total - before - s;
       before += s;
       if (a == 0 || b == 0) continue;
-      const double imb = static_cast<double>(std::max(a, b)) / total;
-      double cost = s;
-      if (imb > 0.65) cost *= 1.0 + 20.0 * (imb - 0.65);
-      if (cost < bestCost) {
-        bestCost = cost;
+      const int big = std::max(a, b);
+      if (big < fallbackBig) {
+        fallbackBig = big;
+        fallbackL = L;
+      }
+      if (10 * static_cast<int64_t>(big) > 7 * static_cast<int64_t>(total)) continue;
+      if (bestL < 0 || s < bestS || (s == bestS && big < bestBig)) {
         bestL = L;
+        bestS = s;
+        bestBig = big;
       }
     }
+    if (bestL < 0) bestL = fallbackL;
     if (bestL < 0) {
       for (int v : q) level[v] = -1;
       orderLeaf(V);
