@@ -61,6 +61,14 @@ class DenseOpts(C.Structure):
                 ("expected_sign", i32p)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("syrk_ms", C.c_double), ("syrk_launches", C.c_longlong), ("schur_flops", C.c_double),
+                ("panel_ms", C.c_double), ("panel_launches", C.c_longlong), ("panel_rows_ms", C.c_double),
+                ("panel_rows_launches", C.c_longlong), ("small_ms", C.c_double), ("small_launches", C.c_longlong),
+                ("assemble_ms", C.c_double), ("assemble_launches", C.c_longlong), ("fwd_ms", C.c_double),
+                ("fwd_launches", C.c_longlong), ("bwd_ms", C.c_double), ("bwd_launches", C.c_longlong)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/symla_hip.h
 SIGNATURES = {
     "symla_last_error": (C.c_char_p, []),
@@ -85,6 +93,8 @@ SIGNATURES = {
     "symla_dense_factor": (C.c_int, [C.c_int, f64p, C.c_int, C.c_int, C.POINTER(DenseOpts), f64p, f64p, i32p,
                                      i32p, i32p, i32p, i32p, i32p, f64p, i32p]),
     "symla_plan_timing": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "symla_plan_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "symla_plan_get_profile": (C.c_int, [C.c_void_p, C.POINTER(Profile)]),
 }
 
 _lib = None

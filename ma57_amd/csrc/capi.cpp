@@ -263,21 +263,21 @@ int symla_plan_get_profile(symla_plan* p, symla_profile* out) {
   return planGuarded(p, [&] {
     if (!p || !out) throw std::invalid_argument("symla_plan_get_profile: null argument");
     const auto r = p->plan->takeProfile();
-    out->syrk_ms = r.ms[Plan::kSyrk];
-    out->syrk_launches = r.launches[Plan::kSyrk];
+    out->syrk_ms = r.ms[Plan::kProfSyrk];
+    out->syrk_launches = r.launches[Plan::kProfSyrk];
     out->schur_flops = r.schurFlops;
-    out->panel_ms = r.ms[Plan::kPanel];
-    out->panel_launches = r.launches[Plan::kPanel];
-    out->panel_rows_ms = r.ms[Plan::kPanelRows];
-    out->panel_rows_launches = r.launches[Plan::kPanelRows];
-    out->small_ms = r.ms[Plan::kSmall];
-    out->small_launches = r.launches[Plan::kSmall];
-    out->assemble_ms = r.ms[Plan::kAssemble];
-    out->assemble_launches = r.launches[Plan::kAssemble];
-    out->fwd_ms = r.ms[Plan::kFwd];
-    out->fwd_launches = r.launches[Plan::kFwd];
-    out->bwd_ms = r.ms[Plan::kBwd];
-    out->bwd_launches = r.launches[Plan::kBwd];
+    out->panel_ms = r.ms[Plan::kProfPanel];
+    out->panel_launches = r.launches[Plan::kProfPanel];
+    out->panel_rows_ms = r.ms[Plan::kProfPanelRows];
+    out->panel_rows_launches = r.launches[Plan::kProfPanelRows];
+    out->small_ms = r.ms[Plan::kProfSmall];
+    out->small_launches = r.launches[Plan::kProfSmall];
+    out->assemble_ms = r.ms[Plan::kProfAssemble];
+    out->assemble_launches = r.launches[Plan::kProfAssemble];
+    out->fwd_ms = r.ms[Plan::kProfFwd];
+    out->fwd_launches = r.launches[Plan::kProfFwd];
+    out->bwd_ms = r.ms[Plan::kProfBwd];
+    out->bwd_launches = r.launches[Plan::kProfBwd];
   });
 }
 
@@ -400,14 +400,17 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     if (n <= kSmallFrontMax) {
       launchFactorSmall(dlist.p, 1, n, f, prm, st);
     } else {
-      launchFrontMaxAbs(0, n, f, st);
-      const int iters = (nE + kPanel - 1) / kPanel;
-      for (int it = 0; it < iters; ++it) {
-        launchPanel(0, f, prm, dws.p, st);
-        launchPanelRows(0, n - nE, f, st);
-        launchSyrk(0, n - it * kPanel, f, st);
-      }
-      launchFrontFinish(0, f, st);
+      std::vector<int> mv{n}, nev{nE};
+      LargeSchedHost lh;
+      LargeSched ls = buildLargeSched({0}, mv, nev, lh);
+      DBuf<int> dl, dr, dt;
+      DBuf<int64_t> dw;
+      dl.upload(lh.lists, st);
+      dr.upload(lh.rowStart, st);
+      dt.upload(lh.tileStart, st);
+      dw.upload(std::vector<int64_t>{0}, st);
+      runLargeSched(ls, dl.p, dr.p, dt.p, f, prm, dws.p, dw.p, st, nullptr);
+      hipCheck(hipStreamSynchronize(st), "dense large front");
     }
     hipCheck(hipMemcpy2DAsync(A, sizeof(double) * lda, dA.p, sizeof(double) * n, sizeof(double) * n, n,
                               hipMemcpyDeviceToHost, st),

@@ -128,6 +128,19 @@ __device__ inline double staticPivot(double a_kk, int sign, double delta, double
   return d;
 }
 
+// Front owning flat block `b` of a batched launch: start[f] <= b < start[f+1].
+__device__ inline int findSegment(const int* start, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -487,14 +500,19 @@ size_t smallFrontSmemBytes(int m) {
 // Large fronts.
 // ---------------------------------------------------------------------------
 
-// max|F| over the lower triangle (dense_kernel.hpp:413-415).
-__global__ __launch_bounds__(256) void k_front_maxabs(int s, DevFronts fd) {
+// max|F| over the lower triangle (dense_kernel.hpp:413-415); `bpf` blocks
+// per listed front.
+__global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count, int bpf, DevFronts fd) {
+  const int f = blockIdx.x / bpf;
+  if (f >= count) return;
+  const int s = list[f];
+  const int part = blockIdx.x % bpf;
   const int m = fd.m[s];
   const double* F = fd.F[s];
   double v = 0.0;
   const int lane = threadIdx.x & 63;
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nwTot = (gridDim.x * blockDim.x) >> 6;
+  const int gw = part * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nwTot = bpf * (blockDim.x >> 6);
   for (int j = gw; j < m; j += nwTot) {
     const double* col = F + static_cast<size_t>(j) * m;
     for (int i = j + lane; i < m; i += 64) v = fmax(v, fabs(col[i]));
@@ -509,7 +527,8 @@ __global__ __launch_bounds__(256) void k_front_maxabs(int s, DevFronts fd) {
 // correction of this panel's pivots, like the reference's setColumnRaw
 // (dense_kernel.hpp:516-545).  Accepted pivots write their multipliers for
 // the eligible rows; the remaining rows are solved by k_panel_rows.
-__global__ __launch_bounds__(kPanelThreads) void k_panel(int s, DevFronts fd, FactorParams prm, double* ws) {
+__global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int count, DevFronts fd, FactorParams prm,
+                                                          double* wsBase, const int64_t* wsOff) {
   constexpr int NT = kPanelThreads;
   __shared__ double wvec[kPanel + 2];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
@@ -518,6 +537,9 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int s, DevFronts fd, Fa
   __shared__ int redI[NT / 64 + 1];
 
   const int tid = threadIdx.x;
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  double* ws = wsBase + wsOff[s];
   if (fd.stopped[s]) {
     if (tid == 0) fd.pcount[s] = 0;
     return;
@@ -751,10 +773,15 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int s, DevFronts fd, Fa
 
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
 // panel: row-parallel forward substitution against U = D L^T of the panel.
-__global__ __launch_bounds__(256) void k_panel_rows(int s, DevFronts fd) {
+__global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, const int* blockStart,
+                                                     DevFronts fd) {
   __shared__ double U[(kPanel + 1) * (kPanel + 1)];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = findSegment(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int rowBlock = blockIdx.x - blockStart[f];
   const int p = fd.pcount[s];
   if (p == 0) return;
   const int m = fd.m[s];
@@ -787,7 +814,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(int s, DevFronts fd) {
     U[qp + q * P1] = u;
   }
   __syncthreads();
-  const int i = nE + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = nE + rowBlock * blockDim.x + threadIdx.x;
   if (i >= m) return;
   double mx = 0.0;
   int q = 0;
@@ -825,13 +852,16 @@ __global__ __launch_bounds__(256) void k_panel_rows(int s, DevFronts fd) {
 // result D[i][j] in lane j + 16*(i%4), register i/4.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
+__global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, const int* tileStart, DevFronts fd) {
   constexpr int TS = 64;
   constexpr int KP = kPanel + 4;  // padded K
   __shared__ double As[TS][KP + 1];
   __shared__ double Bs[TS][KP + 1];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
+  if (static_cast<int>(blockIdx.x) >= tileStart[count]) return;
+  const int f = findSegment(tileStart, count, blockIdx.x);
+  const int s = list[f];
   const int p = fd.pcount[s];
   if (p == 0) return;
   const int m = fd.m[s];
@@ -841,7 +871,7 @@ __global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
   if (N <= 0) return;
   const int T = (N + TS - 1) / TS;
   // lower-triangular tile index
-  const int t = blockIdx.x;
+  const int t = blockIdx.x - tileStart[f];
   int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
@@ -912,8 +942,9 @@ __global__ __launch_bounds__(256) void k_syrk_f64(int s, DevFronts fd) {
   if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
 }
 
-__global__ void k_front_finish(int s, DevFronts fd) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) fd.nF[s] = fd.kcur[s];
+__global__ void k_front_finish(const int* list, int count, DevFronts fd) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < count) fd.nF[list[f]] = fd.kcur[list[f]];
 }
 
 // rowIdx = frontIdx[perm[i]] for every position of the listed fronts.
@@ -985,27 +1016,32 @@ void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd
   }
   k_factor_small<<<count, kSmallThreads, smem, st>>>(list, count, fd, prm);
 }
-void launchFrontMaxAbs(int s, int m, const DevFronts& fd, hipStream_t st) {
-  const int waves = m;
-  int blocks = (waves + 3) / 4;
-  if (blocks > 1024) blocks = 1024;
-  if (blocks < 1) blocks = 1;
-  k_front_maxabs<<<blocks, 256, 0, st>>>(s, fd);
+void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st) {
+  if (count <= 0) return;
+  int bpf = (maxM + 15) / 16;
+  if (bpf > 64) bpf = 64;
+  if (bpf < 1) bpf = 1;
+  k_front_maxabs<<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
 }
-void launchPanel(int s, const DevFronts& fd, const FactorParams& prm, double* ws, hipStream_t st) {
-  k_panel<<<1, kPanelThreads, 0, st>>>(s, fd, prm, ws);
+void launchPanel(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
+                 const int64_t* wsOff, hipStream_t st) {
+  if (count <= 0) return;
+  k_panel<<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
 }
-void launchPanelRows(int s, int rows, const DevFronts& fd, hipStream_t st) {
-  if (rows <= 0) return;
-  k_panel_rows<<<(rows + 255) / 256, 256, 0, st>>>(s, fd);
+void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
+                     hipStream_t st) {
+  if (count <= 0 || totalBlocks <= 0) return;
+  k_panel_rows<<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
 }
-void launchSyrk(int s, int maxTrailing, const DevFronts& fd, hipStream_t st) {
-  if (maxTrailing <= 0) return;
-  const int T = (maxTrailing + 63) / 64;
-  const int tiles = T * (T + 1) / 2;
-  k_syrk_f64<<<tiles, 256, 0, st>>>(s, fd);
+void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
+                hipStream_t st) {
+  if (count <= 0 || totalTiles <= 0) return;
+  k_syrk_f64<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
 }
-void launchFrontFinish(int s, const DevFronts& fd, hipStream_t st) { k_front_finish<<<1, 64, 0, st>>>(s, fd); }
+void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st) {
+  if (count <= 0) return;
+  k_front_finish<<<(count + 255) / 256, 256, 0, st>>>(list, count, fd);
+}
 void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
   k_make_rowidx<<<count, 256, 0, st>>>(list, count, fd);

@@ -16,21 +16,30 @@ void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& f
                     hipStream_t st);
 void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
                        hipStream_t st);
-void launchFrontMaxAbs(int s, int m, const DevFronts& fd, hipStream_t st);
-void launchPanel(int s, const DevFronts& fd, const FactorParams& prm, double* ws, hipStream_t st);
-void launchPanelRows(int s, int rows, const DevFronts& fd, hipStream_t st);
-void launchSyrk(int s, int maxTrailing, const DevFronts& fd, hipStream_t st);
-void launchFrontFinish(int s, const DevFronts& fd, hipStream_t st);
+void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st);
+void launchPanel(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
+                 const int64_t* wsOff, hipStream_t st);
+void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
+                     hipStream_t st);
+void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
+                hipStream_t st);
+void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st);
 void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st);
 void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st);
 
 // solve_kernels.hip
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st);
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st);
-void launchFwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* Z,
-               double* G, double* Dl, const int64_t* gOff, hipStream_t st);
-void launchBwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
-               double* Dl, const int64_t* gOff, hipStream_t st);
+void launchFwdGather(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* G,
+                     double* Dl, const int64_t* gOff, hipStream_t st);
+void launchFwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* Z, double* Dl,
+                   const int64_t* gOff, hipStream_t st);
+void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    const double* G, double* Dl, const int64_t* gOff, hipStream_t st);
+void launchBwdGemv(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                   int nrhs, const double* Z, const double* X, double* Dl, const int64_t* gOff, hipStream_t st);
+void launchBwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* X, double* Dl,
+                   const int64_t* gOff, hipStream_t st);
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);

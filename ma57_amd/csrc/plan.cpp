@@ -336,6 +336,23 @@ void Plan::buildSchedule() {
     ls.taskCount = static_cast<int>(tasks.size() - ls.taskOff);
   }
   levelFrontOff_[levelFronts_.size()] = static_cast<int64_t>(all.size());
+  LargeSchedHost lh;
+  hwsOff_.assign(numSN_, 0);
+  int64_t wsNeed = 2;
+  for (auto& ls : sched_) {
+    ls.lsched = buildLargeSched(ls.large, hm_, hnE_, lh);
+    int64_t off = 0;
+    for (int s : ls.large) {
+      hwsOff_[s] = off;
+      off += 2 * static_cast<int64_t>(hnE_[s]);
+    }
+    wsNeed = std::max(wsNeed, off);
+  }
+  dws_.ensure(static_cast<size_t>(wsNeed));
+  dwsOff_.upload(hwsOff_, stream_);
+  dlargeLists_.upload(lh.lists, stream_);
+  dlargeRowStart_.upload(lh.rowStart, stream_);
+  dlargeTileStart_.upload(lh.tileStart, stream_);
   dsmall_.upload(small, stream_);
   dtasks_.upload(tasks, stream_);
   dallFronts_.upload(all, stream_);
@@ -426,26 +443,62 @@ FactorParams Plan::params(const FactorOptions& o, bool staticMode, double delta,
   return p;
 }
 
-void Plan::factorLargeFront(int s, const FactorParams& prm) {
-  DevFronts fd = devFronts();
-  const int m = hm_[s], nE = hnE_[s];
-  dws_.ensure(2 * static_cast<size_t>(std::max(nE, 1)));
-  launchFrontMaxAbs(s, m, fd, stream_);
-  const int iters = (nE + kPanel - 1) / kPanel;
-  for (int it = 0; it < iters; ++it) {
-    profBegin(kPanel);
-    launchPanel(s, fd, prm, dws_.p, stream_);
-    profEnd(kPanel);
-    if (m > nE) {
-      profBegin(kPanelRows);
-      launchPanelRows(s, m - nE, fd, stream_);
-      profEnd(kPanelRows);
-    }
-    profBegin(kSyrk);
-    launchSyrk(s, m - it * kPanel, fd, stream_);
-    profEnd(kSyrk);
+LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
+                           LargeSchedHost& h) {
+  LargeSched ls;
+  ls.allOff = static_cast<int64_t>(h.lists.size());
+  ls.allCount = static_cast<int>(fronts.size());
+  int maxIters = 0;
+  for (int s : fronts) {
+    h.lists.push_back(s);
+    ls.maxM = std::max(ls.maxM, m[s]);
+    maxIters = std::max(maxIters, (nE[s] + kPanel - 1) / kPanel);
   }
-  launchFrontFinish(s, fd, stream_);
+  for (int it = 0; it < maxIters; ++it) {
+    LargeIter li;
+    li.listOff = static_cast<int64_t>(h.lists.size());
+    li.rowOff = static_cast<int64_t>(h.rowStart.size());
+    li.tileOff = static_cast<int64_t>(h.tileStart.size());
+    int rb = 0, tl = 0;
+    for (int s : fronts) {
+      if ((nE[s] + kPanel - 1) / kPanel <= it) continue;
+      h.lists.push_back(s);
+      h.rowStart.push_back(rb);
+      h.tileStart.push_back(tl);
+      rb += (m[s] - nE[s] + 255) / 256;
+      const int T = (m[s] - it * kPanel + 63) / 64;
+      tl += T * (T + 1) / 2;
+      ++li.count;
+    }
+    h.rowStart.push_back(rb);
+    h.tileStart.push_back(tl);
+    li.rowBlocks = rb;
+    li.tiles = tl;
+    ls.iters.push_back(li);
+  }
+  return ls;
+}
+
+void runLargeSched(const LargeSched& ls, const int* dLists, const int* dRowStart, const int* dTileStart,
+                   const DevFronts& fd, const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st,
+                   Plan* prof) {
+  if (ls.allCount == 0) return;
+  launchFrontMaxAbs(dLists + ls.allOff, ls.allCount, ls.maxM, fd, st);
+  for (const auto& it : ls.iters) {
+    const int* lst = dLists + it.listOff;
+    if (prof) prof->profBegin(Plan::kProfPanel);
+    launchPanel(lst, it.count, fd, prm, ws, wsOff, st);
+    if (prof) prof->profEnd(Plan::kProfPanel);
+    if (it.rowBlocks > 0) {
+      if (prof) prof->profBegin(Plan::kProfPanelRows);
+      launchPanelRows(lst, it.count, dRowStart + it.rowOff, it.rowBlocks, fd, st);
+      if (prof) prof->profEnd(Plan::kProfPanelRows);
+    }
+    if (prof) prof->profBegin(Plan::kProfSyrk);
+    launchSyrk(lst, it.count, dTileStart + it.tileOff, it.tiles, fd, st);
+    if (prof) prof->profEnd(Plan::kProfSyrk);
+  }
+  launchFrontFinish(dLists + ls.allOff, ls.allCount, fd, st);
 }
 
 void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
@@ -459,15 +512,16 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
   launchResetPerm(dallFronts_.p, numSN_, fd, stream_);
   for (size_t L = 0; L < sched_.size(); ++L) {
     const auto& ls = sched_[L];
-    profBegin(kAssemble);
+    profBegin(kProfAssemble);
     launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, 64, fd, aValsDev, 1, stream_);
-    profEnd(kAssemble);
+    profEnd(kProfAssemble);
     if (ls.smallCount > 0) {
-      profBegin(kSmall);
+      profBegin(kProfSmall);
       launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
-      profEnd(kSmall);
+      profEnd(kProfSmall);
     }
-    for (int s : ls.large) factorLargeFront(s, prm);
+    runLargeSched(ls.lsched, dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, fd, prm, dws_.p, dwsOff_.p,
+                  stream_, profiling_ ? this : nullptr);
   }
   launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
   launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
@@ -617,7 +671,25 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     launchResetPerm(dlist.p, static_cast<int>(all.size()), fd, stream_);
     launchAssemble(dtask.p, static_cast<int>(tasks.size()), 1 << 30, fd, aValsDev, 0, stream_);
     launchFactorSmall(dlist.p + all.size(), static_cast<int>(small.size()), maxM, fd, prm, stream_);
-    for (int s : large) factorLargeFront(s, prm);
+    if (!large.empty()) {
+      LargeSchedHost lh;
+      LargeSched lsd = buildLargeSched(large, hm_, hnE_, lh);
+      std::vector<int64_t> wsOffH(numSN_, 0);
+      int64_t off = 0;
+      for (int s : large) {
+        wsOffH[s] = off;
+        off += 2 * static_cast<int64_t>(hnE_[s]);
+      }
+      dws_.ensure(static_cast<size_t>(std::max<int64_t>(off, 2)));
+      DBuf<int> dl, dr, dt;
+      DBuf<int64_t> dw;
+      dl.upload(lh.lists, stream_);
+      dr.upload(lh.rowStart, stream_);
+      dt.upload(lh.tileStart, stream_);
+      dw.upload(wsOffH, stream_);
+      runLargeSched(lsd, dl.p, dr.p, dt.p, fd, prm, dws_.p, dw.p, stream_, nullptr);
+      hipCheck(hipStreamSynchronize(stream_), "dynamic large fronts");
+    }
     launchMakeRowIdx(dlist.p, static_cast<int>(all.size()), fd, stream_);
     // read back pivot counts and row identities of this level
     std::vector<int> nfAll(numSN_);
@@ -751,6 +823,22 @@ void Plan::ensureSolveWs(int nrhs) {
     dgOff_.upload(off, stream_);
     dG_.ensure(static_cast<size_t>(off[numSN_]));
     dDl_.ensure(static_cast<size_t>(off[numSN_]));
+    // per level: 64-row chunks of every front (the GEMM-shaped solve parts)
+    std::vector<int> bs;
+    const int nl = static_cast<int>(levelFronts_.size());
+    solveBlockOff_.assign(nl, 0);
+    solveBlocks_.assign(nl, 0);
+    for (int L = 0; L < nl; ++L) {
+      solveBlockOff_[L] = static_cast<int64_t>(bs.size());
+      int acc = 0;
+      for (int s : levelFronts_[L]) {
+        bs.push_back(acc);
+        acc += (hm_[s] + 63) / 64;
+      }
+      bs.push_back(acc);
+      solveBlocks_[L] = acc;
+    }
+    dsolveBlockStart_.upload(bs, stream_);
     gOffNrhs_ = nrhs;
   }
 }
@@ -762,15 +850,22 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   const int nl = static_cast<int>(levelFronts_.size());
   for (int L = 0; L < nl; ++L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
-    profBegin(kFwd);
-    launchFwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dBp_.p, dZ_.p, dG_.p, dDl_.p, dgOff_.p, stream_);
-    profEnd(kFwd);
+    const int* lst = dallFronts_.p + levelFrontOff_[L];
+    const int* bst = dsolveBlockStart_.p + solveBlockOff_[L];
+    profBegin(kProfFwd);
+    launchFwdGather(lst, cnt, fd, n_, nrhs, dBp_.p, dG_.p, dDl_.p, dgOff_.p, stream_);
+    launchFwdTrsv(lst, cnt, fd, n_, nrhs, dZ_.p, dDl_.p, dgOff_.p, stream_);
+    launchFwdDelta(lst, cnt, bst, solveBlocks_[L], fd, nrhs, dG_.p, dDl_.p, dgOff_.p, stream_);
+    profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
-    profBegin(kBwd);
-    launchBwd(dallFronts_.p + levelFrontOff_[L], cnt, fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, stream_);
-    profEnd(kBwd);
+    const int* lst = dallFronts_.p + levelFrontOff_[L];
+    const int* bst = dsolveBlockStart_.p + solveBlockOff_[L];
+    profBegin(kProfBwd);
+    launchBwdGemv(lst, cnt, bst, solveBlocks_[L], fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, stream_);
+    launchBwdTrsv(lst, cnt, fd, n_, nrhs, dX_.p, dDl_.p, dgOff_.p, stream_);
+    profEnd(kProfBwd);
   }
   launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
 }

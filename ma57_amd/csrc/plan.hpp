@@ -79,6 +79,35 @@ struct RefineResult {
   double initialRel = 0.0, finalRel = 0.0, backwardError = 0.0;
 };
 
+// Batched schedule of the large-front path for one set of independent fronts
+// (one tree level): per panel iteration, the fronts that still have eligible
+// columns, their row blocks and their trailing-update tiles.
+struct LargeIter {
+  int64_t listOff = 0;
+  int count = 0;
+  int64_t rowOff = 0;
+  int rowBlocks = 0;
+  int64_t tileOff = 0;
+  int tiles = 0;
+};
+struct LargeSched {
+  int64_t allOff = 0;
+  int allCount = 0;
+  int maxM = 0;
+  std::vector<LargeIter> iters;
+};
+struct LargeSchedHost {
+  std::vector<int> lists, rowStart, tileStart;
+};
+// m and nE are indexed by front id.
+LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
+                           LargeSchedHost& h);
+class Plan;
+// Launch sequence of a LargeSched; `prof` (may be null) brackets launches with events.
+void runLargeSched(const LargeSched& ls, const int* dLists, const int* dRowStart, const int* dTileStart,
+                   const DevFronts& fd, const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st,
+                   Plan* prof);
+
 class Plan {
  public:
   Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx);
@@ -102,7 +131,7 @@ class Plan {
   std::string lastError;
 
   // per-kernel-class event profile
-  enum ProfCat { kSyrk = 0, kPanel, kPanelRows, kSmall, kAssemble, kFwd, kBwd, kNumCats };
+  enum ProfCat { kProfSyrk = 0, kProfPanel, kProfPanelRows, kProfSmall, kProfAssemble, kProfFwd, kProfBwd, kNumCats };
   struct Profile {
     double ms[kNumCats] = {};
     long long launches[kNumCats] = {};
@@ -110,6 +139,8 @@ class Plan {
   };
   void setProfiling(bool on);
   Profile takeProfile();
+  void profBegin(int cat);
+  void profEnd(int cat);
 
  private:
   // symbolic
@@ -155,6 +186,7 @@ class Plan {
     int64_t taskOff = 0;
     int taskCount = 0;
     std::vector<int> large;
+    LargeSched lsched;
   };
   std::vector<LevelSched> sched_;
   DBuf<int> dsmall_;
@@ -168,6 +200,9 @@ class Plan {
   DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;
   int gOffNrhs_ = -1;
+  DBuf<int> dsolveBlockStart_;
+  std::vector<int64_t> solveBlockOff_;
+  std::vector<int> solveBlocks_;
   // refinement CSR
   DBuf<int64_t> dsymPtr_, dsymSrc_, dstPtr_, dstSrc_;
   DBuf<int> dsymCol_;
@@ -188,8 +223,6 @@ class Plan {
   ProfEvents prof_[kNumCats];
   Profile profAcc_;
   DBuf<unsigned long long> dschur_;
-  void profBegin(int cat);
-  void profEnd(int cat);
   void profHarvest();  // after a stream sync: fold recorded event pairs into profAcc_
 
   void buildStaticLayout();
@@ -200,7 +233,9 @@ class Plan {
   FactorParams params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const;
   void runStatic(const double* aValsDev, const FactorParams& prm);
   void runDynamic(const double* aValsDev, const FactorParams& prm);
-  void factorLargeFront(int s, const FactorParams& prm);
+  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_;
+  DBuf<int64_t> dwsOff_;
+  std::vector<int64_t> hwsOff_;
   void collectStats(FactorInfo& info, bool& anyDelay);
   void finishSingular();
   void ensureSolveWs(int nrhs);

@@ -34,25 +34,43 @@ __global__ void k_permute_out(int n, int nrhs, const int* __restrict__ perm, con
   }
 }
 
-// Forward + diagonal solve of one tree level; one workgroup per front.
-// G (pre-permutation positions) and Dl (post-permutation positions) are
-// m x nrhs per-front work blocks at gOff[s].  Dl rows [0, nF) hold y, rows
-// [nF, m) hold the delta handed to the parent.
-__global__ __launch_bounds__(256) void k_fwd(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                              const double* __restrict__ Bp, double* __restrict__ Z,
-                                              double* __restrict__ G, double* __restrict__ Dl,
-                                              const int64_t* __restrict__ gOff) {
+namespace {
+
+constexpr int TB = 32;   // diagonal block of the triangular solves
+constexpr int GT = 64;   // row / column chunk of the GEMM-shaped parts
+
+// Front owning flat block b: start[f] <= b < start[f+1].
+__device__ inline int segmentOf(const int* start, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace
+
+// Forward step 1 (solve.hpp:327-347): sum the children's deltas into G
+// (pre-permutation positions, fixed child order) and form y0 = Bp - G on the
+// own rows.  One workgroup per front.  G and Dl are m x nrhs per-front work
+// blocks at gOff[s]; Dl rows [0, nF) hold y, rows [nF, m) the delta handed
+// to the parent.
+__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, DevFronts fd, int n, int nrhs,
+                                                     const double* __restrict__ Bp, double* __restrict__ G,
+                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
   if (static_cast<int>(blockIdx.x) >= count) return;
   const int s = list[blockIdx.x];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
   const int64_t io = fd.idxOff[s];
-  const double* F = fd.F[s];
   double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
   const int tid = threadIdx.x, NT = blockDim.x;
   const int64_t mn = static_cast<int64_t>(m) * nrhs;
-
   for (int64_t e = tid; e < mn; e += NT) Gs[e] = 0.0;
   __syncthreads();
   for (int ci = fd.childPtr[s]; ci < fd.childPtr[s + 1]; ++ci) {
@@ -72,68 +90,237 @@ __global__ __launch_bounds__(256) void k_fwd(const int* list, int count, DevFron
     }
     __syncthreads();
   }
-  if (nf > 0) {
-    const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int p = static_cast<int>(e % nf);
-      const int64_t col = e / nf;
-      Ds[p + col * m] = Bp[fd.rowIdx[io + p] + col * n] - Gs[fd.perm[io + p] + col * m];
+  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
+  for (int64_t e = tid; e < tot; e += NT) {
+    const int p = static_cast<int>(e % nf);
+    const int64_t col = e / nf;
+    Ds[p + col * m] = Bp[fd.rowIdx[io + p] + col * n] - Gs[fd.perm[io + p] + col * m];
+  }
+}
+
+// Forward step 2: y <- Ltop^{-1} y (unit lower; the (s+1, s) slot of a 2x2
+// block holds d21 and is skipped), then z = D^{-1} y into Z.  Blocked by TB
+// columns: wave 0 solves the diagonal block with each lane owning one
+// right-hand side, then the workgroup applies the block to the rows below.
+__global__ __launch_bounds__(256) void k_fwd_trsv(const int* list, int count, DevFronts fd, int n, int nrhs,
+                                                   double* __restrict__ Z, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff) {
+  __shared__ double Lb[TB][TB + 1];
+  __shared__ double yb[TB][65];
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int64_t io = fd.idxOff[s];
+  const double* F = fd.F[s];
+  double* Ds = Dl + gOff[s];
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+  for (int b0 = 0; b0 < nf; b0 += TB) {
+    const int bn = min(TB, nf - b0);
+    for (int e = tid; e < TB * TB; e += NT) {
+      const int i = e % TB, j = e / TB;
+      double v = 0.0;
+      if (i > j && i < bn && j < bn) {
+        v = F[(b0 + i) + static_cast<size_t>(b0 + j) * m];
+        if (i == j + 1 && fd.pkind[io + b0 + j] == 2) v = 0.0;
+      }
+      Lb[i][j] = v;
     }
     __syncthreads();
-    // unit-lower substitution with Ltop; the (j+1, j) slot of a 2x2 block
-    // holds d21, not a multiplier (solve.hpp:510-518)
-    for (int j = 0; j + 1 < nf; ++j) {
-      const bool skip = fd.pkind[io + j] == 2;
-      const int len = nf - j - 1;
-      const int64_t tot2 = static_cast<int64_t>(len) * nrhs;
-      const double* Lc = F + static_cast<size_t>(j) * m;
-      for (int64_t e = tid; e < tot2; e += NT) {
-        const int i = j + 1 + static_cast<int>(e % len);
-        const int64_t col = e / len;
-        if (skip && i == j + 1) continue;
-        Ds[i + col * m] = fma(-Lc[i], Ds[j + col * m], Ds[i + col * m]);
+    const bool straddle = (b0 + bn < nf) && fd.pkind[io + b0 + bn - 1] == 2;
+    for (int rg = 0; rg < nrhs; rg += 64) {
+      const int gw = min(64, nrhs - rg);
+      if (tid < 64) {
+        double y[TB];
+        const int c = rg + lane;
+#pragma unroll
+        for (int i = 0; i < TB; ++i) y[i] = (lane < gw && i < bn) ? Ds[(b0 + i) + static_cast<int64_t>(c) * m] : 0.0;
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          const double yj = y[j];
+#pragma unroll
+          for (int i = j + 1; i < TB; ++i) y[i] = fma(-Lb[i][j], yj, y[i]);
+        }
+        if (lane < gw) {
+#pragma unroll
+          for (int i = 0; i < TB; ++i)
+            if (i < bn) {
+              Ds[(b0 + i) + static_cast<int64_t>(c) * m] = y[i];
+              yb[i][lane] = y[i];
+            }
+        }
+      }
+      __syncthreads();
+      const int rows = nf - (b0 + bn);
+      const int64_t tot = static_cast<int64_t>(rows) * gw;
+      for (int64_t e = tid; e < tot; e += NT) {
+        const int i = b0 + bn + static_cast<int>(e % rows);
+        const int cl = static_cast<int>(e / rows);
+        double acc = 0.0;
+        for (int j = 0; j < bn; ++j) {
+          if (straddle && j == bn - 1 && i == b0 + bn) continue;
+          acc = fma(F[i + static_cast<size_t>(b0 + j) * m], yb[j][cl], acc);
+        }
+        double* d = Ds + i + static_cast<int64_t>(rg + cl) * m;
+        *d -= acc;
       }
       __syncthreads();
     }
   }
-  // delta for the rows handed to the parent
-  {
-    const int len = m - nf;
-    const int64_t tot = static_cast<int64_t>(len) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int i = nf + static_cast<int>(e % len);
-      const int64_t col = e / len;
-      double acc = Gs[fd.perm[io + i] + col * m];
-      for (int p = 0; p < nf; ++p) acc = fma(F[i + static_cast<size_t>(p) * m], Ds[p + col * m], acc);
-      Ds[i + col * m] = acc;
-    }
-  }
   // block-diagonal solve (solve.hpp:434-455)
-  {
-    const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int p = static_cast<int>(e % nf);
-      const int64_t col = e / nf;
-      const int kind = fd.pkind[io + p];
-      if (kind == 1) {
-        Z[fd.rowIdx[io + p] + col * n] = Ds[p + col * m] / fd.dDiag[io + p];
-      } else if (kind == 2) {
-        const double d11 = fd.dDiag[io + p], d21 = fd.dOff[io + p], d22 = fd.dDiag[io + p + 1];
-        const double det = d11 * d22 - d21 * d21;
-        const double y0 = Ds[p + col * m], y1 = Ds[p + 1 + col * m];
-        Z[fd.rowIdx[io + p] + col * n] = (d22 * y0 - d21 * y1) / det;
-        Z[fd.rowIdx[io + p + 1] + col * n] = (-d21 * y0 + d11 * y1) / det;
-      }
+  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
+  for (int64_t e = tid; e < tot; e += NT) {
+    const int p = static_cast<int>(e % nf);
+    const int64_t col = e / nf;
+    const int kind = fd.pkind[io + p];
+    if (kind == 1) {
+      Z[fd.rowIdx[io + p] + col * n] = Ds[p + col * m] / fd.dDiag[io + p];
+    } else if (kind == 2) {
+      const double d11 = fd.dDiag[io + p], d21 = fd.dOff[io + p], d22 = fd.dDiag[io + p + 1];
+      const double det = d11 * d22 - d21 * d21;
+      const double y0 = Ds[p + col * m], y1 = Ds[p + 1 + col * m];
+      Z[fd.rowIdx[io + p] + col * n] = (d22 * y0 - d21 * y1) / det;
+      Z[fd.rowIdx[io + p + 1] + col * n] = (-d21 * y0 + d11 * y1) / det;
     }
   }
 }
 
-// Backward solve of one tree level (solve.hpp:483-508); one workgroup per
-// front.  Dl is reused as scratch: rows [nF, m) gather the ancestors' X,
-// rows [0, nF) hold x_own.
-__global__ __launch_bounds__(256) void k_bwd(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                              const double* __restrict__ Z, double* __restrict__ X,
-                                              double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
+// Forward step 3 (solve.hpp:349-361): delta(i) = G(perm(i)) + L(i, 0:nF) y for
+// the rows i in [nF, m).  Flat block b covers rows [b*GT, b*GT+GT) of its
+// front; blocks below nF exit.  L and y tiles are staged in LDS.
+__global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                    int nrhs, const double* __restrict__ G, double* __restrict__ Dl,
+                                                    const int64_t* __restrict__ gOff) {
+  __shared__ double Ls[GT][GT + 1];  // [row][p]
+  __shared__ double ys[GT][65];      // [p][rhs]
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int r0 = (blockIdx.x - blockStart[f]) * GT;
+  const int rs = max(r0, nf);
+  const int re = min(r0 + GT, m);
+  if (rs >= re) return;
+  const int64_t io = fd.idxOff[s];
+  const double* F = fd.F[s];
+  const double* Gs = G + gOff[s];
+  double* Ds = Dl + gOff[s];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int nr = re - rs;
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    double acc[16];
+    const int outs = nr * gw;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int p0 = 0; p0 < nf; p0 += GT) {
+      const int pn = min(GT, nf - p0);
+      for (int e = tid; e < GT * GT; e += NT) {
+        const int r = e % GT, q = e / GT;
+        Ls[r][q] = (r < nr && q < pn) ? F[(rs + r) + static_cast<size_t>(p0 + q) * m] : 0.0;
+      }
+      for (int e = tid; e < GT * 64; e += NT) {
+        const int q = e % GT, c = e / GT;
+        ys[q][c] = (q < pn && c < gw) ? Ds[(p0 + q) + static_cast<int64_t>(rg + c) * m] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int o = tid + k * NT;
+        if (o < outs) {
+          const int r = o % nr, c = o / nr;
+          double a = acc[k];
+          for (int q = 0; q < pn; ++q) a = fma(Ls[r][q], ys[q][c], a);
+          acc[k] = a;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int o = tid + k * NT;
+      if (o < outs) {
+        const int r = o % nr, c = o / nr;
+        const int i = rs + r;
+        Ds[i + static_cast<int64_t>(rg + c) * m] = Gs[fd.perm[io + i] + static_cast<int64_t>(rg + c) * m] + acc[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Backward step 1 (solve.hpp:371-378): x_own(p) = z(p) - sum_{i >= nF}
+// L(i, p) x(row i).  Flat block b covers own columns [b*GT, b*GT+GT); the
+// ancestors' x rows are gathered tile by tile into LDS.  Dl rows [0, nF) of
+// the front receive x_own.
+__global__ __launch_bounds__(256) void k_bwd_gemv(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                   int n, int nrhs, const double* __restrict__ Z,
+                                                   const double* __restrict__ X, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff) {
+  __shared__ double Ls[GT][GT + 1];  // [i][p]
+  __shared__ double xs[GT][65];      // [i][rhs]
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int p0 = (blockIdx.x - blockStart[f]) * GT;
+  if (p0 >= nf) return;
+  const int pn = min(GT, nf - p0);
+  const int64_t io = fd.idxOff[s];
+  const double* F = fd.F[s];
+  double* Ds = Dl + gOff[s];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    const int outs = pn * gw;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int i0 = nf; i0 < m; i0 += GT) {
+      const int in = min(GT, m - i0);
+      for (int e = tid; e < GT * GT; e += NT) {
+        const int i = e % GT, q = e / GT;
+        Ls[i][q] = (i < in && q < pn) ? F[(i0 + i) + static_cast<size_t>(p0 + q) * m] : 0.0;
+      }
+      for (int e = tid; e < GT * 64; e += NT) {
+        const int i = e % GT, c = e / GT;
+        xs[i][c] = (i < in && c < gw) ? X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int o = tid + k * NT;
+        if (o < outs) {
+          const int q = o % pn, c = o / pn;
+          double a = acc[k];
+          for (int i = 0; i < in; ++i) a = fma(Ls[i][q], xs[i][c], a);
+          acc[k] = a;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int o = tid + k * NT;
+      if (o < outs) {
+        const int q = o % pn, c = o / pn;
+        const int p = p0 + q;
+        Ds[p + static_cast<int64_t>(rg + c) * m] = Z[fd.rowIdx[io + p] + static_cast<int64_t>(rg + c) * n] - acc[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Backward step 2 (solve.hpp:380-382): x_own <- Ltop^{-T} x_own, blocked by TB
+// from the bottom (left-looking: each block first subtracts the finished rows
+// below it, then wave 0 solves the block), and x_own is written to X.
+__global__ __launch_bounds__(256) void k_bwd_trsv(const int* list, int count, DevFronts fd, int n, int nrhs,
+                                                   double* __restrict__ X, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff) {
+  __shared__ double Lb[TB][TB + 1];
   if (static_cast<int>(blockIdx.x) >= count) return;
   const int s = list[blockIdx.x];
   const int m = fd.m[s];
@@ -142,47 +329,59 @@ __global__ __launch_bounds__(256) void k_bwd(const int* list, int count, DevFron
   const int64_t io = fd.idxOff[s];
   const double* F = fd.F[s];
   double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x;
-  {
-    const int len = m - nf;
-    const int64_t tot = static_cast<int64_t>(len) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int i = nf + static_cast<int>(e % len);
-      const int64_t col = e / len;
-      Ds[i + col * m] = X[fd.rowIdx[io + i] + col * n];
-    }
-  }
-  __syncthreads();
-  {
-    const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int p = static_cast<int>(e % nf);
-      const int64_t col = e / nf;
-      const double* Lc = F + static_cast<size_t>(p) * m;
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+  for (int b1 = nf; b1 > 0; b1 -= TB) {
+    const int b0 = max(0, b1 - TB);
+    const int bn = b1 - b0;
+    // rows j >= b1 are final: x_i -= sum_j L(j, i) x_j for i in [b0, b1)
+    const bool straddle = (b1 < nf) && fd.pkind[io + b1 - 1] == 2;
+    const int64_t outs = static_cast<int64_t>(bn) * nrhs;
+    for (int64_t o = tid; o < outs; o += NT) {
+      const int i = b0 + static_cast<int>(o % bn);
+      const int64_t c = o / bn;
+      const double* Lc = F + static_cast<size_t>(i) * m;
       double acc = 0.0;
-      for (int i = nf; i < m; ++i) acc = fma(Lc[i], Ds[i + col * m], acc);
-      Ds[p + col * m] = Z[fd.rowIdx[io + p] + col * n] - acc;
+      for (int j = b1; j < nf; ++j) {
+        if (straddle && i == b1 - 1 && j == b1) continue;
+        acc = fma(Lc[j], Ds[j + c * m], acc);
+      }
+      Ds[i + c * m] -= acc;
     }
-  }
-  __syncthreads();
-  // unit-upper substitution with Ltop^T
-  for (int j = nf - 1; j > 0; --j) {
-    const int64_t tot = static_cast<int64_t>(j) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int i = static_cast<int>(e % j);
-      const int64_t col = e / j;
-      if (j == i + 1 && fd.pkind[io + i] == 2) continue;
-      Ds[i + col * m] = fma(-F[j + static_cast<size_t>(i) * m], Ds[j + col * m], Ds[i + col * m]);
+    for (int e = tid; e < TB * TB; e += NT) {
+      const int i = e % TB, j = e / TB;  // Lb[i][j] = L(b0+i, b0+j), i > j
+      double v = 0.0;
+      if (i > j && i < bn && j < bn) {
+        v = F[(b0 + i) + static_cast<size_t>(b0 + j) * m];
+        if (i == j + 1 && fd.pkind[io + b0 + j] == 2) v = 0.0;
+      }
+      Lb[i][j] = v;
+    }
+    __syncthreads();
+    for (int rg = 0; rg < nrhs; rg += 64) {
+      const int gw = min(64, nrhs - rg);
+      if (tid < 64 && lane < gw) {
+        double x[TB];
+        const int c = rg + lane;
+#pragma unroll
+        for (int i = 0; i < TB; ++i) x[i] = (i < bn) ? Ds[(b0 + i) + static_cast<int64_t>(c) * m] : 0.0;
+#pragma unroll
+        for (int j = TB - 1; j >= 0; --j) {
+          const double xj = x[j];
+#pragma unroll
+          for (int i = 0; i < j; ++i) x[i] = fma(-Lb[j][i], xj, x[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < TB; ++i)
+          if (i < bn) Ds[(b0 + i) + static_cast<int64_t>(c) * m] = x[i];
+      }
     }
     __syncthreads();
   }
-  {
-    const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int p = static_cast<int>(e % nf);
-      const int64_t col = e / nf;
-      X[fd.rowIdx[io + p] + col * n] = Ds[p + col * m];
-    }
+  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
+  for (int64_t e = tid; e < tot; e += NT) {
+    const int p = static_cast<int>(e % nf);
+    const int64_t col = e / nf;
+    X[fd.rowIdx[io + p] + col * n] = Ds[p + col * m];
   }
 }
 
@@ -267,15 +466,30 @@ void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t 
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st) {
   k_permute_out<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, X, Xo, ldx);
 }
-void launchFwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* Z,
-               double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
+void launchFwdGather(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* G,
+                     double* Dl, const int64_t* gOff, hipStream_t st) {
   if (count <= 0) return;
-  k_fwd<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Bp, Z, G, Dl, gOff);
+  k_fwd_gather<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Bp, G, Dl, gOff);
 }
-void launchBwd(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
-               double* Dl, const int64_t* gOff, hipStream_t st) {
+void launchFwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* Z, double* Dl,
+                   const int64_t* gOff, hipStream_t st) {
   if (count <= 0) return;
-  k_bwd<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X, Dl, gOff);
+  k_fwd_trsv<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Z, Dl, gOff);
+}
+void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    const double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  k_fwd_delta<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, G, Dl, gOff);
+}
+void launchBwdGemv(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                   int nrhs, const double* Z, const double* X, double* Dl, const int64_t* gOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  k_bwd_gemv<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff);
+}
+void launchBwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* X, double* Dl,
+                   const int64_t* gOff, hipStream_t st) {
+  if (count <= 0) return;
+  k_bwd_trsv<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, X, Dl, gOff);
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {
