@@ -79,9 +79,24 @@ def build_oracle(verbose: bool = False) -> Path:
     return ORACLE
 
 
+FACADE_TEST = ROOT / "tests" / "cpp" / "facade_test"
+
+
+def build_facade_test(verbose: bool = False) -> Path:
+    """C++ program exercising include/symla/solver.hpp (run by the GPU tests)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "facade_test.cpp"),
+           "-L", str(PRODUCT.parent), "-lsymla_hip", f"-Wl,-rpath,{PRODUCT.parent}", "-Wl,-rpath,$ORIGIN/../../ma57_amd",
+           "-o", str(FACADE_TEST)]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    return FACADE_TEST
+
+
 def build_all(verbose: bool = False) -> None:
     build_oracle(verbose)
     build_product(verbose)
+    build_facade_test(verbose)
 
 
 if __name__ == "__main__":

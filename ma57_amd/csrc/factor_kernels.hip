@@ -535,6 +535,9 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   __shared__ int pkd[kPanel + 2];
   __shared__ double redV[NT / 64 + 1];
   __shared__ int redI[NT / 64 + 1];
+  __shared__ double fastV[2][NT / 64];
+  __shared__ int fastI[2][NT / 64];
+  __shared__ double fastDiag[2], fastNext[2];
 
   const int tid = threadIdx.x;
   if (static_cast<int>(blockIdx.x) >= count) return;
@@ -655,57 +658,175 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     ++p;
   };
 
+  // Fast path: the eligible rows of the panel fit RMAX per thread; each
+  // thread keeps the current values of its rows in registers, the newest
+  // multiplier of the next pivot row is recomputed by every thread from
+  // broadcast values, and a pivot accepted at k by the first Bunch-Kaufman
+  // test (or any static pivot) costs one barrier.
+  constexpr int RMAX = 16;
+  const bool regPath = (nE - k0) <= RMAX * NT;
+  double lNext = 0.0;
+  bool haveNext = false;
+  int parity = 0;
+
   while (k < nbEnd) {
-    computeW(k);
-    currentColumn(k, curK);
+    double lv = 0.0;
+    int li = -1;
+    double akk;
+    double nextv = 0.0;
+    double cr[RMAX];
+    if (regPath) {
+#pragma unroll
+      for (int j = 0; j < RMAX; ++j) {
+        const int i = k0 + tid + j * NT;
+        cr[j] = (i >= k && i < nE) ? Fat(i, k) : 0.0;
+      }
+      for (int q = 0; q < p;) {
+        if (pkd[q] == 1) {
+          const double lk = (q == p - 1 && haveNext) ? lNext : Fat(k, k0 + q);
+          const double w = pdd[q] * lk;
+          const double* Lq = F + static_cast<size_t>(k0 + q) * m;
+#pragma unroll
+          for (int j = 0; j < RMAX; ++j) {
+            const int i = k0 + tid + j * NT;
+            if (i >= k && i < nE) cr[j] = fma(-Lq[i], w, cr[j]);
+          }
+          ++q;
+        } else {
+          const double l0 = Fat(k, k0 + q);
+          const double l1 = (q + 1 == p - 1 && haveNext) ? lNext : Fat(k, k0 + q + 1);
+          const double w0 = pdd[q] * l0 + pdo[q] * l1;
+          const double w1 = pdo[q] * l0 + pdd[q + 1] * l1;
+          const double* L0 = F + static_cast<size_t>(k0 + q) * m;
+          const double* L1 = L0 + m;
+#pragma unroll
+          for (int j = 0; j < RMAX; ++j) {
+            const int i = k0 + tid + j * NT;
+            if (i >= k && i < nE) cr[j] = cr[j] - (L0[i] * w0 + L1[i] * w1);
+          }
+          q += 2;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RMAX; ++j) {
+        const int i = k0 + tid + j * NT;
+        if (i > k && i < nE) {
+          const double v = fabs(cr[j]);
+          if (v > 0.0) argmaxCombine(lv, li, v, i);
+        }
+        if (i == k) fastDiag[parity] = cr[j];
+        if (i == k + 1) fastNext[parity] = cr[j];
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(lv, off, 64);
+        const int i2 = __shfl_xor(li, off, 64);
+        argmaxCombine(lv, li, v2, i2);
+      }
+      if ((tid & 63) == 0) {
+        fastV[parity][tid >> 6] = lv;
+        fastI[parity][tid >> 6] = li;
+      }
+      __syncthreads();
+      lv = 0.0;
+      li = -1;
+      for (int w = 0; w < NT / 64; ++w) argmaxCombine(lv, li, fastV[parity][w], fastI[parity][w]);
+      akk = fastDiag[parity];
+      nextv = (k + 1 < nE) ? fastNext[parity] : 0.0;
+      parity ^= 1;
+    } else {
+      computeW(k);
+      currentColumn(k, curK);
+      if (!stat) {
+        for (int i = k + 1 + tid; i < nE; i += NT) {
+          const double v = fabs(curK[i]);
+          if (v > 0.0) argmaxCombine(lv, li, v, i);
+        }
+        blockArgmax<NT>(lv, li, redV, redI);
+      }
+      akk = curK[k];
+    }
+
+    // first Bunch-Kaufman tests (dense_kernel.hpp:601-618); static pivots
+    // are always accepted at k (dense_kernel.hpp:982-1011)
+    int action = -1;  // -1: sigma test needed
+    double dpiv = akk;
     if (stat) {
       const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k]] : 0;
       double applied;
-      const double d = staticPivot(curK[k], sign, delta, sfloor, applied);
+      dpiv = staticPivot(akk, sign, delta, sfloor, applied);
       if (applied != 0.0) {
         pert += fabs(applied);
         ++npert;
       }
-      accept1(curK, d);
-      continue;
-    }
-    double lv = 0.0;
-    int li = -1;
-    for (int i = k + 1 + tid; i < nE; i += NT) {
-      const double v = fabs(curK[i]);
-      if (v > 0.0) argmaxCombine(lv, li, v, i);
-    }
-    blockArgmax<NT>(lv, li, redV, redI);
-    const double akk = curK[k];
-    const int r = li;
-    int action;
-    if (lv <= prm.zeroTol) {
+      action = 1;
+    } else if (lv <= prm.zeroTol) {
       const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
       action = (fabs(akk) <= isolatedFloor) ? 0 : 1;
     } else if (fabs(akk) >= kAlpha * lv) {
       action = 1;
-    } else {
-      computeW(r);
-      currentColumn(r, curR);
-      // the reference mirrors column r into row r, so column k's entry at
-      // row r becomes column r's value at row k (dense_kernel.hpp:543)
-      if (tid == 0) curK[r] = curR[k];
-      __syncthreads();
-      double sg = 0.0;
-      for (int i = k + tid; i < nE; i += NT)
-        if (i != r) sg = fmax(sg, fabs(curR[i]));
-      sg = blockMax<NT>(sg, redV);
-      if (fabs(akk) * sg >= kAlpha * lv * lv)
-        action = 1;
-      else if (fabs(curR[r]) >= kAlpha * sg)
-        action = 2;
-      else
-        action = 3;
     }
     if (action == 0) {
       stop = true;
       break;
     }
+    if (action == 1 && regPath) {
+      const double d = dpiv;
+      pdd[p] = d;  // every thread stores the same value
+      pdo[p] = 0.0;
+      pkd[p] = 1;
+      if (tid == 0) {
+        fd.dDiag[io + k] = d;
+        fd.dOff[io + k] = 0.0;
+        fd.pkind[io + k] = 1;
+        Fat(k, k) = d;
+      }
+      countInertia(d, npos, nneg, nzero);
+      tmax = fmax(tmax, fabs(d));
+#pragma unroll
+      for (int j = 0; j < RMAX; ++j) {
+        const int i = k0 + tid + j * NT;
+        if (i > k && i < nE) {
+          tmax = fmax(tmax, fabs(cr[j]));
+          Fat(i, k) = cr[j] / d;
+        }
+      }
+      lNext = nextv / d;
+      haveNext = true;
+      ++k;
+      ++p;
+      continue;
+    }
+    if (regPath) {
+      // general path below works on the column buffers
+#pragma unroll
+      for (int j = 0; j < RMAX; ++j) {
+        const int i = k0 + tid + j * NT;
+        if (i >= k && i < nE) curK[i] = cr[j];
+      }
+      __syncthreads();
+      haveNext = false;
+    }
+    if (action == 1) {
+      accept1(curK, dpiv);
+      continue;
+    }
+    const int r = li;
+    computeW(r);
+    currentColumn(r, curR);
+    // the reference mirrors column r into row r, so column k's entry at
+    // row r becomes column r's value at row k (dense_kernel.hpp:543)
+    if (tid == 0) curK[r] = curR[k];
+    __syncthreads();
+    double sg = 0.0;
+    for (int i = k + tid; i < nE; i += NT)
+      if (i != r) sg = fmax(sg, fabs(curR[i]));
+    sg = blockMax<NT>(sg, redV);
+    if (fabs(akk) * sg >= kAlpha * lv * lv)
+      action = 1;
+    else if (fabs(curR[r]) >= kAlpha * sg)
+      action = 2;
+    else
+      action = 3;
     if (action == 1) {
       accept1(curK, curK[k]);
       continue;
@@ -756,6 +877,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     k += 2;
     p += 2;
   }
+  __syncthreads();
 
   const double bm = blockMax<NT>(tmax, redV);
   if (tid == 0) {
