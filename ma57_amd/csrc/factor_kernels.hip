@@ -527,9 +527,19 @@ __global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count
 // correction of this panel's pivots, like the reference's setColumnRaw
 // (dense_kernel.hpp:516-545).  Accepted pivots write their multipliers for
 // the eligible rows; the remaining rows are solved by k_panel_rows.
+//
+// Wrow[r][q] = (D L^T)(k0+q, k0+r) holds the correction weights of the
+// panel's first rows, so the weights of the next pivot row are an LDS
+// broadcast.  When every eligible row fits RM slots per thread (RM > 0),
+// each thread keeps its rows of the current column in registers and a
+// pivot accepted by the first Bunch-Kaufman test (every static pivot) costs
+// one barrier; any other decision falls back to the general code, which
+// works on column buffers in global memory.
+template <int RM>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int count, DevFronts fd, FactorParams prm,
                                                           double* wsBase, const int64_t* wsOff) {
   constexpr int NT = kPanelThreads;
+  __shared__ double Wrow[kPanel][kPanel + 1];
   __shared__ double wvec[kPanel + 2];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
@@ -575,7 +585,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   int k = k0, p = 0;
   bool stop = false;
 
-  // wvec[q] = (D L^T)(k0+q, row) for this panel's p pivots
+  // weights of this panel's p pivots for any row (general path)
   auto computeW = [&](int row) {
     for (int q = tid; q < p; q += NT) {
       const int kind = pkd[q];
@@ -588,6 +598,24 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
         w = pdo[q - 1] * Fat(row, k0 + q - 1) + pdd[q] * Fat(row, k0 + q);
       }
       wvec[q] = w;
+    }
+    __syncthreads();
+  };
+  // Wrow from the stored multipliers (after a general-path step)
+  auto rebuildWrow = [&]() {
+    for (int e = tid; e < kPanel * p; e += NT) {
+      const int r = e % kPanel, q = e / kPanel;
+      const int row = k0 + r;
+      if (row >= nE) continue;
+      const int kind = pkd[q];
+      double w;
+      if (kind == 1)
+        w = pdd[q] * Fat(row, k0 + q);
+      else if (kind == 2)
+        w = pdd[q] * Fat(row, k0 + q) + pdo[q] * Fat(row, k0 + q + 1);
+      else
+        w = pdo[q - 1] * Fat(row, k0 + q - 1) + pdd[q] * Fat(row, k0 + q);
+      Wrow[r][q] = w;
     }
     __syncthreads();
   };
@@ -658,14 +686,9 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     ++p;
   };
 
-  // Fast path: the eligible rows of the panel fit RMAX per thread; each
-  // thread keeps the current values of its rows in registers, the newest
-  // multiplier of the next pivot row is recomputed by every thread from
-  // broadcast values, and a pivot accepted at k by the first Bunch-Kaufman
-  // test (or any static pivot) costs one barrier.
-  constexpr int RMAX = 16;
-  const bool regPath = (nE - k0) <= RMAX * NT;
-  double lNext = 0.0;
+  // weight of the newest pivot for the next pivot row, known to every
+  // thread without a barrier (the row's current value in the pivot column)
+  double wNext = 0.0;
   bool haveNext = false;
   int parity = 0;
 
@@ -674,41 +697,51 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     int li = -1;
     double akk;
     double nextv = 0.0;
-    double cr[RMAX];
-    if (regPath) {
+    double cr[RM > 0 ? RM : 1];
+    if (RM > 0) {
+      const int kr = k - k0;
 #pragma unroll
-      for (int j = 0; j < RMAX; ++j) {
+      for (int j = 0; j < RM; ++j) {
         const int i = k0 + tid + j * NT;
         cr[j] = (i >= k && i < nE) ? Fat(i, k) : 0.0;
       }
-      for (int q = 0; q < p;) {
-        if (pkd[q] == 1) {
-          const double lk = (q == p - 1 && haveNext) ? lNext : Fat(k, k0 + q);
-          const double w = pdd[q] * lk;
-          const double* Lq = F + static_cast<size_t>(k0 + q) * m;
+      const int pq = haveNext ? p - 1 : p;
+      int q = 0;
+      for (; q + 4 <= pq; q += 4) {
+        const double w0 = Wrow[kr][q], w1 = Wrow[kr][q + 1], w2 = Wrow[kr][q + 2], w3 = Wrow[kr][q + 3];
+        const double* L0 = F + static_cast<size_t>(k0 + q) * m;
 #pragma unroll
-          for (int j = 0; j < RMAX; ++j) {
-            const int i = k0 + tid + j * NT;
-            if (i >= k && i < nE) cr[j] = fma(-Lq[i], w, cr[j]);
+        for (int j = 0; j < RM; ++j) {
+          const int i = k0 + tid + j * NT;
+          if (i >= k && i < nE) {
+            double c = cr[j];
+            c = fma(-L0[i], w0, c);
+            c = fma(-L0[i + m], w1, c);
+            c = fma(-L0[i + 2 * static_cast<size_t>(m)], w2, c);
+            c = fma(-L0[i + 3 * static_cast<size_t>(m)], w3, c);
+            cr[j] = c;
           }
-          ++q;
-        } else {
-          const double l0 = Fat(k, k0 + q);
-          const double l1 = (q + 1 == p - 1 && haveNext) ? lNext : Fat(k, k0 + q + 1);
-          const double w0 = pdd[q] * l0 + pdo[q] * l1;
-          const double w1 = pdo[q] * l0 + pdd[q + 1] * l1;
-          const double* L0 = F + static_cast<size_t>(k0 + q) * m;
-          const double* L1 = L0 + m;
+        }
+      }
+      for (; q < pq; ++q) {
+        const double w = Wrow[kr][q];
+        const double* Lq = F + static_cast<size_t>(k0 + q) * m;
 #pragma unroll
-          for (int j = 0; j < RMAX; ++j) {
-            const int i = k0 + tid + j * NT;
-            if (i >= k && i < nE) cr[j] = cr[j] - (L0[i] * w0 + L1[i] * w1);
-          }
-          q += 2;
+        for (int j = 0; j < RM; ++j) {
+          const int i = k0 + tid + j * NT;
+          if (i >= k && i < nE) cr[j] = fma(-Lq[i], w, cr[j]);
+        }
+      }
+      if (haveNext) {
+        const double* Lq = F + static_cast<size_t>(k0 + p - 1) * m;
+#pragma unroll
+        for (int j = 0; j < RM; ++j) {
+          const int i = k0 + tid + j * NT;
+          if (i >= k && i < nE) cr[j] = fma(-Lq[i], wNext, cr[j]);
         }
       }
 #pragma unroll
-      for (int j = 0; j < RMAX; ++j) {
+      for (int j = 0; j < RM; ++j) {
         const int i = k0 + tid + j * NT;
         if (i > k && i < nE) {
           const double v = fabs(cr[j]);
@@ -769,7 +802,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       stop = true;
       break;
     }
-    if (action == 1 && regPath) {
+    if (action == 1 && RM > 0) {
       const double d = dpiv;
       pdd[p] = d;  // every thread stores the same value
       pdo[p] = 0.0;
@@ -782,24 +815,28 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       }
       countInertia(d, npos, nneg, nzero);
       tmax = fmax(tmax, fabs(d));
+      const int kq = k - k0;
 #pragma unroll
-      for (int j = 0; j < RMAX; ++j) {
+      for (int j = 0; j < RM; ++j) {
         const int i = k0 + tid + j * NT;
         if (i > k && i < nE) {
-          tmax = fmax(tmax, fabs(cr[j]));
-          Fat(i, k) = cr[j] / d;
+          const double c = cr[j];
+          tmax = fmax(tmax, fabs(c));
+          Fat(i, k) = c / d;
+          // (D L^T)(k, i) = d * l_ik = c
+          if (i - k0 < kPanel) Wrow[i - k0][kq] = c;
         }
       }
-      lNext = nextv / d;
+      wNext = nextv;
       haveNext = true;
       ++k;
       ++p;
       continue;
     }
-    if (regPath) {
+    if (RM > 0) {
       // general path below works on the column buffers
 #pragma unroll
-      for (int j = 0; j < RMAX; ++j) {
+      for (int j = 0; j < RM; ++j) {
         const int i = k0 + tid + j * NT;
         if (i >= k && i < nE) curK[i] = cr[j];
       }
@@ -808,6 +845,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     }
     if (action == 1) {
       accept1(curK, dpiv);
+      if (RM > 0) rebuildWrow();
       continue;
     }
     const int r = li;
@@ -829,11 +867,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       action = 3;
     if (action == 1) {
       accept1(curK, curK[k]);
+      if (RM > 0) rebuildWrow();
       continue;
     }
     if (action == 2) {
       swapSym(k, r);
       accept1(curR, curR[k]);
+      if (RM > 0) rebuildWrow();
       continue;
     }
     // 2x2 at (k, r): bring r to k+1
@@ -876,6 +916,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     __syncthreads();
     k += 2;
     p += 2;
+    if (RM > 0) rebuildWrow();
   }
   __syncthreads();
 
@@ -1079,34 +1120,42 @@ __global__ void k_make_rowidx(const int* list, int count, DevFronts fd) {
 }
 
 // Sum of the per-front statistics (multifrontal.hpp:795-801); one block.
+// Each thread sums a contiguous range of fronts in order and thread 0 adds
+// the partial sums in thread order, so the perturbation total does not
+// depend on timing.
 __global__ void k_reduce_stats(int numSN, DevFronts fd, long long* outI, double* outD) {
   __shared__ long long sI[5][256];
   __shared__ double sD[256];
   long long pos = 0, neg = 0, zero = 0, delayedFronts = 0, npert = 0;
   double pert = 0.0;
-  for (int s = threadIdx.x; s < numSN; s += blockDim.x) {
+  const int chunk = (numSN + blockDim.x - 1) / blockDim.x;
+  const int s0 = threadIdx.x * chunk;
+  const int s1 = min(numSN, s0 + chunk);
+  for (int s = s0; s < s1; ++s) {
     pos += fd.inertia[3 * s];
     neg += fd.inertia[3 * s + 1];
     zero += fd.inertia[3 * s + 2];
     delayedFronts += (fd.nF[s] < fd.nE[s]) ? 1 : 0;
     npert += fd.npert[s];
+    pert += fd.pert[s];
   }
   sI[0][threadIdx.x] = pos;
   sI[1][threadIdx.x] = neg;
   sI[2][threadIdx.x] = zero;
   sI[3][threadIdx.x] = delayedFronts;
   sI[4][threadIdx.x] = npert;
+  sD[threadIdx.x] = pert;
   __syncthreads();
   if (threadIdx.x == 0) {
     long long r[5] = {0, 0, 0, 0, 0};
-    for (int t = 0; t < static_cast<int>(blockDim.x); ++t)
+    double pt = 0.0;
+    for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
       for (int q = 0; q < 5; ++q) r[q] += sI[q][t];
+      pt += sD[t];
+    }
     for (int q = 0; q < 5; ++q) outI[q] = r[q];
-    // perturbation sum in front order (deterministic)
-    for (int s = 0; s < numSN; ++s) pert += fd.pert[s];
-    outD[0] = pert;
+    outD[0] = pt;
   }
-  (void)sD;
 }
 
 // ---------------------------------------------------------------------------
@@ -1145,10 +1194,23 @@ void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd
   if (bpf < 1) bpf = 1;
   k_front_maxabs<<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
 }
-void launchPanel(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
+void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                  const int64_t* wsOff, hipStream_t st) {
   if (count <= 0) return;
-  k_panel<<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  // register slots per thread for the eligible rows of the widest front
+  const int slots = (maxRows + kPanelThreads - 1) / kPanelThreads;
+  if (slots <= 1)
+    k_panel<1><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 2)
+    k_panel<2><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 4)
+    k_panel<4><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 8)
+    k_panel<8><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 16)
+    k_panel<16><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else
+    k_panel<0><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
 }
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st) {

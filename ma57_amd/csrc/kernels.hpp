@@ -17,7 +17,7 @@ void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& f
 void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
                        hipStream_t st);
 void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st);
-void launchPanel(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
+void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                  const int64_t* wsOff, hipStream_t st);
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st);
@@ -30,16 +30,15 @@ void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* 
 // solve_kernels.hip
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st);
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st);
-void launchFwdGather(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* G,
-                     double* Dl, const int64_t* gOff, hipStream_t st);
-void launchFwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* Z, double* Dl,
-                   const int64_t* gOff, hipStream_t st);
-void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                    const double* G, double* Dl, const int64_t* gOff, hipStream_t st);
-void launchBwdGemv(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                   int nrhs, const double* Z, const double* X, double* Dl, const int64_t* gOff, hipStream_t st);
-void launchBwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* X, double* Dl,
-                   const int64_t* gOff, hipStream_t st);
+void launchSolveBegin(unsigned* gen, hipStream_t st);
+void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st);
+void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                    int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st);
+void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                    int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st);
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);
@@ -47,5 +46,6 @@ void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long lo
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st);
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st);
+void launchZeroU32(unsigned* p, int64_t count, hipStream_t st);
 
 }  // namespace symla_hip

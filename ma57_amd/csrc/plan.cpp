@@ -465,6 +465,7 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
       h.lists.push_back(s);
       h.rowStart.push_back(rb);
       h.tileStart.push_back(tl);
+      li.maxNE = std::max(li.maxNE, nE[s]);
       rb += (m[s] - nE[s] + 255) / 256;
       const int T = (m[s] - it * kPanel + 63) / 64;
       tl += T * (T + 1) / 2;
@@ -487,7 +488,7 @@ void runLargeSched(const LargeSched& ls, const int* dLists, const int* dRowStart
   for (const auto& it : ls.iters) {
     const int* lst = dLists + it.listOff;
     if (prof) prof->profBegin(Plan::kProfPanel);
-    launchPanel(lst, it.count, fd, prm, ws, wsOff, st);
+    launchPanel(lst, it.count, it.maxNE, fd, prm, ws, wsOff, st);
     if (prof) prof->profEnd(Plan::kProfPanel);
     if (it.rowBlocks > 0) {
       if (prof) prof->profBegin(Plan::kProfPanelRows);
@@ -823,48 +824,94 @@ void Plan::ensureSolveWs(int nrhs) {
     dgOff_.upload(off, stream_);
     dG_.ensure(static_cast<size_t>(off[numSN_]));
     dDl_.ensure(static_cast<size_t>(off[numSN_]));
-    // per level: 64-row chunks of every front (the GEMM-shaped solve parts)
-    std::vector<int> bs;
+    // Workgroups per front: the gather splits the right-hand sides over up
+    // to one workgroup per 64 rows; the sweeps take one workgroup per 32-row
+    // block (an upper bound: the own/ancestor split is known on the device).
     const int nl = static_cast<int>(levelFronts_.size());
-    solveBlockOff_.assign(nl, 0);
-    solveBlocks_.assign(nl, 0);
+    std::vector<int> gs, fs, bs;
+    gatherOff_.assign(nl, 0);
+    fwdOff_.assign(nl, 0);
+    bwdOff_.assign(nl, 0);
+    gatherBlocks_.assign(nl, 0);
+    fwdBlocks_.assign(nl, 0);
+    bwdBlocks_.assign(nl, 0);
     for (int L = 0; L < nl; ++L) {
-      solveBlockOff_[L] = static_cast<int64_t>(bs.size());
-      int acc = 0;
+      gatherOff_[L] = static_cast<int64_t>(gs.size());
+      fwdOff_[L] = static_cast<int64_t>(fs.size());
+      bwdOff_[L] = static_cast<int64_t>(bs.size());
+      int ga = 0, fa = 0, ba = 0;
       for (int s : levelFronts_[L]) {
-        bs.push_back(acc);
-        acc += (hm_[s] + 63) / 64;
+        gs.push_back(ga);
+        fs.push_back(fa);
+        bs.push_back(ba);
+        const int mm = hm_[s];
+        ga += std::max(1, std::min(nrhs, (mm + 63) / 64));
+        fa += (mm + 31) / 32 + 1;
+        ba += (mm + 31) / 32;
       }
-      bs.push_back(acc);
-      solveBlocks_[L] = acc;
+      gs.push_back(ga);
+      fs.push_back(fa);
+      bs.push_back(ba);
+      gatherBlocks_[L] = ga;
+      fwdBlocks_[L] = fa;
+      bwdBlocks_[L] = ba;
     }
-    dsolveBlockStart_.upload(bs, stream_);
+    dgatherStart_.upload(gs, stream_);
+    dfwdStart_.upload(fs, stream_);
+    dbwdStart_.upload(bs, stream_);
+    std::vector<int64_t> fo(numSN_ + 1, 0);
+    for (int s = 0; s < numSN_; ++s) fo[s + 1] = fo[s] + (hm_[s] + 31) / 32;
+    dflagOff_.upload(fo, stream_);
+    const size_t nflags = static_cast<size_t>(std::max<int64_t>(fo[numSN_], 1));
+    dflagsF_.alloc(nflags);
+    dflagsB_.alloc(nflags);
+    launchZeroU32(dflagsF_.p, static_cast<int64_t>(nflags), stream_);
+    launchZeroU32(dflagsB_.p, static_cast<int64_t>(nflags), stream_);
+    if (dgen_.n == 0) {
+      dgen_.alloc(1);
+      dsolveErr_.alloc(1);
+    }
+    launchZeroU32(dgen_.p, 1, stream_);
+    launchZeroU32(reinterpret_cast<unsigned*>(dsolveErr_.p), 1, stream_);
+    // the uploads above read host vectors local to this block
+    hipCheck(hipStreamSynchronize(stream_), "solve workspace");
     gOffNrhs_ = nrhs;
+  }
+}
+
+void Plan::checkSolveError() {
+  int err = 0;
+  hipCheck(hipMemcpyAsync(&err, dsolveErr_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "solve status");
+  hipCheck(hipStreamSynchronize(stream_), "solve status");
+  if (err != 0) {
+    launchZeroU32(reinterpret_cast<unsigned*>(dsolveErr_.p), 1, stream_);
+    hipCheck(hipStreamSynchronize(stream_), "solve status");
+    throw DeviceError("symla: a solve sweep exceeded its bounded dependency wait");
   }
 }
 
 void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   ensureSolveWs(nrhs);
   DevFronts fd = devFronts();
+  launchSolveBegin(dgen_.p, stream_);
   launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, dBp_.p, stream_);
   const int nl = static_cast<int>(levelFronts_.size());
   for (int L = 0; L < nl; ++L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
     const int* lst = dallFronts_.p + levelFrontOff_[L];
-    const int* bst = dsolveBlockStart_.p + solveBlockOff_[L];
     profBegin(kProfFwd);
-    launchFwdGather(lst, cnt, fd, n_, nrhs, dBp_.p, dG_.p, dDl_.p, dgOff_.p, stream_);
-    launchFwdTrsv(lst, cnt, fd, n_, nrhs, dZ_.p, dDl_.p, dgOff_.p, stream_);
-    launchFwdDelta(lst, cnt, bst, solveBlocks_[L], fd, nrhs, dG_.p, dDl_.p, dgOff_.p, stream_);
+    launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, dBp_.p, dG_.p,
+                    dDl_.p, dgOff_.p, stream_);
+    launchFwdSweep(lst, cnt, dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_, nrhs, dG_.p, dZ_.p, dDl_.p,
+                   dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
     profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
     const int* lst = dallFronts_.p + levelFrontOff_[L];
-    const int* bst = dsolveBlockStart_.p + solveBlockOff_[L];
     profBegin(kProfBwd);
-    launchBwdGemv(lst, cnt, bst, solveBlocks_[L], fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, stream_);
-    launchBwdTrsv(lst, cnt, fd, n_, nrhs, dX_.p, dDl_.p, dgOff_.p, stream_);
+    launchBwdSweep(lst, cnt, dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p,
+                   dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
     profEnd(kProfBwd);
   }
   launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
@@ -898,6 +945,7 @@ void Plan::solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx,
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0_, ev1_);
   solveMs_ = ms;
+  checkSolveError();
 }
 
 RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const double* B, int64_t ldb, int nrhs,
@@ -1000,6 +1048,7 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
                             deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_),
            "X out");
   hipCheck(hipStreamSynchronize(stream_), "refine");
+  checkSolveError();
   return res;
 }
 

@@ -85,6 +85,7 @@ struct RefineResult {
 struct LargeIter {
   int64_t listOff = 0;
   int count = 0;
+  int maxNE = 0;  // widest eligible-row count among the listed fronts
   int64_t rowOff = 0;
   int rowBlocks = 0;
   int64_t tileOff = 0;
@@ -200,9 +201,17 @@ class Plan {
   DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;
   int gOffNrhs_ = -1;
-  DBuf<int> dsolveBlockStart_;
-  std::vector<int64_t> solveBlockOff_;
-  std::vector<int> solveBlocks_;
+  // per level: flat workgroup starts of the gather, forward and backward
+  // sweep launches (offsets into the device arrays, and block totals)
+  DBuf<int> dgatherStart_, dfwdStart_, dbwdStart_;
+  std::vector<int64_t> gatherOff_, fwdOff_, bwdOff_;
+  std::vector<int> gatherBlocks_, fwdBlocks_, bwdBlocks_;
+  // dependency flags of the sweeps (one per own-row block), the solve
+  // generation they are compared with, and the bounded-wait error word
+  DBuf<int64_t> dflagOff_;
+  DBuf<unsigned> dflagsF_, dflagsB_, dgen_;
+  DBuf<int> dsolveErr_;
+  void checkSolveError();
   // refinement CSR
   DBuf<int64_t> dsymPtr_, dsymSrc_, dstPtr_, dstSrc_;
   DBuf<int> dsymCol_;

@@ -7,6 +7,10 @@
 // contributions (fixed child order, no atomics), solves its own rows, and
 // hands a private delta for its remaining rows to its parent.  The backward
 // pass is the reference's per-front step (solve.hpp:364-383) run top-down.
+// Within one tree level, the rows of a front are split into 32-row blocks,
+// each solved by its own workgroup; a block waits on a per-block flag for
+// the blocks it depends on, so a large front is solved by many workgroups in
+// one launch.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -36,8 +40,10 @@ __global__ void k_permute_out(int n, int nrhs, const int* __restrict__ perm, con
 
 namespace {
 
-constexpr int TB = 32;   // diagonal block of the triangular solves
-constexpr int GT = 64;   // row / column chunk of the GEMM-shaped parts
+constexpr int SB = 32;  // own rows per sweep block; also the column tile of the sweeps
+constexpr int DT = 64;  // ancestor-row tile of the backward sweep
+// Polls of a dependency flag before a sweep block gives up (about a second).
+constexpr unsigned kSpinLimit = 1u << 22;
 
 // Front owning flat block b: start[f] <= b < start[f+1].
 __device__ inline int segmentOf(const int* start, int count, int b) {
@@ -52,337 +58,362 @@ __device__ inline int segmentOf(const int* start, int count, int b) {
   return lo;
 }
 
+__device__ inline bool flagReached(const unsigned* f, unsigned gen) {
+  return static_cast<int>(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - gen) >= 0;
+}
+
+// Wait until another workgroup has published block f of this solve.  Thread 0
+// polls with a bounded count; past the bound it raises the error word and
+// stops waiting, so every workgroup still runs to its end.
+__device__ inline void waitBlock(const unsigned* f, unsigned gen, int* err) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (!flagReached(f, gen)) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Publish this workgroup's block: all of its stores happen before the flag.
+__device__ inline void publishBlock(unsigned* f, unsigned gen) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(f, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// L(row, col) of a front with the (s+1, s) slot of a 2x2 pivot block read as
+// zero: that slot stores d21, not a multiplier (multifrontal.hpp:618-628).
+__device__ inline double lval(const double* F, int m, const int* pk, int row, int col) {
+  const double v = F[row + static_cast<size_t>(col) * m];
+  return (row == col + 1 && pk[col] == 2) ? 0.0 : v;
+}
+
 }  // namespace
 
+// One increment per solve: the generation that the sweep flags are compared
+// with, kept on the device so that a recorded solve stays valid on replay.
+__global__ void k_solve_begin(unsigned* gen) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *gen = *gen + 1u;
+}
+
 // Forward step 1 (solve.hpp:327-347): sum the children's deltas into G
-// (pre-permutation positions, fixed child order) and form y0 = Bp - G on the
-// own rows.  One workgroup per front.  G and Dl are m x nrhs per-front work
+// (pre-permutation positions, fixed child order, no atomics) and form
+// y0 = Bp - G on the own rows.  Workgroup q of a front handles the
+// right-hand sides q, q + nq, ...  G and Dl are m x nrhs per-front work
 // blocks at gOff[s]; Dl rows [0, nF) hold y, rows [nF, m) the delta handed
 // to the parent.
-__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                                     const double* __restrict__ Bp, double* __restrict__ G,
-                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
-  if (static_cast<int>(blockIdx.x) >= count) return;
-  const int s = list[blockIdx.x];
+__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                     int n, int nrhs, const double* __restrict__ Bp,
+                                                     double* __restrict__ G, double* __restrict__ Dl,
+                                                     const int64_t* __restrict__ gOff) {
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int q = blockIdx.x - blockStart[f];
+  const int nq = blockStart[f + 1] - blockStart[f];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
   const int64_t io = fd.idxOff[s];
   double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
   const int tid = threadIdx.x, NT = blockDim.x;
-  const int64_t mn = static_cast<int64_t>(m) * nrhs;
-  for (int64_t e = tid; e < mn; e += NT) Gs[e] = 0.0;
+  for (int c = q; c < nrhs; c += nq)
+    for (int t = tid; t < m; t += NT) Gs[t + static_cast<int64_t>(c) * m] = 0.0;
   __syncthreads();
   for (int ci = fd.childPtr[s]; ci < fd.childPtr[s + 1]; ++ci) {
     const int c = fd.childList[ci];
     const int mc = fd.m[c];
     const int nfc = fd.nF[c];
-    const int len = mc - nfc;
-    if (len <= 0) continue;
+    if (mc - nfc <= 0) continue;
     const int* rel = fd.relmap + fd.idxOff[c];
     const double* Dc = Dl + gOff[c];
-    const int64_t tot = static_cast<int64_t>(len) * nrhs;
-    for (int64_t e = tid; e < tot; e += NT) {
-      const int a = nfc + static_cast<int>(e % len);
-      const int64_t col = e / len;
-      const int t = rel[a];
-      if (t >= 0) Gs[t + col * m] += Dc[a + col * mc];
-    }
+    for (int col = q; col < nrhs; col += nq)
+      for (int a = nfc + tid; a < mc; a += NT) {
+        const int t = rel[a];
+        if (t >= 0) Gs[t + static_cast<int64_t>(col) * m] += Dc[a + static_cast<int64_t>(col) * mc];
+      }
     __syncthreads();
   }
-  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-  for (int64_t e = tid; e < tot; e += NT) {
-    const int p = static_cast<int>(e % nf);
-    const int64_t col = e / nf;
-    Ds[p + col * m] = Bp[fd.rowIdx[io + p] + col * n] - Gs[fd.perm[io + p] + col * m];
-  }
+  for (int col = q; col < nrhs; col += nq)
+    for (int p = tid; p < nf; p += NT)
+      Ds[p + static_cast<int64_t>(col) * m] =
+          Bp[fd.rowIdx[io + p] + static_cast<int64_t>(col) * n] - Gs[fd.perm[io + p] + static_cast<int64_t>(col) * m];
 }
 
-// Forward step 2: y <- Ltop^{-1} y (unit lower; the (s+1, s) slot of a 2x2
-// block holds d21 and is skipped), then z = D^{-1} y into Z.  Blocked by TB
-// columns: wave 0 solves the diagonal block with each lane owning one
-// right-hand side, then the workgroup applies the block to the rows below.
-__global__ __launch_bounds__(256) void k_fwd_trsv(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                                   double* __restrict__ Z, double* __restrict__ Dl,
-                                                   const int64_t* __restrict__ gOff) {
-  __shared__ double Lb[TB][TB + 1];
-  __shared__ double yb[TB][65];
-  if (static_cast<int>(blockIdx.x) >= count) return;
-  const int s = list[blockIdx.x];
-  const int m = fd.m[s];
-  const int nf = fd.nF[s];
-  const int64_t io = fd.idxOff[s];
-  const double* F = fd.F[s];
-  double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
-  for (int b0 = 0; b0 < nf; b0 += TB) {
-    const int bn = min(TB, nf - b0);
-    for (int e = tid; e < TB * TB; e += NT) {
-      const int i = e % TB, j = e / TB;
-      double v = 0.0;
-      if (i > j && i < bn && j < bn) {
-        v = F[(b0 + i) + static_cast<size_t>(b0 + j) * m];
-        if (i == j + 1 && fd.pkind[io + b0 + j] == 2) v = 0.0;
-      }
-      Lb[i][j] = v;
-    }
-    __syncthreads();
-    const bool straddle = (b0 + bn < nf) && fd.pkind[io + b0 + bn - 1] == 2;
-    for (int rg = 0; rg < nrhs; rg += 64) {
-      const int gw = min(64, nrhs - rg);
-      if (tid < 64) {
-        double y[TB];
-        const int c = rg + lane;
-#pragma unroll
-        for (int i = 0; i < TB; ++i) y[i] = (lane < gw && i < bn) ? Ds[(b0 + i) + static_cast<int64_t>(c) * m] : 0.0;
-#pragma unroll
-        for (int j = 0; j < TB; ++j) {
-          const double yj = y[j];
-#pragma unroll
-          for (int i = j + 1; i < TB; ++i) y[i] = fma(-Lb[i][j], yj, y[i]);
-        }
-        if (lane < gw) {
-#pragma unroll
-          for (int i = 0; i < TB; ++i)
-            if (i < bn) {
-              Ds[(b0 + i) + static_cast<int64_t>(c) * m] = y[i];
-              yb[i][lane] = y[i];
-            }
-        }
-      }
-      __syncthreads();
-      const int rows = nf - (b0 + bn);
-      const int64_t tot = static_cast<int64_t>(rows) * gw;
-      for (int64_t e = tid; e < tot; e += NT) {
-        const int i = b0 + bn + static_cast<int>(e % rows);
-        const int cl = static_cast<int>(e / rows);
-        double acc = 0.0;
-        for (int j = 0; j < bn; ++j) {
-          if (straddle && j == bn - 1 && i == b0 + bn) continue;
-          acc = fma(F[i + static_cast<size_t>(b0 + j) * m], yb[j][cl], acc);
-        }
-        double* d = Ds + i + static_cast<int64_t>(rg + cl) * m;
-        *d -= acc;
-      }
-      __syncthreads();
-    }
-  }
-  // block-diagonal solve (solve.hpp:434-455)
-  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-  for (int64_t e = tid; e < tot; e += NT) {
-    const int p = static_cast<int>(e % nf);
-    const int64_t col = e / nf;
-    const int kind = fd.pkind[io + p];
-    if (kind == 1) {
-      Z[fd.rowIdx[io + p] + col * n] = Ds[p + col * m] / fd.dDiag[io + p];
-    } else if (kind == 2) {
-      const double d11 = fd.dDiag[io + p], d21 = fd.dOff[io + p], d22 = fd.dDiag[io + p + 1];
-      const double det = d11 * d22 - d21 * d21;
-      const double y0 = Ds[p + col * m], y1 = Ds[p + 1 + col * m];
-      Z[fd.rowIdx[io + p] + col * n] = (d22 * y0 - d21 * y1) / det;
-      Z[fd.rowIdx[io + p + 1] + col * n] = (-d21 * y0 + d11 * y1) / det;
-    }
-  }
-}
-
-// Forward step 3 (solve.hpp:349-361): delta(i) = G(perm(i)) + L(i, 0:nF) y for
-// the rows i in [nF, m).  Flat block b covers rows [b*GT, b*GT+GT) of its
-// front; blocks below nF exit.  L and y tiles are staged in LDS.
-__global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                    int nrhs, const double* __restrict__ G, double* __restrict__ Dl,
-                                                    const int64_t* __restrict__ gOff) {
-  __shared__ double Ls[GT][GT + 1];  // [row][p]
-  __shared__ double ys[GT][65];      // [p][rhs]
+// Forward sweep of one tree level (solve.hpp:349-361 and :434-455).  The
+// workgroups of a front take, in order, the own-row blocks b = 0 .. nOwn-1
+// (SB rows each) and then the blocks of the rows [nF, m).  Own block b
+// subtracts L(b, j) y_j for j < b as the blocks j are published, solves its
+// unit lower diagonal block, applies D^{-1} to its rows and publishes y_b.
+// A block of the rows [nF, m) accumulates L(i, :) y over all own blocks and
+// stores delta(i) = G(perm(i)) + L(i, :) y.  A workgroup only waits for
+// blocks of the same front with a lower index, which were dispatched
+// before it.
+__global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                    int n, int nrhs, const double* __restrict__ G,
+                                                    double* __restrict__ Z, double* __restrict__ Dl,
+                                                    const int64_t* __restrict__ gOff, unsigned* flags,
+                                                    const int64_t* __restrict__ flagOff, const unsigned* genp,
+                                                    int* err) {
+  __shared__ double Ls[SB][SB + 1];  // [row][col]
+  __shared__ double ys[SB][65];      // [row][rhs]
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf(blockStart, count, blockIdx.x);
   const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
-  const int r0 = (blockIdx.x - blockStart[f]) * GT;
-  const int rs = max(r0, nf);
-  const int re = min(r0 + GT, m);
-  if (rs >= re) return;
+  const int nOwn = (nf + SB - 1) / SB;
+  const int nDel = (m - nf + SB - 1) / SB;
+  if (b >= nOwn + nDel) return;
+  const unsigned gen = *genp;
+  const bool own = b < nOwn;
+  const int r0 = own ? b * SB : nf + (b - nOwn) * SB;
+  const int r1 = own ? min(r0 + SB, nf) : min(r0 + SB, m);
+  const int nr = r1 - r0;
+  const int jEnd = own ? b : nOwn;
   const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
   const double* F = fd.F[s];
   const double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x;
-  const int nr = re - rs;
+  unsigned* fl = flags + flagOff[s];
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    double acc[16];
     const int outs = nr * gw;
+    // outputs o = tid + kk * NT (row o % nr, column o / nr); slots past the
+    // end read row 0 / column 0 of the tiles and are never stored
+    double acc[8];
+    int orow[8], ocol[8];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    for (int p0 = 0; p0 < nf; p0 += GT) {
-      const int pn = min(GT, nf - p0);
-      for (int e = tid; e < GT * GT; e += NT) {
-        const int r = e % GT, q = e / GT;
-        Ls[r][q] = (r < nr && q < pn) ? F[(rs + r) + static_cast<size_t>(p0 + q) * m] : 0.0;
+    for (int kk = 0; kk < 8; ++kk) {
+      const int o = tid + kk * NT;
+      acc[kk] = 0.0;
+      orow[kk] = (o < outs) ? o % nr : 0;
+      ocol[kk] = (o < outs) ? o / nr : 0;
+    }
+    for (int j = 0; j < jEnd; ++j) {
+      if (rg == 0) waitBlock(fl + j, gen, err);
+      const int c0 = j * SB, cn = min(SB, nf - c0);
+      for (int e = tid; e < SB * SB; e += NT) {
+        const int r = e % SB, qq = e / SB;
+        Ls[r][qq] = (r < nr && qq < cn) ? lval(F, m, pk, r0 + r, c0 + qq) : 0.0;
       }
-      for (int e = tid; e < GT * 64; e += NT) {
-        const int q = e % GT, c = e / GT;
-        ys[q][c] = (q < pn && c < gw) ? Ds[(p0 + q) + static_cast<int64_t>(rg + c) * m] : 0.0;
+      for (int e = tid; e < SB * 64; e += NT) {
+        const int qq = e % SB, c = e / SB;
+        ys[qq][c] = (qq < cn && c < gw) ? Ds[(c0 + qq) + static_cast<int64_t>(rg + c) * m] : 0.0;
       }
       __syncthreads();
+      for (int qq = 0; qq < cn; ++qq) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int o = tid + k * NT;
-        if (o < outs) {
-          const int r = o % nr, c = o / nr;
-          double a = acc[k];
-          for (int q = 0; q < pn; ++q) a = fma(Ls[r][q], ys[q][c], a);
-          acc[k] = a;
-        }
+        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[orow[kk]][qq], ys[qq][ocol[kk]], acc[kk]);
       }
       __syncthreads();
     }
+    if (!own) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int o = tid + k * NT;
+      for (int kk = 0; kk < 8; ++kk) {
+        const int o = tid + kk * NT;
+        if (o < outs) {
+          const int r = o % nr, c = o / nr;
+          const int i = r0 + r;
+          const int64_t cc = rg + c;
+          Ds[i + cc * m] = Gs[fd.perm[io + i] + cc * m] + acc[kk];
+        }
+      }
+      continue;
+    }
+    // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j)
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int o = tid + kk * NT;
       if (o < outs) {
         const int r = o % nr, c = o / nr;
-        const int i = rs + r;
-        Ds[i + static_cast<int64_t>(rg + c) * m] = Gs[fd.perm[io + i] + static_cast<int64_t>(rg + c) * m] + acc[k];
+        ys[r][c] = Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[kk];
+      }
+    }
+    for (int e = tid; e < SB * SB; e += NT) {
+      const int r = e % SB, qq = e / SB;
+      Ls[r][qq] = (r > qq && r < nr) ? lval(F, m, pk, r0 + r, r0 + qq) : 0.0;
+    }
+    __syncthreads();
+    {
+      // lane row of each half-wave owns one row; the half-waves of the
+      // workgroup take the right-hand sides in turn
+      const int row = lane & 31, half = lane >> 5, wv = tid >> 6;
+      double lrow[SB];
+#pragma unroll
+      for (int j = 0; j < SB; ++j) lrow[j] = Ls[row][j];
+      for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
+        double y = (row < nr) ? ys[row][c] : 0.0;
+#pragma unroll
+        for (int j = 0; j < SB - 1; ++j) {
+          const double yj = __shfl(y, j + 32 * half, 64);
+          if (row > j) y = fma(-lrow[j], yj, y);
+        }
+        if (row < nr) {
+          ys[row][c] = y;
+          Ds[(r0 + row) + static_cast<int64_t>(rg + c) * m] = y;
+        }
+      }
+    }
+    __syncthreads();
+    // z = D^{-1} y for the pivot blocks that end in this block
+    for (int e = tid; e < outs; e += NT) {
+      const int r = e % nr, c = e / nr;
+      const int p = r0 + r;
+      const int64_t cc = rg + c;
+      const int kind = pk[p];
+      if (kind == 1) {
+        Z[fd.rowIdx[io + p] + cc * n] = ys[r][c] / fd.dDiag[io + p];
+      } else if (kind == 2 && r + 1 < nr) {
+        const double d11 = fd.dDiag[io + p], d21 = fd.dOff[io + p], d22 = fd.dDiag[io + p + 1];
+        const double det = d11 * d22 - d21 * d21;
+        const double y0 = ys[r][c], y1 = ys[r + 1][c];
+        Z[fd.rowIdx[io + p] + cc * n] = (d22 * y0 - d21 * y1) / det;
+        Z[fd.rowIdx[io + p + 1] + cc * n] = (-d21 * y0 + d11 * y1) / det;
+      } else if (kind == 0 && r == 0 && p > 0) {
+        // second row of a 2x2 block that starts in the previous block
+        const double d11 = fd.dDiag[io + p - 1], d21 = fd.dOff[io + p - 1], d22 = fd.dDiag[io + p];
+        const double det = d11 * d22 - d21 * d21;
+        const double y0 = Ds[(p - 1) + cc * m], y1 = ys[0][c];
+        Z[fd.rowIdx[io + p - 1] + cc * n] = (d22 * y0 - d21 * y1) / det;
+        Z[fd.rowIdx[io + p] + cc * n] = (-d21 * y0 + d11 * y1) / det;
       }
     }
     __syncthreads();
   }
+  if (own) publishBlock(fl + b, gen);
 }
 
-// Backward step 1 (solve.hpp:371-378): x_own(p) = z(p) - sum_{i >= nF}
-// L(i, p) x(row i).  Flat block b covers own columns [b*GT, b*GT+GT); the
-// ancestors' x rows are gathered tile by tile into LDS.  Dl rows [0, nF) of
-// the front receive x_own.
-__global__ __launch_bounds__(256) void k_bwd_gemv(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                   int n, int nrhs, const double* __restrict__ Z,
-                                                   const double* __restrict__ X, double* __restrict__ Dl,
-                                                   const int64_t* __restrict__ gOff) {
-  __shared__ double Ls[GT][GT + 1];  // [i][p]
-  __shared__ double xs[GT][65];      // [i][rhs]
+// Backward sweep of one tree level (solve.hpp:364-383).  The workgroups of a
+// front take the own-row blocks from the last to the first.  Block w forms
+// x_w = z_w - L(anc, w)^T x_anc - sum_{j > w} L(j, w)^T x_j (the ancestors'
+// x is final from the levels above; the blocks j of the same front are
+// waited for), solves its unit upper diagonal block and publishes x_w.
+__global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                    int n, int nrhs, const double* __restrict__ Z,
+                                                    double* __restrict__ X, double* __restrict__ Dl,
+                                                    const int64_t* __restrict__ gOff, unsigned* flags,
+                                                    const int64_t* __restrict__ flagOff, const unsigned* genp,
+                                                    int* err) {
+  __shared__ double Ls[DT][SB + 1];  // [row i][own column q]
+  __shared__ double xs[DT][65];      // [row i][rhs]
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf(blockStart, count, blockIdx.x);
   const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
-  const int p0 = (blockIdx.x - blockStart[f]) * GT;
-  if (p0 >= nf) return;
-  const int pn = min(GT, nf - p0);
+  const int nOwn = (nf + SB - 1) / SB;
+  if (b >= nOwn) return;
+  const unsigned gen = *genp;
+  const int w = nOwn - 1 - b;
+  const int r0 = w * SB;
+  const int nr = min(r0 + SB, nf) - r0;
   const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
   const double* F = fd.F[s];
   double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x;
+  unsigned* fl = flags + flagOff[s];
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    const int outs = pn * gw;
-    double acc[16];
+    const int outs = nr * gw;
+    // outputs o = tid + kk * NT (own column o % nr, right-hand side o / nr)
+    double acc[8];
+    int orow[8], ocol[8];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    for (int i0 = nf; i0 < m; i0 += GT) {
-      const int in = min(GT, m - i0);
-      for (int e = tid; e < GT * GT; e += NT) {
-        const int i = e % GT, q = e / GT;
-        Ls[i][q] = (i < in && q < pn) ? F[(i0 + i) + static_cast<size_t>(p0 + q) * m] : 0.0;
+    for (int kk = 0; kk < 8; ++kk) {
+      const int o = tid + kk * NT;
+      acc[kk] = 0.0;
+      orow[kk] = (o < outs) ? o % nr : 0;
+      ocol[kk] = (o < outs) ? o / nr : 0;
+    }
+    // rows of the ancestors
+    for (int i0 = nf; i0 < m; i0 += DT) {
+      const int in = min(DT, m - i0);
+      for (int e = tid; e < DT * SB; e += NT) {
+        const int i = e % DT, qq = e / DT;
+        Ls[i][qq] = (i < in && qq < nr) ? F[(i0 + i) + static_cast<size_t>(r0 + qq) * m] : 0.0;
       }
-      for (int e = tid; e < GT * 64; e += NT) {
-        const int i = e % GT, c = e / GT;
+      for (int e = tid; e < DT * 64; e += NT) {
+        const int i = e % DT, c = e / DT;
         xs[i][c] = (i < in && c < gw) ? X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n] : 0.0;
       }
       __syncthreads();
+      for (int i = 0; i < in; ++i) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int o = tid + k * NT;
-        if (o < outs) {
-          const int q = o % pn, c = o / pn;
-          double a = acc[k];
-          for (int i = 0; i < in; ++i) a = fma(Ls[i][q], xs[i][c], a);
-          acc[k] = a;
-        }
+        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[i][orow[kk]], xs[i][ocol[kk]], acc[kk]);
+      }
+      __syncthreads();
+    }
+    // own blocks below this one, as they are published
+    for (int j = nOwn - 1; j > w; --j) {
+      if (rg == 0) waitBlock(fl + j, gen, err);
+      const int i0 = j * SB, in = min(SB, nf - i0);
+      for (int e = tid; e < SB * SB; e += NT) {
+        const int i = e % SB, qq = e / SB;
+        Ls[i][qq] = (i < in && qq < nr) ? lval(F, m, pk, i0 + i, r0 + qq) : 0.0;
+      }
+      for (int e = tid; e < SB * 64; e += NT) {
+        const int i = e % SB, c = e / SB;
+        xs[i][c] = (i < in && c < gw) ? Ds[(i0 + i) + static_cast<int64_t>(rg + c) * m] : 0.0;
+      }
+      __syncthreads();
+      for (int i = 0; i < in; ++i) {
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[i][orow[kk]], xs[i][ocol[kk]], acc[kk]);
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int o = tid + k * NT;
+    for (int kk = 0; kk < 8; ++kk) {
+      const int o = tid + kk * NT;
       if (o < outs) {
-        const int q = o % pn, c = o / pn;
-        const int p = p0 + q;
-        Ds[p + static_cast<int64_t>(rg + c) * m] = Z[fd.rowIdx[io + p] + static_cast<int64_t>(rg + c) * n] - acc[k];
+        const int qq = o % nr, c = o / nr;
+        xs[qq][c] = Z[fd.rowIdx[io + r0 + qq] + static_cast<int64_t>(rg + c) * n] - acc[kk];
       }
+    }
+    for (int e = tid; e < SB * SB; e += NT) {
+      const int i = e % SB, j = e / SB;
+      Ls[i][j] = (i > j && i < nr) ? lval(F, m, pk, r0 + i, r0 + j) : 0.0;
     }
     __syncthreads();
-  }
-}
-
-// Backward step 2 (solve.hpp:380-382): x_own <- Ltop^{-T} x_own, blocked by TB
-// from the bottom (left-looking: each block first subtracts the finished rows
-// below it, then wave 0 solves the block), and x_own is written to X.
-__global__ __launch_bounds__(256) void k_bwd_trsv(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                                   double* __restrict__ X, double* __restrict__ Dl,
-                                                   const int64_t* __restrict__ gOff) {
-  __shared__ double Lb[TB][TB + 1];
-  if (static_cast<int>(blockIdx.x) >= count) return;
-  const int s = list[blockIdx.x];
-  const int m = fd.m[s];
-  const int nf = fd.nF[s];
-  if (nf == 0) return;
-  const int64_t io = fd.idxOff[s];
-  const double* F = fd.F[s];
-  double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
-  for (int b1 = nf; b1 > 0; b1 -= TB) {
-    const int b0 = max(0, b1 - TB);
-    const int bn = b1 - b0;
-    // rows j >= b1 are final: x_i -= sum_j L(j, i) x_j for i in [b0, b1)
-    const bool straddle = (b1 < nf) && fd.pkind[io + b1 - 1] == 2;
-    const int64_t outs = static_cast<int64_t>(bn) * nrhs;
-    for (int64_t o = tid; o < outs; o += NT) {
-      const int i = b0 + static_cast<int>(o % bn);
-      const int64_t c = o / bn;
-      const double* Lc = F + static_cast<size_t>(i) * m;
-      double acc = 0.0;
-      for (int j = b1; j < nf; ++j) {
-        if (straddle && i == b1 - 1 && j == b1) continue;
-        acc = fma(Lc[j], Ds[j + c * m], acc);
-      }
-      Ds[i + c * m] -= acc;
-    }
-    for (int e = tid; e < TB * TB; e += NT) {
-      const int i = e % TB, j = e / TB;  // Lb[i][j] = L(b0+i, b0+j), i > j
-      double v = 0.0;
-      if (i > j && i < bn && j < bn) {
-        v = F[(b0 + i) + static_cast<size_t>(b0 + j) * m];
-        if (i == j + 1 && fd.pkind[io + b0 + j] == 2) v = 0.0;
-      }
-      Lb[i][j] = v;
-    }
-    __syncthreads();
-    for (int rg = 0; rg < nrhs; rg += 64) {
-      const int gw = min(64, nrhs - rg);
-      if (tid < 64 && lane < gw) {
-        double x[TB];
-        const int c = rg + lane;
+    {
+      // lane row of each half-wave owns one row (and column `row` of the
+      // block); the half-waves of the workgroup take the right-hand sides
+      const int row = lane & 31, half = lane >> 5, wv = tid >> 6;
+      double lcol[SB];
 #pragma unroll
-        for (int i = 0; i < TB; ++i) x[i] = (i < bn) ? Ds[(b0 + i) + static_cast<int64_t>(c) * m] : 0.0;
+      for (int j = 0; j < SB; ++j) lcol[j] = Ls[j][row];
+      const int prow = (row < nr) ? fd.rowIdx[io + r0 + row] : 0;
+      for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
+        double x = (row < nr) ? xs[row][c] : 0.0;
 #pragma unroll
-        for (int j = TB - 1; j >= 0; --j) {
-          const double xj = x[j];
-#pragma unroll
-          for (int i = 0; i < j; ++i) x[i] = fma(-Lb[j][i], xj, x[i]);
+        for (int j = SB - 1; j > 0; --j) {
+          const double xj = __shfl(x, j + 32 * half, 64);
+          if (row < j) x = fma(-lcol[j], xj, x);
         }
-#pragma unroll
-        for (int i = 0; i < TB; ++i)
-          if (i < bn) Ds[(b0 + i) + static_cast<int64_t>(c) * m] = x[i];
+        if (row < nr) {
+          Ds[(r0 + row) + static_cast<int64_t>(rg + c) * m] = x;
+          X[prow + static_cast<int64_t>(rg + c) * n] = x;
+        }
       }
     }
     __syncthreads();
   }
-  const int64_t tot = static_cast<int64_t>(nf) * nrhs;
-  for (int64_t e = tid; e < tot; e += NT) {
-    const int p = static_cast<int>(e % nf);
-    const int64_t col = e / nf;
-    X[fd.rowIdx[io + p] + col * n] = Ds[p + col * m];
-  }
+  publishBlock(fl + w, gen);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,6 +482,12 @@ __global__ void k_zero_u64(unsigned long long* p, int count) {
   if (i < count) p[i] = 0ull;
 }
 
+__global__ void k_zero_u32(unsigned* p, int64_t count) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -466,30 +503,25 @@ void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t 
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st) {
   k_permute_out<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, X, Xo, ldx);
 }
-void launchFwdGather(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Bp, double* G,
-                     double* Dl, const int64_t* gOff, hipStream_t st) {
-  if (count <= 0) return;
-  k_fwd_gather<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Bp, G, Dl, gOff);
-}
-void launchFwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* Z, double* Dl,
-                   const int64_t* gOff, hipStream_t st) {
-  if (count <= 0) return;
-  k_fwd_trsv<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, Z, Dl, gOff);
-}
-void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                    const double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
+void launchSolveBegin(unsigned* gen, hipStream_t st) { k_solve_begin<<<1, 64, 0, st>>>(gen); }
+void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
-  k_fwd_delta<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, G, Dl, gOff);
+  k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff);
 }
-void launchBwdGemv(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                   int nrhs, const double* Z, const double* X, double* Dl, const int64_t* gOff, hipStream_t st) {
+void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                    int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
-  k_bwd_gemv<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff);
+  k_fwd_sweep<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff, gen,
+                                      err);
 }
-void launchBwdTrsv(const int* list, int count, const DevFronts& fd, int n, int nrhs, double* X, double* Dl,
-                   const int64_t* gOff, hipStream_t st) {
-  if (count <= 0) return;
-  k_bwd_trsv<<<count, 256, 0, st>>>(list, count, fd, n, nrhs, X, Dl, gOff);
+void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                    int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  k_bwd_sweep<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff, gen,
+                                      err);
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {
@@ -509,6 +541,10 @@ void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const 
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st) {
   if (count <= 0) return;
   k_zero_u64<<<(count + 255) / 256, 256, 0, st>>>(p, count);
+}
+void launchZeroU32(unsigned* p, int64_t count, hipStream_t st) {
+  if (count <= 0) return;
+  k_zero_u32<<<gridFor(count, 256), 256, 0, st>>>(p, count);
 }
 
 }  // namespace symla_hip
