@@ -25,6 +25,10 @@ namespace symla {
 
 enum class Mode { ThresholdPivot, StaticRegularized };       // solver.hpp:20-23
 enum class OrderingType { AMD, Metis, Natural };              // ordering.hpp:31-35
+// Front storage precision (no counterpart in the reference): FP32 keeps the
+// fronts, multipliers and Schur updates in FP32 with FP64 pivots, D and
+// inertia; solveWithRefinement then restores FP64 accuracy.
+enum class Precision { FP64, FP32 };
 
 struct Inertia {  // inertia.hpp:14-18
   int n_pos = 0;
@@ -119,6 +123,7 @@ class SymLDLT<double> {
   ~SymLDLT() { release(); }
 
   void setMode(Mode mode) { mode_ = mode; }
+  void setPrecision(Precision p) { precision_ = p; }
   void setPivotThreshold(double u) { pivot_threshold_ = u; }
   void setOrdering(OrderingType ordering) { ordering_ = ordering; }
   void setKKTBlockSizes(int n1, int n2) {
@@ -181,6 +186,7 @@ class SymLDLT<double> {
     o.growth = regularization_growth_;
     o.static_rel_floor = static_pivot_relative_floor_;
     o.static_abs_floor = static_pivot_absolute_floor_;
+    o.precision = precision_ == Precision::FP32 ? SYMLA_PREC_FP32 : SYMLA_PREC_FP64;
     std::vector<int> signs;
     if (mode_ == Mode::StaticRegularized) {
       signs = buildExpectedSignOriginal(n_);
@@ -258,6 +264,7 @@ class SymLDLT<double> {
 
  private:
   Mode mode_ = Mode::ThresholdPivot;
+  Precision precision_ = Precision::FP64;
   double pivot_threshold_ = 0.01;
   OrderingType ordering_ = OrderingType::AMD;
   bool pattern_analyzed_ = false;

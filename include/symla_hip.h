@@ -33,6 +33,9 @@ enum { SYMLA_ORDER_AMD = 0, SYMLA_ORDER_METIS = 1, SYMLA_ORDER_NATURAL = 2, SYML
 /* Mode (include/symla/solver.hpp:20-23). */
 enum { SYMLA_MODE_THRESHOLD = 0, SYMLA_MODE_STATIC = 1 };
 
+/* Front storage precision (symla_factor_opts.precision). */
+enum { SYMLA_PREC_FP64 = 0, SYMLA_PREC_FP32 = 1 };
+
 typedef struct symla_symbolic symla_symbolic;
 typedef struct symla_plan symla_plan;
 
@@ -85,6 +88,11 @@ typedef struct {
   double growth;                /* 10 */
   double static_rel_floor;      /* 1e-8 */
   double static_abs_floor;      /* 1e-300 */
+  /* Front storage precision (no counterpart in the reference, which factors
+   * in FP64): SYMLA_PREC_FP64, or SYMLA_PREC_FP32 = fronts, multipliers and
+   * Schur updates in FP32 (FP32 MFMA), with pivot tests, 2x2 inversion, D
+   * and inertia in FP64; pair with symla_solve_refined for FP64 accuracy. */
+  int precision;
 } symla_factor_opts;
 
 typedef struct {

@@ -21,6 +21,7 @@ SYMLA_ERR_DEVICE = 4
 
 ORDER_AMD, ORDER_METIS, ORDER_NATURAL, ORDER_USER = 0, 1, 2, 3
 MODE_THRESHOLD, MODE_STATIC = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
 
 i32p = C.POINTER(C.c_int)
 i64p = C.POINTER(C.c_int64)
@@ -36,7 +37,7 @@ class FactorOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("pivot_threshold", C.c_double), ("zero_tolerance", C.c_double),
                 ("relative_pivot_floor", C.c_double), ("expected_sign", i32p), ("initial_delta", C.c_double),
                 ("max_retries", C.c_int), ("growth", C.c_double), ("static_rel_floor", C.c_double),
-                ("static_abs_floor", C.c_double)]
+                ("static_abs_floor", C.c_double), ("precision", C.c_int)]
 
 
 class FactorInfo(C.Structure):

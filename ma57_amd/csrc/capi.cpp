@@ -72,6 +72,7 @@ FactorOptions toFactorOptions(const symla_factor_opts* o, int n) {
   f.growth = o->growth;
   f.staticRelFloor = o->static_rel_floor;
   f.staticAbsFloor = o->static_abs_floor;
+  f.precision = o->precision;
   return f;
 }
 
@@ -319,7 +320,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
         dnpert, dlist, dsign;
     DBuf<int64_t> dio, dacp, daSrc;
     DBuf<double> ddd, ddo, dpert, dws;
-    DBuf<double*> dF;
+    DBuf<void*> dF;
     DBuf<unsigned long long> dom, drm;
     upI(dm, {n});
     upI(dnE, {nE});
@@ -351,7 +352,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     dom.alloc(1);
     drm.alloc(1);
     dws.alloc(2 * static_cast<size_t>(nE) + 2);
-    dF.upload(std::vector<double*>{dA.p}, st);
+    dF.upload(std::vector<void*>{dA.p}, st);
     upI(dlist, {0});
     const int* signPtr = nullptr;
     if (opts && opts->static_mode && opts->expected_sign) {
@@ -363,6 +364,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     f.nE = dnE.p;
     f.ncols = dnc.p;
     f.F = dF.p;
+    f.fp32 = 0;
     f.idxOff = dio.p;
     f.frontIdx = dfi.p;
     f.relmap = drel.p;

@@ -22,7 +22,8 @@ struct DevFronts {
   const int* m;          // front order
   const int* nE;         // eligible (fully summed) columns
   const int* ncols;      // own columns
-  double* const* F;      // front storage, column-major, ld = m
+  void* const* F;        // front storage, column-major, ld = m (double, or float when fp32)
+  int fp32;              // 1: fronts hold float (FP32 factor storage and Schur updates)
   const int64_t* idxOff; // offset into per-position arrays
   const int* frontIdx;   // final-order global index of pre-permutation local position
   const int* relmap;     // child position -> parent pre-permutation position (-1: none)

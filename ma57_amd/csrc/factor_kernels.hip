@@ -178,6 +178,7 @@ __global__ void k_reset_perm(const int* list, int count, DevFronts fd) {
 // (front, column tile), so every target entry has exactly one writer and the
 // summation order is fixed.
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks, int tileW, DevFronts fd,
                                                    const double* __restrict__ aVals, int monotone) {
   const int task = blockIdx.x;
@@ -186,12 +187,12 @@ __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks,
   const int c0 = tasks[task].y;
   const int m = fd.m[s];
   const int c1 = min(c0 + tileW, m);
-  double* F = fd.F[s];
+  T* F = static_cast<T*>(fd.F[s]);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
 
   for (int j = c0 + wv; j < c1; j += nw) {
-    double* col = F + static_cast<size_t>(j) * m;
-    for (int i = j + lane; i < m; i += 64) col[i] = 0.0;
+    T* col = F + static_cast<size_t>(j) * m;
+    for (int i = j + lane; i < m; i += 64) col[i] = T(0);
   }
   __syncthreads();
 
@@ -199,8 +200,9 @@ __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks,
   const int fc = fd.firstCol[s];
   for (int j = c0 + threadIdx.x; j < min(c1, nc); j += blockDim.x) {
     const int gcol = fc + j;
-    double* col = F + static_cast<size_t>(j) * m;
-    for (int64_t t = fd.acolPtr[gcol]; t < fd.acolPtr[gcol + 1]; ++t) col[fd.aRow[t]] += aVals[fd.aSrc[t]];
+    T* col = F + static_cast<size_t>(j) * m;
+    for (int64_t t = fd.acolPtr[gcol]; t < fd.acolPtr[gcol + 1]; ++t)
+      col[fd.aRow[t]] += static_cast<T>(aVals[fd.aSrc[t]]);
   }
   __syncthreads();
 
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks,
     const int c = fd.childList[ci];
     const int mc = fd.m[c];
     const int nfc = fd.nF[c];
-    const double* Fc = fd.F[c];
+    const T* Fc = static_cast<const T*>(fd.F[c]);
     const int* rel = fd.relmap + fd.idxOff[c];
     int bLo = nfc, bHi = mc;
     if (monotone) {
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks,
     for (int b = bLo + wv; b < bHi; b += nw) {
       const int lb = rel[b];
       if (lb < 0) continue;
-      const double* ccol = Fc + static_cast<size_t>(b) * mc;
+      const T* ccol = Fc + static_cast<size_t>(b) * mc;
       for (int a = b + lane; a < mc; a += 64) {
         const int la = rel[a];
         if (la < 0) continue;
@@ -255,6 +257,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks,
 // pivot, which is the reference's panel_size = 1 schedule for fronts with
 // fewer than 256 eligible columns (multifrontal.hpp:564-566).
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(kSmallThreads) void k_factor_small(const int* list, int count, DevFronts fd,
                                                                  FactorParams prm) {
   constexpr int NT = kSmallThreads;
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_factor_small(const int* list,
   const int s = list[blockIdx.x];
   const int m = fd.m[s];
   const int nE = fd.nE[s];
-  double* F = fd.F[s];
+  T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = NT / 64;
 
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_factor_small(const int* list,
 
   double om = 0.0;
   for (int j = wv; j < m; j += nw) {
-    const double* col = F + static_cast<size_t>(j) * m;
+    const T* col = F + static_cast<size_t>(j) * m;
     const int base = P(j, j) - j;
     for (int i = j + lane; i < m; i += 64) {
       const double v = col[i];
@@ -472,9 +475,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_factor_small(const int* list,
 
   // write back
   for (int j = wv; j < m; j += nw) {
-    double* col = F + static_cast<size_t>(j) * m;
+    T* col = F + static_cast<size_t>(j) * m;
     const int base = P(j, j) - j;
-    for (int i = j + lane; i < m; i += 64) col[i] = A[base + i];
+    for (int i = j + lane; i < m; i += 64) col[i] = static_cast<T>(A[base + i]);
   }
   for (int i = tid; i < m; i += NT) fd.perm[io + i] = perm[i];
   const double rm = blockMax<NT>(tmax, redV);
@@ -502,20 +505,21 @@ size_t smallFrontSmemBytes(int m) {
 
 // max|F| over the lower triangle (dense_kernel.hpp:413-415); `bpf` blocks
 // per listed front.
+template <typename T>
 __global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count, int bpf, DevFronts fd) {
   const int f = blockIdx.x / bpf;
   if (f >= count) return;
   const int s = list[f];
   const int part = blockIdx.x % bpf;
   const int m = fd.m[s];
-  const double* F = fd.F[s];
+  const T* F = static_cast<const T*>(fd.F[s]);
   double v = 0.0;
   const int lane = threadIdx.x & 63;
   const int gw = part * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nwTot = bpf * (blockDim.x >> 6);
   for (int j = gw; j < m; j += nwTot) {
-    const double* col = F + static_cast<size_t>(j) * m;
-    for (int i = j + lane; i < m; i += 64) v = fmax(v, fabs(col[i]));
+    const T* col = F + static_cast<size_t>(j) * m;
+    for (int i = j + lane; i < m; i += 64) v = fmax(v, fabs(static_cast<double>(col[i])));
   }
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   if (lane == 0) atomicMaxNonNeg(fd.origMaxBits + s, v);
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count
 // pivot accepted by the first Bunch-Kaufman test (every static pivot) costs
 // one barrier; any other decision falls back to the general code, which
 // works on column buffers in global memory.
-template <int RM>
+template <int RM, typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int count, DevFronts fd, FactorParams prm,
                                                           double* wsBase, const int64_t* wsOff) {
   constexpr int NT = kPanelThreads;
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   }
   const int m = fd.m[s];
   const int nE = fd.nE[s];
-  double* F = fd.F[s];
+  T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   const int k0 = fd.kcur[s];
   if (k0 >= nE) {
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   const int nbEnd = min(k0 + kPanel, nE);
   double* curK = ws;
   double* curR = ws + nE;
-  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+  auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
 
   const bool stat = prm.staticMode != 0;
   double origMax = bitsToDouble(fd.origMaxBits[s]);
@@ -624,7 +628,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     for (int i = k + tid; i < nE; i += NT) {
       const double raw = (i >= col) ? Fat(i, col) : Fat(col, i);
       double acc = 0.0;
-      for (int q = 0; q < p; ++q) acc = fma(Fat(i, k0 + q), wvec[q], acc);
+      for (int q = 0; q < p; ++q) acc = fma(static_cast<double>(Fat(i, k0 + q)), wvec[q], acc);
       out[i] = raw - acc;
     }
     __syncthreads();
@@ -634,19 +638,19 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   auto swapSym = [&](int a, int b) {
     for (int t = tid; t < m; t += NT) {
       if (t < a) {
-        const double x = Fat(a, t);
+        const T x = Fat(a, t);
         Fat(a, t) = Fat(b, t);
         Fat(b, t) = x;
       } else if (t == a) {
-        const double x = Fat(a, a);
+        const T x = Fat(a, a);
         Fat(a, a) = Fat(b, b);
         Fat(b, b) = x;
       } else if (t < b) {
-        const double x = Fat(t, a);
+        const T x = Fat(t, a);
         Fat(t, a) = Fat(b, t);
         Fat(b, t) = x;
       } else if (t > b) {
-        const double x = Fat(t, a);
+        const T x = Fat(t, a);
         Fat(t, a) = Fat(t, b);
         Fat(t, b) = x;
       }
@@ -672,14 +676,14 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       pdd[p] = d;
       pdo[p] = 0.0;
       pkd[p] = 1;
-      Fat(k, k) = d;
+      Fat(k, k) = static_cast<T>(d);
     }
     countInertia(d, npos, nneg, nzero);
     tmax = fmax(tmax, fabs(d));
     for (int i = k + 1 + tid; i < nE; i += NT) {
       const double c = col[i];
       tmax = fmax(tmax, fabs(c));
-      Fat(i, k) = c / d;
+      Fat(i, k) = static_cast<T>(c / d);
     }
     __syncthreads();
     ++k;
@@ -709,35 +713,35 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       int q = 0;
       for (; q + 4 <= pq; q += 4) {
         const double w0 = Wrow[kr][q], w1 = Wrow[kr][q + 1], w2 = Wrow[kr][q + 2], w3 = Wrow[kr][q + 3];
-        const double* L0 = F + static_cast<size_t>(k0 + q) * m;
+        const T* L0 = F + static_cast<size_t>(k0 + q) * m;
 #pragma unroll
         for (int j = 0; j < RM; ++j) {
           const int i = k0 + tid + j * NT;
           if (i >= k && i < nE) {
             double c = cr[j];
-            c = fma(-L0[i], w0, c);
-            c = fma(-L0[i + m], w1, c);
-            c = fma(-L0[i + 2 * static_cast<size_t>(m)], w2, c);
-            c = fma(-L0[i + 3 * static_cast<size_t>(m)], w3, c);
+            c = fma(-static_cast<double>(L0[i]), w0, c);
+            c = fma(-static_cast<double>(L0[i + m]), w1, c);
+            c = fma(-static_cast<double>(L0[i + 2 * static_cast<size_t>(m)]), w2, c);
+            c = fma(-static_cast<double>(L0[i + 3 * static_cast<size_t>(m)]), w3, c);
             cr[j] = c;
           }
         }
       }
       for (; q < pq; ++q) {
         const double w = Wrow[kr][q];
-        const double* Lq = F + static_cast<size_t>(k0 + q) * m;
+        const T* Lq = F + static_cast<size_t>(k0 + q) * m;
 #pragma unroll
         for (int j = 0; j < RM; ++j) {
           const int i = k0 + tid + j * NT;
-          if (i >= k && i < nE) cr[j] = fma(-Lq[i], w, cr[j]);
+          if (i >= k && i < nE) cr[j] = fma(-static_cast<double>(Lq[i]), w, cr[j]);
         }
       }
       if (haveNext) {
-        const double* Lq = F + static_cast<size_t>(k0 + p - 1) * m;
+        const T* Lq = F + static_cast<size_t>(k0 + p - 1) * m;
 #pragma unroll
         for (int j = 0; j < RM; ++j) {
           const int i = k0 + tid + j * NT;
-          if (i >= k && i < nE) cr[j] = fma(-Lq[i], wNext, cr[j]);
+          if (i >= k && i < nE) cr[j] = fma(-static_cast<double>(Lq[i]), wNext, cr[j]);
         }
       }
 #pragma unroll
@@ -811,7 +815,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
         fd.dDiag[io + k] = d;
         fd.dOff[io + k] = 0.0;
         fd.pkind[io + k] = 1;
-        Fat(k, k) = d;
+        Fat(k, k) = static_cast<T>(d);
       }
       countInertia(d, npos, nneg, nzero);
       tmax = fmax(tmax, fabs(d));
@@ -822,7 +826,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
         if (i > k && i < nE) {
           const double c = cr[j];
           tmax = fmax(tmax, fabs(c));
-          Fat(i, k) = c / d;
+          Fat(i, k) = static_cast<T>(c / d);
           // (D L^T)(k, i) = d * l_ik = c
           if (i - k0 < kPanel) Wrow[i - k0][kq] = c;
         }
@@ -901,17 +905,17 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       pdd[p + 1] = d22;
       pdo[p + 1] = 0.0;
       pkd[p + 1] = 0;
-      Fat(k, k) = d11;
-      Fat(k + 1, k) = d21;
-      Fat(k + 1, k + 1) = d22;
+      Fat(k, k) = static_cast<T>(d11);
+      Fat(k + 1, k) = static_cast<T>(d21);
+      Fat(k + 1, k + 1) = static_cast<T>(d22);
     }
     countInertia2(d11, d22, det, npos, nneg);
     const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
     for (int i = k + 2 + tid; i < nE; i += NT) {
       const double a0 = curK[i], a1 = curR[i];
       tmax = fmax(tmax, fmax(fabs(a0), fabs(a1)));
-      Fat(i, k) = a0 * inv11 + a1 * inv21;
-      Fat(i, k + 1) = a0 * inv21 + a1 * inv22;
+      Fat(i, k) = static_cast<T>(a0 * inv11 + a1 * inv21);
+      Fat(i, k + 1) = static_cast<T>(a0 * inv21 + a1 * inv22);
     }
     __syncthreads();
     k += 2;
@@ -936,6 +940,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
 
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
 // panel: row-parallel forward substitution against U = D L^T of the panel.
+template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, const int* blockStart,
                                                      DevFronts fd) {
   __shared__ double U[(kPanel + 1) * (kPanel + 1)];
@@ -951,9 +956,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   const int nE = fd.nE[s];
   const int k1 = fd.kcur[s];
   const int k0 = k1 - p;
-  double* F = fd.F[s];
+  T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
-  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+  auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
   const int P1 = kPanel + 1;
   for (int q = threadIdx.x; q < p; q += blockDim.x) {
     pdd[q] = fd.dDiag[io + k0 + q];
@@ -984,9 +989,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   while (q < p) {
     if (pkd[q] == 1) {
       double x = Fat(i, k0 + q);
-      for (int qp = 0; qp < q; ++qp) x = fma(-Fat(i, k0 + qp), U[qp + q * P1], x);
+      for (int qp = 0; qp < q; ++qp) x = fma(-static_cast<double>(Fat(i, k0 + qp)), U[qp + q * P1], x);
       mx = fmax(mx, fabs(x));
-      Fat(i, k0 + q) = x / pdd[q];
+      Fat(i, k0 + q) = static_cast<T>(x / pdd[q]);
       ++q;
     } else {
       double x0 = Fat(i, k0 + q), x1 = Fat(i, k0 + q + 1);
@@ -999,8 +1004,8 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
       const double d11 = pdd[q], d21 = pdo[q], d22 = pdd[q + 1];
       const double det = d11 * d22 - d21 * d21;
       const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
-      Fat(i, k0 + q) = x0 * inv11 + x1 * inv21;
-      Fat(i, k0 + q + 1) = x0 * inv21 + x1 * inv22;
+      Fat(i, k0 + q) = static_cast<T>(x0 * inv11 + x1 * inv21);
+      Fat(i, k0 + q + 1) = static_cast<T>(x0 * inv21 + x1 * inv22);
       q += 2;
     }
   }
@@ -1043,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, co
   const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
   if (t == 0 && threadIdx.x == 0 && fd.schurFlops)
     atomicAdd(fd.schurFlops, static_cast<unsigned long long>(p) * N * (N + 1));
-  double* F = fd.F[s];
+  double* F = static_cast<double*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
   for (int q = threadIdx.x; q < p; q += blockDim.x) {
@@ -1099,6 +1104,102 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, co
           const double v = Fat(row, col) - acc[bi][bj][rr];
           Fat(row, col) = v;
           mx = fmax(mx, fabs(v));
+        }
+      }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
+}
+
+// FP32 storage: the same update with v_mfma_f32_16x16x4f32 on float
+// operands (L D formed in FP64 from the FP64 pivots, then rounded).  Result
+// layout of the instruction on gfx950 (measured): D[i][j] in lane
+// j + 16*(i/4), register i%4.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_syrk_f32(const int* list, int count, const int* tileStart, DevFronts fd) {
+  constexpr int TS = 64;
+  constexpr int KP = kPanel + 4;  // padded K
+  __shared__ float As[TS][KP + 1];
+  __shared__ float Bs[TS][KP + 1];
+  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
+  __shared__ int pkd[kPanel + 2];
+  if (static_cast<int>(blockIdx.x) >= tileStart[count]) return;
+  const int f = findSegment(tileStart, count, blockIdx.x);
+  const int s = list[f];
+  const int p = fd.pcount[s];
+  if (p == 0) return;
+  const int m = fd.m[s];
+  const int k1 = fd.kcur[s];
+  const int k0 = k1 - p;
+  const int N = m - k1;
+  if (N <= 0) return;
+  const int T = (N + TS - 1) / TS;
+  const int t = blockIdx.x - tileStart[f];
+  int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  if (ti >= T) return;
+  const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
+  if (t == 0 && threadIdx.x == 0 && fd.schurFlops)
+    atomicAdd(fd.schurFlops, static_cast<unsigned long long>(p) * N * (N + 1));
+  float* F = static_cast<float*>(fd.F[s]);
+  const int64_t io = fd.idxOff[s];
+  auto Fat = [&](int i, int j) -> float& { return F[i + static_cast<size_t>(j) * m]; };
+  for (int q = threadIdx.x; q < p; q += blockDim.x) {
+    pdd[q] = fd.dDiag[io + k0 + q];
+    pdo[q] = fd.dOff[io + k0 + q];
+    pkd[q] = fd.pkind[io + k0 + q];
+  }
+  __syncthreads();
+  const int pp = (p + 3) & ~3;
+  for (int e = threadIdx.x; e < TS * pp; e += blockDim.x) {
+    const int r = e % TS, q = e / TS;
+    double a = 0.0;
+    float b = 0.0f;
+    if (q < p) {
+      const int ri = i0 + r, rj = j0 + r;
+      if (ri < m) {
+        const int kind = pkd[q];
+        if (kind == 1)
+          a = Fat(ri, k0 + q) * pdd[q];
+        else if (kind == 2)
+          a = Fat(ri, k0 + q) * pdd[q] + Fat(ri, k0 + q + 1) * pdo[q];
+        else
+          a = Fat(ri, k0 + q - 1) * pdo[q - 1] + Fat(ri, k0 + q) * pdd[q];
+      }
+      if (rj < m) b = Fat(rj, k0 + q);
+    }
+    As[r][q] = static_cast<float>(a);
+    Bs[r][q] = b;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  v4f acc[2][2];
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int kk = 0; kk < pp; kk += 4) {
+    const float a0 = As[wr + lr][kk + lk];
+    const float a1 = As[wr + 16 + lr][kk + lk];
+    const float b0 = Bs[wc + lr][kk + lk];
+    const float b1 = Bs[wc + 16 + lr][kk + lk];
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+  }
+  double mx = 0.0;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = i0 + wr + bi * 16 + 4 * lk + rr;
+        const int col = j0 + wc + bj * 16 + lr;
+        if (row < m && col < m && (ti != tj || row >= col)) {
+          const float v = Fat(row, col) - acc[bi][bj][rr];
+          Fat(row, col) = v;
+          mx = fmax(mx, fabs(static_cast<double>(v)));
         }
       }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -1173,54 +1274,84 @@ void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_
 void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& fd, const double* aVals, int monotone,
                     hipStream_t st) {
   if (ntasks <= 0) return;
-  k_assemble<<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+  if (fd.fp32)
+    k_assemble<float><<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+  else
+    k_assemble<double><<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+}
+template <typename T>
+static void factorSmallT(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
+                         hipStream_t st) {
+  const size_t smem = smallFrontSmemBytes(maxM);
+  static bool attrSet = false;
+  if (!attrSet) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_small<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attrSet = true;
+  }
+  k_factor_small<T><<<count, kSmallThreads, smem, st>>>(list, count, fd, prm);
 }
 void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
                        hipStream_t st) {
   if (count <= 0) return;
-  const size_t smem = smallFrontSmemBytes(maxM);
-  static bool attrSet = false;
-  if (!attrSet) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_small),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attrSet = true;
-  }
-  k_factor_small<<<count, kSmallThreads, smem, st>>>(list, count, fd, prm);
+  if (fd.fp32)
+    factorSmallT<float>(list, count, maxM, fd, prm, st);
+  else
+    factorSmallT<double>(list, count, maxM, fd, prm, st);
 }
 void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
   int bpf = (maxM + 15) / 16;
   if (bpf > 64) bpf = 64;
   if (bpf < 1) bpf = 1;
-  k_front_maxabs<<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
+  if (fd.fp32)
+    k_front_maxabs<float><<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
+  else
+    k_front_maxabs<double><<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
+}
+template <typename T>
+static void panelT(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
+                   const int64_t* wsOff, hipStream_t st) {
+  // register slots per thread for the eligible rows of the widest front
+  const int slots = (maxRows + kPanelThreads - 1) / kPanelThreads;
+  if (slots <= 1)
+    k_panel<1, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 2)
+    k_panel<2, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 4)
+    k_panel<4, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 8)
+    k_panel<8, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 16)
+    k_panel<16, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else if (slots <= 32)
+    k_panel<32, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  else
+    k_panel<0, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
 }
 void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                  const int64_t* wsOff, hipStream_t st) {
   if (count <= 0) return;
-  // register slots per thread for the eligible rows of the widest front
-  const int slots = (maxRows + kPanelThreads - 1) / kPanelThreads;
-  if (slots <= 1)
-    k_panel<1><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
-  else if (slots <= 2)
-    k_panel<2><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
-  else if (slots <= 4)
-    k_panel<4><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
-  else if (slots <= 8)
-    k_panel<8><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
-  else if (slots <= 16)
-    k_panel<16><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+  if (fd.fp32)
+    panelT<float>(list, count, maxRows, fd, prm, ws, wsOff, st);
   else
-    k_panel<0><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelT<double>(list, count, maxRows, fd, prm, ws, wsOff, st);
 }
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st) {
   if (count <= 0 || totalBlocks <= 0) return;
-  k_panel_rows<<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
+  if (fd.fp32)
+    k_panel_rows<float><<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
+  else
+    k_panel_rows<double><<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
 }
 void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
                 hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
-  k_syrk_f64<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
+  if (fd.fp32)
+    k_syrk_f32<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
+  else
+    k_syrk_f64<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
 }
 void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;

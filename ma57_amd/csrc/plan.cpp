@@ -157,7 +157,7 @@ Plan::Profile Plan::takeProfile() {
 Plan::~Plan() {
   for (auto& pc : prof_)
     for (hipEvent_t e : pc.ev) (void)hipEventDestroy(e);
-  for (double* p : dynAllocs_) (void)hipFree(p);
+  for (void* p : dynAllocs_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -252,13 +252,16 @@ void Plan::buildStaticLayout() {
     off[s] = total;
     total += static_cast<size_t>(hm_[s]) * hm_[s];
   }
-  if (pool_.n < total) pool_.alloc(total);
+  // offsets in elements of the current precision; the pool is sized in doubles
+  const size_t es = elemSize();
+  const size_t poolDoubles = (total * es + sizeof(double) - 1) / sizeof(double);
+  if (pool_.n < poolDoubles) pool_.alloc(poolDoubles);
   hF_.resize(numSN_);
-  for (int s = 0; s < numSN_; ++s) hF_[s] = pool_.p + off[s];
+  for (int s = 0; s < numSN_; ++s) hF_[s] = reinterpret_cast<char*>(pool_.p) + off[s] * es;
   maxNE_ = 0;
   for (int s = 0; s < numSN_; ++s) maxNE_ = std::max(maxNE_, hnE_[s]);
   dws_.ensure(2 * static_cast<size_t>(std::max(maxNE_, 1)));
-  for (double* p : dynAllocs_) (void)hipFree(p);
+  for (void* p : dynAllocs_) (void)hipFree(p);
   dynAllocs_.clear();
 }
 
@@ -297,7 +300,7 @@ void Plan::uploadLayout() {
   dacolPtr_.upload(hacolPtr_, stream_);
   daRow_.upload(haRow_, stream_);
   daSrc_.upload(haSrc_, stream_);
-  std::vector<double*> fp(hF_);
+  std::vector<void*> fp(hF_);
   dF_.upload(fp, stream_);
   const size_t cap = std::max<size_t>(static_cast<size_t>(idxUsed_), 1);
   if (dperm_.n < cap) {
@@ -404,6 +407,7 @@ DevFronts Plan::devFronts() {
   f.nE = dnE_.p;
   f.ncols = dncols_.p;
   f.F = dF_.p;
+  f.fp32 = prec_;
   f.idxOff = didxOff_.p;
   f.frontIdx = dfrontIdx_.p;
   f.relmap = drelmap_.p;
@@ -603,9 +607,8 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
         hidxOff_[s] = off;
         hm_[s] = m;
         hnE_[s] = ncols + nExtra;
-        double* np = nullptr;
-        hipCheck(hipMalloc(reinterpret_cast<void**>(&np), sizeof(double) * static_cast<size_t>(m) * m),
-                 "hipMalloc front");
+        void* np = nullptr;
+        hipCheck(hipMalloc(&np, elemSize() * static_cast<size_t>(m) * m), "hipMalloc front");
         dynAllocs_.push_back(np);
         hF_[s] = np;
         // own-column A entries: local rows shift past the extras
@@ -726,6 +729,12 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     vals = dvals_.p;
   }
   info = FactorInfo{};
+  if (opts.precision != 0 && opts.precision != 1) throw std::invalid_argument("symla: unknown precision");
+  if (opts.precision != prec_) {
+    // the front storage changes element size: re-plan the pool
+    prec_ = opts.precision;
+    layoutDirty_ = true;
+  }
   hipCheck(hipEventRecord(ev0_, stream_), "event");
   if (opts.mode == 0) {
     FactorParams prm = params(opts, false, 0.0, nullptr);
@@ -1064,7 +1073,14 @@ void Plan::frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd
   const int64_t io = hidxOff_[si];
   if (rows) hipCheck(hipMemcpy(rows, drowIdx_.p + io, sizeof(int) * mm, hipMemcpyDeviceToHost), "view rows");
   if (L && nf > 0) {
-    hipCheck(hipMemcpy(L, hF_[si], sizeof(double) * static_cast<size_t>(mm) * nf, hipMemcpyDeviceToHost), "view L");
+    const size_t cnt = static_cast<size_t>(mm) * nf;
+    if (prec_) {
+      std::vector<float> tmp(cnt);
+      hipCheck(hipMemcpy(tmp.data(), hF_[si], sizeof(float) * cnt, hipMemcpyDeviceToHost), "view L");
+      for (size_t e = 0; e < cnt; ++e) L[e] = tmp[e];
+    } else {
+      hipCheck(hipMemcpy(L, hF_[si], sizeof(double) * cnt, hipMemcpyDeviceToHost), "view L");
+    }
     for (int j = 0; j < nf; ++j)
       for (int i = 0; i < j && i < mm; ++i) L[i + static_cast<size_t>(j) * mm] = 0.0;  // upper part is not stored
   }

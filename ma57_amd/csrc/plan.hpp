@@ -51,6 +51,7 @@ struct DBuf {
 
 struct FactorOptions {
   int mode = 0;
+  int precision = 0;  // 0: FP64 fronts; 1: FP32 fronts (FP64 pivots, D and refinement)
   double zeroTol = 1e-300;
   double relFloor = 1e-12;
   std::vector<int> expectedSignOrig;  // empty = none
@@ -165,10 +166,10 @@ class Plan {
   std::vector<int> hnF_, hrowIdx_;  // host copies after factorization (dynamic mode / views)
   // device layout
   hipStream_t stream_ = nullptr;
-  DBuf<double> pool_;
-  std::vector<double*> hF_;
-  std::vector<double*> dynAllocs_;
-  DBuf<double*> dF_;
+  DBuf<double> pool_;      // front storage (FP64, or FP32 packed in the same bytes)
+  std::vector<void*> hF_;
+  std::vector<void*> dynAllocs_;
+  DBuf<void*> dF_;
   DBuf<int> dm_, dnE_, dncols_, dfirst_, dchildPtr_, dchildList_, dfrontIdx_, drelmap_, daRow_;
   DBuf<int64_t> didxOff_, dacolPtr_, daSrc_;
   DBuf<int> dperm_, drowIdx_, dpkind_, dnF_, dinertia_, dkcur_, dpcount_, dstopped_, dnpert_;
@@ -222,6 +223,8 @@ class Plan {
   bool singular_ = false;
   std::vector<int> singularColsOrig_;
   bool layoutDirty_ = false;  // dynamic mode changed the layout
+  int prec_ = 0;              // precision of the current front storage
+  size_t elemSize() const { return prec_ ? sizeof(float) : sizeof(double); }
   double factorMs_ = 0.0, solveMs_ = 0.0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool profiling_ = false;

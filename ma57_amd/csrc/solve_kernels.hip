@@ -92,7 +92,8 @@ __device__ inline void publishBlock(unsigned* f, unsigned gen) {
 
 // L(row, col) of a front with the (s+1, s) slot of a 2x2 pivot block read as
 // zero: that slot stores d21, not a multiplier (multifrontal.hpp:618-628).
-__device__ inline double lval(const double* F, int m, const int* pk, int row, int col) {
+template <typename T>
+__device__ inline double lval(const T* F, int m, const int* pk, int row, int col) {
   const double v = F[row + static_cast<size_t>(col) * m];
   return (row == col + 1 && pk[col] == 2) ? 0.0 : v;
 }
@@ -158,6 +159,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
 // stores delta(i) = G(perm(i)) + L(i, :) y.  A workgroup only waits for
 // blocks of the same front with a lower index, which were dispatched
 // before it.
+template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
                                                     int n, int nrhs, const double* __restrict__ G,
                                                     double* __restrict__ Z, double* __restrict__ Dl,
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   const int jEnd = own ? b : nOwn;
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
-  const double* F = fd.F[s];
+  const T* F = static_cast<const T*>(fd.F[s]);
   const double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
@@ -301,6 +303,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
 // x_w = z_w - L(anc, w)^T x_anc - sum_{j > w} L(j, w)^T x_j (the ancestors'
 // x is final from the levels above; the blocks j of the same front are
 // waited for), solves its unit upper diagonal block and publishes x_w.
+template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
                                                     int n, int nrhs, const double* __restrict__ Z,
                                                     double* __restrict__ X, double* __restrict__ Dl,
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   const int nr = min(r0 + SB, nf) - r0;
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
-  const double* F = fd.F[s];
+  const T* F = static_cast<const T*>(fd.F[s]);
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
   const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
       const int in = min(DT, m - i0);
       for (int e = tid; e < DT * SB; e += NT) {
         const int i = e % DT, qq = e / DT;
-        Ls[i][qq] = (i < in && qq < nr) ? F[(i0 + i) + static_cast<size_t>(r0 + qq) * m] : 0.0;
+        Ls[i][qq] = (i < in && qq < nr) ? static_cast<double>(F[(i0 + i) + static_cast<size_t>(r0 + qq) * m]) : 0.0;
       }
       for (int e = tid; e < DT * 64; e += NT) {
         const int i = e % DT, c = e / DT;
@@ -513,15 +516,23 @@ void launchFwdSweep(const int* list, int count, const int* blockStart, int block
                     int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
-  k_fwd_sweep<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff, gen,
-                                      err);
+  if (fd.fp32)
+    k_fwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
+                                               gen, err);
+  else
+    k_fwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
+                                                gen, err);
 }
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
-  k_bwd_sweep<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff, gen,
-                                      err);
+  if (fd.fp32)
+    k_bwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
+                                               gen, err);
+  else
+    k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
+                                                gen, err);
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {

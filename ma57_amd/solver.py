@@ -60,6 +60,11 @@ class OrderingType:
     Natural = _lib.ORDER_NATURAL
 
 
+class Precision:
+    FP64 = _lib.PREC_FP64
+    FP32 = _lib.PREC_FP32
+
+
 @dataclass
 class Inertia:
     n_pos: int = 0
@@ -188,9 +193,17 @@ class SymLDLT:
         self._retries = 0
         self._total_pert = 0.0
         self._num_pert = 0
+        self.precision = Precision.FP64
         self.last_info = None
 
     # --- setters (solver.hpp:36-126) ---
+    def setPrecision(self, precision):
+        """Front storage precision (no counterpart in the reference): Precision.FP64, or Precision.FP32
+        (FP32 fronts and Schur updates, FP64 pivots/D/inertia; use solveWithRefinement for FP64 accuracy)."""
+        if precision not in (Precision.FP64, Precision.FP32):
+            raise InvalidArgument("symla: unknown precision")
+        self.precision = precision
+
     def setMode(self, mode):
         self.mode = mode
 
@@ -303,6 +316,7 @@ class SymLDLT:
         o.growth = self._growth
         o.static_rel_floor = self._static_rel
         o.static_abs_floor = self._static_abs
+        o.precision = self.precision
         return o
 
     def factorize(self, A):
