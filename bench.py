@@ -3,16 +3,21 @@
 
 Workload (one "step"): numeric factorization of the 40^3 7-point Dirichlet
 Laplacian shifted by -0.5*I (n = 64,000, nested-dissection ordering) followed
-by a 16-right-hand-side solve, FP64 throughout, with the matrix values and the
-right-hand sides already resident in HBM.  Symbolic analysis and plan
-creation (device pool, scatter maps) happen once, outside the timed region.
+by a 16-right-hand-side solve, with the matrix values and the right-hand
+sides already resident in HBM.  Two variants are timed on the same plan:
+FP32 fronts (FP32 MFMA Schur updates, FP64 pivots) followed by a solve with
+FP64 iterative refinement, and the plain FP64 factorize + solve.  Symbolic
+analysis and plan creation (device pool, scatter maps) happen once, outside
+the timed regions.  Both variants must reach backward error <= 1e-12 with the
+closed-form inertia.
 
-Prints ONE JSON line.  `value` is seconds per factorize+solve (lower is
-better).  `roofline` covers the dominant kernel (the FP64 MFMA trailing
-update) from HIP events recorded by the library around each of its launches
-on the plan's stream during the timed steps; `cpu_baseline` times the CPU
-oracle (a restatement of the reference's OpenMP multifrontal path) on the
-same workload.
+Prints ONE JSON line.  `value` is seconds per FP32+IR factorize+solve (lower
+is better); `fp64_s` is the plain FP64 step.  `roofline` covers the dominant
+kernel of the FP32 variant (the FP32 MFMA trailing update) from HIP events
+recorded by the library around each of its launches on the plan's stream
+during the timed steps (`roofline_fp64`: the FP64 update of the FP64
+variant); `cpu_baseline` times the CPU oracle (a restatement of the
+reference's OpenMP multifrontal path) on the same workload.
 """
 from __future__ import annotations
 
@@ -29,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 PEAK_HBM_GBS = 8000.0
 
 
@@ -52,7 +58,7 @@ def main() -> int:
     import torch
 
     from ma57_amd import _lib, problems as P
-    from ma57_amd.solver import SymLDLT, OrderingType, csc_arrays
+    from ma57_amd.solver import SymLDLT, OrderingType, Precision, csc_arrays
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
@@ -80,56 +86,79 @@ def main() -> int:
     d_B = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)  # (nrhs, n) row-major == (n, nrhs) column-major
     d_X = torch.empty_like(d_B)
     torch.cuda.synchronize()
-    opts = solver._opts()
-    info = _lib.FactorInfo()
-
-    def step():
-        rc = lib.symla_factorize_device(plan, C.c_void_p(d_vals.data_ptr()), C.byref(opts), C.byref(info))
-        if rc:
-            raise RuntimeError(lib.symla_plan_last_error(plan).decode())
-        rc = lib.symla_solve_device(plan, C.c_void_p(d_B.data_ptr()), n, args.nrhs, C.c_void_p(d_X.data_ptr()), n)
-        if rc:
-            raise RuntimeError(lib.symla_plan_last_error(plan).decode())
-
-    for _ in range(args.warmup):
-        step()
-
-    # timed region (kernel-class events recorded on the plan's stream)
-    lib.symla_plan_set_profiling(plan, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = _lib.Profile()
-    lib.symla_plan_get_profile(plan, C.byref(prof))
-    lib.symla_plan_set_profiling(plan, 0)
-
-    # same steps without event brackets
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    clean = (time.perf_counter() - t1) / args.steps
-
-    fms = C.c_double()
-    sms = C.c_double()
-    lib.symla_plan_timing(plan, C.byref(fms), C.byref(sms))
-
-    X = d_X.cpu().numpy().T
-    be = P.backward_error(A, X, B)
-    inertia = (info.n_pos, info.n_neg, info.n_zero)
     expect = P.closed_form_inertia(P.laplace_eigs_3d(G, G, G), shift)[:3]
-
-    ms_per_step = elapsed / args.steps * 1e3
-    syrk_avg_ms = prof.syrk_ms / max(prof.syrk_launches, 1)
-    flops_per_launch = prof.schur_flops / max(prof.syrk_launches, 1)
-    achieved = flops_per_launch / (syrk_avg_ms * 1e-3) / 1e12 if syrk_avg_ms > 0 else 0.0
-
     front_flops = sum(s.ncols * len(s.rowPattern) ** 2 - s.ncols ** 2 * len(s.rowPattern) + s.ncols ** 3 / 3.0
                       for s in sf.supernodes)
+
+    def leg(precision, refine):
+        """Time `steps` factorize + solve steps of one precision; FP32 solves refine in FP64."""
+        solver.setPrecision(precision)
+        opts = solver._opts()
+        info = _lib.FactorInfo()
+        rinfo = _lib.RefineInfo()
+        ropts = _lib.RefineOpts(10, 1e-14)
+
+        def step():
+            rc = lib.symla_factorize_device(plan, C.c_void_p(d_vals.data_ptr()), C.byref(opts), C.byref(info))
+            if rc:
+                raise RuntimeError(lib.symla_plan_last_error(plan).decode())
+            if refine:
+                rc = lib.symla_solve_refined_device(plan, C.c_void_p(d_vals.data_ptr()), C.c_void_p(d_B.data_ptr()),
+                                                    n, args.nrhs, C.c_void_p(d_X.data_ptr()), n, C.byref(ropts),
+                                                    C.byref(rinfo))
+            else:
+                rc = lib.symla_solve_device(plan, C.c_void_p(d_B.data_ptr()), n, args.nrhs,
+                                            C.c_void_p(d_X.data_ptr()), n)
+            if rc:
+                raise RuntimeError(lib.symla_plan_last_error(plan).decode())
+
+        for _ in range(args.warmup):
+            step()
+        # timed region (kernel-class events recorded by the library on the plan's stream)
+        lib.symla_plan_set_profiling(plan, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = _lib.Profile()
+        lib.symla_plan_get_profile(plan, C.byref(prof))
+        lib.symla_plan_set_profiling(plan, 0)
+        # the same steps without event brackets
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        clean = (time.perf_counter() - t1) / args.steps
+        fms, sms = C.c_double(), C.c_double()
+        lib.symla_plan_timing(plan, C.byref(fms), C.byref(sms))
+        X = d_X.cpu().numpy().T
+        syrk_avg_ms = prof.syrk_ms / max(prof.syrk_launches, 1)
+        flops_per_launch = prof.schur_flops / max(prof.syrk_launches, 1)
+        achieved = flops_per_launch / (syrk_avg_ms * 1e-3) / 1e12 if syrk_avg_ms > 0 else 0.0
+        return {
+            "s": elapsed / args.steps,
+            "s_without_event_brackets": clean,
+            "backward_error": P.backward_error(A, X, B),
+            "inertia": [info.n_pos, info.n_neg, info.n_zero],
+            "ir_iterations": rinfo.iterations if refine else 0,
+            "device_factor_ms": fms.value,
+            "kernel_ms_per_step": {k: getattr(prof, k + "_ms") / args.steps for k in
+                                   ("syrk", "panel", "panel_rows", "small", "assemble", "fwd", "bwd")},
+            "syrk": {"achieved": achieved, "launches_per_step": prof.syrk_launches / args.steps,
+                     "avg_launch_ms": syrk_avg_ms, "flops_per_launch": flops_per_launch},
+        }
+
+    fp64 = leg(Precision.FP64, refine=False)
+    mixed = leg(Precision.FP32, refine=True)
+
+    def roofline(kernel, r, peak):
+        return {"bound": "mfma", "kernel": kernel, "achieved": r["syrk"]["achieved"], "peak": peak,
+                "unit": "TFLOP/s", "frac": r["syrk"]["achieved"] / peak, "traffic": None,
+                "launches_per_step": r["syrk"]["launches_per_step"], "avg_launch_ms": r["syrk"]["avg_launch_ms"],
+                "flops_per_launch": r["syrk"]["flops_per_launch"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -142,46 +171,46 @@ def main() -> int:
         Xc = f.solve(B, parallel=True, threads=threads)
         cpu_s = time.perf_counter() - tc
         cpu = {"value": cpu_s, "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"one full factorize + {args.nrhs}-RHS solve of the same {G}^3 system (OpenMP task DAG "
-                         f"restatement of the reference path, {threads} threads)",
+               "sample": f"one full FP64 factorize + {args.nrhs}-RHS solve of the same {G}^3 system (OpenMP task "
+                         f"DAG restatement of the reference path, {threads} threads)",
                "inertia": list(f.inertia()), "backward_error": P.backward_error(A, Xc, B)}
 
+    ok64 = fp64["backward_error"] <= 1e-12 and tuple(fp64["inertia"]) == tuple(expect)
+    ok32 = mixed["backward_error"] <= 1e-12 and tuple(mixed["inertia"]) == tuple(expect)
     out = {
         "metric": "factorize+solve s, FP32+IR vs FP64 at backward err <=1e-12; Schur TFLOP/s",
-        "value": ms_per_step / 1e3,
+        "value": mixed["s"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "ms_per_step_without_event_brackets": clean * 1e3,
+        "ms_per_step": mixed["s"] * 1e3,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"3D 7-point Laplacian {G}^3 - {shift} I (n={n}), nested dissection, "
-                               f"FP64 factorize + solve with nrhs={args.nrhs}",
-                   "n": n, "nrhs": args.nrhs, "precision": "fp64", "nnz_L": int(sf.nnz_l),
-                   "supernodes": len(sf.supernodes), "max_front": max(len(s.rowPattern) for s in sf.supernodes)},
-        "roofline": {"bound": "mfma", "kernel": "k_syrk_f64", "achieved": achieved,
-                     "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS,
-                     "traffic": None, "launches_per_step": prof.syrk_launches / args.steps,
-                     "avg_launch_ms": syrk_avg_ms, "flops_per_launch": flops_per_launch},
+        "config": {"workload": f"3D 7-point Laplacian {G}^3 - {shift} I (n={n}), nested dissection; factorize + "
+                               f"{args.nrhs}-RHS solve; value: FP32 fronts + FP64 iterative refinement, "
+                               f"fp64: plain FP64 factorize + solve",
+                   "n": n, "nrhs": args.nrhs, "nnz_L": int(sf.nnz_l), "supernodes": len(sf.supernodes),
+                   "max_front": max(len(s.rowPattern) for s in sf.supernodes)},
+        "roofline": roofline("k_syrk_f32", mixed, PEAK_FP32_MFMA_TFLOPS),
         "cpu_baseline": cpu,
-        "backward_error": be,
-        "inertia": list(inertia),
+        "fp32_ir": mixed,
+        "fp64": fp64,
+        "fp64_s": fp64["s"],
+        "speedup_fp32ir_vs_fp64": fp64["s"] / mixed["s"],
+        "roofline_fp64": roofline("k_syrk_f64", fp64, PEAK_FP64_MFMA_TFLOPS),
+        "backward_error": mixed["backward_error"],
+        "inertia": mixed["inertia"],
         "inertia_expected": list(expect),
-        "ir_iterations": 0,
+        "ir_iterations": mixed["ir_iterations"],
         "analyze_s": analyze_s,
-        "device_factor_ms": fms.value,
-        "device_solve_ms": sms.value,
         "factor_gflop": front_flops / 1e9,
-        "kernel_ms_per_step": {k: getattr(prof, k + "_ms") / args.steps for k in
-                               ("syrk", "panel", "panel_rows", "small", "assemble", "fwd", "bwd")},
     }
     print(json.dumps(out))
-    return 0 if (be <= 1e-12 and tuple(inertia) == tuple(expect)) else 1
+    return 0 if (ok64 and ok32) else 1
 
 
 if __name__ == "__main__":

@@ -121,6 +121,15 @@ int symla_factorize_device(symla_plan* p, const double* d_a_vals, const symla_fa
 /* SymLDLT::singularColumns (solver.hpp:251-257): original indices. */
 int symla_singular_cols(const symla_plan* p, int* out, int cap);
 
+/* Inertia and singular-column count per group of original indices (no
+ * counterpart in the reference, which factors one system per call).  For a
+ * block-diagonal matrix assembled from independent systems, group g owns the
+ * original indices [group_start[g], group_start[g+1]) (group_start[0] = 0,
+ * group_start[ngroups] = n); every supernode lies inside one system, so the
+ * per-front counts add up per system.  inertia3: 3 ints per group {pos, neg,
+ * zero}; singular (may be NULL): singular columns per group. */
+int symla_group_inertia(symla_plan* p, int ngroups, const int* group_start, int* inertia3, int* singular);
+
 /* SymLDLT::solve (solver.hpp:198-218) / MultifrontalSolver::solve
  * (solve.hpp:159-431).  B, X: n x nrhs column-major.  Host pointers
  * (symla_solve) or device pointers (symla_solve_device).  SYMLA_ERR_LOGIC

@@ -84,6 +84,7 @@ SIGNATURES = {
     "symla_factorize_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FactorOpts), C.POINTER(FactorInfo)]),
     "symla_singular_cols": (C.c_int, [C.c_void_p, i32p, C.c_int]),
     "symla_solve": (C.c_int, [C.c_void_p, f64p, C.c_int64, C.c_int, f64p, C.c_int64]),
+    "symla_group_inertia": (C.c_int, [C.c_void_p, C.c_int, i32p, i32p, i32p]),
     "symla_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64]),
     "symla_solve_refined": (C.c_int, [C.c_void_p, f64p, f64p, C.c_int64, C.c_int, f64p, C.c_int64,
                                       C.POINTER(RefineOpts), C.POINTER(RefineInfo)]),

@@ -196,6 +196,13 @@ int symla_singular_cols(const symla_plan* p, int* out, int cap) {
   return static_cast<int>(sc.size());
 }
 
+int symla_group_inertia(symla_plan* p, int ngroups, const int* group_start, int* inertia3, int* singular) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_group_inertia: null plan");
+    p->plan->groupInertia(ngroups, group_start, inertia3, singular);
+  });
+}
+
 int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx) {
   return planGuarded(p, [&] {
     if (!p) throw std::invalid_argument("symla_solve: null plan");

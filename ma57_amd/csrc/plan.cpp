@@ -1061,6 +1061,29 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
   return res;
 }
 
+void Plan::groupInertia(int ngroups, const int* groupStart, int* inertia3, int* singular) {
+  if (!factorized_) throw std::logic_error("symla: no factorization");
+  if (ngroups <= 0 || !groupStart || !inertia3) throw std::invalid_argument("symla: bad group arguments");
+  if (groupStart[0] != 0 || groupStart[ngroups] != n_)
+    throw std::invalid_argument("symla: groups must cover the original indices 0..n-1");
+  for (int g = 0; g < ngroups; ++g)
+    if (groupStart[g + 1] < groupStart[g]) throw std::invalid_argument("symla: group boundaries must not decrease");
+  auto groupOf = [&](int orig) {
+    return static_cast<int>(std::upper_bound(groupStart, groupStart + ngroups + 1, orig) - groupStart) - 1;
+  };
+  std::vector<int> fi(3 * static_cast<size_t>(numSN_));
+  hipCheck(hipMemcpy(fi.data(), dinertia_.p, sizeof(int) * fi.size(), hipMemcpyDeviceToHost), "front inertia");
+  std::fill(inertia3, inertia3 + 3 * static_cast<size_t>(ngroups), 0);
+  for (int s = 0; s < numSN_; ++s) {
+    const int g = groupOf(perm_[hfirst_[s]]);
+    for (int q = 0; q < 3; ++q) inertia3[3 * g + q] += fi[3 * static_cast<size_t>(s) + q];
+  }
+  if (singular) {
+    std::fill(singular, singular + ngroups, 0);
+    for (int c : singularColsOrig_) ++singular[groupOf(c)];
+  }
+}
+
 void Plan::frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
                      int* nblocks) {
   if (!factorized_) throw std::logic_error("symla: no factorization");

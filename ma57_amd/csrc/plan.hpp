@@ -126,6 +126,10 @@ class Plan {
   bool factorized() const { return factorized_; }
   bool singular() const { return singular_; }
   const std::vector<int>& singularColsOrig() const { return singularColsOrig_; }
+  // Inertia and singular-column count per group of original indices
+  // [groupStart[g], groupStart[g+1]) (independent systems of a block-diagonal
+  // matrix: every front lies inside one of them).
+  void groupInertia(int ngroups, const int* groupStart, int* inertia3, int* singular);
   void frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
                  int* nblocks);
   double lastFactorMs() const { return factorMs_; }
