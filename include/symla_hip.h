@@ -154,6 +154,23 @@ int symla_solve_refined(symla_plan* p, const double* a_vals, const double* B, in
 int symla_solve_refined_device(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
                                double* dX, int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info);
 
+/* Recorded refactorize + solve (no counterpart in the reference; it serves
+ * callers that refactorize one pattern many times, e.g. inside a time or
+ * Newton loop).  symla_graph_capture records, as one hipGraph on the plan's
+ * stream, the threshold-pivoting factorization of the values at d_a_vals and
+ * the solve of dB into dX (all device pointers, kept by the plan).
+ * symla_graph_replay re-runs both on the current contents of those buffers
+ * and fills info; a replay whose factorization delays pivots is repaired on
+ * the regular path (info->repaired = 1) and solved again, and on a singular
+ * factorization dX is not meaningful (info->singular = 1).  The graph is
+ * re-recorded by itself when a buffer it refers to has moved (another solve
+ * size, a repair).  Static-regularized mode is not recorded (its retry loop
+ * decides on the host): SYMLA_ERR_INVALID. */
+int symla_graph_capture(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs, double* dX,
+                        int64_t ldx, const symla_factor_opts* opts);
+int symla_graph_replay(symla_plan* p, symla_factor_info* info);
+int symla_graph_release(symla_plan* p);
+
 /* NumericFactor::fronts (multifrontal.hpp:118-186), copied to the host on
  * request: one SupernodeFactor.  rows[m] (final-order rowIndices), L
  * (m x npiv column-major, ld m), d_diag/d_off[npiv] (BlockDiagonalD),

@@ -90,6 +90,10 @@ SIGNATURES = {
                                       C.POINTER(RefineOpts), C.POINTER(RefineInfo)]),
     "symla_solve_refined_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
                                              C.c_int64, C.POINTER(RefineOpts), C.POINTER(RefineInfo)]),
+    "symla_graph_capture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
+                                      C.POINTER(FactorOpts)]),
+    "symla_graph_replay": (C.c_int, [C.c_void_p, C.POINTER(FactorInfo)]),
+    "symla_graph_release": (C.c_int, [C.c_void_p]),
     "symla_front_count": (C.c_int, [C.c_void_p]),
     "symla_front_view": (C.c_int, [C.c_void_p, C.c_int, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i32p, i32p]),
     "symla_dense_factor": (C.c_int, [C.c_int, f64p, C.c_int, C.c_int, C.POINTER(DenseOpts), f64p, f64p, i32p,

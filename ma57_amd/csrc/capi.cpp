@@ -203,6 +203,30 @@ int symla_group_inertia(symla_plan* p, int ngroups, const int* group_start, int*
   });
 }
 
+int symla_graph_capture(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs, double* dX,
+                        int64_t ldx, const symla_factor_opts* opts) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_graph_capture: null plan");
+    p->plan->graphCapture(d_a_vals, dB, ldb, nrhs, dX, ldx, toFactorOptions(opts, p->plan->n()));
+  });
+}
+
+int symla_graph_replay(symla_plan* p, symla_factor_info* info) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_graph_replay: null plan");
+    FactorInfo fi;
+    p->plan->graphReplay(fi);
+    fillInfo(fi, info);
+  });
+}
+
+int symla_graph_release(symla_plan* p) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_graph_release: null plan");
+    p->plan->graphRelease();
+  });
+}
+
 int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx) {
   return planGuarded(p, [&] {
     if (!p) throw std::invalid_argument("symla_solve: null plan");

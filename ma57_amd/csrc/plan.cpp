@@ -155,6 +155,7 @@ Plan::Profile Plan::takeProfile() {
 }
 
 Plan::~Plan() {
+  graphRelease();
   for (auto& pc : prof_)
     for (hipEvent_t e : pc.ev) (void)hipEventDestroy(e);
   for (void* p : dynAllocs_) (void)hipFree(p);
@@ -1082,6 +1083,116 @@ void Plan::groupInertia(int ngroups, const int* groupStart, int* inertia3, int* 
     std::fill(singular, singular + ngroups, 0);
     for (int c : singularColsOrig_) ++singular[groupOf(c)];
   }
+}
+
+std::vector<const void*> Plan::graphFingerprint() const {
+  std::vector<const void*> v = {pool_.p,   dF_.p,      dm_.p,       dnE_.p,      dncols_.p,   dfirst_.p,
+                                dchildPtr_.p, dchildList_.p, dfrontIdx_.p, drelmap_.p, daRow_.p, didxOff_.p,
+                                dacolPtr_.p, daSrc_.p,  dperm_.p,    drowIdx_.p,  dpkind_.p,   dnF_.p,
+                                dinertia_.p, dkcur_.p,  dpcount_.p,  dstopped_.p, dnpert_.p,   ddDiag_.p,
+                                ddOff_.p,   dpert_.p,   dorigMax_.p, drunMax_.p,  dstatI_.p,   dstatD_.p,
+                                dws_.p,     dsmall_.p,  dtasks_.p,   dallFronts_.p, dlargeLists_.p,
+                                dlargeRowStart_.p, dlargeTileStart_.p, dwsOff_.p, dpermVec_.p, dBp_.p, dZ_.p,
+                                dX_.p,      dG_.p,      dDl_.p,      dgOff_.p,    dgatherStart_.p, dfwdStart_.p,
+                                dbwdStart_.p, dflagOff_.p, dflagsF_.p, dflagsB_.p, dgen_.p, dsolveErr_.p,
+                                dschur_.p};
+  for (void* f : hF_) v.push_back(f);
+  return v;
+}
+
+void Plan::graphRecord() {
+  if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graphExec_ = nullptr;
+  graph_ = nullptr;
+  const GraphArgs& g = graphArgs_;
+  if (g.opts.precision != prec_) {
+    prec_ = g.opts.precision;
+    layoutDirty_ = true;
+  }
+  if (layoutDirty_) {
+    buildStaticLayout();
+    uploadLayout();
+    layoutDirty_ = false;
+  }
+  ensureSolveWs(g.nrhs);
+  const bool prof = profiling_;
+  profiling_ = false;  // no event records inside the graph
+  FactorParams prm = params(g.opts, false, 0.0, nullptr);
+  hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    runStatic(g.vals, prm);
+    solveCore(g.B, g.ldb, g.nrhs, g.X, g.ldx);
+  } catch (...) {
+    hipGraph_t tmp = nullptr;
+    (void)hipStreamEndCapture(stream_, &tmp);
+    if (tmp) (void)hipGraphDestroy(tmp);
+    profiling_ = prof;
+    throw;
+  }
+  hipCheck(hipStreamEndCapture(stream_, &graph_), "end capture");
+  profiling_ = prof;
+  hipCheck(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "instantiate graph");
+  graphPrint_ = graphFingerprint();
+}
+
+void Plan::graphCapture(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                        const FactorOptions& opts) {
+  if (opts.mode != 0) throw std::invalid_argument("symla: graph capture covers threshold-pivoting factorizations");
+  if (!dVals || !dB || !dX) throw std::invalid_argument("symla: graph capture needs device buffers");
+  if (nrhs <= 0 || ldb < n_ || ldx < n_) throw std::invalid_argument("symla: bad right-hand-side layout");
+  if (opts.precision != 0 && opts.precision != 1) throw std::invalid_argument("symla: unknown precision");
+  graphArgs_ = GraphArgs{dVals, dB, ldb, nrhs, dX, ldx, opts};
+  graphRecord();
+}
+
+void Plan::graphRelease() {
+  if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graphExec_ = nullptr;
+  graph_ = nullptr;
+  graphPrint_.clear();
+}
+
+void Plan::graphReplay(FactorInfo& info) {
+  if (!graphExec_) throw std::logic_error("symla: no recorded graph (call symla_graph_capture first)");
+  factorized_ = false;
+  // a repair pass or another solve size may have moved what the graph refers to
+  if (layoutDirty_ || graphArgs_.opts.precision != prec_ || graphFingerprint() != graphPrint_) {
+    graphRecord();
+  } else {
+    ensureSolveWs(graphArgs_.nrhs);
+    if (graphFingerprint() != graphPrint_) graphRecord();
+  }
+  info = FactorInfo{};
+  singular_ = false;
+  singularColsOrig_.clear();
+  hipCheck(hipEventRecord(ev0_, stream_), "event");
+  hipCheck(hipGraphLaunch(graphExec_, stream_), "graph launch");
+  bool anyDelay = false;
+  collectStats(info, anyDelay);
+  checkSolveError();
+  if (anyDelay) {
+    // the recorded schedule assumed no delays: repair and solve again
+    FactorParams prm = params(graphArgs_.opts, false, 0.0, nullptr);
+    runDynamic(graphArgs_.vals, prm);
+    collectStats(info, anyDelay);
+    info.repaired = true;
+    info.singular = singular_;
+    info.nSingular = static_cast<int>(singularColsOrig_.size());
+    if (!singular_) {
+      solveCore(graphArgs_.B, graphArgs_.ldb, graphArgs_.nrhs, graphArgs_.X, graphArgs_.ldx);
+      hipCheck(hipStreamSynchronize(stream_), "repaired solve");
+      checkSolveError();
+    }
+  }
+  hipCheck(hipEventRecord(ev1_, stream_), "event");
+  hipCheck(hipEventSynchronize(ev1_), "event");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  factorMs_ = ms;
+  solveMs_ = 0.0;
+  factorized_ = true;
 }
 
 void Plan::frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,

@@ -130,6 +130,17 @@ class Plan {
   // [groupStart[g], groupStart[g+1]) (independent systems of a block-diagonal
   // matrix: every front lies inside one of them).
   void groupInertia(int ngroups, const int* groupStart, int* inertia3, int* singular);
+
+  // hipGraph of one factorization (threshold pivoting, static schedule) and
+  // one solve on fixed device buffers.  Replay re-runs both on the current
+  // buffer contents; a replay whose factorization delayed pivots repairs it
+  // on the regular path and solves again.  The graph is re-recorded when a
+  // buffer it refers to has moved.
+  void graphCapture(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                    const FactorOptions& opts);
+  void graphReplay(FactorInfo& info);
+  void graphRelease();
+  bool hasGraph() const { return graphExec_ != nullptr; }
   void frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
                  int* nblocks);
   double lastFactorMs() const { return factorMs_; }
@@ -231,6 +242,21 @@ class Plan {
   size_t elemSize() const { return prec_ ? sizeof(float) : sizeof(double); }
   double factorMs_ = 0.0, solveMs_ = 0.0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  // recorded refactorize + solve
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graphExec_ = nullptr;
+  struct GraphArgs {
+    const double* vals = nullptr;
+    const double* B = nullptr;
+    int64_t ldb = 0;
+    int nrhs = 0;
+    double* X = nullptr;
+    int64_t ldx = 0;
+    FactorOptions opts;
+  } graphArgs_;
+  std::vector<const void*> graphPrint_;
+  std::vector<const void*> graphFingerprint() const;
+  void graphRecord();
   bool profiling_ = false;
   struct ProfEvents {
     std::vector<hipEvent_t> ev;

@@ -391,6 +391,32 @@ class SymLDLT:
                                 ri_.backward_error)
         return RefinedSolve(X.reshape(-1) if vec else X, diag)
 
+    # --- recorded refactorize + solve (no reference counterpart) ---
+    def captureGraph(self, d_vals: int, d_B: int, nrhs: int, d_X: int):
+        """Record factorize(values at device address d_vals) + solve(d_B -> d_X), n x nrhs column-major."""
+        if not self._pattern_analyzed:
+            raise LogicError("symla::SymLDLT::factorize: analyzePattern must be called first")
+        o = self._opts()
+        rc = self._lib.symla_graph_capture(self._plan, C.c_void_p(d_vals), C.c_void_p(d_B), self._n, nrhs,
+                                           C.c_void_p(d_X), self._n, C.byref(o))
+        _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
+
+    def replayGraph(self):
+        """Re-run the recorded factorize + solve on the current buffer contents; returns the factor info."""
+        info = _lib.FactorInfo()
+        self._factorized = False
+        rc = self._lib.symla_graph_replay(self._plan, C.byref(info))
+        _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
+        self.last_info = info
+        self._inertia = Inertia(info.n_pos, info.n_neg, info.n_zero)
+        self._singular = bool(info.singular)
+        self._factorized = True
+        return info
+
+    def releaseGraph(self):
+        rc = self._lib.symla_graph_release(self._plan)
+        _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
+
     def inertia(self) -> Inertia:
         return self._inertia
 
