@@ -197,6 +197,29 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
                        double* d_off, int* perm, int* pivot_start, int* pivot_kind, int* n_pivot_blocks,
                        int* n_factored, int* inertia3, double* total_perturbation, int* num_perturbed);
 
+/* Scheduling choices of a plan (no counterpart in the reference, whose
+ * front kernel is one CPU routine): fronts of order <= small_front_max
+ * (at most 192) are factored whole in LDS, larger ones by the panel path;
+ * assembly runs in column tiles of assemble_tile (a power of two in
+ * [16, 1024]).  symla_plan_autotune times factorize + solve of the values at
+ * d_a_vals (device) with nrhs fixed pseudo-random right-hand sides, best of
+ * reps, over a grid of choices in the given precision, keeps the fastest
+ * (returned in *chosen) and reports every trial (up to cap records,
+ * *ntrials = grid size).  The plan then holds a factorization of those
+ * values. */
+typedef struct {
+  int small_front_max;
+  int assemble_tile;
+} symla_tuning;
+typedef struct {
+  symla_tuning tuning;
+  double ms;
+} symla_tuning_trial;
+int symla_plan_get_tuning(const symla_plan* p, symla_tuning* out);
+int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t);
+int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                        symla_tuning* chosen, symla_tuning_trial* trials, int cap, int* ntrials);
+
 /* Device-side timing of the last factorize / solve (milliseconds, HIP
  * events on the plan's stream). */
 int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms);

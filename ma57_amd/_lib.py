@@ -62,6 +62,14 @@ class DenseOpts(C.Structure):
                 ("expected_sign", i32p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("small_front_max", C.c_int), ("assemble_tile", C.c_int)]
+
+
+class TuningTrial(C.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", C.c_double)]
+
+
 class Profile(C.Structure):
     _fields_ = [("syrk_ms", C.c_double), ("syrk_launches", C.c_longlong), ("schur_flops", C.c_double),
                 ("panel_ms", C.c_double), ("panel_launches", C.c_longlong), ("panel_rows_ms", C.c_double),
@@ -94,6 +102,10 @@ SIGNATURES = {
                                       C.POINTER(FactorOpts)]),
     "symla_graph_replay": (C.c_int, [C.c_void_p, C.POINTER(FactorInfo)]),
     "symla_graph_release": (C.c_int, [C.c_void_p]),
+    "symla_plan_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "symla_plan_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "symla_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
+                                      C.POINTER(TuningTrial), C.c_int, i32p]),
     "symla_front_count": (C.c_int, [C.c_void_p]),
     "symla_front_view": (C.c_int, [C.c_void_p, C.c_int, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i32p, i32p]),
     "symla_dense_factor": (C.c_int, [C.c_int, f64p, C.c_int, C.c_int, C.POINTER(DenseOpts), f64p, f64p, i32p,

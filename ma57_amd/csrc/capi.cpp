@@ -227,6 +227,38 @@ int symla_graph_release(symla_plan* p) {
   });
 }
 
+int symla_plan_get_tuning(const symla_plan* p, symla_tuning* out) {
+  if (!p || !out) return fail(SYMLA_ERR_INVALID, "symla_plan_get_tuning: null argument");
+  out->small_front_max = p->plan->tuning().smallMax;
+  out->assemble_tile = p->plan->tuning().asmTile;
+  return SYMLA_OK;
+}
+
+int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t) {
+  return planGuarded(p, [&] {
+    if (!p || !t) throw std::invalid_argument("symla_plan_set_tuning: null argument");
+    p->plan->setTuning(Plan::Tuning{t->small_front_max, t->assemble_tile});
+  });
+}
+
+int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                        symla_tuning* chosen, symla_tuning_trial* trials, int cap, int* ntrials) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_plan_autotune: null plan");
+    const auto tr = p->plan->autotune(d_a_vals, nrhs, precision, reps);
+    if (chosen) {
+      chosen->small_front_max = p->plan->tuning().smallMax;
+      chosen->assemble_tile = p->plan->tuning().asmTile;
+    }
+    if (ntrials) *ntrials = static_cast<int>(tr.size());
+    for (int i = 0; trials && i < cap && i < static_cast<int>(tr.size()); ++i) {
+      trials[i].tuning.small_front_max = tr[i].t.smallMax;
+      trials[i].tuning.assemble_tile = tr[i].t.asmTile;
+      trials[i].ms = tr[i].ms;
+    }
+  });
+}
+
 int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx) {
   return planGuarded(p, [&] {
     if (!p) throw std::invalid_argument("symla_solve: null plan");

@@ -10,6 +10,8 @@
 // actually receives (multifrontal.hpp:453-479).
 #include "plan.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -328,8 +330,8 @@ void Plan::buildSchedule() {
     levelFrontOff_[L] = static_cast<int64_t>(all.size());
     for (int s : levelFronts_[L]) {
       all.push_back(s);
-      for (int c0 = 0; c0 < hm_[s]; c0 += 64) tasks.push_back(make_int2(s, c0));
-      if (hm_[s] <= kSmallFrontMax) {
+      for (int c0 = 0; c0 < hm_[s]; c0 += tune_.asmTile) tasks.push_back(make_int2(s, c0));
+      if (hm_[s] <= tune_.smallMax) {
         small.push_back(s);
         ls.smallMaxM = std::max(ls.smallMaxM, hm_[s]);
       } else {
@@ -519,7 +521,7 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
   for (size_t L = 0; L < sched_.size(); ++L) {
     const auto& ls = sched_[L];
     profBegin(kProfAssemble);
-    launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, 64, fd, aValsDev, 1, stream_);
+    launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, tune_.asmTile, fd, aValsDev, 1, stream_);
     profEnd(kProfAssemble);
     if (ls.smallCount > 0) {
       profBegin(kProfSmall);
@@ -662,7 +664,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     for (int s : fronts) {
       all.push_back(s);
       tasks.push_back(make_int2(s, 0));
-      if (hm_[s] <= kSmallFrontMax) {
+      if (hm_[s] <= tune_.smallMax) {
         small.push_back(s);
         maxM = std::max(maxM, hm_[s]);
       } else {
@@ -1193,6 +1195,62 @@ void Plan::graphReplay(FactorInfo& info) {
   factorMs_ = ms;
   solveMs_ = 0.0;
   factorized_ = true;
+}
+
+void Plan::setTuning(const Tuning& t) {
+  if (t.smallMax < 0 || t.smallMax > kSmallFrontMax)
+    throw std::invalid_argument("symla: small-front bound must lie in [0, 192]");
+  if (t.asmTile < 16 || t.asmTile > 1024 || (t.asmTile & (t.asmTile - 1)) != 0)
+    throw std::invalid_argument("symla: assembly tile must be a power of two in [16, 1024]");
+  tune_ = t;
+  buildSchedule();
+  graphPrint_.clear();  // a recorded graph holds the old schedule: re-record on replay
+}
+
+std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int precision, int reps) {
+  if (!dVals) throw std::invalid_argument("symla: autotune needs device values");
+  if (nrhs <= 0 || reps <= 0) throw std::invalid_argument("symla: autotune needs nrhs > 0 and reps > 0");
+  const Tuning keep = tune_;
+  std::vector<double> hb(static_cast<size_t>(n_) * nrhs);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (double& v : hb) {  // fixed pseudo-random right-hand sides in [-1, 1)
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    v = static_cast<double>(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+  DBuf<double> dB, dXt;
+  dB.alloc(hb.size());
+  dXt.alloc(hb.size());
+  hipCheck(hipMemcpy(dB.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice), "autotune rhs");
+  FactorOptions o;
+  o.precision = precision;
+  std::vector<TuneTrial> trials;
+  const int smalls[] = {64, 96, 128, 160, 192};
+  const int tiles[] = {32, 64, 128};
+  for (int sm : smalls)
+    for (int tw : tiles) {
+      setTuning(Tuning{sm, tw});
+      FactorInfo info;
+      factorize(nullptr, dVals, o, info);  // warm-up: workspaces, kernels
+      solve(dB.p, n_, nrhs, dXt.p, n_, true);
+      double best = 1e300;
+      for (int r = 0; r < reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        factorize(nullptr, dVals, o, info);
+        solve(dB.p, n_, nrhs, dXt.p, n_, true);
+        const auto t1 = std::chrono::steady_clock::now();
+        best = std::min(best, std::chrono::duration<double, std::milli>(t1 - t0).count());
+      }
+      trials.push_back(TuneTrial{tune_, best});
+    }
+  Tuning bestT = keep;
+  double bestMs = 1e300;
+  for (const auto& tr : trials)
+    if (tr.ms < bestMs) {
+      bestMs = tr.ms;
+      bestT = tr.t;
+    }
+  setTuning(bestT);
+  return trials;
 }
 
 void Plan::frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,

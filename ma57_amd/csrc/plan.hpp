@@ -141,6 +141,22 @@ class Plan {
   void graphReplay(FactorInfo& info);
   void graphRelease();
   bool hasGraph() const { return graphExec_ != nullptr; }
+
+  // Scheduling choices: fronts of order <= smallMax take the LDS kernel,
+  // larger ones the panel path; assembly runs in column tiles of asmTile.
+  struct Tuning {
+    int smallMax = kSmallFrontMax;
+    int asmTile = 64;
+  };
+  struct TuneTrial {
+    Tuning t;
+    double ms = 0.0;
+  };
+  const Tuning& tuning() const { return tune_; }
+  void setTuning(const Tuning& t);
+  // Times factorize + solve (values at dVals, nrhs random right-hand sides)
+  // for every candidate tuning, keeps the fastest and returns all trials.
+  std::vector<TuneTrial> autotune(const double* dVals, int nrhs, int precision, int reps);
   void frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
                  int* nblocks);
   double lastFactorMs() const { return factorMs_; }
@@ -255,6 +271,7 @@ class Plan {
     FactorOptions opts;
   } graphArgs_;
   std::vector<const void*> graphPrint_;
+  Tuning tune_;
   std::vector<const void*> graphFingerprint() const;
   void graphRecord();
   bool profiling_ = false;

@@ -391,6 +391,34 @@ class SymLDLT:
                                 ri_.backward_error)
         return RefinedSolve(X.reshape(-1) if vec else X, diag)
 
+    # --- scheduling choices (no reference counterpart) ---
+    def tuning(self):
+        t = _lib.Tuning()
+        rc = self._lib.symla_plan_get_tuning(self._plan, C.byref(t))
+        _raise(rc, "symla_plan_get_tuning failed")
+        return (t.small_front_max, t.assemble_tile)
+
+    def setTuning(self, small_front_max: int, assemble_tile: int):
+        t = _lib.Tuning(small_front_max, assemble_tile)
+        rc = self._lib.symla_plan_set_tuning(self._plan, C.byref(t))
+        _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
+
+    def autotune(self, d_vals: int, nrhs: int = 1, reps: int = 2):
+        """Time factorize + solve over the tuning grid on device values; keep the fastest.  Returns
+        (chosen (small_front_max, assemble_tile), [((small_front_max, assemble_tile), ms), ...])."""
+        if not self._pattern_analyzed:
+            raise LogicError("symla::SymLDLT::factorize: analyzePattern must be called first")
+        chosen = _lib.Tuning()
+        trials = (_lib.TuningTrial * 64)()
+        nt = C.c_int(0)
+        rc = self._lib.symla_plan_autotune(self._plan, C.c_void_p(d_vals), nrhs, self.precision, reps,
+                                           C.byref(chosen), trials, 64, C.byref(nt))
+        _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
+        self._factorized = False  # the trials factored the tuning values: factorize() before solving
+        return ((chosen.small_front_max, chosen.assemble_tile),
+                [((trials[i].tuning.small_front_max, trials[i].tuning.assemble_tile), trials[i].ms)
+                 for i in range(min(nt.value, 64))])
+
     # --- recorded refactorize + solve (no reference counterpart) ---
     def captureGraph(self, d_vals: int, d_B: int, nrhs: int, d_X: int):
         """Record factorize(values at device address d_vals) + solve(d_B -> d_X), n x nrhs column-major."""
