@@ -711,20 +711,24 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       }
       const int pq = haveNext ? p - 1 : p;
       int q = 0;
-      for (; q + 4 <= pq; q += 4) {
-        const double w0 = Wrow[kr][q], w1 = Wrow[kr][q + 1], w2 = Wrow[kr][q + 2], w3 = Wrow[kr][q + 3];
+      // U multipliers per row in flight: the loads of a group are issued
+      // together, so a pivot costs about p / U memory latencies
+      constexpr int U = RM <= 2 ? 16 : (RM <= 4 ? 8 : (RM <= 8 ? 4 : (RM <= 16 ? 2 : 1)));
+      for (; q + U <= pq; q += U) {
         const T* L0 = F + static_cast<size_t>(k0 + q) * m;
+        double lv[RM > 0 ? RM : 1][U];
 #pragma unroll
         for (int j = 0; j < RM; ++j) {
           const int i = k0 + tid + j * NT;
-          if (i >= k && i < nE) {
-            double c = cr[j];
-            c = fma(-static_cast<double>(L0[i]), w0, c);
-            c = fma(-static_cast<double>(L0[i + m]), w1, c);
-            c = fma(-static_cast<double>(L0[i + 2 * static_cast<size_t>(m)]), w2, c);
-            c = fma(-static_cast<double>(L0[i + 3 * static_cast<size_t>(m)]), w3, c);
-            cr[j] = c;
-          }
+          const bool act = i >= k && i < nE;
+#pragma unroll
+          for (int u = 0; u < U; ++u) lv[j][u] = act ? static_cast<double>(L0[i + u * static_cast<size_t>(m)]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const double w = Wrow[kr][q + u];
+#pragma unroll
+          for (int j = 0; j < RM; ++j) cr[j] = fma(-lv[j][u], w, cr[j]);
         }
       }
       for (; q < pq; ++q) {

@@ -62,6 +62,16 @@ __device__ inline bool flagReached(const unsigned* f, unsigned gen) {
   return static_cast<int>(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - gen) >= 0;
 }
 
+// Values handed between workgroups of one launch (the solved rows of a
+// block) are written and read with device-coherent accesses, so publishing a
+// block needs no cache write-back and waiting for one no cache invalidation.
+__device__ inline void storeShared(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double loadShared(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Wait until another workgroup has published block f of this solve.  Thread 0
 // polls with a bounded count; past the bound it raises the error word and
 // stops waiting, so every workgroup still runs to its end.
@@ -78,16 +88,14 @@ __device__ inline void waitBlock(const unsigned* f, unsigned gen, int* err) {
     }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Publish this workgroup's block: all of its stores happen before the flag.
+// Publish this workgroup's block: every wave waits until its device-coherent
+// stores have completed, then one flag store releases the block.
 __device__ inline void publishBlock(unsigned* f, unsigned gen) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(f, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) __hip_atomic_store(f, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // L(row, col) of a front with the (s+1, s) slot of a 2x2 pivot block read as
@@ -213,7 +221,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int qq = e % SB, c = e / SB;
-        ys[qq][c] = (qq < cn && c < gw) ? Ds[(c0 + qq) + static_cast<int64_t>(rg + c) * m] : 0.0;
+        ys[qq][c] = (qq < cn && c < gw) ? loadShared(Ds + (c0 + qq) + static_cast<int64_t>(rg + c) * m) : 0.0;
       }
       __syncthreads();
       for (int qq = 0; qq < cn; ++qq) {
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
         }
         if (row < nr) {
           ys[row][c] = y;
-          Ds[(r0 + row) + static_cast<int64_t>(rg + c) * m] = y;
+          storeShared(Ds + (r0 + row) + static_cast<int64_t>(rg + c) * m, y);
         }
       }
     }
@@ -288,7 +296,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
         // second row of a 2x2 block that starts in the previous block
         const double d11 = fd.dDiag[io + p - 1], d21 = fd.dOff[io + p - 1], d22 = fd.dDiag[io + p];
         const double det = d11 * d22 - d21 * d21;
-        const double y0 = Ds[(p - 1) + cc * m], y1 = ys[0][c];
+        const double y0 = loadShared(Ds + (p - 1) + cc * m), y1 = ys[0][c];
         Z[fd.rowIdx[io + p - 1] + cc * n] = (d22 * y0 - d21 * y1) / det;
         Z[fd.rowIdx[io + p] + cc * n] = (-d21 * y0 + d11 * y1) / det;
       }
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int i = e % SB, c = e / SB;
-        xs[i][c] = (i < in && c < gw) ? Ds[(i0 + i) + static_cast<int64_t>(rg + c) * m] : 0.0;
+        xs[i][c] = (i < in && c < gw) ? loadShared(Ds + (i0 + i) + static_cast<int64_t>(rg + c) * m) : 0.0;
       }
       __syncthreads();
       for (int i = 0; i < in; ++i) {
@@ -409,7 +417,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
           if (row < j) x = fma(-lcol[j], xj, x);
         }
         if (row < nr) {
-          Ds[(r0 + row) + static_cast<int64_t>(rg + c) * m] = x;
+          storeShared(Ds + (r0 + row) + static_cast<int64_t>(rg + c) * m, x);
           X[prow + static_cast<int64_t>(rg + c) * n] = x;
         }
       }
