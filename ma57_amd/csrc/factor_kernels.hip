@@ -526,24 +526,34 @@ __global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count
 }
 
 // Panel pivoting on the eligible rows [k, nE) of one front.  Columns not yet
-// pivoted are never modified in place: the current value of a candidate
-// column is recomputed from its stored (pre-panel) values minus the
-// correction of this panel's pivots, like the reference's setColumnRaw
+// pivoted are never modified in place in global memory: the current value of
+// a candidate column is its stored (pre-panel) value minus the correction of
+// this panel's pivots, like the reference's setColumnRaw
 // (dense_kernel.hpp:516-545).  Accepted pivots write their multipliers for
 // the eligible rows; the remaining rows are solved by k_panel_rows.
 //
-// Wrow[r][q] = (D L^T)(k0+q, k0+r) holds the correction weights of the
-// panel's first rows, so the weights of the next pivot row are an LDS
-// broadcast.  When every eligible row fits RM slots per thread (RM > 0),
-// each thread keeps its rows of the current column in registers and a
-// pivot accepted by the first Bunch-Kaufman test (every static pivot) costs
-// one barrier; any other decision falls back to the general code, which
-// works on column buffers in global memory.
+// When every eligible row fits RM slots per thread (RM > 0), the next SW
+// columns from the current pivot (a sub-panel) are loaded into LDS with the
+// correction of the panel's earlier pivots applied once, then factored
+// right-looking inside LDS: a pivot accepted by the first Bunch-Kaufman test
+// (every static pivot) costs two barriers and no global reads.  Any other
+// decision runs the general code, which works on column buffers in global
+// memory, and the sub-panel is reloaded after it.
+template <int RM>
+struct PanelShape {
+  static constexpr int SW = RM <= 4 ? 8 : (RM <= 8 ? 4 : (RM <= 16 ? 2 : 1));  // sub-panel width
+  static constexpr int ROWS = RM > 0 ? RM * kPanelThreads : 0;
+  static constexpr size_t lds() { return sizeof(double) * static_cast<size_t>(ROWS) * SW; }
+};
+
 template <int RM, typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int count, DevFronts fd, FactorParams prm,
                                                           double* wsBase, const int64_t* wsOff) {
   constexpr int NT = kPanelThreads;
-  __shared__ double Wrow[kPanel][kPanel + 1];
+  constexpr int SW = PanelShape<RM>::SW;
+  constexpr int CSR = PanelShape<RM>::ROWS;
+  extern __shared__ double Cs[];  // [SW][CSR]: sub-panel column j, row k0 + r at Cs[j * CSR + r]
+  __shared__ double Wsub[SW][kPanel + 2];
   __shared__ double wvec[kPanel + 2];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
@@ -551,7 +561,6 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   __shared__ int redI[NT / 64 + 1];
   __shared__ double fastV[2][NT / 64];
   __shared__ int fastI[2][NT / 64];
-  __shared__ double fastDiag[2], fastNext[2];
 
   const int tid = threadIdx.x;
   if (static_cast<int>(blockIdx.x) >= count) return;
@@ -589,38 +598,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   int k = k0, p = 0;
   bool stop = false;
 
-  // weights of this panel's p pivots for any row (general path)
-  auto computeW = [&](int row) {
-    for (int q = tid; q < p; q += NT) {
-      const int kind = pkd[q];
-      double w;
-      if (kind == 1) {
-        w = pdd[q] * Fat(row, k0 + q);
-      } else if (kind == 2) {
-        w = pdd[q] * Fat(row, k0 + q) + pdo[q] * Fat(row, k0 + q + 1);
-      } else {
-        w = pdo[q - 1] * Fat(row, k0 + q - 1) + pdd[q] * Fat(row, k0 + q);
-      }
-      wvec[q] = w;
-    }
-    __syncthreads();
+  // (D L^T)(q, row) for this panel's pivot q and a given row
+  auto weight = [&](int q, int row) -> double {
+    const int kind = pkd[q];
+    if (kind == 1) return pdd[q] * static_cast<double>(Fat(row, k0 + q));
+    if (kind == 2) return pdd[q] * static_cast<double>(Fat(row, k0 + q)) + pdo[q] * static_cast<double>(Fat(row, k0 + q + 1));
+    return pdo[q - 1] * static_cast<double>(Fat(row, k0 + q - 1)) + pdd[q] * static_cast<double>(Fat(row, k0 + q));
   };
-  // Wrow from the stored multipliers (after a general-path step)
-  auto rebuildWrow = [&]() {
-    for (int e = tid; e < kPanel * p; e += NT) {
-      const int r = e % kPanel, q = e / kPanel;
-      const int row = k0 + r;
-      if (row >= nE) continue;
-      const int kind = pkd[q];
-      double w;
-      if (kind == 1)
-        w = pdd[q] * Fat(row, k0 + q);
-      else if (kind == 2)
-        w = pdd[q] * Fat(row, k0 + q) + pdo[q] * Fat(row, k0 + q + 1);
-      else
-        w = pdo[q - 1] * Fat(row, k0 + q - 1) + pdd[q] * Fat(row, k0 + q);
-      Wrow[r][q] = w;
-    }
+  // weights of this panel's p pivots for one row (general path)
+  auto computeW = [&](int row) {
+    for (int q = tid; q < p; q += NT) wvec[q] = weight(q, row);
     __syncthreads();
   };
   // out[i] = stored column `col` (rows >= k) minus the panel correction
@@ -690,73 +677,66 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     ++p;
   };
 
-  // weight of the newest pivot for the next pivot row, known to every
-  // thread without a barrier (the row's current value in the pivot column)
-  double wNext = 0.0;
-  bool haveNext = false;
+  // Sub-panel [c0, c0 + sw): current values of the eligible rows >= k.
+  int c0 = k, sw = 0;
+  auto loadSub = [&]() {
+    c0 = k;
+    sw = min(SW, nbEnd - k);
+    for (int e = tid; e < sw * p; e += NT) {
+      const int j = e % sw, q = e / sw;
+      Wsub[j][q] = weight(q, c0 + j);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < (RM > 0 ? RM : 1); ++jj) {
+      const int i = k0 + tid + jj * NT;
+      if (i < k || i >= nE) continue;
+      double acc[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) {
+        const int col = c0 + j;
+        acc[j] = (j < sw) ? static_cast<double>(i >= col ? Fat(i, col) : Fat(col, i)) : 0.0;
+      }
+      constexpr int U = 8;
+      int q = 0;
+      for (; q + U <= p; q += U) {
+        double lv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) lv[u] = static_cast<double>(Fat(i, k0 + q + u));
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < SW; ++j) acc[j] = fma(-lv[u], Wsub[j][q + u], acc[j]);
+      }
+      for (; q < p; ++q) {
+        const double l = Fat(i, k0 + q);
+#pragma unroll
+        for (int j = 0; j < SW; ++j) acc[j] = fma(-l, Wsub[j][q], acc[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < SW; ++j)
+        if (j < sw) Cs[j * CSR + (i - k0)] = acc[j];
+    }
+    __syncthreads();
+  };
+
   int parity = 0;
+  if (RM > 0) loadSub();
 
   while (k < nbEnd) {
     double lv = 0.0;
     int li = -1;
     double akk;
-    double nextv = 0.0;
-    double cr[RM > 0 ? RM : 1];
+    const int kk = k - c0;
     if (RM > 0) {
-      const int kr = k - k0;
+      const double* col = Cs + kk * CSR;
 #pragma unroll
-      for (int j = 0; j < RM; ++j) {
-        const int i = k0 + tid + j * NT;
-        cr[j] = (i >= k && i < nE) ? Fat(i, k) : 0.0;
-      }
-      const int pq = haveNext ? p - 1 : p;
-      int q = 0;
-      // U multipliers per row in flight: the loads of a group are issued
-      // together, so a pivot costs about p / U memory latencies
-      constexpr int U = RM <= 2 ? 16 : (RM <= 4 ? 8 : (RM <= 8 ? 4 : (RM <= 16 ? 2 : 1)));
-      for (; q + U <= pq; q += U) {
-        const T* L0 = F + static_cast<size_t>(k0 + q) * m;
-        double lv[RM > 0 ? RM : 1][U];
-#pragma unroll
-        for (int j = 0; j < RM; ++j) {
-          const int i = k0 + tid + j * NT;
-          const bool act = i >= k && i < nE;
-#pragma unroll
-          for (int u = 0; u < U; ++u) lv[j][u] = act ? static_cast<double>(L0[i + u * static_cast<size_t>(m)]) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const double w = Wrow[kr][q + u];
-#pragma unroll
-          for (int j = 0; j < RM; ++j) cr[j] = fma(-lv[j][u], w, cr[j]);
-        }
-      }
-      for (; q < pq; ++q) {
-        const double w = Wrow[kr][q];
-        const T* Lq = F + static_cast<size_t>(k0 + q) * m;
-#pragma unroll
-        for (int j = 0; j < RM; ++j) {
-          const int i = k0 + tid + j * NT;
-          if (i >= k && i < nE) cr[j] = fma(-static_cast<double>(Lq[i]), w, cr[j]);
-        }
-      }
-      if (haveNext) {
-        const T* Lq = F + static_cast<size_t>(k0 + p - 1) * m;
-#pragma unroll
-        for (int j = 0; j < RM; ++j) {
-          const int i = k0 + tid + j * NT;
-          if (i >= k && i < nE) cr[j] = fma(-static_cast<double>(Lq[i]), wNext, cr[j]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < RM; ++j) {
-        const int i = k0 + tid + j * NT;
+      for (int jj = 0; jj < (RM > 0 ? RM : 1); ++jj) {
+        const int i = k0 + tid + jj * NT;
         if (i > k && i < nE) {
-          const double v = fabs(cr[j]);
+          const double v = fabs(col[i - k0]);
           if (v > 0.0) argmaxCombine(lv, li, v, i);
         }
-        if (i == k) fastDiag[parity] = cr[j];
-        if (i == k + 1) fastNext[parity] = cr[j];
       }
       for (int off = 32; off > 0; off >>= 1) {
         const double v2 = __shfl_xor(lv, off, 64);
@@ -771,9 +751,8 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       lv = 0.0;
       li = -1;
       for (int w = 0; w < NT / 64; ++w) argmaxCombine(lv, li, fastV[parity][w], fastI[parity][w]);
-      akk = fastDiag[parity];
-      nextv = (k + 1 < nE) ? fastNext[parity] : 0.0;
       parity ^= 1;
+      akk = col[k - k0];
     } else {
       computeW(k);
       currentColumn(k, curK);
@@ -812,6 +791,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     }
     if (action == 1 && RM > 0) {
       const double d = dpiv;
+      const double* col = Cs + kk * CSR;
+      // (D L^T)(k, c0 + j) for the sub-panel columns right of k, as the
+      // general path would form it from the stored multiplier
+      double wj[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j)
+        wj[j] = (j > kk && j < sw) ? d * static_cast<double>(static_cast<T>(col[c0 + j - k0] / d)) : 0.0;
       pdd[p] = d;  // every thread stores the same value
       pdo[p] = 0.0;
       pkd[p] = 1;
@@ -823,37 +809,35 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       }
       countInertia(d, npos, nneg, nzero);
       tmax = fmax(tmax, fabs(d));
-      const int kq = k - k0;
 #pragma unroll
-      for (int j = 0; j < RM; ++j) {
-        const int i = k0 + tid + j * NT;
+      for (int jj = 0; jj < (RM > 0 ? RM : 1); ++jj) {
+        const int i = k0 + tid + jj * NT;
         if (i > k && i < nE) {
-          const double c = cr[j];
+          const double c = col[i - k0];
           tmax = fmax(tmax, fabs(c));
-          Fat(i, k) = static_cast<T>(c / d);
-          // (D L^T)(k, i) = d * l_ik = c
-          if (i - k0 < kPanel) Wrow[i - k0][kq] = c;
+          const T lt = static_cast<T>(c / d);
+          Fat(i, k) = lt;
+          const double l = lt;
+#pragma unroll
+          for (int j = 0; j < SW; ++j)
+            if (j > kk && j < sw) Cs[j * CSR + (i - k0)] = fma(-l, wj[j], Cs[j * CSR + (i - k0)]);
         }
       }
-      wNext = nextv;
-      haveNext = true;
+      __syncthreads();
       ++k;
       ++p;
+      if (k - c0 >= sw && k < nbEnd) loadSub();
       continue;
     }
     if (RM > 0) {
-      // general path below works on the column buffers
-#pragma unroll
-      for (int j = 0; j < RM; ++j) {
-        const int i = k0 + tid + j * NT;
-        if (i >= k && i < nE) curK[i] = cr[j];
-      }
+      // the general path below works on the column buffers
+      const double* col = Cs + kk * CSR;
+      for (int i = k + tid; i < nE; i += NT) curK[i] = col[i - k0];
       __syncthreads();
-      haveNext = false;
     }
     if (action == 1) {
       accept1(curK, dpiv);
-      if (RM > 0) rebuildWrow();
+      if (RM > 0 && k < nbEnd) loadSub();
       continue;
     }
     const int r = li;
@@ -875,13 +859,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       action = 3;
     if (action == 1) {
       accept1(curK, curK[k]);
-      if (RM > 0) rebuildWrow();
+      if (RM > 0 && k < nbEnd) loadSub();
       continue;
     }
     if (action == 2) {
       swapSym(k, r);
       accept1(curR, curR[k]);
-      if (RM > 0) rebuildWrow();
+      if (RM > 0 && k < nbEnd) loadSub();
       continue;
     }
     // 2x2 at (k, r): bring r to k+1
@@ -924,7 +908,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
     __syncthreads();
     k += 2;
     p += 2;
-    if (RM > 0) rebuildWrow();
+    if (RM > 0 && k < nbEnd) loadSub();
   }
   __syncthreads();
 
@@ -1313,25 +1297,37 @@ void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd
   else
     k_front_maxabs<double><<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
 }
+template <int RM, typename T>
+static void panelLaunch(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
+                        const int64_t* wsOff, hipStream_t st) {
+  const size_t lds = PanelShape<RM>::lds();
+  static bool attrSet = false;
+  if (!attrSet) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_panel<RM, T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attrSet = true;
+  }
+  k_panel<RM, T><<<count, kPanelThreads, lds, st>>>(list, count, fd, prm, ws, wsOff);
+}
 template <typename T>
 static void panelT(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                    const int64_t* wsOff, hipStream_t st) {
   // register slots per thread for the eligible rows of the widest front
   const int slots = (maxRows + kPanelThreads - 1) / kPanelThreads;
   if (slots <= 1)
-    k_panel<1, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<1, T>(list, count, fd, prm, ws, wsOff, st);
   else if (slots <= 2)
-    k_panel<2, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<2, T>(list, count, fd, prm, ws, wsOff, st);
   else if (slots <= 4)
-    k_panel<4, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<4, T>(list, count, fd, prm, ws, wsOff, st);
   else if (slots <= 8)
-    k_panel<8, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<8, T>(list, count, fd, prm, ws, wsOff, st);
   else if (slots <= 16)
-    k_panel<16, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<16, T>(list, count, fd, prm, ws, wsOff, st);
   else if (slots <= 32)
-    k_panel<32, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<32, T>(list, count, fd, prm, ws, wsOff, st);
   else
-    k_panel<0, T><<<count, kPanelThreads, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    panelLaunch<0, T>(list, count, fd, prm, ws, wsOff, st);
 }
 void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                  const int64_t* wsOff, hipStream_t st) {
