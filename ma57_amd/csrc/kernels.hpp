@@ -39,6 +39,12 @@ void launchFwdSweep(const int* list, int count, const int* blockStart, int block
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st);
+// fronts of at most kTinyFront rows: one wave per front, no dependency flags
+constexpr int kTinyFront = 64;
+void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
+                   double* Dl, const int64_t* gOff, hipStream_t st);
+void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
+                   hipStream_t st);
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);

@@ -847,6 +847,11 @@ void Plan::ensureSolveWs(int nrhs) {
     gatherBlocks_.assign(nl, 0);
     fwdBlocks_.assign(nl, 0);
     bwdBlocks_.assign(nl, 0);
+    std::vector<int> lists;
+    sweepOff_.assign(nl, 0);
+    tinyOff_.assign(nl, 0);
+    sweepCnt_.assign(nl, 0);
+    tinyCnt_.assign(nl, 0);
     for (int L = 0; L < nl; ++L) {
       gatherOff_[L] = static_cast<int64_t>(gs.size());
       fwdOff_[L] = static_cast<int64_t>(fs.size());
@@ -854,13 +859,23 @@ void Plan::ensureSolveWs(int nrhs) {
       int ga = 0, fa = 0, ba = 0;
       for (int s : levelFronts_[L]) {
         gs.push_back(ga);
+        ga += std::max(1, std::min(nrhs, (hm_[s] + 63) / 64));
+      }
+      sweepOff_[L] = static_cast<int64_t>(lists.size());
+      for (int s : levelFronts_[L]) {
+        const int mm = hm_[s];
+        if (mm <= kTinyFront) continue;
+        lists.push_back(s);
         fs.push_back(fa);
         bs.push_back(ba);
-        const int mm = hm_[s];
-        ga += std::max(1, std::min(nrhs, (mm + 63) / 64));
         fa += (mm + 31) / 32 + 1;
         ba += (mm + 31) / 32;
       }
+      sweepCnt_[L] = static_cast<int>(lists.size() - sweepOff_[L]);
+      tinyOff_[L] = static_cast<int64_t>(lists.size());
+      for (int s : levelFronts_[L])
+        if (hm_[s] <= kTinyFront) lists.push_back(s);
+      tinyCnt_[L] = static_cast<int>(lists.size() - tinyOff_[L]);
       gs.push_back(ga);
       fs.push_back(fa);
       bs.push_back(ba);
@@ -868,6 +883,7 @@ void Plan::ensureSolveWs(int nrhs) {
       fwdBlocks_[L] = fa;
       bwdBlocks_[L] = ba;
     }
+    dsolveLists_.upload(lists, stream_);
     dgatherStart_.upload(gs, stream_);
     dfwdStart_.upload(fs, stream_);
     dbwdStart_.upload(bs, stream_);
@@ -914,16 +930,16 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     profBegin(kProfFwd);
     launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, dBp_.p, dG_.p,
                     dDl_.p, dgOff_.p, stream_);
-    launchFwdSweep(lst, cnt, dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_, nrhs, dG_.p, dZ_.p, dDl_.p,
-                   dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+    launchFwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_,
+                   nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+    launchFwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, stream_);
     profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
-    const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
-    const int* lst = dallFronts_.p + levelFrontOff_[L];
     profBegin(kProfBwd);
-    launchBwdSweep(lst, cnt, dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_, nrhs, dZ_.p, dX_.p, dDl_.p,
-                   dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+    launchBwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_,
+                   nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+    launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dZ_.p, dX_.p, stream_);
     profEnd(kProfBwd);
   }
   launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
@@ -1097,7 +1113,7 @@ std::vector<const void*> Plan::graphFingerprint() const {
                                 dlargeRowStart_.p, dlargeTileStart_.p, dwsOff_.p, dpermVec_.p, dBp_.p, dZ_.p,
                                 dX_.p,      dG_.p,      dDl_.p,      dgOff_.p,    dgatherStart_.p, dfwdStart_.p,
                                 dbwdStart_.p, dflagOff_.p, dflagsF_.p, dflagsB_.p, dgen_.p, dsolveErr_.p,
-                                dschur_.p};
+                                dschur_.p,  dsolveLists_.p};
   for (void* f : hF_) v.push_back(f);
   return v;
 }

@@ -238,6 +238,11 @@ class Plan {
   DBuf<int> dgatherStart_, dfwdStart_, dbwdStart_;
   std::vector<int64_t> gatherOff_, fwdOff_, bwdOff_;
   std::vector<int> gatherBlocks_, fwdBlocks_, bwdBlocks_;
+  // per level: fronts solved by the sweeps, then fronts of at most
+  // kTinyFront rows solved one wave each (offsets into dsolveLists_)
+  DBuf<int> dsolveLists_;
+  std::vector<int64_t> sweepOff_, tinyOff_;
+  std::vector<int> sweepCnt_, tinyCnt_;
   // dependency flags of the sweeps (one per own-row block), the solve
   // generation they are compared with, and the bounded-wait error word
   DBuf<int64_t> dflagOff_;

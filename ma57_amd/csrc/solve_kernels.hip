@@ -427,6 +427,113 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   publishBlock(fl + w, gen);
 }
 
+// Fronts of at most 64 rows: one wave per front, lane i owning row i, and
+// the right-hand sides in groups of 8 held in registers.  Forward: the
+// unit-lower solve of the own rows, D^{-1} into Z, and the delta of the
+// remaining rows (solve.hpp:303-362, :434-455) in one pass; no flags needed.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, DevFronts fd, int n, int nrhs,
+                                                   const double* __restrict__ G, double* __restrict__ Z,
+                                                   double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
+  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (f >= count) return;
+  const int s = list[f];
+  const int lane = threadIdx.x & 63;
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const double* Gs = G + gOff[s];
+  double* Ds = Dl + gOff[s];
+  const bool row = lane < m;
+  const bool own = lane < nf;
+  const int kind = own ? pk[lane] : 0;
+  const int prow = row ? fd.rowIdx[io + lane] : 0;
+  const int gpos = row ? fd.perm[io + lane] : 0;
+  for (int cg = 0; cg < nrhs; cg += 8) {
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = (own && cg + c < nrhs) ? Ds[lane + static_cast<int64_t>(cg + c) * m] : 0.0;
+    for (int j = 0; j < nf; ++j) {
+      const double l = (row && lane > j) ? lval(F, m, pk, lane, j) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double yj = __shfl(v[c], j, 64);
+        v[c] = fma(-l, yj, v[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (cg + c >= nrhs) break;
+      const int64_t cc = cg + c;
+      // lanes below nF hold y; the others hold -L(i, :) y
+      const double up = __shfl(v[c], min(lane + 1, 63), 64);
+      const double dn = __shfl(v[c], max(lane - 1, 0), 64);
+      if (own) {
+        Ds[lane + cc * m] = v[c];
+        if (kind == 1) {
+          Z[prow + cc * n] = v[c] / fd.dDiag[io + lane];
+        } else if (kind == 2) {
+          const double d11 = fd.dDiag[io + lane], d21 = fd.dOff[io + lane], d22 = fd.dDiag[io + lane + 1];
+          const double det = d11 * d22 - d21 * d21;
+          Z[prow + cc * n] = (d22 * v[c] - d21 * up) / det;
+        } else {
+          const double d11 = fd.dDiag[io + lane - 1], d21 = fd.dOff[io + lane - 1], d22 = fd.dDiag[io + lane];
+          const double det = d11 * d22 - d21 * d21;
+          Z[prow + cc * n] = (-d21 * dn + d11 * v[c]) / det;
+        }
+      } else if (row) {
+        Ds[lane + cc * m] = Gs[gpos + cc * m] - v[c];
+      }
+    }
+  }
+}
+
+// Backward pass of fronts of at most 64 rows: lane p owns own row p;
+// x_own = z - L(anc, :)^T x_anc, then the unit upper solve with L_top^T
+// (solve.hpp:364-383).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, DevFronts fd, int n, int nrhs,
+                                                   const double* __restrict__ Z, double* __restrict__ X) {
+  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (f >= count) return;
+  const int s = list[f];
+  const int lane = threadIdx.x & 63;
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const bool own = lane < nf;
+  const int prow = own ? fd.rowIdx[io + lane] : 0;
+  for (int cg = 0; cg < nrhs; cg += 8) {
+    double x[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) x[c] = (own && cg + c < nrhs) ? Z[prow + static_cast<int64_t>(cg + c) * n] : 0.0;
+    for (int i = nf; i < m; ++i) {
+      const double l = own ? static_cast<double>(F[i + static_cast<size_t>(lane) * m]) : 0.0;
+      const int arow = fd.rowIdx[io + i];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double xa = (cg + c < nrhs) ? X[arow + static_cast<int64_t>(cg + c) * n] : 0.0;
+        x[c] = fma(-l, xa, x[c]);
+      }
+    }
+    for (int j = nf - 1; j > 0; --j) {
+      const double l = (lane < j) ? lval(F, m, pk, j, lane) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double xj = __shfl(x[c], j, 64);
+        x[c] = fma(-l, xj, x[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (own && cg + c < nrhs) X[prow + static_cast<int64_t>(cg + c) * n] = x[c];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Refinement support (refine.hpp:69-144).
 // ---------------------------------------------------------------------------
@@ -541,6 +648,24 @@ void launchBwdSweep(const int* list, int count, const int* blockStart, int block
   else
     k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
                                                 gen, err);
+}
+void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
+                   double* Dl, const int64_t* gOff, hipStream_t st) {
+  if (count <= 0) return;
+  const int blocks = (count + 3) / 4;
+  if (fd.fp32)
+    k_fwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
+  else
+    k_fwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
+}
+void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
+                   hipStream_t st) {
+  if (count <= 0) return;
+  const int blocks = (count + 3) / 4;
+  if (fd.fp32)
+    k_bwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
+  else
+    k_bwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {
