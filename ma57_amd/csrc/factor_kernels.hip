@@ -172,21 +172,21 @@ __global__ void k_reset_perm(const int* list, int count, DevFronts fd) {
 }
 
 // ---------------------------------------------------------------------------
-// Assembly of one column tile of one front (multifrontal.hpp:498-535):
+// Assembly of one column tile [c0, c1) of one front (multifrontal.hpp:498-535):
 // zero the lower triangle, add A's own-column entries, then extend-add every
 // child's contribution block in child order.  One workgroup owns each
 // (front, column tile), so every target entry has exactly one writer and the
 // summation order is fixed.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_assemble(const int2* tasks, int ntasks, int tileW, DevFronts fd,
+__global__ __launch_bounds__(256) void k_assemble(const int4* tasks, int ntasks, DevFronts fd,
                                                    const double* __restrict__ aVals, int monotone) {
   const int task = blockIdx.x;
   if (task >= ntasks) return;
   const int s = tasks[task].x;
   const int c0 = tasks[task].y;
   const int m = fd.m[s];
-  const int c1 = min(c0 + tileW, m);
+  const int c1 = min(tasks[task].z, m);
   T* F = static_cast<T*>(fd.F[s]);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
 
@@ -927,11 +927,18 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
 }
 
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
-// panel: row-parallel forward substitution against U = D L^T of the panel.
+// panel.  With U = D L^T of the panel, the multipliers of a row are
+// f U^{-1} = (f L^{-T}) D^{-1}: every workgroup forms M = L^{-1} of the
+// panel's unit lower block in LDS (one column per thread, no barriers), then
+// each thread maps one row: y = f M^T in registers, the running-max check on
+// y, and l = y D^{-1} for the 1x1 and 2x2 pivot blocks (dense_kernel.hpp:
+// 772-787 applied to the whole panel at once).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, const int* blockStart,
                                                      DevFronts fd) {
-  __shared__ double U[(kPanel + 1) * (kPanel + 1)];
+  constexpr int P = kPanel + 1;  // a 2x2 pivot at the panel's last column makes it 65
+  __shared__ double Lp[P][P + 1];
+  __shared__ double Mi[P][P + 1];
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
@@ -947,54 +954,75 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
-  const int P1 = kPanel + 1;
   for (int q = threadIdx.x; q < p; q += blockDim.x) {
     pdd[q] = fd.dDiag[io + k0 + q];
     pdo[q] = fd.dOff[io + k0 + q];
     pkd[q] = fd.pkind[io + k0 + q];
   }
   __syncthreads();
-  // U(q', q) for q' before the block containing q
+  // unit lower L of the panel; the (s+1, s) slot of a 2x2 block holds d21
   for (int t = threadIdx.x; t < p * p; t += blockDim.x) {
-    const int qp = t % p, q = t / p;
-    double u = 0.0;
-    if (qp < q) {
-      const int kind = pkd[qp];
-      if (kind == 1)
-        u = pdd[qp] * Fat(k0 + q, k0 + qp);
-      else if (kind == 2)
-        u = (q > qp + 1) ? pdd[qp] * Fat(k0 + q, k0 + qp) + pdo[qp] * Fat(k0 + q, k0 + qp + 1) : 0.0;
-      else
-        u = pdo[qp - 1] * Fat(k0 + q, k0 + qp - 1) + pdd[qp] * Fat(k0 + q, k0 + qp);
+    const int r = t % p, c = t / p;
+    double v = 0.0;
+    if (r > c && !(r == c + 1 && pkd[c] == 2)) v = Fat(k0 + r, k0 + c);
+    Lp[r][c] = v;
+  }
+  __syncthreads();
+  // column c of M = L^{-1}: M[c][c] = 1, M[r][c] = -sum_{c <= t < r} L[r][t] M[t][c]
+  if (static_cast<int>(threadIdx.x) < p) {
+    const int c = threadIdx.x;
+    Mi[c][c] = 1.0;
+    for (int r = c + 1; r < p; ++r) {
+      double a0 = Lp[r][c], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int t = c + 1;
+      for (; t + 4 <= r; t += 4) {
+        a0 = fma(Lp[r][t], Mi[t][c], a0);
+        a1 = fma(Lp[r][t + 1], Mi[t + 1][c], a1);
+        a2 = fma(Lp[r][t + 2], Mi[t + 2][c], a2);
+        a3 = fma(Lp[r][t + 3], Mi[t + 3][c], a3);
+      }
+      for (; t < r; ++t) a0 = fma(Lp[r][t], Mi[t][c], a0);
+      Mi[r][c] = -((a0 + a1) + (a2 + a3));
     }
-    U[qp + q * P1] = u;
   }
   __syncthreads();
   const int i = nE + rowBlock * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  double mx = 0.0;
-  int q = 0;
-  while (q < p) {
-    if (pkd[q] == 1) {
-      double x = Fat(i, k0 + q);
-      for (int qp = 0; qp < q; ++qp) x = fma(-static_cast<double>(Fat(i, k0 + qp)), U[qp + q * P1], x);
-      mx = fmax(mx, fabs(x));
-      Fat(i, k0 + q) = static_cast<T>(x / pdd[q]);
-      ++q;
-    } else {
-      double x0 = Fat(i, k0 + q), x1 = Fat(i, k0 + q + 1);
+  double x[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) x[q] = (q < p) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
+  // y = x M^T in place, last column first (y_q needs x_0 .. x_q only)
+#pragma unroll
+  for (int q = P - 1; q > 0; --q) {
+    if (q < p) {
+      double a0 = x[q], a1 = 0.0;
+#pragma unroll
       for (int qp = 0; qp < q; ++qp) {
-        const double l = Fat(i, k0 + qp);
-        x0 = fma(-l, U[qp + q * P1], x0);
-        x1 = fma(-l, U[qp + (q + 1) * P1], x1);
+        if (qp & 1)
+          a1 = fma(x[qp], Mi[q][qp], a1);
+        else
+          a0 = fma(x[qp], Mi[q][qp], a0);
       }
-      mx = fmax(mx, fmax(fabs(x0), fabs(x1)));
-      const double d11 = pdd[q], d21 = pdo[q], d22 = pdd[q + 1];
-      const double det = d11 * d22 - d21 * d21;
-      const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
-      Fat(i, k0 + q) = static_cast<T>(x0 * inv11 + x1 * inv21);
-      Fat(i, k0 + q + 1) = static_cast<T>(x0 * inv21 + x1 * inv22);
-      q += 2;
+      x[q] = a0 + a1;
+    }
+  }
+  double mx = 0.0;
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (q < p) {
+      const int kind = pkd[q];
+      if (kind == 1) {
+        mx = fmax(mx, fabs(x[q]));
+        Fat(i, k0 + q) = static_cast<T>(x[q] / pdd[q]);
+      } else if (kind == 2 && q + 1 < P) {
+        const double x0 = x[q], x1 = x[q + 1];
+        mx = fmax(mx, fmax(fabs(x0), fabs(x1)));
+        const double d11 = pdd[q], d21 = pdo[q], d22 = pdd[q + 1];
+        const double det = d11 * d22 - d21 * d21;
+        const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
+        Fat(i, k0 + q) = static_cast<T>(x0 * inv11 + x1 * inv21);
+        Fat(i, k0 + q + 1) = static_cast<T>(x0 * inv21 + x1 * inv22);
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -1259,13 +1287,13 @@ void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_
   if (count <= 0) return;
   k_reset_perm<<<count, 256, 0, st>>>(list, count, fd);
 }
-void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& fd, const double* aVals, int monotone,
+void launchAssemble(const int4* tasks, int ntasks, const DevFronts& fd, const double* aVals, int monotone,
                     hipStream_t st) {
   if (ntasks <= 0) return;
   if (fd.fp32)
-    k_assemble<float><<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+    k_assemble<float><<<ntasks, 256, 0, st>>>(tasks, ntasks, fd, aVals, monotone);
   else
-    k_assemble<double><<<ntasks, 256, 0, st>>>(tasks, ntasks, tileW, fd, aVals, monotone);
+    k_assemble<double><<<ntasks, 256, 0, st>>>(tasks, ntasks, fd, aVals, monotone);
 }
 template <typename T>
 static void factorSmallT(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,

@@ -12,7 +12,7 @@ namespace symla_hip {
 size_t smallFrontSmemBytes(int m);
 void launchResetFronts(int numSN, const DevFronts& fd, hipStream_t st);
 void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_t st);
-void launchAssemble(const int2* tasks, int ntasks, int tileW, const DevFronts& fd, const double* aVals, int monotone,
+void launchAssemble(const int4* tasks, int ntasks, const DevFronts& fd, const double* aVals, int monotone,
                     hipStream_t st);
 void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
                        hipStream_t st);

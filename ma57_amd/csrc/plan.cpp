@@ -320,7 +320,7 @@ void Plan::uploadLayout() {
 
 void Plan::buildSchedule() {
   std::vector<int> small, all;
-  std::vector<int2> tasks;
+  std::vector<int4> tasks;
   sched_.assign(levelFronts_.size(), {});
   levelFrontOff_.assign(levelFronts_.size() + 1, 0);
   for (size_t L = 0; L < levelFronts_.size(); ++L) {
@@ -330,7 +330,11 @@ void Plan::buildSchedule() {
     levelFrontOff_[L] = static_cast<int64_t>(all.size());
     for (int s : levelFronts_[L]) {
       all.push_back(s);
-      for (int c0 = 0; c0 < hm_[s]; c0 += tune_.asmTile) tasks.push_back(make_int2(s, c0));
+      // column tiles: a large front gets narrower tiles so that it alone
+      // spreads over at least 128 workgroups
+      int w = tune_.asmTile;
+      while (w > 8 && (hm_[s] + w - 1) / w < 128 && hm_[s] > 4 * tune_.asmTile) w >>= 1;
+      for (int c0 = 0; c0 < hm_[s]; c0 += w) tasks.push_back(make_int4(s, c0, std::min(c0 + w, hm_[s]), 0));
       if (hm_[s] <= tune_.smallMax) {
         small.push_back(s);
         ls.smallMaxM = std::max(ls.smallMaxM, hm_[s]);
@@ -521,7 +525,7 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
   for (size_t L = 0; L < sched_.size(); ++L) {
     const auto& ls = sched_[L];
     profBegin(kProfAssemble);
-    launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, tune_.asmTile, fd, aValsDev, 1, stream_);
+    launchAssemble(dtasks_.p + ls.taskOff, ls.taskCount, fd, aValsDev, 1, stream_);
     profEnd(kProfAssemble);
     if (ls.smallCount > 0) {
       profBegin(kProfSmall);
@@ -580,7 +584,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
   DevFronts fd = devFronts();
   launchResetFronts(numSN_, fd, stream_);
   DBuf<int> dlist;
-  DBuf<int2> dtask;
+  DBuf<int4> dtask;
   for (size_t L = 0; L < levelFronts_.size(); ++L) {
     const auto& fronts = levelFronts_[L];
     for (int s : fronts) {
@@ -658,12 +662,12 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     uploadLayout();
     fd = devFronts();
     std::vector<int> small, all;
-    std::vector<int2> tasks;
+    std::vector<int4> tasks;
     int maxM = 0;
     std::vector<int> large;
     for (int s : fronts) {
       all.push_back(s);
-      tasks.push_back(make_int2(s, 0));
+      tasks.push_back(make_int4(s, 0, hm_[s], 0));
       if (hm_[s] <= tune_.smallMax) {
         small.push_back(s);
         maxM = std::max(maxM, hm_[s]);
@@ -676,7 +680,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     dlist.upload(lists, stream_);
     dtask.upload(tasks, stream_);
     launchResetPerm(dlist.p, static_cast<int>(all.size()), fd, stream_);
-    launchAssemble(dtask.p, static_cast<int>(tasks.size()), 1 << 30, fd, aValsDev, 0, stream_);
+    launchAssemble(dtask.p, static_cast<int>(tasks.size()), fd, aValsDev, 0, stream_);
     launchFactorSmall(dlist.p + all.size(), static_cast<int>(small.size()), maxM, fd, prm, stream_);
     if (!large.empty()) {
       LargeSchedHost lh;

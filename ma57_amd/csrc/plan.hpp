@@ -223,7 +223,7 @@ class Plan {
   };
   std::vector<LevelSched> sched_;
   DBuf<int> dsmall_;
-  DBuf<int2> dtasks_;
+  DBuf<int4> dtasks_;
   DBuf<int> dallFronts_;
   DBuf<int> dlevelFronts_;
   std::vector<int64_t> levelFrontOff_;
