@@ -54,3 +54,24 @@ def test_host_only_error_paths():
     lib.symla_symbolic_free(out)
     rc = lib.symla_dense_factor(-1, None, 1, -1, None, None, None, None, None, None, None, None, None, None, None)
     assert rc == _lib.SYMLA_ERR_INVALID
+
+
+def test_null_plan_arguments_are_rejected():
+    lib = _lib.load()
+    t = _lib.Tuning(64, 64)
+    assert lib.symla_plan_set_tuning(None, C.byref(t)) == _lib.SYMLA_ERR_INVALID
+    assert lib.symla_plan_get_tuning(None, C.byref(t)) == _lib.SYMLA_ERR_INVALID
+    assert lib.symla_graph_replay(None, None) == _lib.SYMLA_ERR_INVALID
+    starts = np.array([0, 1], dtype=np.int32)
+    inr = np.zeros(3, dtype=np.int32)
+    assert lib.symla_group_inertia(None, 1, _lib.ptr(starts, _lib.i32p), _lib.ptr(inr, _lib.i32p),
+                                   None) == _lib.SYMLA_ERR_INVALID
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one MI355X" in r.stderr
