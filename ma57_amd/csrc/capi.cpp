@@ -37,6 +37,9 @@ int guarded(F&& f) {
   try {
     f();
     g_lastError.clear();
+    // every HIP call and launch above was checked; leave no per-thread
+    // error state behind for the caller's own HIP work
+    (void)hipGetLastError();
     return SYMLA_OK;
   } catch (const std::invalid_argument& e) {
     return fail(SYMLA_ERR_INVALID, e.what());

@@ -1282,10 +1282,12 @@ void launchResetFronts(int numSN, const DevFronts& fd, hipStream_t st) {
   if (numSN <= 0) return;
   const int blocks = (numSN + 255) / 256;
   k_reset_fronts<<<blocks, 256, 0, st>>>(numSN, fd);
+  hipCheck(hipGetLastError(), "launchResetFronts");
 }
 void launchResetPerm(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
   k_reset_perm<<<count, 256, 0, st>>>(list, count, fd);
+  hipCheck(hipGetLastError(), "launchResetPerm");
 }
 void launchAssemble(const int4* tasks, int ntasks, const DevFronts& fd, const double* aVals, int monotone,
                     hipStream_t st) {
@@ -1294,6 +1296,7 @@ void launchAssemble(const int4* tasks, int ntasks, const DevFronts& fd, const do
     k_assemble<float><<<ntasks, 256, 0, st>>>(tasks, ntasks, fd, aVals, monotone);
   else
     k_assemble<double><<<ntasks, 256, 0, st>>>(tasks, ntasks, fd, aVals, monotone);
+  hipCheck(hipGetLastError(), "launchAssemble");
 }
 template <typename T>
 static void factorSmallT(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
@@ -1301,11 +1304,14 @@ static void factorSmallT(const int* list, int count, int maxM, const DevFronts& 
   const size_t smem = smallFrontSmemBytes(maxM);
   static bool attrSet = false;
   if (!attrSet) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_small<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipCheck(hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_small<T>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(smallFrontSmemBytes(kSmallFrontMax))),
+             "k_factor_small LDS size");
     attrSet = true;
   }
   k_factor_small<T><<<count, kSmallThreads, smem, st>>>(list, count, fd, prm);
+  hipCheck(hipGetLastError(), "factorSmallT");
 }
 void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd, const FactorParams& prm,
                        hipStream_t st) {
@@ -1314,6 +1320,7 @@ void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd
     factorSmallT<float>(list, count, maxM, fd, prm, st);
   else
     factorSmallT<double>(list, count, maxM, fd, prm, st);
+  hipCheck(hipGetLastError(), "launchFactorSmall");
 }
 void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
@@ -1324,18 +1331,21 @@ void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd
     k_front_maxabs<float><<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
   else
     k_front_maxabs<double><<<count * bpf, 256, 0, st>>>(list, count, bpf, fd);
+  hipCheck(hipGetLastError(), "launchFrontMaxAbs");
 }
 template <int RM, typename T>
 static void panelLaunch(const int* list, int count, const DevFronts& fd, const FactorParams& prm, double* ws,
                         const int64_t* wsOff, hipStream_t st) {
   const size_t lds = PanelShape<RM>::lds();
   static bool attrSet = false;
-  if (!attrSet) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_panel<RM, T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (!attrSet && lds > 0) {
+    hipCheck(hipFuncSetAttribute(reinterpret_cast<const void*>(k_panel<RM, T>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+             "k_panel LDS size");
     attrSet = true;
   }
   k_panel<RM, T><<<count, kPanelThreads, lds, st>>>(list, count, fd, prm, ws, wsOff);
+  hipCheck(hipGetLastError(), "panelLaunch");
 }
 template <typename T>
 static void panelT(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
@@ -1364,6 +1374,7 @@ void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, c
     panelT<float>(list, count, maxRows, fd, prm, ws, wsOff, st);
   else
     panelT<double>(list, count, maxRows, fd, prm, ws, wsOff, st);
+  hipCheck(hipGetLastError(), "launchPanel");
 }
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st) {
@@ -1372,6 +1383,7 @@ void launchPanelRows(const int* list, int count, const int* blockStart, int tota
     k_panel_rows<float><<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
   else
     k_panel_rows<double><<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
+  hipCheck(hipGetLastError(), "launchPanelRows");
 }
 void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
                 hipStream_t st) {
@@ -1380,17 +1392,21 @@ void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles
     k_syrk_f32<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
   else
     k_syrk_f64<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
+  hipCheck(hipGetLastError(), "launchSyrk");
 }
 void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
   k_front_finish<<<(count + 255) / 256, 256, 0, st>>>(list, count, fd);
+  hipCheck(hipGetLastError(), "launchFrontFinish");
 }
 void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;
   k_make_rowidx<<<count, 256, 0, st>>>(list, count, fd);
+  hipCheck(hipGetLastError(), "launchMakeRowIdx");
 }
 void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st) {
   k_reduce_stats<<<1, 256, 0, st>>>(numSN, fd, outI, outD);
+  hipCheck(hipGetLastError(), "launchReduceStats");
 }
 
 }  // namespace symla_hip

@@ -8,6 +8,9 @@
 
 namespace symla_hip {
 
+// Throws DeviceError for a failed HIP call (defined with the plan).
+void hipCheck(hipError_t e, const char* what);
+
 // factor_kernels.hip
 size_t smallFrontSmemBytes(int m);
 void launchResetFronts(int numSN, const DevFronts& fd, hipStream_t st);

@@ -617,15 +617,21 @@ static int gridFor(int64_t total, int threads) {
 }
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st) {
   k_permute_in<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, B, ldb, Bp);
+  hipCheck(hipGetLastError(), "launchPermuteIn");
 }
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st) {
   k_permute_out<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, X, Xo, ldx);
+  hipCheck(hipGetLastError(), "launchPermuteOut");
 }
-void launchSolveBegin(unsigned* gen, hipStream_t st) { k_solve_begin<<<1, 64, 0, st>>>(gen); }
+void launchSolveBegin(unsigned* gen, hipStream_t st) {
+  k_solve_begin<<<1, 64, 0, st>>>(gen);
+  hipCheck(hipGetLastError(), "launchSolveBegin");
+}
 void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                      int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
   k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff);
+  hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
@@ -637,6 +643,7 @@ void launchFwdSweep(const int* list, int count, const int* blockStart, int block
   else
     k_fwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
                                                 gen, err);
+  hipCheck(hipGetLastError(), "launchFwdSweep");
 }
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
@@ -648,6 +655,7 @@ void launchBwdSweep(const int* list, int count, const int* blockStart, int block
   else
     k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
                                                 gen, err);
+  hipCheck(hipGetLastError(), "launchBwdSweep");
 }
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
                    double* Dl, const int64_t* gOff, hipStream_t st) {
@@ -657,6 +665,7 @@ void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
     k_fwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
   else
     k_fwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
+  hipCheck(hipGetLastError(), "launchFwdTiny");
 }
 void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
                    hipStream_t st) {
@@ -666,29 +675,36 @@ void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
     k_bwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
   else
     k_bwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
+  hipCheck(hipGetLastError(), "launchBwdTiny");
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {
   k_residual<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, rowPtr, colIdx, valSrc, aVals,
                                                                            B, X, R);
+  hipCheck(hipGetLastError(), "launchResidual");
 }
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st) {
   k_axpy<<<gridFor(len, 256), 256, 0, st>>>(len, x, dx, out);
+  hipCheck(hipGetLastError(), "launchAxpy");
 }
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st) {
   k_absmax<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, v, ld, outBits);
+  hipCheck(hipGetLastError(), "launchAbsMax");
 }
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st) {
   k_rowsum_max<<<gridFor(n, 256), 256, 0, st>>>(n, rowPtr, valSrc, aVals, outBits);
+  hipCheck(hipGetLastError(), "launchRowSumMax");
 }
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st) {
   if (count <= 0) return;
   k_zero_u64<<<(count + 255) / 256, 256, 0, st>>>(p, count);
+  hipCheck(hipGetLastError(), "launchZeroU64");
 }
 void launchZeroU32(unsigned* p, int64_t count, hipStream_t st) {
   if (count <= 0) return;
   k_zero_u32<<<gridFor(count, 256), 256, 0, st>>>(p, count);
+  hipCheck(hipGetLastError(), "launchZeroU32");
 }
 
 }  // namespace symla_hip
