@@ -792,7 +792,13 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     hipCheck(hipStreamSynchronize(stream_), "maxabs");
     double maxAbs = bitsToDoubleHost(b);
     if (maxAbs <= 0.0) maxAbs = 1.0;
-    delta = std::sqrt(std::numeric_limits<double>::epsilon()) * maxAbs;
+    // sqrt(eps) * max|A| (solver.hpp:352-357) with eps of the precision the
+    // fronts are stored in: FP32 fronts cannot carry the element growth a
+    // FP64-sized perturbation of a tiny pivot causes, and refinement in FP64
+    // removes the larger perturbation's error
+    const double eps = prec_ ? static_cast<double>(std::numeric_limits<float>::epsilon())
+                             : std::numeric_limits<double>::epsilon();
+    delta = std::sqrt(eps) * maxAbs;
   }
   const int maxAttempts = std::max(1, opts.maxRetries + 1);
   for (int attempt = 0; attempt < maxAttempts; ++attempt) {
