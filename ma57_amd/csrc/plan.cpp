@@ -792,7 +792,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     hipCheck(hipStreamSynchronize(stream_), "maxabs");
     double maxAbs = bitsToDoubleHost(b);
     if (maxAbs <= 0.0) maxAbs = 1.0;
-    // sqrt(eps) * max|A| (solver.hpp:352-357) with eps of the precision the
+    // sqrt(eps) * max|A| (solver.hpp:346-358) with eps of the precision the
     // fronts are stored in: FP32 fronts cannot carry the element growth a
     // FP64-sized perturbation of a tiny pivot causes, and refinement in FP64
     // removes the larger perturbation's error
