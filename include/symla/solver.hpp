@@ -153,6 +153,11 @@ class SymLDLT<double> {
   int regularizationRetriesUsed() const { return last_info_.retries_used; }
   double totalPerturbation() const { return last_info_.total_perturbation; }
   int numPerturbedPivots() const { return last_info_.num_perturbed; }
+  // Front precision of the last factorization (FP32 static-regularized
+  // factorizations may fall back to FP64; no reference counterpart).
+  Precision precisionUsed() const {
+    return last_info_.precision_used == SYMLA_PREC_FP32 ? Precision::FP32 : Precision::FP64;
+  }
 
   void analyzePattern(const SparseMatrix& A, const SymbolicFactorOptions& options = SymbolicFactorOptions()) {
     if (A.rows() != A.cols()) throw std::invalid_argument("symla::computeOrdering: matrix must be square");

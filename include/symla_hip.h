@@ -105,6 +105,10 @@ typedef struct {
   double last_delta;            /* lastRegularizationDelta() */
   int delayed_fronts;           /* fronts that handed delayed columns to their parent */
   int repaired;                 /* 1 when the delayed-pivot repair pass ran */
+  /* SYMLA_PREC_* of the returned factorization.  Static-regularized mode
+   * with FP32 fronts falls back to the reference's FP64 retry sequence when
+   * two FP32 attempts miss the expected inertia (no reference counterpart). */
+  int precision_used;
 } symla_factor_info;
 
 /* SymLDLT::factorize / refactorize (solver.hpp:140-172) through

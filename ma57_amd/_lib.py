@@ -44,7 +44,7 @@ class FactorInfo(C.Structure):
     _fields_ = [("n_pos", C.c_int), ("n_neg", C.c_int), ("n_zero", C.c_int), ("singular", C.c_int),
                 ("n_singular", C.c_int), ("total_perturbation", C.c_double), ("num_perturbed", C.c_int),
                 ("retries_used", C.c_int), ("last_delta", C.c_double), ("delayed_fronts", C.c_int),
-                ("repaired", C.c_int)]
+                ("repaired", C.c_int), ("precision_used", C.c_int)]
 
 
 class RefineOpts(C.Structure):

@@ -92,6 +92,7 @@ void fillInfo(const FactorInfo& fi, symla_factor_info* info) {
   info->last_delta = fi.lastDelta;
   info->delayed_fronts = fi.delayedFronts;
   info->repaired = fi.repaired ? 1 : 0;
+  info->precision_used = fi.precisionUsed;
 }
 
 }  // namespace

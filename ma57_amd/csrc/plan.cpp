@@ -757,6 +757,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     }
     info.singular = singular_;
     info.nSingular = static_cast<int>(singularColsOrig_.size());
+    info.precisionUsed = prec_;
     hipCheck(hipEventRecord(ev1_, stream_), "event");
     hipCheck(hipEventSynchronize(ev1_), "event");
     float ms = 0.f;
@@ -783,56 +784,71 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     else if (v < 0)
       ++expectedNeg;
   }
-  double delta = opts.initialDelta;
-  if (delta <= 0.0) {
+  // max|A| for the default perturbation sqrt(eps) * max|A| (solver.hpp:346-358)
+  double maxAbs = 1.0;
+  if (opts.initialDelta <= 0.0) {
     launchZeroU64(dnorms_.p, 1, stream_);
     if (nnzA_ > 0) launchAbsMax(static_cast<int>(nnzA_), 1, vals, nnzA_, dnorms_.p, stream_);
     unsigned long long b = 0;
     hipCheck(hipMemcpyAsync(&b, dnorms_.p, sizeof(b), hipMemcpyDeviceToHost, stream_), "maxabs");
     hipCheck(hipStreamSynchronize(stream_), "maxabs");
-    double maxAbs = bitsToDoubleHost(b);
+    maxAbs = bitsToDoubleHost(b);
     if (maxAbs <= 0.0) maxAbs = 1.0;
-    // sqrt(eps) * max|A| (solver.hpp:346-358) with eps of the precision the
-    // fronts are stored in: FP32 fronts cannot carry the element growth a
-    // FP64-sized perturbation of a tiny pivot causes, and refinement in FP64
-    // removes the larger perturbation's error
-    const double eps = prec_ ? static_cast<double>(std::numeric_limits<float>::epsilon())
-                             : std::numeric_limits<double>::epsilon();
-    delta = std::sqrt(eps) * maxAbs;
   }
+  const bool fullCoverage = (expectedPos + expectedNeg == n_);
+  // one inertia-controlled retry sequence in the current front precision;
+  // returns true when the factorization holds the expected inertia
+  auto retryLoop = [&](int maxAttempts, double eps) {
+    double delta = opts.initialDelta > 0.0 ? opts.initialDelta : std::sqrt(eps) * maxAbs;
+    for (int attempt = 0; attempt < maxAttempts; ++attempt) {
+      FactorParams prm = params(opts, true, delta, dsignFinal_.p);
+      singular_ = false;
+      singularColsOrig_.clear();
+      runStatic(vals, prm);
+      bool anyDelay = false;
+      collectStats(info, anyDelay);
+      info.lastDelta = delta;
+      info.retriesUsed = attempt;
+      info.precisionUsed = prec_;
+      const bool ok = !haveExp || (info.zero == 0 && !singular_ &&
+                                   (!fullCoverage || (info.pos == expectedPos && info.neg == expectedNeg)));
+      if (ok) return true;
+      if (attempt + 1 < maxAttempts) delta *= opts.growth;
+    }
+    return false;
+  };
+  auto finish = [&]() {
+    hipCheck(hipEventRecord(ev1_, stream_), "event");
+    hipCheck(hipEventSynchronize(ev1_), "event");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+    factorMs_ = ms;
+    factorized_ = true;
+  };
   const int maxAttempts = std::max(1, opts.maxRetries + 1);
-  for (int attempt = 0; attempt < maxAttempts; ++attempt) {
-    FactorParams prm = params(opts, true, delta, dsignFinal_.p);
-    singular_ = false;
-    singularColsOrig_.clear();
-    runStatic(vals, prm);
-    bool anyDelay = false;
-    collectStats(info, anyDelay);
-    info.lastDelta = delta;
-    info.retriesUsed = attempt;
-    const bool fullCoverage = (expectedPos + expectedNeg == n_);
-    const bool ok = !haveExp || (info.zero == 0 && !singular_ &&
-                                 (!fullCoverage || (info.pos == expectedPos && info.neg == expectedNeg)));
-    if (ok) {
-      hipCheck(hipEventRecord(ev1_, stream_), "event");
-      hipCheck(hipEventSynchronize(ev1_), "event");
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, ev0_, ev1_);
-      factorMs_ = ms;
-      factorized_ = true;
+  if (prec_ == 1) {
+    // FP32 fronts with a perturbation sized for FP32 (eps of float); if they
+    // do not hold the expected inertia within two attempts, the element
+    // growth of the perturbed pivots is beyond FP32 and the reference's own
+    // FP64 sequence runs instead (reported as precision_used = FP64)
+    if (retryLoop(std::min(2, maxAttempts), static_cast<double>(std::numeric_limits<float>::epsilon()))) {
+      finish();
       return;
     }
-    if (attempt + 1 >= maxAttempts) {
-      std::ostringstream oss;
-      oss << "symla::SymLDLT::factorize (Mode::StaticRegularized): inertia-controlled regularization retry budget ("
-          << opts.maxRetries << " retries) exhausted without matching the expected inertia. expected (n_pos="
-          << expectedPos << ", n_neg=" << expectedNeg << ", n_zero=0), got (n_pos=" << info.pos
-          << ", n_neg=" << info.neg << ", n_zero=" << info.zero << ", singular=" << (singular_ ? "true" : "false")
-          << ") at delta=" << delta << ". Refusing to report a wrong-inertia factorization as successful.";
-      throw std::runtime_error(oss.str());
-    }
-    delta *= opts.growth;
+    prec_ = 0;
+    layoutDirty_ = true;
   }
+  if (retryLoop(maxAttempts, std::numeric_limits<double>::epsilon())) {
+    finish();
+    return;
+  }
+  std::ostringstream oss;
+  oss << "symla::SymLDLT::factorize (Mode::StaticRegularized): inertia-controlled regularization retry budget ("
+      << opts.maxRetries << " retries) exhausted without matching the expected inertia. expected (n_pos="
+      << expectedPos << ", n_neg=" << expectedNeg << ", n_zero=0), got (n_pos=" << info.pos << ", n_neg=" << info.neg
+      << ", n_zero=" << info.zero << ", singular=" << (singular_ ? "true" : "false") << ") at delta=" << info.lastDelta
+      << ". Refusing to report a wrong-inertia factorization as successful.";
+  throw std::runtime_error(oss.str());
 }
 
 void Plan::ensureSolveWs(int nrhs) {
@@ -1199,6 +1215,7 @@ void Plan::graphReplay(FactorInfo& info) {
   hipCheck(hipGraphLaunch(graphExec_, stream_), "graph launch");
   bool anyDelay = false;
   collectStats(info, anyDelay);
+  info.precisionUsed = prec_;
   checkSolveError();
   if (anyDelay) {
     // the recorded schedule assumed no delays: repair and solve again

@@ -73,6 +73,7 @@ struct FactorInfo {
   double lastDelta = 0.0;
   int delayedFronts = 0;
   bool repaired = false;
+  int precisionUsed = 0;  // front precision of the returned factorization
 };
 
 struct RefineResult {

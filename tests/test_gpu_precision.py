@@ -88,6 +88,9 @@ def test_c3_sqd_kkt_static(precision):
     s = _factor(K, precision, OrderingType.AMD, kkt=(n1, n2))
     assert _inertia(s) == (n2, n1, 0)
     assert s.regularizationRetriesUsed() <= 1
+    # FP32 fronts either hold the expected inertia, or the factorization reports the FP64 fallback
+    assert s.last_info.precision_used in ((Precision.FP64,) if precision == Precision.FP64
+                                          else (Precision.FP32, Precision.FP64))
     B = np.random.default_rng(0x53594D4C41 + 3).standard_normal((n1 + n2, 4))
     r = s.solveWithRefinement(K, B, RefinementOptions(10, 1e-14))
     assert P.backward_error(K, r.x, B) <= BE_TOL
