@@ -201,7 +201,9 @@ def main() -> int:
                      "schur_gflop_per_step": prof.schur_flops / args.steps / 1e9},
         }
 
-    fp64 = leg(Precision.FP64, refine=args.config == "c3")
+    # the plain FP64 variant solves without refinement, except where the matrix needs it to reach the
+    # 1e-12 bar (static perturbations on c3; the 27-point system's conditioning on c5)
+    fp64 = leg(Precision.FP64, refine=args.config in ("c3", "c5"))
     mixed = leg(Precision.FP32, refine=True)
     if expect is None:
         expect = tuple(fp64["inertia"])  # no closed form: the FP64 factorization of the same matrix

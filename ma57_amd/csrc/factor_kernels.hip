@@ -1240,9 +1240,11 @@ __global__ void k_make_rowidx(const int* list, int count, DevFronts fd) {
 // Each thread sums a contiguous range of fronts in order and thread 0 adds
 // the partial sums in thread order, so the perturbation total does not
 // depend on timing.
-__global__ void k_reduce_stats(int numSN, DevFronts fd, long long* outI, double* outD) {
-  __shared__ long long sI[5][256];
-  __shared__ double sD[256];
+constexpr int kStatsThreads = 1024;
+__global__ __launch_bounds__(kStatsThreads) void k_reduce_stats(int numSN, DevFronts fd, long long* outI,
+                                                                double* outD) {
+  __shared__ long long sI[5][kStatsThreads];
+  __shared__ double sD[kStatsThreads];
   long long pos = 0, neg = 0, zero = 0, delayedFronts = 0, npert = 0;
   double pert = 0.0;
   const int chunk = (numSN + blockDim.x - 1) / blockDim.x;
@@ -1405,7 +1407,7 @@ void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream
   hipCheck(hipGetLastError(), "launchMakeRowIdx");
 }
 void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st) {
-  k_reduce_stats<<<1, 256, 0, st>>>(numSN, fd, outI, outD);
+  k_reduce_stats<<<1, kStatsThreads, 0, st>>>(numSN, fd, outI, outD);
   hipCheck(hipGetLastError(), "launchReduceStats");
 }
 

@@ -52,6 +52,9 @@ void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, c
                     const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st);
+// per-column max |v| into outBits[c * stride] (bits of non-negative doubles)
+void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, int stride,
+                      hipStream_t st);
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st);
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st);

@@ -1079,11 +1079,9 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
   // backward error per column with the symmetric infinity norm
   {
     launchZeroU64(dnorms_.p + 8, 3 * nrhs, stream_);
-    for (int c = 0; c < nrhs; ++c) {
-      norm(Rc + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 8 + 3 * c);
-      norm(Xc + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 9 + 3 * c);
-      norm(dBr_.p + static_cast<size_t>(c) * n_, 1, n_, dnorms_.p + 10 + 3 * c);
-    }
+    launchAbsMaxCols(n_, nrhs, Rc, n_, dnorms_.p + 8, 3, stream_);
+    launchAbsMaxCols(n_, nrhs, Xc, n_, dnorms_.p + 9, 3, stream_);
+    launchAbsMaxCols(n_, nrhs, dBr_.p, n_, dnorms_.p + 10, 3, stream_);
     std::vector<unsigned long long> h(3 * static_cast<size_t>(nrhs));
     hipCheck(hipMemcpyAsync(h.data(), dnorms_.p + 8, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             stream_),

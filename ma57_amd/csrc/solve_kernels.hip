@@ -578,6 +578,21 @@ __global__ void k_absmax(int n, int nrhs, const double* __restrict__ v, int64_t 
     atomicMax(outBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
 }
 
+// Per-column max |v| of an n x nrhs block: workgroup (c, part) covers rows
+// part, part + parts, ... of column c; outBits[c * stride] receives the max.
+__global__ void k_absmax_cols(int n, int nrhs, const double* __restrict__ v, int64_t ld, int parts,
+                              unsigned long long* outBits, int stride) {
+  const int c = blockIdx.x / parts;
+  const int part = blockIdx.x % parts;
+  if (c >= nrhs) return;
+  const double* col = v + static_cast<int64_t>(c) * ld;
+  double mx = 0.0;
+  for (int i = part * blockDim.x + threadIdx.x; i < n; i += parts * blockDim.x) mx = fmax(mx, fabs(col[i]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0)
+    atomicMax(outBits + static_cast<int64_t>(c) * stride, static_cast<unsigned long long>(__double_as_longlong(mx)));
+}
+
 // max over rows of sum |a| over a row-grouped entry list.  Over the
 // symmetric CSR this is the true infinity norm; over the caller's stored
 // entries grouped by row it is the reference's sparseMaxAbsRowSum
@@ -690,6 +705,15 @@ void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hip
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st) {
   k_absmax<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, v, ld, outBits);
   hipCheck(hipGetLastError(), "launchAbsMax");
+}
+void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, int stride,
+                      hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return;
+  int parts = (n + 256 * 16 - 1) / (256 * 16);
+  if (parts > 64) parts = 64;
+  if (parts < 1) parts = 1;
+  k_absmax_cols<<<nrhs * parts, 256, 0, st>>>(n, nrhs, v, ld, parts, outBits, stride);
+  hipCheck(hipGetLastError(), "launchAbsMaxCols");
 }
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st) {
