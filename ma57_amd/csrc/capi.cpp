@@ -472,14 +472,30 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
       std::vector<int> mv{n}, nev{nE};
       LargeSchedHost lh;
       LargeSched ls = buildLargeSched({0}, mv, nev, lh);
-      DBuf<int> dl, dr, dt;
+      DBuf<int> dl, dr, dt, dmw, derr;
       DBuf<int64_t> dw;
+      DBuf<MwSync> dsync;
       dl.upload(lh.lists, st);
       dr.upload(lh.rowStart, st);
       dt.upload(lh.tileStart, st);
+      dmw.upload(lh.mwStart, st);
       dw.upload(std::vector<int64_t>{0}, st);
-      runLargeSched(ls, dl.p, dr.p, dt.p, f, prm, dws.p, dw.p, st, nullptr);
+      dsync.alloc(kMwSlots);
+      derr.alloc(1);
+      hipCheck(hipMemsetAsync(dsync.p, 0, sizeof(MwSync) * kMwSlots, st), "memset");
+      hipCheck(hipMemsetAsync(derr.p, 0, sizeof(int), st), "memset");
+      LargeDev dv;
+      dv.lists = dl.p;
+      dv.rowStart = dr.p;
+      dv.tileStart = dt.p;
+      dv.mwStart = dmw.p;
+      dv.sync = dsync.p;
+      dv.err = derr.p;
+      runLargeSched(ls, dv, f, prm, dws.p, dw.p, st, nullptr);
+      int herr = 0;
+      hipCheck(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, st), "error word");
       hipCheck(hipStreamSynchronize(st), "dense large front");
+      if (herr != 0) throw DeviceError("symla: a cooperative panel step exceeded its bounded wait");
     }
     hipCheck(hipMemcpy2DAsync(A, sizeof(double) * lda, dA.p, sizeof(double) * n, sizeof(double) * n, n,
                               hipMemcpyDeviceToHost, st),

@@ -51,6 +51,22 @@ struct DevFronts {
   unsigned long long* schurFlops;  // algorithmic flops of the MFMA trailing updates (p*r*(r+1) per update)
 };
 
+// Cooperative panel of one front whose eligible rows exceed what one
+// workgroup holds: W workgroups of kPanelThreads threads, kMwRows rows each,
+// exchange per pivot through one MwSync record in device memory.
+constexpr int kMwRows = 2 * kPanelThreads;
+constexpr int kMwMaxW = 32;
+constexpr int kMwSlots = 16;  // giant fronts per launch
+struct MwSync {
+  unsigned long long flag[kMwMaxW];  // tag of the last step each workgroup posted
+  double lv[2][kMwMaxW];             // per-step local max |a_ik| (double-buffered by step parity)
+  int li[2][kMwMaxW];                // and its row
+  double extra[2][12];               // workgroup 0: a_kk and the sub-panel rows of the pivot column
+  int state[4];                      // workgroup 0 after a general step: k, p, stop
+  unsigned epoch;                    // launches so far on this record (tags stay unique)
+  int pad;
+};
+
 struct FactorParams {
   double zeroTol;        // DenseLDLTOptions::zero_tolerance
   double relFloor;       // DenseLDLTOptions::relative_pivot_floor

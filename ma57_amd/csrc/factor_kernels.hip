@@ -926,6 +926,477 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   }
 }
 
+// Device-coherent accesses for values exchanged between workgroups of one
+// launch (no cache write-back or invalidation needed around them).
+template <typename T>
+__device__ inline void storeCoherent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T loadCoherent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Cooperative panel of a front with more eligible rows than one workgroup
+// holds (the same decisions as k_panel, dense_kernel.hpp:585-792).  The W
+// workgroups of a front own kMwRows consecutive eligible rows each; the
+// sub-panel (SW columns from the current pivot) of those rows lives in each
+// workgroup's LDS.  Per pivot every workgroup posts its local max |a_ik|
+// (workgroup 0 also a_kk and the pivot column's sub-panel rows) and reads
+// everyone's, so all reach the same decision.  A decision other than "1x1
+// at k" is carried out by workgroup 0 alone on the column buffers in global
+// memory (the general path of k_panel) while the others wait; then all
+// reload their sub-panel.  Waits are bounded (the error word is raised and
+// the wait abandoned), so every workgroup runs to its end.
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int count, const int* wgStart,
+                                                             DevFronts fd, FactorParams prm, double* wsBase,
+                                                             const int64_t* wsOff, MwSync* syncBase, int* err) {
+  constexpr int NT = kPanelThreads;
+  constexpr int SW = 8;
+  constexpr int CSR = kMwRows;
+  extern __shared__ double Cs[];  // [SW][CSR]
+  __shared__ double Wsub[SW][kPanel + 2];
+  __shared__ double wvec[kPanel + 2];
+  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
+  __shared__ int pkd[kPanel + 2];
+  __shared__ double redV[NT / 64 + 1];
+  __shared__ int redI[NT / 64 + 1];
+  __shared__ double bcast[16];
+  __shared__ int bcastI[4];
+  __shared__ int abandon;
+
+  const int tid = threadIdx.x;
+  if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
+  const int f = findSegment(wgStart, count, blockIdx.x);
+  const int s = list[f];
+  const int w = blockIdx.x - wgStart[f];
+  const int W = wgStart[f + 1] - wgStart[f];
+  MwSync* sy = syncBase + f;
+  const unsigned long long base = static_cast<unsigned long long>(loadCoherent(&sy->epoch)) << 16;
+  const bool lead = (w == 0);
+  double* ws = wsBase + wsOff[s];
+  const int m = fd.m[s];
+  const int nE = fd.nE[s];
+  T* F = static_cast<T*>(fd.F[s]);
+  const int64_t io = fd.idxOff[s];
+  const int k0 = fd.kcur[s];
+  const bool skip = fd.stopped[s] != 0 || k0 >= nE;
+  if (skip || nE - k0 > W * kMwRows) {
+    if (!skip && tid == 0) atomicExch(err, 2);  // the launch does not cover the rows (never planned so)
+    if (lead && tid == 0) fd.pcount[s] = 0;
+    if (lead && tid == 0) storeCoherent(&sy->epoch, static_cast<unsigned>((base >> 16) + 1));
+    return;
+  }
+  const int nbEnd = min(k0 + kPanel, nE);
+  const int r0 = k0 + w * kMwRows;  // first row of this workgroup
+  double* curK = ws;
+  double* curR = ws + nE;
+  auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
+
+  const bool stat = prm.staticMode != 0;
+  double origMax = bitsToDouble(fd.origMaxBits[s]);
+  if (!stat && origMax < prm.zeroTol) origMax = 1.0;
+  if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
+  const double runMax0 = fmax(origMax, bitsToDouble(fd.runMaxBits[s]));
+  double tmax = 0.0;
+  const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+  const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+  int npos = 0, nneg = 0, nzero = 0, npert = 0;
+  double pert = 0.0;
+  int k = k0, p = 0;
+  bool stop = false;
+  unsigned step = 0;
+  if (tid == 0) abandon = 0;
+  __syncthreads();
+
+  // post this workgroup's step, then wait until every workgroup posted it
+  auto post = [&](unsigned st) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) storeCoherent(&sy->flag[w], base | st);
+  };
+  auto waitFor = [&](int from, int to, unsigned st) {
+    if (tid >= from && tid < to) {
+      unsigned spins = 0;
+      while (loadCoherent(&sy->flag[tid]) < (base | st)) {
+        if (loadCoherent(err) != 0) {
+          abandon = 1;
+          break;
+        }
+        if (++spins > (1u << 22)) {
+          atomicExch(err, 3);
+          abandon = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+  };
+
+  // (D L^T)(q, row) of this panel's pivot q for a row
+  auto weightCoherent = [&](int q, int row) -> double {
+    auto L = [&](int c) { return static_cast<double>(loadCoherent(&Fat(row, k0 + c))); };
+    const int kind = pkd[q];
+    if (kind == 1) return pdd[q] * L(q);
+    if (kind == 2) return pdd[q] * L(q) + pdo[q] * L(q + 1);
+    return pdo[q - 1] * L(q - 1) + pdd[q] * L(q);
+  };
+  int c0 = k, sw = 0;
+  auto loadSub = [&]() {
+    c0 = k;
+    sw = min(SW, nbEnd - k);
+    for (int e = tid; e < sw * p; e += NT) {
+      const int j = e % sw, q = e / sw;
+      Wsub[j][q] = weightCoherent(q, c0 + j);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int i = r0 + tid + jj * NT;
+      if (i < k || i >= nE) continue;
+      double acc[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) {
+        const int col = c0 + j;
+        acc[j] = (j < sw) ? static_cast<double>(i >= col ? Fat(i, col) : Fat(col, i)) : 0.0;
+      }
+      constexpr int U = 8;
+      int q = 0;
+      for (; q + U <= p; q += U) {
+        double lv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) lv[u] = static_cast<double>(Fat(i, k0 + q + u));
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < SW; ++j) acc[j] = fma(-lv[u], Wsub[j][q + u], acc[j]);
+      }
+      for (; q < p; ++q) {
+        const double l = Fat(i, k0 + q);
+#pragma unroll
+        for (int j = 0; j < SW; ++j) acc[j] = fma(-l, Wsub[j][q], acc[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < SW; ++j)
+        if (j < sw) Cs[j * CSR + (i - r0)] = acc[j];
+    }
+    __syncthreads();
+  };
+
+  // ---- general step (workgroup 0 alone; column buffers in global memory) ----
+  auto computeW = [&](int row) {
+    for (int q = tid; q < p; q += NT) {
+      const int kind = pkd[q];
+      double wq;
+      if (kind == 1)
+        wq = pdd[q] * static_cast<double>(Fat(row, k0 + q));
+      else if (kind == 2)
+        wq = pdd[q] * static_cast<double>(Fat(row, k0 + q)) + pdo[q] * static_cast<double>(Fat(row, k0 + q + 1));
+      else
+        wq = pdo[q - 1] * static_cast<double>(Fat(row, k0 + q - 1)) + pdd[q] * static_cast<double>(Fat(row, k0 + q));
+      wvec[q] = wq;
+    }
+    __syncthreads();
+  };
+  auto currentColumn = [&](int col, double* out) {
+    for (int i = k + tid; i < nE; i += NT) {
+      const double raw = (i >= col) ? Fat(i, col) : Fat(col, i);
+      double acc = 0.0;
+      for (int q = 0; q < p; ++q) acc = fma(static_cast<double>(Fat(i, k0 + q)), wvec[q], acc);
+      out[i] = raw - acc;
+    }
+    __syncthreads();
+  };
+  auto swapSym = [&](int a, int b) {
+    for (int t = tid; t < m; t += NT) {
+      if (t < a) {
+        const T x = Fat(a, t);
+        Fat(a, t) = Fat(b, t);
+        Fat(b, t) = x;
+      } else if (t == a) {
+        const T x = Fat(a, a);
+        Fat(a, a) = Fat(b, b);
+        Fat(b, b) = x;
+      } else if (t < b) {
+        const T x = Fat(t, a);
+        Fat(t, a) = Fat(b, t);
+        Fat(b, t) = x;
+      } else if (t > b) {
+        const T x = Fat(t, a);
+        Fat(t, a) = Fat(t, b);
+        Fat(t, b) = x;
+      }
+    }
+    if (tid == 0) {
+      const int tp = fd.perm[io + a];
+      fd.perm[io + a] = fd.perm[io + b];
+      fd.perm[io + b] = tp;
+      double x = curK[a];
+      curK[a] = curK[b];
+      curK[b] = x;
+      x = curR[a];
+      curR[a] = curR[b];
+      curR[b] = x;
+    }
+    __syncthreads();
+  };
+  auto accept1 = [&](const double* col, double d) {
+    if (tid == 0) {
+      fd.dDiag[io + k] = d;
+      fd.dOff[io + k] = 0.0;
+      fd.pkind[io + k] = 1;
+      pdd[p] = d;
+      pdo[p] = 0.0;
+      pkd[p] = 1;
+      Fat(k, k) = static_cast<T>(d);
+    }
+    countInertia(d, npos, nneg, nzero);
+    tmax = fmax(tmax, fabs(d));
+    for (int i = k + 1 + tid; i < nE; i += NT) {
+      const double c = col[i];
+      tmax = fmax(tmax, fabs(c));
+      Fat(i, k) = static_cast<T>(c / d);
+    }
+    __syncthreads();
+    ++k;
+    ++p;
+  };
+  // one general step at pivot k (sigma test, swaps, 1x1 or 2x2, or a stop);
+  // curK holds the current column k for the rows [k, nE)
+  auto generalStep = [&](double lv, int li, double akk) {
+    const int r = li;
+    computeW(r);
+    currentColumn(r, curR);
+    if (tid == 0) curK[r] = curR[k];  // dense_kernel.hpp:543
+    __syncthreads();
+    double sg = 0.0;
+    for (int i = k + tid; i < nE; i += NT)
+      if (i != r) sg = fmax(sg, fabs(curR[i]));
+    sg = blockMax<NT>(sg, redV);
+    int action;
+    if (fabs(akk) * sg >= kAlpha * lv * lv)
+      action = 1;
+    else if (fabs(curR[r]) >= kAlpha * sg)
+      action = 2;
+    else
+      action = 3;
+    if (action == 1) {
+      accept1(curK, curK[k]);
+      return;
+    }
+    if (action == 2) {
+      swapSym(k, r);
+      accept1(curR, curR[k]);
+      return;
+    }
+    if (r != k + 1) swapSym(k + 1, r);
+    const double d11 = curK[k];
+    const double d21 = curK[k + 1];
+    const double d22 = curR[k + 1];
+    const double det = d11 * d22 - d21 * d21;
+    const double rmax = fmax(runMax0, blockMax<NT>(tmax, redV));
+    if (fabs(det) <= prm.zeroTol * fmax(1.0, rmax) * fmax(1.0, rmax)) {
+      stop = true;
+      return;
+    }
+    tmax = fmax(tmax, fmax(fabs(d11), fmax(fabs(d21), fabs(d22))));
+    if (tid == 0) {
+      fd.dDiag[io + k] = d11;
+      fd.dOff[io + k] = d21;
+      fd.dDiag[io + k + 1] = d22;
+      fd.dOff[io + k + 1] = 0.0;
+      fd.pkind[io + k] = 2;
+      fd.pkind[io + k + 1] = 0;
+      pdd[p] = d11;
+      pdo[p] = d21;
+      pkd[p] = 2;
+      pdd[p + 1] = d22;
+      pdo[p + 1] = 0.0;
+      pkd[p + 1] = 0;
+      Fat(k, k) = static_cast<T>(d11);
+      Fat(k + 1, k) = static_cast<T>(d21);
+      Fat(k + 1, k + 1) = static_cast<T>(d22);
+    }
+    countInertia2(d11, d22, det, npos, nneg);
+    const double inv11 = d22 / det, inv21 = -d21 / det, inv22 = d11 / det;
+    for (int i = k + 2 + tid; i < nE; i += NT) {
+      const double a0 = curK[i], a1 = curR[i];
+      tmax = fmax(tmax, fmax(fabs(a0), fabs(a1)));
+      Fat(i, k) = static_cast<T>(a0 * inv11 + a1 * inv21);
+      Fat(i, k + 1) = static_cast<T>(a0 * inv21 + a1 * inv22);
+    }
+    __syncthreads();
+    k += 2;
+    p += 2;
+  };
+
+  loadSub();
+  while (k < nbEnd && !abandon) {
+    const int kk = k - c0;
+    const double* col = Cs + kk * CSR;
+    // local candidates of column k
+    double lv = 0.0;
+    int li = -1;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int i = r0 + tid + jj * NT;
+      if (i > k && i < nE) {
+        const double v = fabs(col[i - r0]);
+        if (v > 0.0) argmaxCombine(lv, li, v, i);
+      }
+    }
+    blockArgmax<NT>(lv, li, redV, redI);
+    const int par = step & 1;
+    ++step;
+    if (tid == 0) {
+      storeCoherent(&sy->lv[par][w], lv);
+      storeCoherent(&sy->li[par][w], li);
+      if (lead) {
+        storeCoherent(&sy->extra[par][0], col[k - r0]);
+        for (int j = kk + 1; j < sw; ++j) storeCoherent(&sy->extra[par][1 + j], col[c0 + j - r0]);
+      }
+    }
+    post(step);
+    waitFor(0, W, step);
+    if (abandon) break;
+    if (tid == 0) {
+      double bv = 0.0;
+      int bi = -1;
+      for (int v = 0; v < W; ++v) argmaxCombine(bv, bi, loadCoherent(&sy->lv[par][v]), loadCoherent(&sy->li[par][v]));
+      bcast[0] = bv;
+      bcastI[0] = bi;
+      bcast[1] = loadCoherent(&sy->extra[par][0]);
+      for (int j = kk + 1; j < sw; ++j) bcast[2 + j] = loadCoherent(&sy->extra[par][1 + j]);
+    }
+    __syncthreads();
+    lv = bcast[0];
+    li = bcastI[0];
+    const double akk = bcast[1];
+
+    // first Bunch-Kaufman tests (dense_kernel.hpp:601-618); static pivots
+    // are always accepted at k (dense_kernel.hpp:982-1011)
+    int action = -1;
+    double dpiv = akk;
+    if (stat) {
+      const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k]] : 0;
+      double applied;
+      dpiv = staticPivot(akk, sign, delta, sfloor, applied);
+      if (applied != 0.0) {
+        pert += fabs(applied);
+        ++npert;
+      }
+      action = 1;
+    } else if (lv <= prm.zeroTol) {
+      const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
+      action = (fabs(akk) <= isolatedFloor) ? 0 : 1;
+    } else if (fabs(akk) >= kAlpha * lv) {
+      action = 1;
+    }
+    if (action == 0) {
+      stop = true;
+      break;
+    }
+    if (action == 1) {
+      const double d = dpiv;
+      double wj[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j)
+        wj[j] = (j > kk && j < sw) ? d * static_cast<double>(static_cast<T>(bcast[2 + j] / d)) : 0.0;
+      pdd[p] = d;
+      pdo[p] = 0.0;
+      pkd[p] = 1;
+      if (lead && tid == 0) {
+        fd.dDiag[io + k] = d;
+        fd.dOff[io + k] = 0.0;
+        fd.pkind[io + k] = 1;
+        Fat(k, k) = static_cast<T>(d);
+      }
+      if (lead) countInertia(d, npos, nneg, nzero);
+      tmax = fmax(tmax, fabs(d));
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        const int i = r0 + tid + jj * NT;
+        if (i > k && i < nE) {
+          const double c = col[i - r0];
+          tmax = fmax(tmax, fabs(c));
+          const T lt = static_cast<T>(c / d);
+          // rows of the panel are read by every workgroup (their weights)
+          if (i < k0 + kPanel + 2)
+            storeCoherent(&Fat(i, k), lt);
+          else
+            Fat(i, k) = lt;
+          const double l = lt;
+#pragma unroll
+          for (int j = 0; j < SW; ++j)
+            if (j > kk && j < sw) Cs[j * CSR + (i - r0)] = fma(-l, wj[j], Cs[j * CSR + (i - r0)]);
+        }
+      }
+      __syncthreads();
+      ++k;
+      ++p;
+      if (k - c0 >= sw && k < nbEnd) loadSub();
+      continue;
+    }
+    // general step: everyone hands its rows of column k to curK, workgroup 0
+    // decides and updates, then everyone reloads its sub-panel
+    for (int jj = 0; jj < 2; ++jj) {
+      const int i = r0 + tid + jj * NT;
+      if (i >= k && i < nE) curK[i] = col[i - r0];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    ++step;
+    post(step);
+    if (lead) {
+      waitFor(0, W, step);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (!abandon) generalStep(lv, li, akk);
+      if (tid == 0) {
+        storeCoherent(&sy->state[0], k);
+        storeCoherent(&sy->state[1], p);
+        storeCoherent(&sy->state[2], stop ? 1 : 0);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      ++step;
+      post(step);
+    } else {
+      ++step;
+      waitFor(0, 1, step);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const int nk = loadCoherent(&sy->state[0]);
+      const int np = loadCoherent(&sy->state[1]);
+      stop = loadCoherent(&sy->state[2]) != 0;
+      // the pivots workgroup 0 took in the general step
+      for (int q = p + tid; q < np; q += NT) {
+        pdd[q] = fd.dDiag[io + k0 + q];
+        pdo[q] = fd.dOff[io + k0 + q];
+        pkd[q] = fd.pkind[io + k0 + q];
+      }
+      k = nk;
+      p = np;
+      __syncthreads();
+    }
+    if (stop || abandon) break;
+    if (k < nbEnd) loadSub();
+  }
+  __syncthreads();
+
+  const double bm = blockMax<NT>(tmax, redV);
+  if (tid == 0) atomicMaxNonNeg(fd.runMaxBits + s, fmax(bm, origMax));
+  if (lead && tid == 0) {
+    fd.kcur[s] = k;
+    fd.pcount[s] = p;
+    fd.stopped[s] = stop ? 1 : 0;
+    fd.inertia[3 * s] += npos;
+    fd.inertia[3 * s + 1] += nneg;
+    fd.inertia[3 * s + 2] += nzero;
+    fd.pert[s] += pert;
+    fd.npert[s] += npert;
+    storeCoherent(&sy->epoch, static_cast<unsigned>((base >> 16) + 1));
+  }
+}
+
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
 // panel.  With U = D L^T of the panel, the multipliers of a row are
 // f U^{-1} = (f L^{-T}) D^{-1}: every workgroup forms M = L^{-1} of the
@@ -1377,6 +1848,31 @@ void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, c
   else
     panelT<double>(list, count, maxRows, fd, prm, ws, wsOff, st);
   hipCheck(hipGetLastError(), "launchPanel");
+}
+void launchPanelMw(const int* list, int count, const int* wgStart, int blocks, const DevFronts& fd,
+                   const FactorParams& prm, double* ws, const int64_t* wsOff, MwSync* sync, int* err,
+                   hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  constexpr size_t lds = sizeof(double) * 8 * kMwRows;
+  static bool attrSet[2] = {false, false};
+  if (fd.fp32) {
+    if (!attrSet[1]) {
+      hipCheck(hipFuncSetAttribute(reinterpret_cast<const void*>(k_panel_mw<float>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+               "k_panel_mw LDS size");
+      attrSet[1] = true;
+    }
+    k_panel_mw<float><<<blocks, kPanelThreads, lds, st>>>(list, count, wgStart, fd, prm, ws, wsOff, sync, err);
+  } else {
+    if (!attrSet[0]) {
+      hipCheck(hipFuncSetAttribute(reinterpret_cast<const void*>(k_panel_mw<double>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+               "k_panel_mw LDS size");
+      attrSet[0] = true;
+    }
+    k_panel_mw<double><<<blocks, kPanelThreads, lds, st>>>(list, count, wgStart, fd, prm, ws, wsOff, sync, err);
+  }
+  hipCheck(hipGetLastError(), "launchPanelMw");
 }
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st) {

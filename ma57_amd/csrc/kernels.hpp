@@ -22,6 +22,10 @@ void launchFactorSmall(const int* list, int count, int maxM, const DevFronts& fd
 void launchFrontMaxAbs(const int* list, int count, int maxM, const DevFronts& fd, hipStream_t st);
 void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, const FactorParams& prm, double* ws,
                  const int64_t* wsOff, hipStream_t st);
+// cooperative panels: list[f] factored by workgroups [wgStart[f], wgStart[f+1]) with sync[f]
+void launchPanelMw(const int* list, int count, const int* wgStart, int blocks, const DevFronts& fd,
+                   const FactorParams& prm, double* ws, const int64_t* wsOff, MwSync* sync, int* err,
+                   hipStream_t st);
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st);
 void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,

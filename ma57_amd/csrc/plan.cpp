@@ -107,6 +107,11 @@ Plan::Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx) 
   dnorms_.alloc(16);
   dschur_.alloc(1);
   hipCheck(hipMemset(dschur_.p, 0, sizeof(unsigned long long)), "memset");
+  // records of the cooperative panels (tags only grow; zero once)
+  dmwSync_.alloc(kMwSlots);
+  hipCheck(hipMemset(dmwSync_.p, 0, sizeof(MwSync) * kMwSlots), "memset");
+  dfactErr_.alloc(1);
+  hipCheck(hipMemset(dfactErr_.p, 0, sizeof(int)), "memset");
   hipCheck(hipStreamSynchronize(stream_), "plan setup");
 }
 
@@ -363,6 +368,7 @@ void Plan::buildSchedule() {
   dlargeLists_.upload(lh.lists, stream_);
   dlargeRowStart_.upload(lh.rowStart, stream_);
   dlargeTileStart_.upload(lh.tileStart, stream_);
+  dlargeMwStart_.upload(lh.mwStart, stream_);
   dsmall_.upload(small, stream_);
   dtasks_.upload(tasks, stream_);
   dallFronts_.upload(all, stream_);
@@ -455,7 +461,7 @@ FactorParams Plan::params(const FactorOptions& o, bool staticMode, double delta,
 }
 
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h) {
+                           LargeSchedHost& h, bool allowMw) {
   LargeSched ls;
   ls.allOff = static_cast<int64_t>(h.lists.size());
   ls.allCount = static_cast<int>(fronts.size());
@@ -471,46 +477,85 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
     li.rowOff = static_cast<int64_t>(h.rowStart.size());
     li.tileOff = static_cast<int64_t>(h.tileStart.size());
     int rb = 0, tl = 0;
+    std::vector<int> reg, mw, mwW;
+    int mwTotal = 0;
     for (int s : fronts) {
       if ((nE[s] + kPanel - 1) / kPanel <= it) continue;
       h.lists.push_back(s);
       h.rowStart.push_back(rb);
       h.tileStart.push_back(tl);
-      li.maxNE = std::max(li.maxNE, nE[s]);
       rb += (m[s] - nE[s] + 255) / 256;
       const int T = (m[s] - it * kPanel + 63) / 64;
       tl += T * (T + 1) / 2;
       ++li.count;
+      // eligible rows left at this iteration's first pivot (no delays: it * kPanel)
+      const int rows = nE[s] - it * kPanel;
+      const int W = (rows + kMwRows - 1) / kMwRows;
+      if (allowMw && rows > kMwThreshold && W <= kMwMaxW && static_cast<int>(mw.size()) < kMwSlots &&
+          mwTotal + W <= kMwMaxBlocks) {
+        mw.push_back(s);
+        mwW.push_back(W);
+        mwTotal += W;
+      } else {
+        reg.push_back(s);
+        li.regMaxNE = std::max(li.regMaxNE, nE[s]);
+      }
     }
     h.rowStart.push_back(rb);
     h.tileStart.push_back(tl);
     li.rowBlocks = rb;
     li.tiles = tl;
+    li.regOff = static_cast<int64_t>(h.lists.size());
+    li.regCount = static_cast<int>(reg.size());
+    h.lists.insert(h.lists.end(), reg.begin(), reg.end());
+    li.mwOff = static_cast<int64_t>(h.lists.size());
+    li.mwCount = static_cast<int>(mw.size());
+    h.lists.insert(h.lists.end(), mw.begin(), mw.end());
+    li.mwStartOff = static_cast<int64_t>(h.mwStart.size());
+    int acc = 0;
+    for (int W : mwW) {
+      h.mwStart.push_back(acc);
+      acc += W;
+    }
+    h.mwStart.push_back(acc);
+    li.mwBlocks = acc;
     ls.iters.push_back(li);
   }
   return ls;
 }
 
-void runLargeSched(const LargeSched& ls, const int* dLists, const int* dRowStart, const int* dTileStart,
-                   const DevFronts& fd, const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st,
-                   Plan* prof) {
+void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
+                   const int64_t* wsOff, hipStream_t st, Plan* prof) {
   if (ls.allCount == 0) return;
-  launchFrontMaxAbs(dLists + ls.allOff, ls.allCount, ls.maxM, fd, st);
+  launchFrontMaxAbs(dv.lists + ls.allOff, ls.allCount, ls.maxM, fd, st);
   for (const auto& it : ls.iters) {
-    const int* lst = dLists + it.listOff;
+    const int* lst = dv.lists + it.listOff;
     if (prof) prof->profBegin(Plan::kProfPanel);
-    launchPanel(lst, it.count, it.maxNE, fd, prm, ws, wsOff, st);
+    launchPanel(dv.lists + it.regOff, it.regCount, it.regMaxNE, fd, prm, ws, wsOff, st);
+    launchPanelMw(dv.lists + it.mwOff, it.mwCount, dv.mwStart + it.mwStartOff, it.mwBlocks, fd, prm, ws, wsOff,
+                  dv.sync, dv.err, st);
     if (prof) prof->profEnd(Plan::kProfPanel);
     if (it.rowBlocks > 0) {
       if (prof) prof->profBegin(Plan::kProfPanelRows);
-      launchPanelRows(lst, it.count, dRowStart + it.rowOff, it.rowBlocks, fd, st);
+      launchPanelRows(lst, it.count, dv.rowStart + it.rowOff, it.rowBlocks, fd, st);
       if (prof) prof->profEnd(Plan::kProfPanelRows);
     }
     if (prof) prof->profBegin(Plan::kProfSyrk);
-    launchSyrk(lst, it.count, dTileStart + it.tileOff, it.tiles, fd, st);
+    launchSyrk(lst, it.count, dv.tileStart + it.tileOff, it.tiles, fd, st);
     if (prof) prof->profEnd(Plan::kProfSyrk);
   }
-  launchFrontFinish(dLists + ls.allOff, ls.allCount, fd, st);
+  launchFrontFinish(dv.lists + ls.allOff, ls.allCount, fd, st);
+}
+
+LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart) {
+  LargeDev d;
+  d.lists = lists;
+  d.rowStart = rowStart;
+  d.tileStart = tileStart;
+  d.mwStart = mwStart;
+  d.sync = dmwSync_.p;
+  d.err = dfactErr_.p;
+  return d;
 }
 
 void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
@@ -532,7 +577,8 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
       launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
       profEnd(kProfSmall);
     }
-    runLargeSched(ls.lsched, dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, fd, prm, dws_.p, dwsOff_.p,
+    runLargeSched(ls.lsched, largeDev(dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, dlargeMwStart_.p), fd,
+                  prm, dws_.p, dwsOff_.p,
                   stream_, profiling_ ? this : nullptr);
   }
   launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
@@ -548,9 +594,16 @@ void Plan::collectStats(FactorInfo& info, bool& anyDelay) {
   }
   long long si[8];
   double sd[4];
+  int ferr = 0;
   hipCheck(hipMemcpyAsync(si, dstatI_.p, 5 * sizeof(long long), hipMemcpyDeviceToHost, stream_), "stats");
   hipCheck(hipMemcpyAsync(sd, dstatD_.p, sizeof(double), hipMemcpyDeviceToHost, stream_), "stats");
+  hipCheck(hipMemcpyAsync(&ferr, dfactErr_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "stats");
   hipCheck(hipStreamSynchronize(stream_), "factorize");
+  if (ferr != 0) {
+    hipCheck(hipMemset(dfactErr_.p, 0, sizeof(int)), "error word");
+    throw DeviceError("symla: a cooperative panel step exceeded its bounded wait (code " + std::to_string(ferr) +
+                      ")");
+  }
   info.pos = static_cast<int>(si[0]);
   info.neg = static_cast<int>(si[1]);
   info.zero = static_cast<int>(si[2]);
@@ -692,13 +745,14 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
         off += 2 * static_cast<int64_t>(hnE_[s]);
       }
       dws_.ensure(static_cast<size_t>(std::max<int64_t>(off, 2)));
-      DBuf<int> dl, dr, dt;
+      DBuf<int> dl, dr, dt, dmw;
       DBuf<int64_t> dw;
       dl.upload(lh.lists, stream_);
       dr.upload(lh.rowStart, stream_);
       dt.upload(lh.tileStart, stream_);
+      dmw.upload(lh.mwStart, stream_);
       dw.upload(wsOffH, stream_);
-      runLargeSched(lsd, dl.p, dr.p, dt.p, fd, prm, dws_.p, dw.p, stream_, nullptr);
+      runLargeSched(lsd, largeDev(dl.p, dr.p, dt.p, dmw.p), fd, prm, dws_.p, dw.p, stream_, nullptr);
       hipCheck(hipStreamSynchronize(stream_), "dynamic large fronts");
     }
     launchMakeRowIdx(dlist.p, static_cast<int>(all.size()), fd, stream_);
@@ -1137,7 +1191,7 @@ std::vector<const void*> Plan::graphFingerprint() const {
                                 dlargeRowStart_.p, dlargeTileStart_.p, dwsOff_.p, dpermVec_.p, dBp_.p, dZ_.p,
                                 dX_.p,      dG_.p,      dDl_.p,      dgOff_.p,    dgatherStart_.p, dfwdStart_.p,
                                 dbwdStart_.p, dflagOff_.p, dflagsF_.p, dflagsB_.p, dgen_.p, dsolveErr_.p,
-                                dschur_.p,  dsolveLists_.p};
+                                dschur_.p,  dsolveLists_.p, dlargeMwStart_.p, dmwSync_.p, dfactErr_.p};
   for (void* f : hF_) v.push_back(f);
   return v;
 }

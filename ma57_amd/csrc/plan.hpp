@@ -83,15 +83,25 @@ struct RefineResult {
 
 // Batched schedule of the large-front path for one set of independent fronts
 // (one tree level): per panel iteration, the fronts that still have eligible
-// columns, their row blocks and their trailing-update tiles.
+// columns, their row blocks and their trailing-update tiles.  Fronts with
+// more than kMwThreshold eligible rows left take the cooperative panel
+// (several workgroups per front); the others one workgroup each.
+constexpr int kMwThreshold = 4096;
+constexpr int kMwMaxBlocks = 240;  // cooperative workgroups per launch (all resident at once)
 struct LargeIter {
-  int64_t listOff = 0;
+  int64_t listOff = 0;  // all fronts of the iteration (rows, trailing update)
   int count = 0;
-  int maxNE = 0;  // widest eligible-row count among the listed fronts
   int64_t rowOff = 0;
   int rowBlocks = 0;
   int64_t tileOff = 0;
   int tiles = 0;
+  int64_t regOff = 0;  // single-workgroup panels
+  int regCount = 0;
+  int regMaxNE = 0;    // widest eligible-row count among them
+  int64_t mwOff = 0;   // cooperative panels: fronts, and workgroup starts at mwStartOff
+  int mwCount = 0;
+  int64_t mwStartOff = 0;
+  int mwBlocks = 0;
 };
 struct LargeSched {
   int64_t allOff = 0;
@@ -100,16 +110,24 @@ struct LargeSched {
   std::vector<LargeIter> iters;
 };
 struct LargeSchedHost {
-  std::vector<int> lists, rowStart, tileStart;
+  std::vector<int> lists, rowStart, tileStart, mwStart;
 };
-// m and nE are indexed by front id.
+// Device copies of a LargeSchedHost plus the cooperative-panel records.
+struct LargeDev {
+  const int* lists = nullptr;
+  const int* rowStart = nullptr;
+  const int* tileStart = nullptr;
+  const int* mwStart = nullptr;
+  MwSync* sync = nullptr;
+  int* err = nullptr;
+};
+// m and nE are indexed by front id; allowMw = false keeps every panel on one workgroup.
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h);
+                           LargeSchedHost& h, bool allowMw = true);
 class Plan;
 // Launch sequence of a LargeSched; `prof` (may be null) brackets launches with events.
-void runLargeSched(const LargeSched& ls, const int* dLists, const int* dRowStart, const int* dTileStart,
-                   const DevFronts& fd, const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st,
-                   Plan* prof);
+void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
+                   const int64_t* wsOff, hipStream_t st, Plan* prof);
 
 class Plan {
  public:
@@ -298,7 +316,10 @@ class Plan {
   FactorParams params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const;
   void runStatic(const double* aValsDev, const FactorParams& prm);
   void runDynamic(const double* aValsDev, const FactorParams& prm);
-  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_;
+  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_, dlargeMwStart_;
+  DBuf<MwSync> dmwSync_;
+  DBuf<int> dfactErr_;  // cooperative-panel wait errors
+  LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart);
   DBuf<int64_t> dwsOff_;
   std::vector<int64_t> hwsOff_;
   void collectStats(FactorInfo& info, bool& anyDelay);

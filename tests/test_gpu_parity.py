@@ -121,6 +121,28 @@ def test_dense_front_schur_complement(n, ne, seed, oracle):
     assert np.abs(Sg - So).max() <= 1e-11 * max(1.0, np.abs(So).max())
 
 
+@pytest.mark.parametrize("n,ne", [(5000, 5000), (5200, 4800)])
+def test_dense_cooperative_panel(n, ne):
+    """Fronts with more than 4096 eligible rows take the cooperative (multi-workgroup) panel: an indefinite
+    random front needs 2x2 pivots and swaps (the general step); checked against LAPACK's inertia
+    (Sylvester), the reconstruction of the eligible block, and the Schur complement of the rest."""
+    rng = np.random.default_rng(n)
+    R = rng.standard_normal((n, n))
+    A = (R + R.T) / 2.0
+    g = dense_factor(A, n_eligible=ne)
+    assert g["n_factored"] == ne
+    assert any(k == 2 for _, k in g["pivots"])
+    A11 = A[:ne, :ne]
+    pos, neg = _lapack_inertia(A11)
+    assert g["inertia"] == (pos, neg, 0)
+    p = g["perm"][:ne]
+    assert np.linalg.norm(_reconstruct(g, ne) - A11[np.ix_(p, p)]) / np.linalg.norm(A11) < 1e-10
+    if ne < n:
+        S = A[ne:, ne:] - A[ne:, :ne] @ np.linalg.solve(A11, A[:ne, ne:])
+        Sg = np.tril(g["A"][ne:, ne:])
+        assert np.abs(Sg - np.tril(S)).max() <= 1e-8 * max(1.0, np.abs(S).max())
+
+
 @pytest.mark.parametrize("n", [30, 250, 400])
 def test_dense_static_matches_oracle(n, oracle):
     """Quasi-definite front (diagonal dominant with the expected signs) plus a few wrong-signed and tiny
