@@ -61,7 +61,7 @@ struct MwSync {
   unsigned long long flag[kMwMaxW];  // tag of the last step each workgroup posted
   double lv[2][kMwMaxW];             // per-step local max |a_ik| (double-buffered by step parity)
   int li[2][kMwMaxW];                // and its row
-  double extra[2][12];               // workgroup 0: a_kk and the sub-panel rows of the pivot column
+  double extra[2][12];               // workgroup 0: a_kk and the pivot column at rows k + 1 .. k + 8
   int state[4];                      // workgroup 0 after a general step: k, p, stop
   unsigned epoch;                    // launches so far on this record (tags stay unique)
   int pad;

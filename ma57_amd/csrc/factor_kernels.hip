@@ -964,6 +964,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
   __shared__ int redI[NT / 64 + 1];
   __shared__ double bcast[16];
   __shared__ int bcastI[4];
+  __shared__ double lastL[SW + 1];  // multipliers of the latest fast pivot at rows k + 1 .. k + SW
   __shared__ int abandon;
 
   const int tid = threadIdx.x;
@@ -1043,13 +1044,18 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
     if (kind == 2) return pdd[q] * L(q) + pdo[q] * L(q + 1);
     return pdo[q - 1] * L(q - 1) + pdd[q] * L(q);
   };
+  // lastFast: the pivot just taken (p - 1, at row c0 - 1) was a fast 1x1
+  // whose multipliers workgroup 0 may still be storing, so its weights come
+  // from lastL (the same values, formed from the broadcast pivot column);
+  // every earlier pivot's multipliers were stored before workgroup 0 posted
+  // the step that followed it.
   int c0 = k, sw = 0;
-  auto loadSub = [&]() {
+  auto loadSub = [&](bool lastFast) {
     c0 = k;
     sw = min(SW, nbEnd - k);
     for (int e = tid; e < sw * p; e += NT) {
       const int j = e % sw, q = e / sw;
-      Wsub[j][q] = weightCoherent(q, c0 + j);
+      Wsub[j][q] = (lastFast && q == p - 1) ? pdd[q] * lastL[1 + j] : weightCoherent(q, c0 + j);
     }
     __syncthreads();
 #pragma unroll
@@ -1062,19 +1068,22 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
         const int col = c0 + j;
         acc[j] = (j < sw) ? static_cast<double>(i >= col ? Fat(i, col) : Fat(col, i)) : 0.0;
       }
+      // the panel's own rows were stored device-coherent: read them the same way
+      const bool panelRow = i < k0 + kPanel + 2;
       constexpr int U = 8;
       int q = 0;
       for (; q + U <= p; q += U) {
         double lv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) lv[u] = static_cast<double>(Fat(i, k0 + q + u));
+        for (int u = 0; u < U; ++u)
+          lv[u] = static_cast<double>(panelRow ? loadCoherent(&Fat(i, k0 + q + u)) : Fat(i, k0 + q + u));
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
           for (int j = 0; j < SW; ++j) acc[j] = fma(-lv[u], Wsub[j][q + u], acc[j]);
       }
       for (; q < p; ++q) {
-        const double l = Fat(i, k0 + q);
+        const double l = panelRow ? loadCoherent(&Fat(i, k0 + q)) : Fat(i, k0 + q);
 #pragma unroll
         for (int j = 0; j < SW; ++j) acc[j] = fma(-l, Wsub[j][q], acc[j]);
       }
@@ -1232,7 +1241,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
     p += 2;
   };
 
-  loadSub();
+  loadSub(false);
   while (k < nbEnd && !abandon) {
     const int kk = k - c0;
     const double* col = Cs + kk * CSR;
@@ -1254,8 +1263,9 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       storeCoherent(&sy->lv[par][w], lv);
       storeCoherent(&sy->li[par][w], li);
       if (lead) {
+        // a_kk and the pivot column's next SW rows (rows k + t < nbEnd)
         storeCoherent(&sy->extra[par][0], col[k - r0]);
-        for (int j = kk + 1; j < sw; ++j) storeCoherent(&sy->extra[par][1 + j], col[c0 + j - r0]);
+        for (int t = 1; t <= SW && k + t < nbEnd; ++t) storeCoherent(&sy->extra[par][t], col[k + t - r0]);
       }
     }
     post(step);
@@ -1268,7 +1278,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       bcast[0] = bv;
       bcastI[0] = bi;
       bcast[1] = loadCoherent(&sy->extra[par][0]);
-      for (int j = kk + 1; j < sw; ++j) bcast[2 + j] = loadCoherent(&sy->extra[par][1 + j]);
+      for (int t = 1; t <= SW && k + t < nbEnd; ++t) bcast[1 + t] = loadCoherent(&sy->extra[par][t]);
     }
     __syncthreads();
     lv = bcast[0];
@@ -1303,10 +1313,11 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       double wj[SW];
 #pragma unroll
       for (int j = 0; j < SW; ++j)
-        wj[j] = (j > kk && j < sw) ? d * static_cast<double>(static_cast<T>(bcast[2 + j] / d)) : 0.0;
+        wj[j] = (j > kk && j < sw) ? d * static_cast<double>(static_cast<T>(bcast[1 + j - kk] / d)) : 0.0;
       pdd[p] = d;
       pdo[p] = 0.0;
       pkd[p] = 1;
+      if (tid >= 1 && tid <= SW && k + tid < nbEnd) lastL[tid] = static_cast<double>(static_cast<T>(bcast[1 + tid] / d));
       if (lead && tid == 0) {
         fd.dDiag[io + k] = d;
         fd.dOff[io + k] = 0.0;
@@ -1336,7 +1347,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       __syncthreads();
       ++k;
       ++p;
-      if (k - c0 >= sw && k < nbEnd) loadSub();
+      if (k - c0 >= sw && k < nbEnd) loadSub(true);
       continue;
     }
     // general step: everyone hands its rows of column k to curK, workgroup 0
@@ -1378,7 +1389,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       __syncthreads();
     }
     if (stop || abandon) break;
-    if (k < nbEnd) loadSub();
+    if (k < nbEnd) loadSub(false);
   }
   __syncthreads();
 
