@@ -101,6 +101,10 @@ def main() -> int:
     ap.add_argument("--replays", type=int, default=10, help="c5: recorded refactorize + solve replays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: min(16, cpu count))")
+    ap.add_argument("--exact-panel", action="store_true",
+                    help="panels without the speculative blocked step (pivot-by-pivot kernel only)")
+    ap.add_argument("--spec-min-rows", type=int, default=None,
+                    help="fronts with more eligible rows than this take the speculative panel step")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,6 +130,10 @@ def main() -> int:
     n = A.shape[0]
     sf = solver.symbolicFactor()
     plan = solver._plan
+    if args.exact_panel or args.spec_min_rows is not None:
+        small, tile = solver.tuning()
+        solver.setTuning(small, tile, not args.exact_panel, args.spec_min_rows)
+    tuning = solver._get_tuning()
     B = np.random.default_rng(SEED + int(args.config[1])).standard_normal((n, nrhs))
 
     _, _, _, vals = csc_arrays(A)
@@ -271,7 +279,9 @@ def main() -> int:
                                     f"refinement, fp64: plain FP64 factorize + solve",
                         "name": args.config, "n": n, "nrhs": nrhs, "nnz_L": int(sf.nnz_l),
                         "supernodes": len(sf.supernodes),
-                        "max_front": max(len(s.rowPattern) for s in sf.supernodes)}, **extra),
+                        "max_front": max(len(s.rowPattern) for s in sf.supernodes),
+                        "speculative_panel": bool(tuning.speculative_panel),
+                        "speculative_min_rows": tuning.speculative_min_rows}, **extra),
         "roofline": roofline("k_syrk_f32", mixed, PEAK_FP32_MFMA_TFLOPS),
         "cpu_baseline": cpu,
         "fp32_ir": mixed,

@@ -196,6 +196,7 @@ typedef struct {
   double static_rel_floor;
   double static_abs_floor;
   const int* expected_sign; /* static mode, size n or NULL */
+  int exact_panel;          /* 1: no speculative panel step (pivot-by-pivot kernel only) */
 } symla_dense_opts;
 int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_dense_opts* opts, double* d_diag,
                        double* d_off, int* perm, int* pivot_start, int* pivot_kind, int* n_pivot_blocks,
@@ -205,7 +206,11 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
  * front kernel is one CPU routine): fronts of order <= small_front_max
  * (at most 192) are factored whole in LDS, larger ones by the panel path;
  * assembly runs in column tiles of assemble_tile (a power of two in
- * [16, 1024]).  symla_plan_autotune times factorize + solve of the values at
+ * [16, 1024]).  With speculative_panel set, each panel of a large front is
+ * first factored as 1x1 pivots in order and the threshold tests checked for
+ * all its columns at once; the exact pivot-by-pivot kernel continues from
+ * the first column that fails (the same pivot decisions, up to rounding of
+ * the tested values).  symla_plan_autotune times factorize + solve of the values at
  * d_a_vals (device) with nrhs fixed pseudo-random right-hand sides, best of
  * reps, over a grid of choices in the given precision, keeps the fastest
  * (returned in *chosen) and reports every trial (up to cap records,
@@ -214,6 +219,8 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
 typedef struct {
   int small_front_max;
   int assemble_tile;
+  int speculative_panel;    /* 1 (default): speculative blocked step on large fronts; 0: exact steps only */
+  int speculative_min_rows; /* fronts with more eligible rows than this take it (default 2048) */
 } symla_tuning;
 typedef struct {
   symla_tuning tuning;

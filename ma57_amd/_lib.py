@@ -59,11 +59,12 @@ class RefineInfo(C.Structure):
 class DenseOpts(C.Structure):
     _fields_ = [("zero_tolerance", C.c_double), ("relative_pivot_floor", C.c_double), ("static_mode", C.c_int),
                 ("static_delta", C.c_double), ("static_rel_floor", C.c_double), ("static_abs_floor", C.c_double),
-                ("expected_sign", i32p)]
+                ("expected_sign", i32p), ("exact_panel", C.c_int)]
 
 
 class Tuning(C.Structure):
-    _fields_ = [("small_front_max", C.c_int), ("assemble_tile", C.c_int)]
+    _fields_ = [("small_front_max", C.c_int), ("assemble_tile", C.c_int), ("speculative_panel", C.c_int),
+                ("speculative_min_rows", C.c_int)]
 
 
 class TuningTrial(C.Structure):

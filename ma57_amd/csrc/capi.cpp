@@ -235,13 +235,18 @@ int symla_plan_get_tuning(const symla_plan* p, symla_tuning* out) {
   if (!p || !out) return fail(SYMLA_ERR_INVALID, "symla_plan_get_tuning: null argument");
   out->small_front_max = p->plan->tuning().smallMax;
   out->assemble_tile = p->plan->tuning().asmTile;
+  out->speculative_panel = p->plan->tuning().specPanel;
+  out->speculative_min_rows = p->plan->tuning().specMinRows;
   return SYMLA_OK;
 }
 
 int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t) {
   return planGuarded(p, [&] {
     if (!p || !t) throw std::invalid_argument("symla_plan_set_tuning: null argument");
-    p->plan->setTuning(Plan::Tuning{t->small_front_max, t->assemble_tile});
+    if (t->speculative_panel != 0 && t->speculative_panel != 1)
+      throw std::invalid_argument("symla_plan_set_tuning: speculative_panel must be 0 or 1");
+    p->plan->setTuning(
+        Plan::Tuning{t->small_front_max, t->assemble_tile, t->speculative_panel, t->speculative_min_rows});
   });
 }
 
@@ -253,11 +258,15 @@ int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int pre
     if (chosen) {
       chosen->small_front_max = p->plan->tuning().smallMax;
       chosen->assemble_tile = p->plan->tuning().asmTile;
+      chosen->speculative_panel = p->plan->tuning().specPanel;
+      chosen->speculative_min_rows = p->plan->tuning().specMinRows;
     }
     if (ntrials) *ntrials = static_cast<int>(tr.size());
     for (int i = 0; trials && i < cap && i < static_cast<int>(tr.size()); ++i) {
       trials[i].tuning.small_front_max = tr[i].t.smallMax;
       trials[i].tuning.assemble_tile = tr[i].t.asmTile;
+      trials[i].tuning.speculative_panel = tr[i].t.specPanel;
+      trials[i].tuning.speculative_min_rows = tr[i].t.specMinRows;
       trials[i].ms = tr[i].ms;
     }
   });
@@ -384,7 +393,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
                               hipMemcpyHostToDevice, st),
              "A upload");
     DBuf<int> dm, dnE, dnc, dfirst, dfi, drel, dcp, dcl, daRow, dperm, drow, dpk, dnF, dinert, dkcur, dpc, dstop,
-        dnpert, dlist, dsign;
+        dnpert, dlist, dsign, dcb;
     DBuf<int64_t> dio, dacp, daSrc;
     DBuf<double> ddd, ddo, dpert, dws;
     DBuf<void*> dF;
@@ -414,11 +423,12 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     dkcur.alloc(1);
     dpc.alloc(1);
     dstop.alloc(1);
+    dcb.alloc(1);
     dnpert.alloc(1);
     dpert.alloc(1);
     dom.alloc(1);
     drm.alloc(1);
-    dws.alloc(2 * static_cast<size_t>(nE) + 2);
+    dws.alloc(static_cast<size_t>(panelWorkspace(nE)));
     dF.upload(std::vector<void*>{dA.p}, st);
     upI(dlist, {0});
     const int* signPtr = nullptr;
@@ -453,6 +463,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     f.kcur = dkcur.p;
     f.pcount = dpc.p;
     f.stopped = dstop.p;
+    f.cbDone = dcb.p;
     f.pert = dpert.p;
     f.npert = dnpert.p;
     f.schurFlops = nullptr;
@@ -464,6 +475,8 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     prm.staticRelFloor = opts ? opts->static_rel_floor : 1e-8;
     prm.staticAbsFloor = opts ? opts->static_abs_floor : 1e-300;
     prm.expectedSign = signPtr;
+    prm.specPanel = (opts && opts->exact_panel) ? 0 : 1;
+    prm.specMinRows = 0;  // every panel of the dense entry takes the speculative step
     launchResetFronts(1, f, st);
     launchResetPerm(dlist.p, 1, f, st);
     if (n <= kSmallFrontMax) {
@@ -471,14 +484,15 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     } else {
       std::vector<int> mv{n}, nev{nE};
       LargeSchedHost lh;
-      LargeSched ls = buildLargeSched({0}, mv, nev, lh);
-      DBuf<int> dl, dr, dt, dmw, derr;
+      LargeSched ls = buildLargeSched({0}, mv, nev, lh, (opts && opts->exact_panel) ? -1 : 0);
+      DBuf<int> dl, dr, dt, dmw, dsp, derr;
       DBuf<int64_t> dw;
       DBuf<MwSync> dsync;
       dl.upload(lh.lists, st);
       dr.upload(lh.rowStart, st);
       dt.upload(lh.tileStart, st);
       dmw.upload(lh.mwStart, st);
+      dsp.upload(lh.specStart, st);
       dw.upload(std::vector<int64_t>{0}, st);
       dsync.alloc(kMwSlots);
       derr.alloc(1);
@@ -489,6 +503,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
       dv.rowStart = dr.p;
       dv.tileStart = dt.p;
       dv.mwStart = dmw.p;
+      dv.specStart = dsp.p;
       dv.sync = dsync.p;
       dv.err = derr.p;
       runLargeSched(ls, dv, f, prm, dws.p, dw.p, st, nullptr);

@@ -46,6 +46,7 @@ struct DevFronts {
   int* kcur;             // next pivot position (panel path)
   int* pcount;           // pivots in the last panel (panel path)
   int* stopped;          // 1 when a front stopped on a delay
+  int* cbDone;           // 1 when the speculative panel also wrote the last panel's rows [nE, m)
   double* pert;          // static mode: sum |perturbation|
   int* npert;            // static mode: perturbed pivots
   unsigned long long* schurFlops;  // algorithmic flops of the MFMA trailing updates (p*r*(r+1) per update)
@@ -75,6 +76,26 @@ struct FactorParams {
   double staticRelFloor;
   double staticAbsFloor;
   const int* expectedSign;  // final-order expected signs (static mode) or null
+  int specPanel;         // 1: panels of fronts with more than specMinRows eligible rows
+  int specMinRows;       //    start with the speculative blocked step
 };
+__host__ __device__ inline bool specFront(const FactorParams& p, int nE) { return p.specPanel && nE > p.specMinRows; }
+
+// Speculative panel step: a panel's pivots are first taken in order as 1x1
+// without interchanges (diagonal block factored in LDS, the rows below by
+// forward substitution), then the threshold tests are checked for every
+// column at once; the prefix that passes is kept and the exact panel kernel
+// continues from the first column that fails.  Its per-front record lives in
+// the front's workspace after the two column buffers (2 * nE doubles):
+//   colmax[2][kPanel]  max |c_iq| over the eligible rows below the block
+//                      (bits of non-negative doubles; double-buffered by
+//                      panel iteration parity)
+//   cdiag[kPanel]      a_qq at pivot q (before any static perturbation)
+//   dval[kPanel]       the pivot d_q
+//   bmax[kPanel]       max |c_iq| over the block rows below q
+//   lblk[kPanel * kPanel]  multipliers of the block, l_iq at [i * kPanel + q]
+constexpr int kSpecRec = 5 * kPanel + kPanel * kPanel;
+constexpr int kSpecRows = 256;  // rows below the block per workgroup
+__host__ __device__ inline int64_t panelWorkspace(int nE) { return 2 * static_cast<int64_t>(nE) + kSpecRec; }
 
 }  // namespace symla_hip

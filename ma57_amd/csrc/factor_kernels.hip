@@ -157,6 +157,7 @@ __global__ void k_reset_fronts(int numSN, DevFronts fd) {
     fd.kcur[s] = 0;
     fd.pcount[s] = 0;
     fd.stopped[s] = 0;
+    fd.cbDone[s] = 0;
     fd.pert[s] = 0.0;
     fd.npert[s] = 0;
   }
@@ -574,12 +575,23 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   const int nE = fd.nE[s];
   T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
-  const int k0 = fd.kcur[s];
-  if (k0 >= nE) {
-    if (tid == 0) fd.pcount[s] = 0;
+  // after a speculative step the panel started p0 pivots before kcur
+  const int kc = fd.kcur[s];
+  const bool sp = specFront(prm, nE);
+  const int p0 = sp ? fd.pcount[s] : 0;
+  const int k0 = kc - p0;
+  if (kc >= nE) {
+    if (tid == 0 && !sp) fd.pcount[s] = 0;
     return;
   }
   const int nbEnd = min(k0 + kPanel, nE);
+  if (kc >= nbEnd) return;
+  for (int q = tid; q < p0; q += NT) {
+    pdd[q] = fd.dDiag[io + k0 + q];
+    pdo[q] = fd.dOff[io + k0 + q];
+    pkd[q] = fd.pkind[io + k0 + q];
+  }
+  __syncthreads();
   double* curK = ws;
   double* curR = ws + nE;
   auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
@@ -595,7 +607,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
 
   int npos = 0, nneg = 0, nzero = 0, npert = 0;
   double pert = 0.0;
-  int k = k0, p = 0;
+  int k = kc, p = p0;
   bool stop = false;
 
   // (D L^T)(q, row) for this panel's pivot q and a given row
@@ -981,15 +993,19 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
   const int nE = fd.nE[s];
   T* F = static_cast<T*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
-  const int k0 = fd.kcur[s];
-  const bool skip = fd.stopped[s] != 0 || k0 >= nE;
+  // after a speculative step the panel started p0 pivots before kcur
+  const int kc = fd.kcur[s];
+  const bool sp = specFront(prm, nE);
+  const int p0 = sp ? fd.pcount[s] : 0;
+  const int k0 = kc - p0;
+  const int nbEnd = min(k0 + kPanel, nE);
+  const bool skip = fd.stopped[s] != 0 || kc >= nbEnd;
   if (skip || nE - k0 > W * kMwRows) {
     if (!skip && tid == 0) atomicExch(err, 2);  // the launch does not cover the rows (never planned so)
-    if (lead && tid == 0) fd.pcount[s] = 0;
+    if (lead && tid == 0 && !sp) fd.pcount[s] = 0;
     if (lead && tid == 0) storeCoherent(&sy->epoch, static_cast<unsigned>((base >> 16) + 1));
     return;
   }
-  const int nbEnd = min(k0 + kPanel, nE);
   const int r0 = k0 + w * kMwRows;  // first row of this workgroup
   double* curK = ws;
   double* curR = ws + nE;
@@ -1005,10 +1021,15 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
   const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
   int npos = 0, nneg = 0, nzero = 0, npert = 0;
   double pert = 0.0;
-  int k = k0, p = 0;
+  int k = kc, p = p0;
   bool stop = false;
   unsigned step = 0;
   if (tid == 0) abandon = 0;
+  for (int q = tid; q < p0; q += NT) {
+    pdd[q] = fd.dDiag[io + k0 + q];
+    pdo[q] = fd.dOff[io + k0 + q];
+    pkd[q] = fd.pkind[io + k0 + q];
+  }
   __syncthreads();
 
   // post this workgroup's step, then wait until every workgroup posted it
@@ -1408,6 +1429,314 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// Speculative panel step.  The threshold Bunch-Kaufman panel
+// (dense_kernel.hpp:585-792) takes pivot k as a 1x1 without an interchange
+// whenever |a_kk| >= alpha * lambda_k (lambda_k = max over the eligible rows
+// below k of |a_ik|, current values), or lambda_k <= zero_tolerance and
+// |a_kk| is above the isolated-pivot floor; static pivots are always taken
+// at k (dense_kernel.hpp:982-1011).  When every pivot of a panel is taken
+// that way, the panel is an unpivoted LDL^T of its diagonal block followed by
+// the substitution of the rows below, and lambda_k is max |c_ik| over the
+// values c_ik the substitution forms.  k_spec_factor does that factorization
+// and the column maxima without writing the front; k_spec_apply finds the
+// first column whose test fails, keeps the columns before it, and leaves the
+// rest (still holding current values) to the exact panel kernel, which
+// continues from there.
+// ---------------------------------------------------------------------------
+struct SpecRec {
+  unsigned long long* colmax;  // [2][kPanel]
+  double* cdiag;
+  double* dval;
+  double* bmax;
+  double* lblk;
+};
+__device__ inline SpecRec specRec(double* ws, int nE) {
+  double* r = ws + 2 * static_cast<int64_t>(nE);
+  SpecRec x;
+  x.colmax = reinterpret_cast<unsigned long long*>(r);
+  x.cdiag = r + 2 * kPanel;
+  x.dval = r + 3 * kPanel;
+  x.bmax = r + 4 * kPanel;
+  x.lblk = r + 5 * kPanel;
+  return x;
+}
+
+// One row below the diagonal block: x holds its raw values in the panel
+// columns; for q < lim in order, c_q = x_q - sum_{j<q} l_j W[q][j] with
+// W[q][j] = d_j l_qj, then x[q] = l_q = T(c_q / d_q).  onC(q, c_q) sees every
+// current value (the same arithmetic in both passes, so both see the same c).
+template <typename T, typename OnC>
+__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1], const double* dv, int lim,
+                               OnC onC) {
+#pragma unroll
+  for (int q = 0; q < kPanel; ++q) {
+    if (q < lim) {
+      double c0 = x[q], c1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < q; ++j) {
+        if (j & 1)
+          c1 = fma(-x[j], W[q][j], c1);
+        else
+          c0 = fma(-x[j], W[q][j], c0);
+      }
+      const double c = c0 + c1;
+      onC(q, c);
+      x[q] = static_cast<double>(static_cast<T>(c / dv[q]));
+    }
+  }
+}
+
+// Pass 1: every workgroup of a front factors the diagonal block (pw x pw,
+// pw = min(kPanel, nE - k0)) in registers, 1x1 pivots in order, and takes
+// kSpecRows eligible rows below it; the column maxima of the rows go to
+// colmax[parity] with atomics.  Workgroup 0 also stores the block's
+// multipliers, pivots, a_qq and in-block column maxima in the record.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count, const int* wgStart, int parity,
+                                                     DevFronts fd, FactorParams prm, double* wsBase,
+                                                     const int64_t* wsOff) {
+  constexpr int P = kPanel;
+  __shared__ double W[P][P + 1];
+  __shared__ double cvec[2][P];
+  __shared__ double dv[P];
+  __shared__ int sgn[P];
+  __shared__ double red[4][P];
+  if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
+  const int f = findSegment(wgStart, count, blockIdx.x);
+  const int s = list[f];
+  const int w = blockIdx.x - wgStart[f];
+  const int nE = fd.nE[s];
+  if (!specFront(prm, nE) || fd.stopped[s]) return;
+  const int m = fd.m[s];
+  const int k0 = fd.kcur[s];
+  if (k0 >= nE) return;
+  const int pw = min(P, nE - k0);
+  const int rBeg = k0 + pw + w * kSpecRows;
+  if (w > 0 && rBeg >= nE) return;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  auto Fat = [&](int i, int j) -> double { return static_cast<double>(F[i + static_cast<size_t>(j) * m]); };
+  const int64_t io = fd.idxOff[s];
+  const SpecRec rec = specRec(wsBase + wsOff[s], nE);
+  const int tid = threadIdx.x;
+  const bool stat = prm.staticMode != 0;
+  double origMax = bitsToDouble(fd.origMaxBits[s]);
+  if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
+  const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+  const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+  for (int q = tid; q < pw; q += 256)
+    sgn[q] = (stat && prm.expectedSign) ? prm.expectedSign[fd.frontIdx[io + k0 + q]] : 0;
+  // block entry (i, j), i >= j: column j = tid % 64, rows i = tid / 64 + 4 r
+  const int j = tid & 63, ib = tid >> 6;
+  double a[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = ib + 4 * r;
+    a[r] = (i < pw && j < pw && i >= j) ? Fat(k0 + i, k0 + j) : 0.0;
+  }
+  __syncthreads();
+  // Right-looking, one column per step: a_ij -= c_iq l_jq (c_iq = d_q l_iq
+  // up to the rounding of the stored multiplier), one division per thread.
+  for (int q = 0; q < pw; ++q) {
+    double* cv = cvec[q & 1];
+    if (j == q) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = ib + 4 * r;
+        if (i >= q && i < pw) cv[i] = a[r];
+      }
+    }
+    __syncthreads();
+    double d = cv[q];
+    if (stat) {
+      double applied;
+      d = staticPivot(d, sgn[q], delta, sfloor, applied);
+    }
+    if (tid == 0) dv[q] = d;
+    if (j > q && j < pw) {
+      const double lj = static_cast<double>(static_cast<T>(cv[j] / d));
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = ib + 4 * r;
+        if (i >= j && i < pw) a[r] = fma(-cv[i], lj, a[r]);
+      }
+    }
+  }
+  __syncthreads();
+  // a now holds c_ij (current value at pivot j) for i > j and a_jj^(j) on the diagonal
+  double bm = 0.0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = ib + 4 * r;
+    if (j < pw && i < pw) {
+      if (i > j) {
+        const double l = static_cast<double>(static_cast<T>(a[r] / dv[j]));
+        W[i][j] = dv[j] * l;
+        bm = fmax(bm, fabs(a[r]));
+        if (w == 0) rec.lblk[i * P + j] = l;
+      } else if (i == j && w == 0) {
+        rec.cdiag[j] = a[r];
+      }
+    }
+  }
+  red[ib][j] = bm;
+  __syncthreads();
+  if (w == 0 && tid < pw) {
+    rec.dval[tid] = dv[tid];
+    rec.bmax[tid] = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+  }
+  // eligible rows below the block: column maxima of the current values
+  const int i = rBeg + tid;
+  const bool valid = i < nE;
+  double x[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) x[q] = (valid && q < pw) ? Fat(i, k0 + q) : 0.0;
+  __syncthreads();  // red is reused below
+  specRow<T>(x, W, dv, pw, [&](int q, double c) {
+    double v = fabs(c);
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6][q] = v;
+  });
+  __syncthreads();
+  if (tid < pw) {
+    const double v = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+    if (v > 0.0) atomicMaxNonNeg(rec.colmax + parity * P + tid, v);
+  }
+}
+
+// Pass 2: every workgroup finds the first column q* whose test fails (all
+// of them when static), writes the multipliers of its rows for the columns
+// before q* (and of the rows [nE, m) for the whole panel when q* = pw), and
+// workgroup 0 commits the block's columns before q*, the pivots, inertia and
+// perturbation counts, sets kcur = k0 + q*, pcount = q*, and clears the
+// other colmax buffer for the next panel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, const int* wgStart, int parity,
+                                                    DevFronts fd, FactorParams prm, double* wsBase,
+                                                    const int64_t* wsOff) {
+  constexpr int P = kPanel;
+  __shared__ double W[P][P + 1];
+  __shared__ double dv[P];
+  __shared__ int qsh;
+  __shared__ double redV[4];
+  if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
+  const int f = findSegment(wgStart, count, blockIdx.x);
+  const int s = list[f];
+  const int w = blockIdx.x - wgStart[f];
+  const int tid = threadIdx.x;
+  const int m = fd.m[s];
+  const int nE = fd.nE[s];
+  if (!specFront(prm, nE)) return;  // this front's panels are all exact
+  const int k0 = fd.kcur[s];
+  if (fd.stopped[s] || k0 >= nE) {
+    if (w == 0 && tid == 0) {
+      fd.pcount[s] = 0;
+      fd.cbDone[s] = 0;
+    }
+    return;
+  }
+  const int pw = min(P, nE - k0);
+  const int rBeg = k0 + pw + w * kSpecRows;
+  T* F = static_cast<T*>(fd.F[s]);
+  auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
+  const int64_t io = fd.idxOff[s];
+  const SpecRec rec = specRec(wsBase + wsOff[s], nE);
+  const bool stat = prm.staticMode != 0;
+  double origMax = bitsToDouble(fd.origMaxBits[s]);
+  if (!stat && origMax < prm.zeroTol) origMax = 1.0;
+  if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
+  for (int q = tid; q < pw; q += 256) dv[q] = rec.dval[q];
+  for (int e = tid; e < pw * pw; e += 256) {
+    const int q = e / pw, jj = e % pw;
+    W[q][jj] = (jj < q) ? rec.dval[jj] * rec.lblk[q * P + jj] : 0.0;
+  }
+  if (tid == 0) {
+    int qs = pw;
+    if (!stat) {
+      const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
+      for (int q = 0; q < pw; ++q) {
+        const double lam = fmax(rec.bmax[q], bitsToDouble(rec.colmax[parity * P + q]));
+        const double akk = rec.cdiag[q];
+        const bool ok = (lam <= prm.zeroTol) ? fabs(akk) > isolatedFloor : fabs(akk) >= kAlpha * lam;
+        if (!ok) {
+          qs = q;
+          break;
+        }
+      }
+    }
+    qsh = qs;
+  }
+  __syncthreads();
+  const int qs = qsh;
+  double tmax = 0.0;
+  const int i = rBeg + tid;
+  const int lim = (i < nE) ? qs : ((i < m && qs == pw) ? pw : 0);
+  if (lim > 0) {
+    double x[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) x[q] = (q < lim) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
+    specRow<T>(x, W, dv, lim, [&](int, double c) { tmax = fmax(tmax, fabs(c)); });
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < lim) Fat(i, k0 + q) = static_cast<T>(x[q]);
+  }
+  if (w == 0) {
+    for (int e = tid; e < pw * pw; e += 256) {
+      const int q = e / pw, r = e % pw;  // column q, row r of the block
+      if (q < qs && r >= q) Fat(k0 + r, k0 + q) = static_cast<T>(r == q ? dv[q] : rec.lblk[r * P + q]);
+    }
+    for (int q = tid; q < qs; q += 256) {
+      fd.dDiag[io + k0 + q] = dv[q];
+      fd.dOff[io + k0 + q] = 0.0;
+      fd.pkind[io + k0 + q] = 1;
+      tmax = fmax(tmax, fmax(fabs(dv[q]), rec.bmax[q]));
+    }
+    if (tid < P) rec.colmax[(parity ^ 1) * P + tid] = 0ull;
+    if (tid == 0) {
+      int npos = 0, nneg = 0, nzero = 0, npert = 0;
+      double pert = 0.0;
+      const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+      const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+      for (int q = 0; q < qs; ++q) {
+        countInertia(dv[q], npos, nneg, nzero);
+        if (stat) {
+          const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k0 + q]] : 0;
+          double applied;
+          (void)staticPivot(rec.cdiag[q], sign, delta, sfloor, applied);
+          if (applied != 0.0) {
+            pert += fabs(applied);
+            ++npert;
+          }
+        }
+      }
+      fd.inertia[3 * s] += npos;
+      fd.inertia[3 * s + 1] += nneg;
+      fd.inertia[3 * s + 2] += nzero;
+      fd.pert[s] += pert;
+      fd.npert[s] += npert;
+      fd.kcur[s] = k0 + qs;
+      fd.pcount[s] = qs;
+      fd.cbDone[s] = (qs == pw) ? 1 : 0;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+  if ((tid & 63) == 0) redV[tid >> 6] = tmax;
+  __syncthreads();
+  if (tid == 0) {
+    const double v = fmax(fmax(redV[0], redV[1]), fmax(redV[2], redV[3]));
+    atomicMaxNonNeg(fd.runMaxBits + s, fmax(v, origMax));
+  }
+}
+
+// Clears both colmax buffers of the speculative records of a level's fronts.
+__global__ void k_spec_reset(const int* list, int count, DevFronts fd, double* wsBase, const int64_t* wsOff) {
+  const int f = blockIdx.x;
+  if (f >= count) return;
+  const int s = list[f];
+  const SpecRec rec = specRec(wsBase + wsOff[s], fd.nE[s]);
+  for (int t = threadIdx.x; t < 2 * kPanel; t += blockDim.x) rec.colmax[t] = 0ull;
+}
+
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
 // panel.  With U = D L^T of the panel, the multipliers of a row are
 // f U^{-1} = (f L^{-T}) D^{-1}: every workgroup forms M = L^{-1} of the
@@ -1428,7 +1757,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   const int s = list[f];
   const int rowBlock = blockIdx.x - blockStart[f];
   const int p = fd.pcount[s];
-  if (p == 0) return;
+  if (p == 0 || fd.cbDone[s]) return;  // the speculative step already wrote these rows
   const int m = fd.m[s];
   const int nE = fd.nE[s];
   const int k1 = fd.kcur[s];
@@ -1884,6 +2213,26 @@ void launchPanelMw(const int* list, int count, const int* wgStart, int blocks, c
     k_panel_mw<double><<<blocks, kPanelThreads, lds, st>>>(list, count, wgStart, fd, prm, ws, wsOff, sync, err);
   }
   hipCheck(hipGetLastError(), "launchPanelMw");
+}
+void launchSpecReset(const int* list, int count, const DevFronts& fd, double* ws, const int64_t* wsOff,
+                     hipStream_t st) {
+  if (count <= 0) return;
+  k_spec_reset<<<count, 128, 0, st>>>(list, count, fd, ws, wsOff);
+  hipCheck(hipGetLastError(), "launchSpecReset");
+}
+void launchSpecPanel(const int* list, int count, const int* wgStart, int blocks, int parity, const DevFronts& fd,
+                     const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  if (fd.fp32) {
+    k_spec_factor<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    hipCheck(hipGetLastError(), "k_spec_factor");
+    k_spec_apply<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+  } else {
+    k_spec_factor<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    hipCheck(hipGetLastError(), "k_spec_factor");
+    k_spec_apply<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+  }
+  hipCheck(hipGetLastError(), "launchSpecPanel");
 }
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st) {

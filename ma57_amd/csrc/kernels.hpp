@@ -26,6 +26,13 @@ void launchPanel(const int* list, int count, int maxRows, const DevFronts& fd, c
 void launchPanelMw(const int* list, int count, const int* wgStart, int blocks, const DevFronts& fd,
                    const FactorParams& prm, double* ws, const int64_t* wsOff, MwSync* sync, int* err,
                    hipStream_t st);
+// Speculative panel step (both passes) for the fronts of one panel iteration;
+// wgStart[f] .. wgStart[f+1] are front f's workgroups (kSpecRows rows each,
+// at least one), parity = iteration index & 1.
+void launchSpecPanel(const int* list, int count, const int* wgStart, int blocks, int parity, const DevFronts& fd,
+                     const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st);
+void launchSpecReset(const int* list, int count, const DevFronts& fd, double* ws, const int64_t* wsOff,
+                     hipStream_t st);
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st);
 void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
@@ -52,9 +59,13 @@ void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
                    double* Dl, const int64_t* gOff, hipStream_t st);
 void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
                    hipStream_t st);
-void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
-                    const double* aVals, const double* B, const double* X, double* R, hipStream_t st);
-void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st);
+// R = B - A X over symmetric CSR values aSym; max |R| into *rmaxBits if non-null
+void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
+                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st);
+void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, double* aSym, hipStream_t st);
+// out = x + dx; max |out| into *maxBits if non-null
+void launchAxpy(int64_t len, const double* x, const double* dx, double* out, unsigned long long* maxBits,
+                hipStream_t st);
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st);
 // per-column max |v| into outBits[c * stride] (bits of non-negative doubles)
 void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, int stride,

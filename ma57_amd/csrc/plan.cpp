@@ -90,6 +90,7 @@ Plan::Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx) 
   dkcur_.alloc(numSN_);
   dpcount_.alloc(numSN_);
   dstopped_.alloc(numSN_);
+  dcbDone_.alloc(numSN_);
   dnpert_.alloc(numSN_);
   dpert_.alloc(numSN_);
   dorigMax_.alloc(numSN_);
@@ -355,11 +356,11 @@ void Plan::buildSchedule() {
   hwsOff_.assign(numSN_, 0);
   int64_t wsNeed = 2;
   for (auto& ls : sched_) {
-    ls.lsched = buildLargeSched(ls.large, hm_, hnE_, lh);
+    ls.lsched = buildLargeSched(ls.large, hm_, hnE_, lh, tune_.specPanel ? tune_.specMinRows : -1);
     int64_t off = 0;
     for (int s : ls.large) {
       hwsOff_[s] = off;
-      off += 2 * static_cast<int64_t>(hnE_[s]);
+      off += panelWorkspace(hnE_[s]);
     }
     wsNeed = std::max(wsNeed, off);
   }
@@ -369,6 +370,7 @@ void Plan::buildSchedule() {
   dlargeRowStart_.upload(lh.rowStart, stream_);
   dlargeTileStart_.upload(lh.tileStart, stream_);
   dlargeMwStart_.upload(lh.mwStart, stream_);
+  dlargeSpecStart_.upload(lh.specStart, stream_);
   dsmall_.upload(small, stream_);
   dtasks_.upload(tasks, stream_);
   dallFronts_.upload(all, stream_);
@@ -407,6 +409,7 @@ void Plan::buildRefineCsr() {
         symSrc[ps[c]++] = t;
       }
     }
+  nnzSym_ = symCnt[n_];
   dsymPtr_.upload(symCnt, stream_);
   dsymCol_.upload(symCol, stream_);
   dsymSrc_.upload(symSrc, stream_);
@@ -442,6 +445,7 @@ DevFronts Plan::devFronts() {
   f.kcur = dkcur_.p;
   f.pcount = dpcount_.p;
   f.stopped = dstopped_.p;
+  f.cbDone = dcbDone_.p;
   f.pert = dpert_.p;
   f.npert = dnpert_.p;
   f.schurFlops = dschur_.p;
@@ -457,11 +461,13 @@ FactorParams Plan::params(const FactorOptions& o, bool staticMode, double delta,
   p.staticRelFloor = o.staticRelFloor;
   p.staticAbsFloor = o.staticAbsFloor;
   p.expectedSign = signDev;
+  p.specPanel = tune_.specPanel ? 1 : 0;
+  p.specMinRows = tune_.specMinRows;
   return p;
 }
 
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h, bool allowMw) {
+                           LargeSchedHost& h, int specMinRows, bool allowMw) {
   LargeSched ls;
   ls.allOff = static_cast<int64_t>(h.lists.size());
   ls.allCount = static_cast<int>(fronts.size());
@@ -476,7 +482,8 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
     li.listOff = static_cast<int64_t>(h.lists.size());
     li.rowOff = static_cast<int64_t>(h.rowStart.size());
     li.tileOff = static_cast<int64_t>(h.tileStart.size());
-    int rb = 0, tl = 0;
+    int rb = 0, tl = 0, sb = 0;
+    li.specOff = static_cast<int64_t>(h.specStart.size());
     std::vector<int> reg, mw, mwW;
     int mwTotal = 0;
     for (int s : fronts) {
@@ -484,7 +491,11 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
       h.lists.push_back(s);
       h.rowStart.push_back(rb);
       h.tileStart.push_back(tl);
+      h.specStart.push_back(sb);
       rb += (m[s] - nE[s] + 255) / 256;
+      // rows below this iteration's diagonal block (the block starts at or after it * kPanel)
+      const int below = m[s] - it * kPanel - std::min(kPanel, nE[s] - it * kPanel);
+      if (specMinRows >= 0 && nE[s] > specMinRows) sb += std::max(1, (below + kSpecRows - 1) / kSpecRows);
       const int T = (m[s] - it * kPanel + 63) / 64;
       tl += T * (T + 1) / 2;
       ++li.count;
@@ -503,6 +514,8 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
     }
     h.rowStart.push_back(rb);
     h.tileStart.push_back(tl);
+    h.specStart.push_back(sb);
+    li.specBlocks = sb;
     li.rowBlocks = rb;
     li.tiles = tl;
     li.regOff = static_cast<int64_t>(h.lists.size());
@@ -528,9 +541,14 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
                    const int64_t* wsOff, hipStream_t st, Plan* prof) {
   if (ls.allCount == 0) return;
   launchFrontMaxAbs(dv.lists + ls.allOff, ls.allCount, ls.maxM, fd, st);
-  for (const auto& it : ls.iters) {
+  if (prm.specPanel) launchSpecReset(dv.lists + ls.allOff, ls.allCount, fd, ws, wsOff, st);
+  for (size_t t = 0; t < ls.iters.size(); ++t) {
+    const LargeIter& it = ls.iters[t];
     const int* lst = dv.lists + it.listOff;
     if (prof) prof->profBegin(Plan::kProfPanel);
+    if (prm.specPanel && it.specBlocks > 0)
+      launchSpecPanel(lst, it.count, dv.specStart + it.specOff, it.specBlocks, static_cast<int>(t & 1), fd, prm, ws,
+                      wsOff, st);
     launchPanel(dv.lists + it.regOff, it.regCount, it.regMaxNE, fd, prm, ws, wsOff, st);
     launchPanelMw(dv.lists + it.mwOff, it.mwCount, dv.mwStart + it.mwStartOff, it.mwBlocks, fd, prm, ws, wsOff,
                   dv.sync, dv.err, st);
@@ -547,8 +565,10 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
   launchFrontFinish(dv.lists + ls.allOff, ls.allCount, fd, st);
 }
 
-LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart) {
+LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart,
+                        const int* specStart) {
   LargeDev d;
+  d.specStart = specStart;
   d.lists = lists;
   d.rowStart = rowStart;
   d.tileStart = tileStart;
@@ -577,7 +597,8 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
       launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
       profEnd(kProfSmall);
     }
-    runLargeSched(ls.lsched, largeDev(dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, dlargeMwStart_.p), fd,
+    runLargeSched(ls.lsched, largeDev(dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, dlargeMwStart_.p,
+                                    dlargeSpecStart_.p), fd,
                   prm, dws_.p, dwsOff_.p,
                   stream_, profiling_ ? this : nullptr);
   }
@@ -737,22 +758,23 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     launchFactorSmall(dlist.p + all.size(), static_cast<int>(small.size()), maxM, fd, prm, stream_);
     if (!large.empty()) {
       LargeSchedHost lh;
-      LargeSched lsd = buildLargeSched(large, hm_, hnE_, lh);
+      LargeSched lsd = buildLargeSched(large, hm_, hnE_, lh, tune_.specPanel ? tune_.specMinRows : -1);
       std::vector<int64_t> wsOffH(numSN_, 0);
       int64_t off = 0;
       for (int s : large) {
         wsOffH[s] = off;
-        off += 2 * static_cast<int64_t>(hnE_[s]);
+        off += panelWorkspace(hnE_[s]);
       }
       dws_.ensure(static_cast<size_t>(std::max<int64_t>(off, 2)));
-      DBuf<int> dl, dr, dt, dmw;
+      DBuf<int> dl, dr, dt, dmw, dsp;
       DBuf<int64_t> dw;
       dl.upload(lh.lists, stream_);
       dr.upload(lh.rowStart, stream_);
       dt.upload(lh.tileStart, stream_);
       dmw.upload(lh.mwStart, stream_);
+      dsp.upload(lh.specStart, stream_);
       dw.upload(wsOffH, stream_);
-      runLargeSched(lsd, largeDev(dl.p, dr.p, dt.p, dmw.p), fd, prm, dws_.p, dw.p, stream_, nullptr);
+      runLargeSched(lsd, largeDev(dl.p, dr.p, dt.p, dmw.p, dsp.p), fd, prm, dws_.p, dw.p, stream_, nullptr);
       hipCheck(hipStreamSynchronize(stream_), "dynamic large fronts");
     }
     launchMakeRowIdx(dlist.p, static_cast<int>(all.size()), fd, stream_);
@@ -1100,11 +1122,12 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
   const double normAsym = bitsToDoubleHost(hb[1]);
   const double normB = bitsToDoubleHost(hb[2]);
 
+  dsymVal_.ensure(static_cast<size_t>(nnzSym_));
+  launchGatherVals(nnzSym_, dsymSrc_.p, vals, dsymVal_.p, stream_);
   solveCore(dBr_.p, n_, nrhs, Xc, n_);
-  auto relRes = [&](const double* R, const double* Xv) {
-    launchZeroU64(dnorms_.p + 3, 2, stream_);
-    norm(R, nrhs, n_, dnorms_.p + 3);
-    norm(Xv, nrhs, n_, dnorms_.p + 4);
+  // max |R| and max |X| were accumulated into slots 3 and 4 by the residual
+  // and the update that produced them
+  auto relRes = [&]() {
     unsigned long long h2[2];
     hipCheck(hipMemcpyAsync(h2, dnorms_.p + 3, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_),
              "norms");
@@ -1113,16 +1136,19 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
     const double denom = normA * nX + normB;
     return denom > 0.0 ? nR / denom : nR;
   };
-  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymSrc_.p, vals, dBr_.p, Xc, Rc, stream_);
-  double r0 = relRes(Rc, Xc);
+  launchZeroU64(dnorms_.p + 3, 2, stream_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xc, Rc, dnorms_.p + 3, stream_);
+  norm(Xc, nrhs, n_, dnorms_.p + 4);
+  double r0 = relRes();
   res.initialRel = r0;
   res.finalRel = r0;
   for (int it = 0; it < maxIt; ++it) {  // refine.hpp:121-141
     if (r0 <= tol) break;
     solveCore(Rc, n_, nrhs, dDx_.p, n_);
-    launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, Xn, stream_);
-    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymSrc_.p, vals, dBr_.p, Xn, Rn, stream_);
-    const double rNew = relRes(Rn, Xn);
+    launchZeroU64(dnorms_.p + 3, 2, stream_);
+    launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, Xn, dnorms_.p + 4, stream_);
+    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xn, Rn, dnorms_.p + 3, stream_);
+    const double rNew = relRes();
     if (!(rNew < r0)) break;
     std::swap(Xc, Xn);
     std::swap(Rc, Rn);
@@ -1191,7 +1217,8 @@ std::vector<const void*> Plan::graphFingerprint() const {
                                 dlargeRowStart_.p, dlargeTileStart_.p, dwsOff_.p, dpermVec_.p, dBp_.p, dZ_.p,
                                 dX_.p,      dG_.p,      dDl_.p,      dgOff_.p,    dgatherStart_.p, dfwdStart_.p,
                                 dbwdStart_.p, dflagOff_.p, dflagsF_.p, dflagsB_.p, dgen_.p, dsolveErr_.p,
-                                dschur_.p,  dsolveLists_.p, dlargeMwStart_.p, dmwSync_.p, dfactErr_.p};
+                                dschur_.p,  dsolveLists_.p, dlargeMwStart_.p, dmwSync_.p, dfactErr_.p,
+                                dlargeSpecStart_.p, dcbDone_.p};
   for (void* f : hF_) v.push_back(f);
   return v;
 }
@@ -1297,6 +1324,7 @@ void Plan::setTuning(const Tuning& t) {
     throw std::invalid_argument("symla: small-front bound must lie in [0, 192]");
   if (t.asmTile < 16 || t.asmTile > 1024 || (t.asmTile & (t.asmTile - 1)) != 0)
     throw std::invalid_argument("symla: assembly tile must be a power of two in [16, 1024]");
+  if (t.specMinRows < 0) throw std::invalid_argument("symla: speculative-panel row bound must be >= 0");
   tune_ = t;
   buildSchedule();
   graphPrint_.clear();  // a recorded graph holds the old schedule: re-record on replay
@@ -1323,7 +1351,7 @@ std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int p
   const int tiles[] = {32, 64, 128};
   for (int sm : smalls)
     for (int tw : tiles) {
-      setTuning(Tuning{sm, tw});
+      setTuning(Tuning{sm, tw, keep.specPanel, keep.specMinRows});
       FactorInfo info;
       factorize(nullptr, dVals, o, info);  // warm-up: workspaces, kernels
       solve(dB.p, n_, nrhs, dXt.p, n_, true);

@@ -87,6 +87,8 @@ struct RefineResult {
 // more than kMwThreshold eligible rows left take the cooperative panel
 // (several workgroups per front); the others one workgroup each.
 constexpr int kMwThreshold = 4096;
+// Fronts with more eligible rows than this take the speculative panel step by default.
+constexpr int kSpecMinRowsDefault = 2048;
 constexpr int kMwMaxBlocks = 240;  // cooperative workgroups per launch (all resident at once)
 struct LargeIter {
   int64_t listOff = 0;  // all fronts of the iteration (rows, trailing update)
@@ -102,6 +104,8 @@ struct LargeIter {
   int mwCount = 0;
   int64_t mwStartOff = 0;
   int mwBlocks = 0;
+  int64_t specOff = 0;  // speculative step: workgroup starts of the iteration's fronts
+  int specBlocks = 0;
 };
 struct LargeSched {
   int64_t allOff = 0;
@@ -110,7 +114,7 @@ struct LargeSched {
   std::vector<LargeIter> iters;
 };
 struct LargeSchedHost {
-  std::vector<int> lists, rowStart, tileStart, mwStart;
+  std::vector<int> lists, rowStart, tileStart, mwStart, specStart;
 };
 // Device copies of a LargeSchedHost plus the cooperative-panel records.
 struct LargeDev {
@@ -118,12 +122,14 @@ struct LargeDev {
   const int* rowStart = nullptr;
   const int* tileStart = nullptr;
   const int* mwStart = nullptr;
+  const int* specStart = nullptr;
   MwSync* sync = nullptr;
   int* err = nullptr;
 };
 // m and nE are indexed by front id; allowMw = false keeps every panel on one workgroup.
+// specMinRows < 0: no speculative step; else fronts with nE > specMinRows get its workgroups.
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h, bool allowMw = true);
+                           LargeSchedHost& h, int specMinRows, bool allowMw = true);
 class Plan;
 // Launch sequence of a LargeSched; `prof` (may be null) brackets launches with events.
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
@@ -166,6 +172,8 @@ class Plan {
   struct Tuning {
     int smallMax = kSmallFrontMax;
     int asmTile = 64;
+    int specPanel = 1;
+    int specMinRows = kSpecMinRowsDefault;
   };
   struct TuneTrial {
     Tuning t;
@@ -206,6 +214,7 @@ class Plan {
   std::vector<int> colToSN_;
   // A pattern
   int64_t nnzA_ = 0;
+  int64_t nnzSym_ = 0;  // entries of the symmetric CSR (both triangles)
   std::vector<int64_t> aColptr_;
   std::vector<int> aRowidx_;
   // host layout mirrors
@@ -222,7 +231,7 @@ class Plan {
   DBuf<void*> dF_;
   DBuf<int> dm_, dnE_, dncols_, dfirst_, dchildPtr_, dchildList_, dfrontIdx_, drelmap_, daRow_;
   DBuf<int64_t> didxOff_, dacolPtr_, daSrc_;
-  DBuf<int> dperm_, drowIdx_, dpkind_, dnF_, dinertia_, dkcur_, dpcount_, dstopped_, dnpert_;
+  DBuf<int> dperm_, drowIdx_, dpkind_, dnF_, dinertia_, dkcur_, dpcount_, dstopped_, dnpert_, dcbDone_;
   DBuf<double> ddDiag_, ddOff_, dpert_;
   DBuf<unsigned long long> dorigMax_, drunMax_;
   DBuf<long long> dstatI_;
@@ -271,6 +280,7 @@ class Plan {
   // refinement CSR
   DBuf<int64_t> dsymPtr_, dsymSrc_, dstPtr_, dstSrc_;
   DBuf<int> dsymCol_;
+  DBuf<double> dsymVal_;  // symmetric CSR values of the matrix being refined against
   DBuf<double> dR_, dDx_, dXn_, dRn_, dBr_;
   DBuf<unsigned long long> dnorms_;
   // state
@@ -316,10 +326,11 @@ class Plan {
   FactorParams params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const;
   void runStatic(const double* aValsDev, const FactorParams& prm);
   void runDynamic(const double* aValsDev, const FactorParams& prm);
-  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_, dlargeMwStart_;
+  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_, dlargeMwStart_, dlargeSpecStart_;
   DBuf<MwSync> dmwSync_;
   DBuf<int> dfactErr_;  // cooperative-panel wait errors
-  LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart);
+  LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart,
+                    const int* specStart);
   DBuf<int64_t> dwsOff_;
   std::vector<int64_t> hwsOff_;
   void collectStats(FactorInfo& info, bool& anyDelay);

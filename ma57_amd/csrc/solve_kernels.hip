@@ -11,6 +11,8 @@
 // each solved by its own workgroup; a block waits on a per-block flag for
 // the blocks it depends on, so a large front is solved by many workgroups in
 // one launch.
+#include <algorithm>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -42,6 +44,7 @@ namespace {
 
 constexpr int SB = 32;  // own rows per sweep block; also the column tile of the sweeps
 constexpr int DT = 64;  // ancestor-row tile of the backward sweep
+constexpr int YW = 65;  // row stride of the right-hand-side tiles in LDS (64 columns + 1)
 // Polls of a dependency flag before a sweep block gives up (about a second).
 constexpr unsigned kSpinLimit = 1u << 22;
 
@@ -105,6 +108,30 @@ __device__ inline double lval(const T* F, int m, const int* pk, int row, int col
   const double v = F[row + static_cast<size_t>(col) * m];
   return (row == col + 1 && pk[col] == 2) ? 0.0 : v;
 }
+
+// The sweeps' tile products on the matrix cores, in the storage type of the
+// factor: v_mfma_f64_16x16x4f64 for FP64 fronts, v_mfma_f32_16x16x4f32 for
+// FP32 fronts (operands rounded to FP32).  A[i][k] comes from lane i + 16k,
+// B[k][j] from lane j + 16k; row(lane, r) is the row i of the result held in
+// register r of a lane (its column j is lane % 16).
+template <typename T>
+struct SweepMfma;
+template <>
+struct SweepMfma<double> {
+  using E = double;
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct SweepMfma<float> {
+  using E = float;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
 }  // namespace
 
@@ -174,8 +201,12 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
                                                     const int64_t* __restrict__ gOff, unsigned* flags,
                                                     const int64_t* __restrict__ flagOff, const unsigned* genp,
                                                     int* err) {
-  __shared__ double Ls[SB][SB + 1];  // [row][col]
-  __shared__ double ys[SB][65];      // [row][rhs]
+  using M = SweepMfma<T>;
+  using E = typename M::E;
+  __shared__ E Lt[SB][SB + 1];          // [row][col] operand tile of L
+  __shared__ double ybuf[SB * YW];      // y_j operand tile (E, [q][rhs]); then this block's y (double)
+  E* yt = reinterpret_cast<E*>(ybuf);
+  double (*ys)[YW] = reinterpret_cast<double (*)[YW]>(ybuf);
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf(blockStart, count, blockIdx.x);
   const int s = list[f];
@@ -197,73 +228,69 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   const double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  // wave wv owns output rows tr .. tr + 15 and right-hand sides tc .. tc + 31
+  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    const int outs = nr * gw;
-    // outputs o = tid + kk * NT (row o % nr, column o / nr); slots past the
-    // end read row 0 / column 0 of the tiles and are never stored
-    double acc[8];
-    int orow[8], ocol[8];
-#pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      const int o = tid + kk * NT;
-      acc[kk] = 0.0;
-      orow[kk] = (o < outs) ? o % nr : 0;
-      ocol[kk] = (o < outs) ? o / nr : 0;
-    }
+    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
     for (int j = 0; j < jEnd; ++j) {
       if (rg == 0) waitBlock(fl + j, gen, err);
       const int c0 = j * SB, cn = min(SB, nf - c0);
       for (int e = tid; e < SB * SB; e += NT) {
         const int r = e % SB, qq = e / SB;
-        Ls[r][qq] = (r < nr && qq < cn) ? lval(F, m, pk, r0 + r, c0 + qq) : 0.0;
+        Lt[r][qq] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int qq = e % SB, c = e / SB;
-        ys[qq][c] = (qq < cn && c < gw) ? loadShared(Ds + (c0 + qq) + static_cast<int64_t>(rg + c) * m) : 0.0;
+        yt[qq * YW + c] =
+            (qq < cn && c < gw) ? static_cast<E>(loadShared(Ds + (c0 + qq) + static_cast<int64_t>(rg + c) * m)) : E(0);
       }
       __syncthreads();
-      for (int qq = 0; qq < cn; ++qq) {
+      typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
-        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[orow[kk]][qq], ys[qq][ocol[kk]], acc[kk]);
+      for (int kk = 0; kk < SB; kk += 4) {
+        const E av = Lt[tr + lr][kk + lk];
+        part[0] = M::mma(av, yt[(kk + lk) * YW + tc + lr], part[0]);
+        part[1] = M::mma(av, yt[(kk + lk) * YW + tc + 16 + lr], part[1]);
       }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
       __syncthreads();
     }
     if (!own) {
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) {
-        const int o = tid + kk * NT;
-        if (o < outs) {
-          const int r = o % nr, c = o / nr;
-          const int i = r0 + r;
-          const int64_t cc = rg + c;
-          Ds[i + cc * m] = Gs[fd.perm[io + i] + cc * m] + acc[kk];
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+          if (r < nr && c < gw) {
+            const int i = r0 + r;
+            const int64_t cc = rg + c;
+            Ds[i + cc * m] = Gs[fd.perm[io + i] + cc * m] + acc[t][rr];
+          }
         }
-      }
       continue;
     }
     // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j)
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      const int o = tid + kk * NT;
-      if (o < outs) {
-        const int r = o % nr, c = o / nr;
-        ys[r][c] = Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[kk];
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+        if (r < nr && c < gw) ys[r][c] = Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[t][rr];
       }
-    }
-    for (int e = tid; e < SB * SB; e += NT) {
-      const int r = e % SB, qq = e / SB;
-      Ls[r][qq] = (r > qq && r < nr) ? lval(F, m, pk, r0 + r, r0 + qq) : 0.0;
-    }
     __syncthreads();
     {
       // lane row of each half-wave owns one row; the half-waves of the
       // workgroup take the right-hand sides in turn
-      const int row = lane & 31, half = lane >> 5, wv = tid >> 6;
+      const int row = lane & 31, half = lane >> 5;
       double lrow[SB];
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lrow[j] = Ls[row][j];
+      for (int j = 0; j < SB; ++j) lrow[j] = (row > j && row < nr) ? lval(F, m, pk, r0 + row, r0 + j) : 0.0;
       for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
         double y = (row < nr) ? ys[row][c] : 0.0;
 #pragma unroll
@@ -279,7 +306,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
     }
     __syncthreads();
     // z = D^{-1} y for the pivot blocks that end in this block
-    for (int e = tid; e < outs; e += NT) {
+    for (int e = tid; e < nr * gw; e += NT) {
       const int r = e % nr, c = e / nr;
       const int p = r0 + r;
       const int64_t cc = rg + c;
@@ -318,8 +345,12 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
                                                     const int64_t* __restrict__ gOff, unsigned* flags,
                                                     const int64_t* __restrict__ flagOff, const unsigned* genp,
                                                     int* err) {
-  __shared__ double Ls[DT][SB + 1];  // [row i][own column q]
-  __shared__ double xs[DT][65];      // [row i][rhs]
+  using M = SweepMfma<T>;
+  using E = typename M::E;
+  __shared__ E Lt[DT][SB + 1];          // [row i][own column q] operand tile of L
+  __shared__ double xbuf[DT * YW];      // x operand tile (E, [row i][rhs]); then this block's x (double)
+  E* xt = reinterpret_cast<E*>(xbuf);
+  double (*xs)[YW] = reinterpret_cast<double (*)[YW]>(xbuf);
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf(blockStart, count, blockIdx.x);
   const int s = list[f];
@@ -337,36 +368,40 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   const T* F = static_cast<const T*>(fd.F[s]);
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  // wave wv owns own columns tr .. tr + 15 and right-hand sides tc .. tc + 31
+  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    const int outs = nr * gw;
-    // outputs o = tid + kk * NT (own column o % nr, right-hand side o / nr)
-    double acc[8];
-    int orow[8], ocol[8];
+    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
+    // acc(q, c) += sum over rows i of the tile of L(i, r0 + q) x(i, c)
+    auto tileProduct = [&](int kn) {
+      typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+      for (int kk = 0; kk < kn; kk += 4) {
+        const E av = Lt[kk + lk][tr + lr];
+        part[0] = M::mma(av, xt[(kk + lk) * YW + tc + lr], part[0]);
+        part[1] = M::mma(av, xt[(kk + lk) * YW + tc + 16 + lr], part[1]);
+      }
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      const int o = tid + kk * NT;
-      acc[kk] = 0.0;
-      orow[kk] = (o < outs) ? o % nr : 0;
-      ocol[kk] = (o < outs) ? o / nr : 0;
-    }
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
+    };
     // rows of the ancestors
     for (int i0 = nf; i0 < m; i0 += DT) {
       const int in = min(DT, m - i0);
       for (int e = tid; e < DT * SB; e += NT) {
         const int i = e % DT, qq = e / DT;
-        Ls[i][qq] = (i < in && qq < nr) ? static_cast<double>(F[(i0 + i) + static_cast<size_t>(r0 + qq) * m]) : 0.0;
+        Lt[i][qq] = (i < in && qq < nr) ? F[(i0 + i) + static_cast<size_t>(r0 + qq) * m] : T(0);
       }
       for (int e = tid; e < DT * 64; e += NT) {
         const int i = e % DT, c = e / DT;
-        xs[i][c] = (i < in && c < gw) ? X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n] : 0.0;
+        xt[i * YW + c] = (i < in && c < gw) ? static_cast<E>(X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n])
+                                            : E(0);
       }
       __syncthreads();
-      for (int i = 0; i < in; ++i) {
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[i][orow[kk]], xs[i][ocol[kk]], acc[kk]);
-      }
+      tileProduct((in + 3) & ~3);
       __syncthreads();
     }
     // own blocks below this one, as they are published
@@ -375,39 +410,33 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
       const int i0 = j * SB, in = min(SB, nf - i0);
       for (int e = tid; e < SB * SB; e += NT) {
         const int i = e % SB, qq = e / SB;
-        Ls[i][qq] = (i < in && qq < nr) ? lval(F, m, pk, i0 + i, r0 + qq) : 0.0;
+        Lt[i][qq] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int i = e % SB, c = e / SB;
-        xs[i][c] = (i < in && c < gw) ? loadShared(Ds + (i0 + i) + static_cast<int64_t>(rg + c) * m) : 0.0;
+        xt[i * YW + c] =
+            (i < in && c < gw) ? static_cast<E>(loadShared(Ds + (i0 + i) + static_cast<int64_t>(rg + c) * m)) : E(0);
       }
       __syncthreads();
-      for (int i = 0; i < in; ++i) {
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) acc[kk] = fma(Ls[i][orow[kk]], xs[i][ocol[kk]], acc[kk]);
-      }
+      tileProduct((in + 3) & ~3);
       __syncthreads();
     }
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      const int o = tid + kk * NT;
-      if (o < outs) {
-        const int qq = o % nr, c = o / nr;
-        xs[qq][c] = Z[fd.rowIdx[io + r0 + qq] + static_cast<int64_t>(rg + c) * n] - acc[kk];
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+        if (qq < nr && c < gw)
+          xs[qq][c] = Z[fd.rowIdx[io + r0 + qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr];
       }
-    }
-    for (int e = tid; e < SB * SB; e += NT) {
-      const int i = e % SB, j = e / SB;
-      Ls[i][j] = (i > j && i < nr) ? lval(F, m, pk, r0 + i, r0 + j) : 0.0;
-    }
     __syncthreads();
     {
       // lane row of each half-wave owns one row (and column `row` of the
       // block); the half-waves of the workgroup take the right-hand sides
-      const int row = lane & 31, half = lane >> 5, wv = tid >> 6;
+      const int row = lane & 31, half = lane >> 5;
       double lcol[SB];
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lcol[j] = Ls[j][row];
+      for (int j = 0; j < SB; ++j) lcol[j] = (j > row && j < nr) ? lval(F, m, pk, r0 + j, r0 + row) : 0.0;
       const int prow = (row < nr) ? fd.rowIdx[io + r0 + row] : 0;
       for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
         double x = (row < nr) ? xs[row][c] : 0.0;
@@ -538,40 +567,92 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
 // Refinement support (refine.hpp:69-144).
 // ---------------------------------------------------------------------------
 
-// R = B - A X with A given as the full symmetric CSR built from the caller's
-// lower-triangle CSC values (selfadjointView<Lower> semantics).
-__global__ void k_residual(int n, int nrhs, const int64_t* __restrict__ rowPtr, const int* __restrict__ colIdx,
-                           const int64_t* __restrict__ valSrc, const double* __restrict__ aVals,
-                           const double* __restrict__ B, const double* __restrict__ X, double* __restrict__ R) {
-  const int64_t total = static_cast<int64_t>(n) * nrhs;
-  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(e % n);
-    const int64_t c = e / n;
-    const double* x = X + c * n;
-    double acc = 0.0;
-    for (int64_t t = rowPtr[i]; t < rowPtr[i + 1]; ++t) acc = fma(aVals[valSrc[t]], x[colIdx[t]], acc);
-    R[e] = B[e] - acc;
-  }
+// Symmetric CSR values in row order, aSym[t] = aVals[valSrc[t]]: the full
+// matrix built from the caller's lower-triangle CSC values
+// (selfadjointView<Lower> semantics).
+__global__ void k_gather_vals(int64_t nnz, const int64_t* __restrict__ valSrc, const double* __restrict__ aVals,
+                              double* __restrict__ aSym) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < nnz;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    aSym[t] = aVals[valSrc[t]];
 }
 
+// R = B - A X for column-major n x nrhs blocks (leading dimension n), A as
+// symmetric CSR (both triangles).  One thread per row; the right-hand sides
+// go in chunks of kResChunk so the row's indices and values are read once
+// per chunk rather than once per column.  max |R| goes to *rmaxBits when
+// given (the slot holds the bits of a non-negative double).
+constexpr int kResChunk = 16;
+__global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t* __restrict__ rowPtr,
+                                                  const int* __restrict__ colIdx, const double* __restrict__ aSym,
+                                                  const double* __restrict__ B, const double* __restrict__ X,
+                                                  double* __restrict__ R, unsigned long long* rmaxBits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0;
+  if (i < n) {
+    const int64_t t0 = rowPtr[i], t1 = rowPtr[i + 1];
+    for (int c0 = 0; c0 < nrhs; c0 += kResChunk) {
+      const int rc = min(kResChunk, nrhs - c0);
+      const double* x = X + static_cast<int64_t>(c0) * n;
+      double acc[kResChunk];
+#pragma unroll
+      for (int u = 0; u < kResChunk; ++u) acc[u] = 0.0;
+      if (rc == kResChunk) {
+        for (int64_t t = t0; t < t1; ++t) {
+          const double a = aSym[t];
+          const double* xc = x + colIdx[t];
+#pragma unroll
+          for (int u = 0; u < kResChunk; ++u) acc[u] = fma(a, xc[static_cast<int64_t>(u) * n], acc[u]);
+        }
+      } else {
+        for (int64_t t = t0; t < t1; ++t) {
+          const double a = aSym[t];
+          const double* xc = x + colIdx[t];
+#pragma unroll
+          for (int u = 0; u < kResChunk; ++u)
+            if (u < rc) acc[u] = fma(a, xc[static_cast<int64_t>(u) * n], acc[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kResChunk; ++u)
+        if (u < rc) {
+          const int64_t e = static_cast<int64_t>(c0 + u) * n + i;
+          const double r = B[e] - acc[u];
+          R[e] = r;
+          mx = fmax(mx, fabs(r));
+        }
+    }
+  }
+  if (!rmaxBits) return;
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0)
+    atomicMax(rmaxBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+}
+
+// out = x + dx, and max |out| into *maxBits when given.
 __global__ void k_axpy(int64_t len, const double* __restrict__ x, const double* __restrict__ dx,
-                       double* __restrict__ out) {
+                       double* __restrict__ out, unsigned long long* maxBits) {
+  double mx = 0.0;
   for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < len;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    out[e] = x[e] + dx[e];
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double v = x[e] + dx[e];
+    out[e] = v;
+    mx = fmax(mx, fabs(v));
+  }
+  if (!maxBits) return;
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0)
+    atomicMax(maxBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
 }
 
 // max |v| over a strided n x nrhs block, into *outBits (must start at 0).
 __global__ void k_absmax(int n, int nrhs, const double* __restrict__ v, int64_t ld,
                          unsigned long long* outBits) {
-  const int64_t total = static_cast<int64_t>(n) * nrhs;
   double mx = 0.0;
-  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(e % n);
-    const int64_t c = e / n;
-    mx = fmax(mx, fabs(v[i + c * ld]));
+  for (int c = 0; c < nrhs; ++c) {
+    const double* col = v + static_cast<int64_t>(c) * ld;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+      mx = fmax(mx, fabs(col[i]));
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
   if ((threadIdx.x & 63) == 0 && mx > 0.0)
@@ -692,18 +773,25 @@ void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
     k_bwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
   hipCheck(hipGetLastError(), "launchBwdTiny");
 }
-void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const int64_t* valSrc,
-                    const double* aVals, const double* B, const double* X, double* R, hipStream_t st) {
-  k_residual<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, rowPtr, colIdx, valSrc, aVals,
-                                                                           B, X, R);
+void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
+                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return;
+  k_residual<<<(n + 255) / 256, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, X, R, rmaxBits);
   hipCheck(hipGetLastError(), "launchResidual");
 }
-void launchAxpy(int64_t len, const double* x, const double* dx, double* out, hipStream_t st) {
-  k_axpy<<<gridFor(len, 256), 256, 0, st>>>(len, x, dx, out);
+void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, double* aSym, hipStream_t st) {
+  if (nnz <= 0) return;
+  k_gather_vals<<<gridFor(nnz, 256), 256, 0, st>>>(nnz, valSrc, aVals, aSym);
+  hipCheck(hipGetLastError(), "launchGatherVals");
+}
+void launchAxpy(int64_t len, const double* x, const double* dx, double* out, unsigned long long* maxBits,
+                hipStream_t st) {
+  k_axpy<<<gridFor(len, 256), 256, 0, st>>>(len, x, dx, out, maxBits);
   hipCheck(hipGetLastError(), "launchAxpy");
 }
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st) {
-  k_absmax<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, v, ld, outBits);
+  if (n <= 0 || nrhs <= 0) return;
+  k_absmax<<<gridFor(std::min<int64_t>(n, 256 * 1024), 256), 256, 0, st>>>(n, nrhs, v, ld, outBits);
   hipCheck(hipGetLastError(), "launchAbsMax");
 }
 void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, int stride,
