@@ -537,12 +537,13 @@ __global__ __launch_bounds__(256) void k_front_maxabs(const int* list, int count
 // columns from the current pivot (a sub-panel) are loaded into LDS with the
 // correction of the panel's earlier pivots applied once, then factored
 // right-looking inside LDS: a pivot accepted by the first Bunch-Kaufman test
-// (every static pivot) costs two barriers and no global reads.  Any other
+// (every static pivot) costs one barrier and no global reads.  Any other
 // decision runs the general code, which works on column buffers in global
 // memory, and the sub-panel is reloaded after it.
 template <int RM>
 struct PanelShape {
-  static constexpr int SW = RM <= 4 ? 8 : (RM <= 8 ? 4 : (RM <= 16 ? 2 : 1));  // sub-panel width
+  // sub-panel width (a wider one saves reloads but costs LDS traffic on every pivot)
+  static constexpr int SW = RM <= 4 ? 8 : (RM <= 8 ? 4 : (RM <= 16 ? 2 : 1));
   static constexpr int ROWS = RM > 0 ? RM * kPanelThreads : 0;
   static constexpr size_t lds() { return sizeof(double) * static_cast<size_t>(ROWS) * SW; }
 };
@@ -835,10 +836,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
             if (j > kk && j < sw) Cs[j * CSR + (i - k0)] = fma(-l, wj[j], Cs[j * CSR + (i - k0)]);
         }
       }
-      __syncthreads();
+      // No barrier here: a thread updated only its own rows, and the values
+      // the next step reads from other threads' rows (a_kk, the sub-panel
+      // rows of the pivot column) are read after the next reduction's
+      // barrier.  A reload reads the multipliers just stored: it waits.
       ++k;
       ++p;
-      if (k - c0 >= sw && k < nbEnd) loadSub();
+      if (k - c0 >= sw && k < nbEnd) {
+        __syncthreads();
+        loadSub();
+      }
       continue;
     }
     if (RM > 0) {

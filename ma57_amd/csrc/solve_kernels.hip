@@ -203,8 +203,12 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
                                                     int* err) {
   using M = SweepMfma<T>;
   using E = typename M::E;
+  constexpr int NT = 256;
   __shared__ E Lt[SB][SB + 1];          // [row][col] operand tile of L
   __shared__ double ybuf[SB * YW];      // y_j operand tile (E, [q][rhs]); then this block's y (double)
+  __shared__ double Ld[SB][SB + 1];     // own block: its unit lower diagonal block
+  __shared__ double dA[SB + 1], dB[SB + 1];  // own block: d_diag, d_off of positions r0 - 1 .. r0 + SB - 1
+  __shared__ int pkA[SB + 1], rixA[SB + 1];  // and their pivot kinds and final-order rows
   E* yt = reinterpret_cast<E*>(ybuf);
   double (*ys)[YW] = reinterpret_cast<double (*)[YW]>(ybuf);
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
@@ -228,25 +232,56 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   const double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // wave wv owns output rows tr .. tr + 15 and right-hand sides tc .. tc + 31
   const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
+  // What the own block's tail needs does not depend on other blocks: stage
+  // it before the first wait (visible after the next barrier).
+  if (own) {
+    for (int e = tid; e < SB * SB; e += NT) {
+      const int r = e % SB, j = e / SB;
+      Ld[r][j] = (r > j && r < nr) ? lval(F, m, pk, r0 + r, r0 + j) : 0.0;
+    }
+    for (int r = tid; r <= SB; r += NT) {
+      const int p = r0 - 1 + r;
+      const bool ok = p >= 0 && p < r1;
+      dA[r] = ok ? fd.dDiag[io + p] : 0.0;
+      dB[r] = ok ? fd.dOff[io + p] : 0.0;
+      pkA[r] = ok ? pk[p] : 0;
+      rixA[r] = ok ? fd.rowIdx[io + p] : 0;
+    }
+  }
+  // L(b, j) tile of the workgroup's rows: four elements per thread, loaded
+  // one tile ahead so the load overlaps the wait and the products.
+  E lpre[4];
+  auto loadL = [&](int j) {
+    const int c0 = j * SB, cn = min(SB, nf - c0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      const int r = e % SB, qq = e / SB;
+      lpre[u] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
+    }
+  };
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
     double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
+    if (jEnd > 0) loadL(0);
     for (int j = 0; j < jEnd; ++j) {
       if (rg == 0) waitBlock(fl + j, gen, err);
       const int c0 = j * SB, cn = min(SB, nf - c0);
-      for (int e = tid; e < SB * SB; e += NT) {
-        const int r = e % SB, qq = e / SB;
-        Lt[r][qq] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = tid + u * NT;
+        Lt[e % SB][e / SB] = lpre[u];
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int qq = e % SB, c = e / SB;
         yt[qq * YW + c] =
             (qq < cn && c < gw) ? static_cast<E>(loadShared(Ds + (c0 + qq) + static_cast<int64_t>(rg + c) * m)) : E(0);
       }
+      if (j + 1 < jEnd) loadL(j + 1);
       __syncthreads();
       typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
@@ -290,7 +325,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
       const int row = lane & 31, half = lane >> 5;
       double lrow[SB];
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lrow[j] = (row > j && row < nr) ? lval(F, m, pk, r0 + row, r0 + j) : 0.0;
+      for (int j = 0; j < SB; ++j) lrow[j] = Ld[row][j];
       for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
         double y = (row < nr) ? ys[row][c] : 0.0;
 #pragma unroll
@@ -305,32 +340,34 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
       }
     }
     __syncthreads();
-    // z = D^{-1} y for the pivot blocks that end in this block
+    // the later blocks need y only: release it before forming z
+    if (rg + 64 >= nrhs) publishBlock(fl + b, gen);
+    // z = D^{-1} y for the pivot blocks that end in this block (slot r + 1
+    // of the staged pivot data is position r0 + r)
     for (int e = tid; e < nr * gw; e += NT) {
       const int r = e % nr, c = e / nr;
       const int p = r0 + r;
       const int64_t cc = rg + c;
-      const int kind = pk[p];
+      const int kind = pkA[r + 1];
       if (kind == 1) {
-        Z[fd.rowIdx[io + p] + cc * n] = ys[r][c] / fd.dDiag[io + p];
+        Z[rixA[r + 1] + cc * n] = ys[r][c] / dA[r + 1];
       } else if (kind == 2 && r + 1 < nr) {
-        const double d11 = fd.dDiag[io + p], d21 = fd.dOff[io + p], d22 = fd.dDiag[io + p + 1];
+        const double d11 = dA[r + 1], d21 = dB[r + 1], d22 = dA[r + 2];
         const double det = d11 * d22 - d21 * d21;
         const double y0 = ys[r][c], y1 = ys[r + 1][c];
-        Z[fd.rowIdx[io + p] + cc * n] = (d22 * y0 - d21 * y1) / det;
-        Z[fd.rowIdx[io + p + 1] + cc * n] = (-d21 * y0 + d11 * y1) / det;
+        Z[rixA[r + 1] + cc * n] = (d22 * y0 - d21 * y1) / det;
+        Z[rixA[r + 2] + cc * n] = (-d21 * y0 + d11 * y1) / det;
       } else if (kind == 0 && r == 0 && p > 0) {
         // second row of a 2x2 block that starts in the previous block
-        const double d11 = fd.dDiag[io + p - 1], d21 = fd.dOff[io + p - 1], d22 = fd.dDiag[io + p];
+        const double d11 = dA[0], d21 = dB[0], d22 = dA[1];
         const double det = d11 * d22 - d21 * d21;
         const double y0 = loadShared(Ds + (p - 1) + cc * m), y1 = ys[0][c];
-        Z[fd.rowIdx[io + p - 1] + cc * n] = (d22 * y0 - d21 * y1) / det;
-        Z[fd.rowIdx[io + p] + cc * n] = (-d21 * y0 + d11 * y1) / det;
+        Z[rixA[0] + cc * n] = (d22 * y0 - d21 * y1) / det;
+        Z[rixA[1] + cc * n] = (-d21 * y0 + d11 * y1) / det;
       }
     }
     __syncthreads();
   }
-  if (own) publishBlock(fl + b, gen);
 }
 
 // Backward sweep of one tree level (solve.hpp:364-383).  The workgroups of a
@@ -347,8 +384,11 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
                                                     int* err) {
   using M = SweepMfma<T>;
   using E = typename M::E;
+  constexpr int NT = 256;
   __shared__ E Lt[DT][SB + 1];          // [row i][own column q] operand tile of L
   __shared__ double xbuf[DT * YW];      // x operand tile (E, [row i][rhs]); then this block's x (double)
+  __shared__ double Ld[SB][SB + 1];     // the block's unit lower diagonal block, [i][j] = L(r0 + i, r0 + j)
+  __shared__ int rixA[SB];              // final-order rows of the block
   E* xt = reinterpret_cast<E*>(xbuf);
   double (*xs)[YW] = reinterpret_cast<double (*)[YW]>(xbuf);
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
@@ -368,10 +408,38 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   const T* F = static_cast<const T*>(fd.F[s]);
   double* Ds = Dl + gOff[s];
   unsigned* fl = flags + flagOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // wave wv owns own columns tr .. tr + 15 and right-hand sides tc .. tc + 31
   const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
+  // staged before any wait (visible after the next barrier)
+  for (int e = tid; e < SB * SB; e += NT) {
+    const int i = e % SB, j = e / SB;
+    Ld[i][j] = (i > j && i < nr) ? lval(F, m, pk, r0 + i, r0 + j) : 0.0;
+  }
+  for (int q = tid; q < SB; q += NT) rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
+  // L tiles loaded one tile ahead: the ancestors' rows (DT x SB, eight
+  // elements per thread), then the own blocks below this one (SB x SB, four)
+  E apre[8];
+  auto loadA = [&](int i0) {
+    const int in = min(DT, m - i0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + u * NT;
+      const int i = e % DT, qq = e / DT;
+      apre[u] = (i < in && qq < nr) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(r0 + qq) * m]) : E(0);
+    }
+  };
+  E opre[4];
+  auto loadO = [&](int j) {
+    const int i0 = j * SB, in = min(SB, nf - i0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      const int i = e % SB, qq = e / SB;
+      opre[u] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
+    }
+  };
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
     double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
@@ -388,35 +456,41 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
     };
-    // rows of the ancestors
+    // rows of the ancestors (their x is final from the levels above)
+    if (nf < m) loadA(nf);
     for (int i0 = nf; i0 < m; i0 += DT) {
       const int in = min(DT, m - i0);
-      for (int e = tid; e < DT * SB; e += NT) {
-        const int i = e % DT, qq = e / DT;
-        Lt[i][qq] = (i < in && qq < nr) ? F[(i0 + i) + static_cast<size_t>(r0 + qq) * m] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * NT;
+        Lt[e % DT][e / DT] = apre[u];
       }
       for (int e = tid; e < DT * 64; e += NT) {
         const int i = e % DT, c = e / DT;
         xt[i * YW + c] = (i < in && c < gw) ? static_cast<E>(X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n])
                                             : E(0);
       }
+      if (i0 + DT < m) loadA(i0 + DT);
       __syncthreads();
       tileProduct((in + 3) & ~3);
       __syncthreads();
     }
     // own blocks below this one, as they are published
+    if (nOwn - 1 > w) loadO(nOwn - 1);
     for (int j = nOwn - 1; j > w; --j) {
       if (rg == 0) waitBlock(fl + j, gen, err);
       const int i0 = j * SB, in = min(SB, nf - i0);
-      for (int e = tid; e < SB * SB; e += NT) {
-        const int i = e % SB, qq = e / SB;
-        Lt[i][qq] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = tid + u * NT;
+        Lt[e % SB][e / SB] = opre[u];
       }
       for (int e = tid; e < SB * 64; e += NT) {
         const int i = e % SB, c = e / SB;
         xt[i * YW + c] =
             (i < in && c < gw) ? static_cast<E>(loadShared(Ds + (i0 + i) + static_cast<int64_t>(rg + c) * m)) : E(0);
       }
+      if (j - 1 > w) loadO(j - 1);
       __syncthreads();
       tileProduct((in + 3) & ~3);
       __syncthreads();
@@ -426,8 +500,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (qq < nr && c < gw)
-          xs[qq][c] = Z[fd.rowIdx[io + r0 + qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr];
+        if (qq < nr && c < gw) xs[qq][c] = Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr];
       }
     __syncthreads();
     {
@@ -436,8 +509,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
       const int row = lane & 31, half = lane >> 5;
       double lcol[SB];
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lcol[j] = (j > row && j < nr) ? lval(F, m, pk, r0 + j, r0 + row) : 0.0;
-      const int prow = (row < nr) ? fd.rowIdx[io + r0 + row] : 0;
+      for (int j = 0; j < SB; ++j) lcol[j] = (j > row) ? Ld[j][row] : 0.0;
       for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
         double x = (row < nr) ? xs[row][c] : 0.0;
 #pragma unroll
@@ -446,14 +518,20 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
           if (row < j) x = fma(-lcol[j], xj, x);
         }
         if (row < nr) {
+          xs[row][c] = x;
           storeShared(Ds + (r0 + row) + static_cast<int64_t>(rg + c) * m, x);
-          X[prow + static_cast<int64_t>(rg + c) * n] = x;
         }
       }
     }
     __syncthreads();
+    // the earlier blocks need x only through Ds: release it, then write X
+    if (rg + 64 >= nrhs) publishBlock(fl + w, gen);
+    for (int e = tid; e < nr * gw; e += NT) {
+      const int r = e % nr, c = e / nr;
+      X[rixA[r] + static_cast<int64_t>(rg + c) * n] = xs[r][c];
+    }
+    __syncthreads();
   }
-  publishBlock(fl + w, gen);
 }
 
 // Fronts of at most 64 rows: one wave per front, lane i owning row i, and
