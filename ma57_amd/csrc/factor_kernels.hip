@@ -1509,6 +1509,8 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   __shared__ double dv[P];
   __shared__ int sgn[P];
   __shared__ double red[4][P];
+  __shared__ float cb[16][kSpecRows + 1];  // |c| of 16 columns x the workgroup's rows, rounded up
+  __shared__ float part[4][P];
   if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
   const int f = findSegment(wgStart, count, blockIdx.x);
   const int s = list[f];
@@ -1570,11 +1572,14 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
     }
   }
   __syncthreads();
+  // columns past the block: a neutral pivot for the branch-free row solve
+  for (int q = pw + tid; q < P; q += 256) dv[q] = 1.0;
   // a now holds c_ij (current value at pivot j) for i > j and a_jj^(j) on the diagonal
   double bm = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int i = ib + 4 * r;
+    if (i > j && (i >= pw || j >= pw)) W[i][j] = 0.0;
     if (j < pw && i < pw) {
       if (i > j) {
         const double l = static_cast<double>(static_cast<T>(a[r] / dv[j]));
@@ -1598,15 +1603,28 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   double x[P];
 #pragma unroll
   for (int q = 0; q < P; ++q) x[q] = (valid && q < pw) ? Fat(i, k0 + q) : 0.0;
-  __syncthreads();  // red is reused below
+  // Column maxima through LDS, 16 columns at a time: |c| rounded up to
+  // float (a bound that is never below the value, so a borderline column can
+  // only fail the test here and be decided by the exact kernel).
   specRow<T>(x, W, dv, pw, [&](int q, double c) {
-    double v = fabs(c);
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    if ((tid & 63) == 0) red[tid >> 6][q] = v;
+    const double ac = fabs(c);
+    float f = static_cast<float>(ac);
+    if (static_cast<double>(f) < ac) f = nextafterf(f, INFINITY);
+    cb[q & 15][tid] = f;
+    if ((q & 15) == 15 || q == pw - 1) {
+      __syncthreads();
+      const int col = tid & 15, seg = tid >> 4;  // 16 rows per thread
+      float v = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v = fmaxf(v, cb[col][seg * 16 + r]);
+      v = fmaxf(v, __shfl_xor(v, 16, 64));
+      v = fmaxf(v, __shfl_xor(v, 32, 64));
+      if ((tid & 63) < 16) part[tid >> 6][(q & ~15) + col] = v;
+      __syncthreads();
+    }
   });
-  __syncthreads();
   if (tid < pw) {
-    const double v = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+    const double v = fmax(fmax(part[0][tid], part[1][tid]), fmax(part[2][tid], part[3][tid]));
     if (v > 0.0) atomicMaxNonNeg(rec.colmax + parity * P + tid, v);
   }
 }
@@ -1652,26 +1670,23 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   double origMax = bitsToDouble(fd.origMaxBits[s]);
   if (!stat && origMax < prm.zeroTol) origMax = 1.0;
   if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
-  for (int q = tid; q < pw; q += 256) dv[q] = rec.dval[q];
-  for (int e = tid; e < pw * pw; e += 256) {
-    const int q = e / pw, jj = e % pw;
-    W[q][jj] = (jj < q) ? rec.dval[jj] * rec.lblk[q * P + jj] : 0.0;
+  for (int q = tid; q < P; q += 256) dv[q] = (q < pw) ? rec.dval[q] : 1.0;
+  for (int e = tid; e < P * P; e += 256) {
+    const int q = e / P, jj = e % P;
+    W[q][jj] = (jj < q && q < pw) ? rec.dval[jj] * rec.lblk[q * P + jj] : 0.0;
   }
-  if (tid == 0) {
-    int qs = pw;
-    if (!stat) {
+  // first failing column: the first wave tests one column per lane
+  if (tid < 64) {
+    const int q = tid;
+    bool fail = false;
+    if (!stat && q < pw) {
       const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
-      for (int q = 0; q < pw; ++q) {
-        const double lam = fmax(rec.bmax[q], bitsToDouble(rec.colmax[parity * P + q]));
-        const double akk = rec.cdiag[q];
-        const bool ok = (lam <= prm.zeroTol) ? fabs(akk) > isolatedFloor : fabs(akk) >= kAlpha * lam;
-        if (!ok) {
-          qs = q;
-          break;
-        }
-      }
+      const double lam = fmax(rec.bmax[q], bitsToDouble(rec.colmax[parity * P + q]));
+      const double akk = rec.cdiag[q];
+      fail = !((lam <= prm.zeroTol) ? fabs(akk) > isolatedFloor : fabs(akk) >= kAlpha * lam);
     }
-    qsh = qs;
+    const unsigned long long bad = __ballot(fail);
+    if (tid == 0) qsh = bad ? min(pw, __ffsll(static_cast<long long>(bad)) - 1) : pw;
   }
   __syncthreads();
   const int qs = qsh;
@@ -1682,7 +1697,9 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
     double x[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) x[q] = (q < lim) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
-    specRow<T>(x, W, dv, lim, [&](int, double c) { tmax = fmax(tmax, fabs(c)); });
+    specRow<T>(x, W, dv, lim, [&](int q, double c) {
+      if (q < lim) tmax = fmax(tmax, fabs(c));
+    });
 #pragma unroll
     for (int q = 0; q < P; ++q)
       if (q < lim) Fat(i, k0 + q) = static_cast<T>(x[q]);
@@ -1699,31 +1716,42 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
       tmax = fmax(tmax, fmax(fabs(dv[q]), rec.bmax[q]));
     }
     if (tid < P) rec.colmax[(parity ^ 1) * P + tid] = 0ull;
-    if (tid == 0) {
+    if (tid < 64) {
+      // inertia and perturbation of the kept pivots: one column per lane
+      const int q = tid;
       int npos = 0, nneg = 0, nzero = 0, npert = 0;
       double pert = 0.0;
-      const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
-      const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
-      for (int q = 0; q < qs; ++q) {
+      if (q < qs) {
         countInertia(dv[q], npos, nneg, nzero);
         if (stat) {
+          const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+          const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
           const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k0 + q]] : 0;
           double applied;
           (void)staticPivot(rec.cdiag[q], sign, delta, sfloor, applied);
           if (applied != 0.0) {
-            pert += fabs(applied);
-            ++npert;
+            pert = fabs(applied);
+            npert = 1;
           }
         }
       }
-      fd.inertia[3 * s] += npos;
-      fd.inertia[3 * s + 1] += nneg;
-      fd.inertia[3 * s + 2] += nzero;
-      fd.pert[s] += pert;
-      fd.npert[s] += npert;
-      fd.kcur[s] = k0 + qs;
-      fd.pcount[s] = qs;
-      fd.cbDone[s] = (qs == pw) ? 1 : 0;
+      for (int off = 32; off > 0; off >>= 1) {
+        npos += __shfl_xor(npos, off, 64);
+        nneg += __shfl_xor(nneg, off, 64);
+        nzero += __shfl_xor(nzero, off, 64);
+        npert += __shfl_xor(npert, off, 64);
+        pert += __shfl_xor(pert, off, 64);
+      }
+      if (tid == 0) {
+        fd.inertia[3 * s] += npos;
+        fd.inertia[3 * s + 1] += nneg;
+        fd.inertia[3 * s + 2] += nzero;
+        fd.pert[s] += pert;
+        fd.npert[s] += npert;
+        fd.kcur[s] = k0 + qs;
+        fd.pcount[s] = qs;
+        fd.cbDone[s] = (qs == pw) ? 1 : 0;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
@@ -1885,6 +1913,24 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, co
   double* F = static_cast<double*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  // The product is formed transposed (L operand first), so the result of a
+  // register holds 16 consecutive rows of 4 columns: the C tile is read and
+  // written in whole 128-byte column segments.  Its values are loaded before
+  // the products and arrive while they run.
+  double cv[2][2][4];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = i0 + wr + bi * 16 + lr;
+        const int col = j0 + wc + bj * 16 + 4 * rr + lk;
+        cv[bi][bj][rr] = (row < m && col < m && (ti != tj || row >= col)) ? Fat(row, col) : 0.0;
+      }
   for (int q = threadIdx.x; q < p; q += blockDim.x) {
     pdd[q] = fd.dDiag[io + k0 + q];
     pdo[q] = fd.dOff[io + k0 + q];
@@ -1892,50 +1938,70 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, co
   }
   __syncthreads();
   const int pp = (p + 3) & ~3;
-  for (int e = threadIdx.x; e < TS * pp; e += blockDim.x) {
-    const int r = e % TS, q = e / TS;
-    double a = 0.0, b = 0.0;
-    if (q < p) {
-      const int ri = i0 + r, rj = j0 + r;
-      if (ri < m) {
+  // Operand tiles: thread (r, qb) stages row r of both tiles for the columns
+  // q = qb + 4u.  Every load is issued before any is used (no dependent
+  // branches between them); the L D scaling, which needs the neighbouring
+  // column of a 2x2 block, is applied from LDS afterwards.
+  {
+    const int r = threadIdx.x & 63, qb = threadIdx.x >> 6;
+    const int ri = i0 + r, rj = j0 + r;
+    double la[KP / 4], lb[KP / 4];
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      const int q = qb + 4 * u;
+      la[u] = (q < p && ri < m) ? Fat(ri, k0 + q) : 0.0;
+      lb[u] = (q < p && rj < m) ? Fat(rj, k0 + q) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      As[r][qb + 4 * u] = la[u];
+      Bs[r][qb + 4 * u] = lb[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      const int q = qb + 4 * u;
+      double a = 0.0;
+      if (q < p) {
         const int kind = pkd[q];
         if (kind == 1)
-          a = Fat(ri, k0 + q) * pdd[q];
+          a = la[u] * pdd[q];
         else if (kind == 2)
-          a = Fat(ri, k0 + q) * pdd[q] + Fat(ri, k0 + q + 1) * pdo[q];
-        else
-          a = Fat(ri, k0 + q - 1) * pdo[q - 1] + Fat(ri, k0 + q) * pdd[q];
+          a = la[u] * pdd[q] + As[r][q + 1] * pdo[q];
+        else if (q > 0)
+          a = As[r][q - 1] * pdo[q - 1] + la[u] * pdd[q];
       }
-      if (rj < m) b = Fat(rj, k0 + q);
+      la[u] = a;
     }
-    As[r][q] = a;
-    Bs[r][q] = b;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) As[r][qb + 4 * u] = la[u];
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   v4d acc[2][2];
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
-  const int lr = lane & 15, lk = lane >> 4;
   for (int kk = 0; kk < pp; kk += 4) {
     const double a0 = As[wr + lr][kk + lk];
     const double a1 = As[wr + 16 + lr][kk + lk];
     const double b0 = Bs[wc + lr][kk + lk];
     const double b1 = Bs[wc + 16 + lr][kk + lk];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
   }
   double mx = 0.0;
+#pragma unroll
   for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
     for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + 4 * rr + lk;
-        const int col = j0 + wc + bj * 16 + lr;
+        const int row = i0 + wr + bi * 16 + lr;
+        const int col = j0 + wc + bj * 16 + 4 * rr + lk;
         if (row < m && col < m && (ti != tj || row >= col)) {
-          const double v = Fat(row, col) - acc[bi][bj][rr];
+          const double v = cv[bi][bj][rr] - acc[bi][bj][rr];
           Fat(row, col) = v;
           mx = fmax(mx, fabs(v));
         }
@@ -1980,6 +2046,22 @@ __global__ __launch_bounds__(256) void k_syrk_f32(const int* list, int count, co
   float* F = static_cast<float*>(fd.F[s]);
   const int64_t io = fd.idxOff[s];
   auto Fat = [&](int i, int j) -> float& { return F[i + static_cast<size_t>(j) * m]; };
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  // transposed product and early C loads, as in k_syrk_f64 (here a register
+  // holds 16 consecutive rows of one column: 64-byte segments)
+  float cv[2][2][4];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = i0 + wr + bi * 16 + lr;
+        const int col = j0 + wc + bj * 16 + 4 * lk + rr;
+        cv[bi][bj][rr] = (row < m && col < m && (ti != tj || row >= col)) ? Fat(row, col) : 0.0f;
+      }
   for (int q = threadIdx.x; q < p; q += blockDim.x) {
     pdd[q] = fd.dDiag[io + k0 + q];
     pdo[q] = fd.dOff[io + k0 + q];
@@ -1987,51 +2069,67 @@ __global__ __launch_bounds__(256) void k_syrk_f32(const int* list, int count, co
   }
   __syncthreads();
   const int pp = (p + 3) & ~3;
-  for (int e = threadIdx.x; e < TS * pp; e += blockDim.x) {
-    const int r = e % TS, q = e / TS;
-    double a = 0.0;
-    float b = 0.0f;
-    if (q < p) {
-      const int ri = i0 + r, rj = j0 + r;
-      if (ri < m) {
+  // operand tiles staged as in k_syrk_f64; L D formed in FP64 and rounded
+  {
+    const int r = threadIdx.x & 63, qb = threadIdx.x >> 6;
+    const int ri = i0 + r, rj = j0 + r;
+    float la[KP / 4], lb[KP / 4];
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      const int q = qb + 4 * u;
+      la[u] = (q < p && ri < m) ? Fat(ri, k0 + q) : 0.0f;
+      lb[u] = (q < p && rj < m) ? Fat(rj, k0 + q) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      As[r][qb + 4 * u] = la[u];
+      Bs[r][qb + 4 * u] = lb[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) {
+      const int q = qb + 4 * u;
+      double a = 0.0;
+      if (q < p) {
         const int kind = pkd[q];
         if (kind == 1)
-          a = Fat(ri, k0 + q) * pdd[q];
+          a = la[u] * pdd[q];
         else if (kind == 2)
-          a = Fat(ri, k0 + q) * pdd[q] + Fat(ri, k0 + q + 1) * pdo[q];
-        else
-          a = Fat(ri, k0 + q - 1) * pdo[q - 1] + Fat(ri, k0 + q) * pdd[q];
+          a = la[u] * pdd[q] + static_cast<double>(As[r][q + 1]) * pdo[q];
+        else if (q > 0)
+          a = static_cast<double>(As[r][q - 1]) * pdo[q - 1] + la[u] * pdd[q];
       }
-      if (rj < m) b = Fat(rj, k0 + q);
+      la[u] = static_cast<float>(a);
     }
-    As[r][q] = static_cast<float>(a);
-    Bs[r][q] = b;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP / 4; ++u) As[r][qb + 4 * u] = la[u];
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   v4f acc[2][2];
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lk = lane >> 4;
   for (int kk = 0; kk < pp; kk += 4) {
     const float a0 = As[wr + lr][kk + lk];
     const float a1 = As[wr + 16 + lr][kk + lk];
     const float b0 = Bs[wc + lr][kk + lk];
     const float b1 = Bs[wc + 16 + lr][kk + lk];
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, acc[1][1], 0, 0, 0);
   }
   double mx = 0.0;
+#pragma unroll
   for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
     for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + 4 * lk + rr;
-        const int col = j0 + wc + bj * 16 + lr;
+        const int row = i0 + wr + bi * 16 + lr;
+        const int col = j0 + wc + bj * 16 + 4 * lk + rr;
         if (row < m && col < m && (ti != tj || row >= col)) {
-          const float v = Fat(row, col) - acc[bi][bj][rr];
+          const float v = cv[bi][bj][rr] - acc[bi][bj][rr];
           Fat(row, col) = v;
           mx = fmax(mx, fabs(static_cast<double>(v)));
         }
