@@ -101,6 +101,9 @@ def main() -> int:
     ap.add_argument("--replays", type=int, default=10, help="c5: recorded refactorize + solve replays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: min(16, cpu count))")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device to device (the known-size kernel that scales the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--exact-panel", action="store_true",
                     help="panels without the speculative blocked step (pivot-by-pivot kernel only)")
     ap.add_argument("--spec-min-rows", type=int, default=None,
@@ -112,6 +115,14 @@ def main() -> int:
         return 2
 
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": src.numel() * 8}))
+        return 0
 
     from ma57_amd import _lib, problems as P
     from ma57_amd.batch import BatchedLDLT
@@ -216,9 +227,19 @@ def main() -> int:
     if expect is None:
         expect = tuple(fp64["inertia"])  # no closed form: the FP64 factorization of the same matrix
 
+    def traffic(kernel):
+        """HBM bytes per dispatch from the committed counter passes of the same workload, or None."""
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_syrk_laplace3d_40.json")
+        if args.config != "c2" or nrhs != 16 or not os.path.exists(path):
+            return None
+        with open(path) as fh:
+            rec = json.load(fh)
+        k = rec.get("kernels", {}).get(kernel)
+        return k["hbm_bytes_per_dispatch"] if k else None
+
     def roofline(kernel, r, peak):
         return {"bound": "mfma", "kernel": kernel, "achieved": r["syrk"]["achieved"], "peak": peak,
-                "unit": "TFLOP/s", "frac": r["syrk"]["achieved"] / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": r["syrk"]["achieved"] / peak, "traffic": traffic(kernel),
                 "launches_per_step": r["syrk"]["launches_per_step"], "avg_launch_ms": r["syrk"]["avg_launch_ms"],
                 "flops_per_launch": r["syrk"]["flops_per_launch"]}
 

@@ -645,6 +645,19 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
 // Refinement support (refine.hpp:69-144).
 // ---------------------------------------------------------------------------
 
+// Max of v over a 256-thread workgroup into *bits (bits of a non-negative
+// double) with one atomic per workgroup; every thread of the group calls it.
+__device__ inline void groupAtomicMax(double v, unsigned long long* bits) {
+  __shared__ double wmax[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    if (mx > 0.0) atomicMax(bits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  }
+}
+
 // Symmetric CSR values in row order, aSym[t] = aVals[valSrc[t]]: the full
 // matrix built from the caller's lower-triangle CSC values
 // (selfadjointView<Lower> semantics).
@@ -701,10 +714,7 @@ __global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t
         }
     }
   }
-  if (!rmaxBits) return;
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0 && mx > 0.0)
-    atomicMax(rmaxBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  if (rmaxBits) groupAtomicMax(mx, rmaxBits);
 }
 
 // out = x + dx, and max |out| into *maxBits when given.
@@ -717,10 +727,7 @@ __global__ void k_axpy(int64_t len, const double* __restrict__ x, const double* 
     out[e] = v;
     mx = fmax(mx, fabs(v));
   }
-  if (!maxBits) return;
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0 && mx > 0.0)
-    atomicMax(maxBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  if (maxBits) groupAtomicMax(mx, maxBits);
 }
 
 // max |v| over a strided n x nrhs block, into *outBits (must start at 0).
@@ -732,9 +739,7 @@ __global__ void k_absmax(int n, int nrhs, const double* __restrict__ v, int64_t 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
       mx = fmax(mx, fabs(col[i]));
   }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0 && mx > 0.0)
-    atomicMax(outBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  groupAtomicMax(mx, outBits);
 }
 
 // Per-column max |v| of an n x nrhs block: workgroup (c, part) covers rows
@@ -764,9 +769,7 @@ __global__ void k_rowsum_max(int n, const int64_t* __restrict__ rowPtr, const in
     for (int64_t t = rowPtr[i]; t < rowPtr[i + 1]; ++t) sacc += fabs(aVals[valSrc[t]]);
     mx = fmax(mx, sacc);
   }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0 && mx > 0.0)
-    atomicMax(outBits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  groupAtomicMax(mx, outBits);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int count) {
@@ -864,7 +867,7 @@ void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, d
 }
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, unsigned long long* maxBits,
                 hipStream_t st) {
-  k_axpy<<<gridFor(len, 256), 256, 0, st>>>(len, x, dx, out, maxBits);
+  k_axpy<<<std::min(gridFor(len, 256), 2048), 256, 0, st>>>(len, x, dx, out, maxBits);
   hipCheck(hipGetLastError(), "launchAxpy");
 }
 void launchAbsMax(int n, int nrhs, const double* v, int64_t ld, unsigned long long* outBits, hipStream_t st) {
@@ -883,7 +886,7 @@ void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned lon
 }
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st) {
-  k_rowsum_max<<<gridFor(n, 256), 256, 0, st>>>(n, rowPtr, valSrc, aVals, outBits);
+  k_rowsum_max<<<std::min(gridFor(n, 256), 2048), 256, 0, st>>>(n, rowPtr, valSrc, aVals, outBits);
   hipCheck(hipGetLastError(), "launchRowSumMax");
 }
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st) {
