@@ -7,6 +7,8 @@
 // Pivot decision logic follows the reference's DenseLDLT::factor
 // (include/symla/dense_kernel.hpp:585-792) and DenseLDLT::factorStatic
 // (dense_kernel.hpp:962-1035); see the comments at each step.
+#include <climits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -38,13 +40,24 @@ __device__ inline void argmaxCombine(double& v, int& i, double v2, int i2) {
   }
 }
 
+// Wavefront reductions of the device library (cross-lane DPP steps instead
+// of LDS permutes).
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_max_f64(double);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_min_i32(int);
+
+// argmaxCombine over the 64 lanes of a wave (every lane active; a lane with
+// no candidate holds v = 0, idx = -1): the largest value and, among lanes
+// holding it, the smallest row.
+__device__ inline void waveArgmax(double& v, int& idx) {
+  const double wm = __ockl_wfred_max_f64(v);
+  const int wi = __ockl_wfred_min_i32((idx >= 0 && v == wm) ? idx : INT_MAX);
+  v = wm;
+  idx = (wi == INT_MAX) ? -1 : wi;
+}
+
 template <int NT>
 __device__ inline void blockArgmax(double& v, int& idx, double* sv, int* si) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double v2 = __shfl_xor(v, off, 64);
-    const int i2 = __shfl_xor(idx, off, 64);
-    argmaxCombine(v, idx, v2, i2);
-  }
+  waveArgmax(v, idx);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
     sv[w] = v;
@@ -66,7 +79,7 @@ __device__ inline void blockArgmax(double& v, int& idx, double* sv, int* si) {
 
 template <int NT>
 __device__ inline double blockMax(double v, double* sv) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  v = __ockl_wfred_max_f64(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sv[w] = v;
   __syncthreads();
@@ -751,11 +764,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
           if (v > 0.0) argmaxCombine(lv, li, v, i);
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(lv, off, 64);
-        const int i2 = __shfl_xor(li, off, 64);
-        argmaxCombine(lv, li, v2, i2);
-      }
+      waveArgmax(lv, li);
       if ((tid & 63) == 0) {
         fastV[parity][tid >> 6] = lv;
         fastI[parity][tid >> 6] = li;
