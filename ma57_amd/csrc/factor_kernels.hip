@@ -576,6 +576,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   __shared__ int redI[NT / 64 + 1];
   __shared__ double fastV[2][NT / 64];
   __shared__ int fastI[2][NT / 64];
+  __shared__ double blkV[NT / 64][SW];  // per-wave column maxima of a blocked step
 
   const int tid = threadIdx.x;
   if (static_cast<int>(blockIdx.x) >= count) return;
@@ -749,7 +750,155 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
   int parity = 0;
   if (RM > 0) loadSub();
 
+  // Blocked first test over the sub-panel's remaining columns [k, c0 + sw):
+  // every thread factors that block (at most SW x SW, from LDS) with 1x1
+  // pivots in order and runs the same recurrence over its own rows,
+  // c_ij = a_ij - sum_{q<j} l_iq d_q l_(k+j)q, so one reduction gives
+  // lambda for every column at once.  The pivots before the first column
+  // whose test fails are kept (the same decisions as pivot by pivot, up to
+  // the order of rounding); that column goes through the single-pivot path
+  // below.  Returns the number of pivots kept.
+  auto blockStep = [&]() -> int {
+    const int kk0 = k - c0, bw = sw - kk0;
+    double Wb[SW][SW], dq[SW], aq[SW];
+    {
+      double B[SW][SW];
+#pragma unroll
+      for (int a = 0; a < SW; ++a)
+#pragma unroll
+        for (int b = 0; b <= a; ++b) B[a][b] = (a < bw) ? Cs[(kk0 + b) * CSR + (k + a - k0)] : 0.0;
+#pragma unroll
+      for (int q = 0; q < SW; ++q) {
+        dq[q] = 1.0;
+        aq[q] = 0.0;
+        if (q < bw) {
+          aq[q] = B[q][q];
+          double d = aq[q];
+          if (stat) {
+            const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k + q]] : 0;
+            double applied;
+            d = staticPivot(d, sign, delta, sfloor, applied);
+          }
+          dq[q] = d;
+          double la[SW];
+#pragma unroll
+          for (int a = q + 1; a < SW; ++a) {
+            la[a] = (a < bw) ? static_cast<double>(static_cast<T>(B[a][q] / d)) : 0.0;
+            Wb[a][q] = d * la[a];
+          }
+#pragma unroll
+          for (int a = q + 1; a < SW; ++a)
+#pragma unroll
+            for (int b = q + 1; b <= a; ++b) B[a][b] = fma(-la[a], Wb[b][q], B[a][b]);
+        }
+      }
+    }
+    // pass 1: column maxima over this thread's rows below each pivot
+    double vmax[SW];
+#pragma unroll
+    for (int j = 0; j < SW; ++j) vmax[j] = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < (RM > 0 ? RM : 1); ++jj) {
+      const int i = k0 + tid + jj * NT;
+      if (i < k || i >= nE) continue;
+      double l[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) {
+        if (j < bw) {
+          double c = Cs[(kk0 + j) * CSR + (i - k0)];
+#pragma unroll
+          for (int q = 0; q < j; ++q) c = fma(-l[q], Wb[j][q], c);
+          if (i > k + j) vmax[j] = fmax(vmax[j], fabs(c));
+          l[j] = static_cast<double>(static_cast<T>(c / dq[j]));
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SW; ++j) {
+      const double v = __ockl_wfred_max_f64(vmax[j]);
+      if ((tid & 63) == 0) blkV[tid >> 6][j] = v;
+    }
+    __syncthreads();
+    int qs = bw;
+    {
+      const double isolatedFloor = fmax(prm.zeroTol, prm.relFloor * origMax);
+      for (int j = 0; j < bw; ++j) {
+        double lam = 0.0;
+        for (int w = 0; w < NT / 64; ++w) lam = fmax(lam, blkV[w][j]);
+        const bool ok = stat || ((lam <= prm.zeroTol) ? fabs(aq[j]) > isolatedFloor : fabs(aq[j]) >= kAlpha * lam);
+        if (!ok) {
+          qs = j;
+          break;
+        }
+      }
+    }
+    if (qs == 0) return 0;  // blkV is rewritten only after the next block step's barrier
+    // pass 2: multipliers of the kept columns, current values of the others
+#pragma unroll
+    for (int jj = 0; jj < (RM > 0 ? RM : 1); ++jj) {
+      const int i = k0 + tid + jj * NT;
+      if (i < k || i >= nE) continue;
+      double x[SW], l[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) x[j] = (j < bw) ? Cs[(kk0 + j) * CSR + (i - k0)] : 0.0;
+#pragma unroll
+      for (int j = 0; j < SW; ++j) {
+        if (j < qs) {
+          double c = x[j];
+#pragma unroll
+          for (int q = 0; q < j; ++q) c = fma(-l[q], Wb[j][q], c);
+          l[j] = static_cast<double>(static_cast<T>(c / dq[j]));
+          if (i > k + j) {
+            tmax = fmax(tmax, fabs(c));
+            Fat(i, k + j) = static_cast<T>(l[j]);
+          }
+        } else if (j < bw) {
+          double y = x[j];
+#pragma unroll
+          for (int q = 0; q < SW; ++q)
+            if (q < qs) y = fma(-l[q], Wb[j][q], y);
+          Cs[(kk0 + j) * CSR + (i - k0)] = y;
+        }
+      }
+    }
+    for (int q = 0; q < qs; ++q) {
+      pdd[p + q] = dq[q];
+      pdo[p + q] = 0.0;
+      pkd[p + q] = 1;
+      countInertia(dq[q], npos, nneg, nzero);
+      tmax = fmax(tmax, fabs(dq[q]));
+      if (stat && dq[q] != aq[q]) {
+        pert += fabs(dq[q] - aq[q]);
+        ++npert;
+      }
+    }
+    if (tid == 0)
+      for (int q = 0; q < qs; ++q) {
+        fd.dDiag[io + k + q] = dq[q];
+        fd.dOff[io + k + q] = 0.0;
+        fd.pkind[io + k + q] = 1;
+        Fat(k + q, k + q) = static_cast<T>(dq[q]);
+      }
+    __syncthreads();  // the kept rows' Cs values are read by other threads next
+    return qs;
+  };
+  int skipAt = -1;
+
   while (k < nbEnd) {
+    if (RM > 0 && SW > 1 && k != skipAt && k - c0 + 1 < sw) {
+      const int kept = blockStep();
+      if (kept == 0) {
+        skipAt = k;
+      } else {
+        k += kept;
+        p += kept;
+        if (k - c0 >= sw && k < nbEnd) {
+          __syncthreads();
+          loadSub();
+        }
+      }
+      continue;
+    }
     double lv = 0.0;
     int li = -1;
     double akk;
