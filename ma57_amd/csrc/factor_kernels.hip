@@ -2034,266 +2034,228 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
 }
 
 // Trailing update T(k1:m, k1:m) -= (L D) L^T over the lower triangle, for
-// the last panel's p pivots, with v_mfma_f64_16x16x4f64.  64x64 output tile
-// per workgroup, four waves of 32x32.  Operand layout of the instruction on
-// gfx950 (measured): A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
-// result D[i][j] in lane j + 16*(i%4), register i/4.
-typedef double v4d __attribute__((ext_vector_type(4)));
+// the last panel's p pivots, on the matrix cores: v_mfma_f64_16x16x4f64 for
+// FP64 fronts, v_mfma_f32_16x16x4f32 on FP32 operands for FP32 fronts (L D
+// formed in FP64 from the FP64 pivots, then rounded).  Operand layout of the
+// instructions on gfx950 (measured): A[i][k] from lane i + 16k, B[k][j] from
+// lane j + 16k; the result D[i][j] is in lane j + 16*(i%4), register i/4
+// (f64) or lane j + 16*(i/4), register i%4 (f32).
+//
+// 64x64 lower-triangle output tiles, four waves of 32x32.  The workgroups are
+// persistent: each walks the tiles t = blockIdx.x, + gridDim.x, ...; while a
+// tile's products and write-back run, the next tile's operand rows and C
+// values are already being loaded into registers, and a front's pivots are
+// read only when the walk enters a new front.  The product is formed
+// transposed (L operand first), so a register holds 16 consecutive rows of a
+// column: C is read and written in whole column segments.
+template <typename T>
+struct SchurMfma;
+template <>
+struct SchurMfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // column (within the 16 x 16 block) of register r of this lane; the row is lane % 16
+  static __device__ inline int col(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct SchurMfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ inline int col(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
-__global__ __launch_bounds__(256) void k_syrk_f64(const int* list, int count, const int* tileStart, DevFronts fd) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_schur(const int* list, int count, const int* tileStart, DevFronts fd) {
+  using M = SchurMfma<T>;
   constexpr int TS = 64;
-  constexpr int KP = kPanel + 4;  // padded K
-  __shared__ double As[TS][KP + 1];
-  __shared__ double Bs[TS][KP + 1];
+  constexpr int KP = kPanel + 4;  // padded K (a 2x2 at the panel end makes p = 65)
+  constexpr int KU = KP / 4;      // staged columns per thread
+  __shared__ T As[TS][KP + 1];    // L D rows of the tile's row block
+  __shared__ T Bs[TS][KP + 1];    // L rows of the tile's column block
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
-  if (static_cast<int>(blockIdx.x) >= tileStart[count]) return;
-  const int f = findSegment(tileStart, count, blockIdx.x);
-  const int s = list[f];
-  const int p = fd.pcount[s];
-  if (p == 0) return;
-  const int m = fd.m[s];
-  const int k1 = fd.kcur[s];
-  const int k0 = k1 - p;
-  const int N = m - k1;
-  if (N <= 0) return;
-  const int T = (N + TS - 1) / TS;
-  // lower-triangular tile index
-  const int t = blockIdx.x - tileStart[f];
-  int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  while (ti * (ti + 1) / 2 > t) --ti;
-  const int tj = t - ti * (ti + 1) / 2;
-  if (ti >= T) return;
-  const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
-  if (t == 0 && threadIdx.x == 0 && fd.schurFlops)
-    atomicAdd(fd.schurFlops, static_cast<unsigned long long>(p) * N * (N + 1));
-  double* F = static_cast<double*>(fd.F[s]);
-  const int64_t io = fd.idxOff[s];
-  auto Fat = [&](int i, int j) -> double& { return F[i + static_cast<size_t>(j) * m]; };
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int total = tileStart[count];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  // The product is formed transposed (L operand first), so the result of a
-  // register holds 16 consecutive rows of 4 columns: the C tile is read and
-  // written in whole 128-byte column segments.  Its values are loaded before
-  // the products and arrive while they run.
-  double cv[2][2][4];
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + lr;
-        const int col = j0 + wc + bj * 16 + 4 * rr + lk;
-        cv[bi][bj][rr] = (row < m && col < m && (ti != tj || row >= col)) ? Fat(row, col) : 0.0;
-      }
-  for (int q = threadIdx.x; q < p; q += blockDim.x) {
-    pdd[q] = fd.dDiag[io + k0 + q];
-    pdo[q] = fd.dOff[io + k0 + q];
-    pkd[q] = fd.pkind[io + k0 + q];
-  }
-  __syncthreads();
-  const int pp = (p + 3) & ~3;
-  // Operand tiles: thread (r, qb) stages row r of both tiles for the columns
-  // q = qb + 4u.  Every load is issued before any is used (no dependent
-  // branches between them); the L D scaling, which needs the neighbouring
-  // column of a 2x2 block, is applied from LDS afterwards.
-  {
-    const int r = threadIdx.x & 63, qb = threadIdx.x >> 6;
-    const int ri = i0 + r, rj = j0 + r;
-    double la[KP / 4], lb[KP / 4];
-#pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
-      const int q = qb + 4 * u;
-      la[u] = (q < p && ri < m) ? Fat(ri, k0 + q) : 0.0;
-      lb[u] = (q < p && rj < m) ? Fat(rj, k0 + q) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
-      As[r][qb + 4 * u] = la[u];
-      Bs[r][qb + 4 * u] = lb[u];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
-      const int q = qb + 4 * u;
-      double a = 0.0;
-      if (q < p) {
-        const int kind = pkd[q];
-        if (kind == 1)
-          a = la[u] * pdd[q];
-        else if (kind == 2)
-          a = la[u] * pdd[q] + As[r][q + 1] * pdo[q];
-        else if (q > 0)
-          a = As[r][q - 1] * pdo[q - 1] + la[u] * pdd[q];
-      }
-      la[u] = a;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < KP / 4; ++u) As[r][qb + 4 * u] = la[u];
-  }
-  __syncthreads();
-  v4d acc[2][2];
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int kk = 0; kk < pp; kk += 4) {
-    const double a0 = As[wr + lr][kk + lk];
-    const double a1 = As[wr + 16 + lr][kk + lk];
-    const double b0 = Bs[wc + lr][kk + lk];
-    const double b1 = Bs[wc + 16 + lr][kk + lk];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
-  }
-  double mx = 0.0;
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + lr;
-        const int col = j0 + wc + bj * 16 + 4 * rr + lk;
-        if (row < m && col < m && (ti != tj || row >= col)) {
-          const double v = cv[bi][bj][rr] - acc[bi][bj][rr];
-          Fat(row, col) = v;
-          mx = fmax(mx, fabs(v));
-        }
-      }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
-}
+  const int sr = tid & 63, qb = tid >> 6;  // staging: row sr of both operand tiles, columns qb + 4u
 
-// FP32 storage: the same update with v_mfma_f32_16x16x4f32 on float
-// operands (L D formed in FP64 from the FP64 pivots, then rounded).  Result
-// layout of the instruction on gfx950 (measured): D[i][j] in lane
-// j + 16*(i/4), register i%4.
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__global__ __launch_bounds__(256) void k_syrk_f32(const int* list, int count, const int* tileStart, DevFronts fd) {
-  constexpr int TS = 64;
-  constexpr int KP = kPanel + 4;  // padded K
-  __shared__ float As[TS][KP + 1];
-  __shared__ float Bs[TS][KP + 1];
-  __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
-  __shared__ int pkd[kPanel + 2];
-  if (static_cast<int>(blockIdx.x) >= tileStart[count]) return;
-  const int f = findSegment(tileStart, count, blockIdx.x);
-  const int s = list[f];
-  const int p = fd.pcount[s];
-  if (p == 0) return;
-  const int m = fd.m[s];
-  const int k1 = fd.kcur[s];
-  const int k0 = k1 - p;
-  const int N = m - k1;
-  if (N <= 0) return;
-  const int T = (N + TS - 1) / TS;
-  const int t = blockIdx.x - tileStart[f];
-  int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  while (ti * (ti + 1) / 2 > t) --ti;
-  const int tj = t - ti * (ti + 1) / 2;
-  if (ti >= T) return;
-  const int i0 = k1 + ti * TS, j0 = k1 + tj * TS;
-  if (t == 0 && threadIdx.x == 0 && fd.schurFlops)
-    atomicAdd(fd.schurFlops, static_cast<unsigned long long>(p) * N * (N + 1));
-  float* F = static_cast<float*>(fd.F[s]);
-  const int64_t io = fd.idxOff[s];
-  auto Fat = [&](int i, int j) -> float& { return F[i + static_cast<size_t>(j) * m]; };
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-  const int lr = lane & 15, lk = lane >> 4;
-  // transposed product and early C loads, as in k_syrk_f64 (here a register
-  // holds 16 consecutive rows of one column: 64-byte segments)
-  float cv[2][2][4];
+  // Geometry of tile t.  A front's tiles are numbered column block by
+  // column block (tj, then ti = tj .. Tp-1 down the column), so the
+  // workgroups running at the same time read and write consecutive pieces of
+  // the same columns.  Tp is the planned tiles per side (the launch covers
+  // Tp (Tp + 1) / 2 tiles per front); the front's metadata is read again only
+  // when the walk enters another front.
+  struct Tile {
+    int f, s, p, m, k0, k1, Tp, lo, hi, ti, tj;
+    T* F;
+  };
+  auto tileOf = [&](int t, const Tile* prev) -> Tile {
+    Tile x;
+    if (prev && t >= prev->lo && t < prev->hi) {
+      x = *prev;
+    } else {
+      x.f = findSegment(tileStart, count, t);
+      x.lo = tileStart[x.f];
+      x.hi = tileStart[x.f + 1];
+      x.s = list[x.f];
+      x.p = fd.pcount[x.s];
+      x.m = fd.m[x.s];
+      x.k1 = fd.kcur[x.s];
+      x.k0 = x.k1 - x.p;
+      x.F = static_cast<T*>(fd.F[x.s]);
+      const int nt = x.hi - x.lo;
+      int Tp = static_cast<int>((sqrt(8.0 * nt + 1.0) - 1.0) * 0.5);
+      while ((Tp + 1) * (Tp + 2) / 2 <= nt) ++Tp;
+      while (Tp * (Tp + 1) / 2 > nt) --Tp;
+      x.Tp = Tp;
+    }
+    // column block tj starts at S(tj) = tj Tp - tj (tj - 1) / 2
+    const int lt = t - x.lo;
+    const double b = x.Tp + 0.5;
+    int tj = static_cast<int>(b - sqrt(fmax(b * b - 2.0 * lt, 0.0)));
+    auto S = [&](int c) { return c * x.Tp - c * (c - 1) / 2; };
+    while (tj > 0 && S(tj) > lt) --tj;
+    while (tj + 1 < x.Tp && S(tj + 1) <= lt) ++tj;
+    x.tj = tj;
+    x.ti = tj + (lt - S(tj));
+    return x;
+  };
+  // a tile does work when its front took pivots and the tile lies in the trailing block
+  auto live = [&](const Tile& x) { return x.p > 0 && x.k1 + x.ti * TS < x.m; };
+  T la[KU], lb[KU], cv[2][2][4];
+  auto loadTile = [&](const Tile& x) {
+    const int i0 = x.k1 + x.ti * TS, j0 = x.k1 + x.tj * TS;
+    const int ri = i0 + sr, rj = j0 + sr;
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + lr;
-        const int col = j0 + wc + bj * 16 + 4 * lk + rr;
-        cv[bi][bj][rr] = (row < m && col < m && (ti != tj || row >= col)) ? Fat(row, col) : 0.0f;
-      }
-  for (int q = threadIdx.x; q < p; q += blockDim.x) {
-    pdd[q] = fd.dDiag[io + k0 + q];
-    pdo[q] = fd.dOff[io + k0 + q];
-    pkd[q] = fd.pkind[io + k0 + q];
-  }
-  __syncthreads();
-  const int pp = (p + 3) & ~3;
-  // operand tiles staged as in k_syrk_f64; L D formed in FP64 and rounded
-  {
-    const int r = threadIdx.x & 63, qb = threadIdx.x >> 6;
-    const int ri = i0 + r, rj = j0 + r;
-    float la[KP / 4], lb[KP / 4];
-#pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
+    for (int u = 0; u < KU; ++u) {
       const int q = qb + 4 * u;
-      la[u] = (q < p && ri < m) ? Fat(ri, k0 + q) : 0.0f;
-      lb[u] = (q < p && rj < m) ? Fat(rj, k0 + q) : 0.0f;
+      la[u] = (q < x.p && ri < x.m) ? x.F[ri + static_cast<size_t>(x.k0 + q) * x.m] : T(0);
+      lb[u] = (q < x.p && rj < x.m) ? x.F[rj + static_cast<size_t>(x.k0 + q) * x.m] : T(0);
     }
 #pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
-      As[r][qb + 4 * u] = la[u];
-      Bs[r][qb + 4 * u] = lb[u];
-    }
-    __syncthreads();
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int u = 0; u < KP / 4; ++u) {
-      const int q = qb + 4 * u;
-      double a = 0.0;
-      if (q < p) {
-        const int kind = pkd[q];
-        if (kind == 1)
-          a = la[u] * pdd[q];
-        else if (kind == 2)
-          a = la[u] * pdd[q] + static_cast<double>(As[r][q + 1]) * pdo[q];
-        else if (q > 0)
-          a = static_cast<double>(As[r][q - 1]) * pdo[q - 1] + la[u] * pdd[q];
-      }
-      la[u] = static_cast<float>(a);
-    }
-    __syncthreads();
+      for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-    for (int u = 0; u < KP / 4; ++u) As[r][qb + 4 * u] = la[u];
-  }
-  __syncthreads();
-  v4f acc[2][2];
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int kk = 0; kk < pp; kk += 4) {
-    const float a0 = As[wr + lr][kk + lk];
-    const float a1 = As[wr + 16 + lr][kk + lk];
-    const float b0 = Bs[wc + lr][kk + lk];
-    const float b1 = Bs[wc + 16 + lr][kk + lk];
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, acc[1][1], 0, 0, 0);
-  }
-  double mx = 0.0;
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = i0 + wr + bi * 16 + lr;
-        const int col = j0 + wc + bj * 16 + 4 * lk + rr;
-        if (row < m && col < m && (ti != tj || row >= col)) {
-          const float v = cv[bi][bj][rr] - acc[bi][bj][rr];
-          Fat(row, col) = v;
-          mx = fmax(mx, fabs(static_cast<double>(v)));
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = i0 + wr + bi * 16 + lr;
+          const int col = j0 + wc + bj * 16 + M::col(lane, rr);
+          cv[bi][bj][rr] = (row < x.m && col < x.m && (x.ti != x.tj || row >= col))
+                               ? x.F[row + static_cast<size_t>(col) * x.m]
+                               : T(0);
         }
+  };
+
+  int t = blockIdx.x;
+  if (t >= total) return;
+  Tile cur = tileOf(t, nullptr);
+  int pivFront = -1;  // front whose pivots are in pdd / pdo / pkd
+  if (live(cur)) loadTile(cur);
+  while (t < total) {
+    const int tn = t + gridDim.x;
+    if (live(cur)) {
+      if (cur.s != pivFront) {
+        for (int q = tid; q < cur.p; q += blockDim.x) {
+          const int64_t io = fd.idxOff[cur.s];
+          pdd[q] = fd.dDiag[io + cur.k0 + q];
+          pdo[q] = fd.dOff[io + cur.k0 + q];
+          pkd[q] = fd.pkind[io + cur.k0 + q];
+        }
+        pivFront = cur.s;
+        if (cur.ti == 0 && cur.tj == 0 && tid == 0 && fd.schurFlops) {
+          const unsigned long long N = static_cast<unsigned long long>(cur.m - cur.k1);
+          atomicAdd(fd.schurFlops, static_cast<unsigned long long>(cur.p) * N * (N + 1));
+        }
+      } else if (cur.ti == 0 && cur.tj == 0 && tid == 0 && fd.schurFlops) {
+        const unsigned long long N = static_cast<unsigned long long>(cur.m - cur.k1);
+        atomicAdd(fd.schurFlops, static_cast<unsigned long long>(cur.p) * N * (N + 1));
       }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + s, mx);
+      // stage the operands: raw L rows, then the L D scaling (a 2x2 block
+      // needs its neighbouring column, read back from LDS)
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        As[sr][qb + 4 * u] = la[u];
+        Bs[sr][qb + 4 * u] = lb[u];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int q = qb + 4 * u;
+        double a = 0.0;
+        if (q < cur.p) {
+          const int kind = pkd[q];
+          const double l = static_cast<double>(la[u]);
+          if (kind == 1)
+            a = l * pdd[q];
+          else if (kind == 2)
+            a = l * pdd[q] + static_cast<double>(As[sr][q + 1]) * pdo[q];
+          else if (q > 0)
+            a = static_cast<double>(As[sr][q - 1]) * pdo[q - 1] + l * pdd[q];
+        }
+        la[u] = static_cast<T>(a);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < KU; ++u) As[sr][qb + 4 * u] = la[u];
+      __syncthreads();
+    }
+    // the next tile's operands and C travel while this tile computes
+    T cn[2][2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) cn[bi][bj][rr] = cv[bi][bj][rr];
+    Tile nxt;
+    if (tn < total) {
+      nxt = tileOf(tn, &cur);
+      if (live(nxt)) loadTile(nxt);
+    }
+    if (live(cur)) {
+      const int pp = (cur.p + 3) & ~3;
+      typename M::v4 acc[2][2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = typename M::v4{0, 0, 0, 0};
+#pragma unroll 4
+      for (int kk = 0; kk < pp; kk += 4) {
+        const T a0 = As[wr + lr][kk + lk];
+        const T a1 = As[wr + 16 + lr][kk + lk];
+        const T b0 = Bs[wc + lr][kk + lk];
+        const T b1 = Bs[wc + 16 + lr][kk + lk];
+        acc[0][0] = M::mma(b0, a0, acc[0][0]);
+        acc[0][1] = M::mma(b1, a0, acc[0][1]);
+        acc[1][0] = M::mma(b0, a1, acc[1][0]);
+        acc[1][1] = M::mma(b1, a1, acc[1][1]);
+      }
+      const int i0 = cur.k1 + cur.ti * TS, j0 = cur.k1 + cur.tj * TS;
+      double mx = 0.0;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = i0 + wr + bi * 16 + lr;
+            const int col = j0 + wc + bj * 16 + M::col(lane, rr);
+            if (row < cur.m && col < cur.m && (cur.ti != cur.tj || row >= col)) {
+              const T v = static_cast<T>(cn[bi][bj][rr] - acc[bi][bj][rr]);
+              cur.F[row + static_cast<size_t>(col) * cur.m] = v;
+              mx = fmax(mx, fabs(static_cast<double>(v)));
+            }
+          }
+      mx = __ockl_wfred_max_f64(mx);
+      if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + cur.s, mx);
+      __syncthreads();  // the LDS tiles are restaged for the next tile
+    }
+    t = tn;
+    if (t < total) cur = nxt;
+  }
 }
 
 __global__ void k_front_finish(const int* list, int count, DevFronts fd) {
@@ -2509,10 +2471,12 @@ void launchPanelRows(const int* list, int count, const int* blockStart, int tota
 void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
                 hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
+  // persistent workgroups: two per CU (the operand tiles take 70 KB of LDS in FP64)
+  const int blocks = std::min(totalTiles, 2 * 256);
   if (fd.fp32)
-    k_syrk_f32<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
+    k_schur<float><<<blocks, 256, 0, st>>>(list, count, tileStart, fd);
   else
-    k_syrk_f64<<<totalTiles, 256, 0, st>>>(list, count, tileStart, fd);
+    k_schur<double><<<blocks, 256, 0, st>>>(list, count, tileStart, fd);
   hipCheck(hipGetLastError(), "launchSyrk");
 }
 void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st) {
