@@ -233,7 +233,7 @@ def main() -> int:
 
     def traffic(kernel):
         """HBM bytes per dispatch from the committed counter passes of the same workload, or None."""
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_syrk_laplace3d_40.json")
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_schur_laplace3d_40.json")
         if args.config != "c2" or nrhs != 16 or not os.path.exists(path):
             return None
         with open(path) as fh:
@@ -307,13 +307,13 @@ def main() -> int:
                         "max_front": max(len(s.rowPattern) for s in sf.supernodes),
                         "speculative_panel": bool(tuning.speculative_panel),
                         "speculative_min_rows": tuning.speculative_min_rows}, **extra),
-        "roofline": roofline("k_syrk_f32", mixed, PEAK_FP32_MFMA_TFLOPS),
+        "roofline": roofline("k_schur<float>", mixed, PEAK_FP32_MFMA_TFLOPS),
         "cpu_baseline": cpu,
         "fp32_ir": mixed,
         "fp64": fp64,
         "fp64_s": fp64["s"],
         "speedup_fp32ir_vs_fp64": fp64["s"] / mixed["s"],
-        "roofline_fp64": roofline("k_syrk_f64", fp64, PEAK_FP64_MFMA_TFLOPS),
+        "roofline_fp64": roofline("k_schur<double>", fp64, PEAK_FP64_MFMA_TFLOPS),
         "backward_error": mixed["backward_error"],
         "inertia": mixed["inertia"],
         "inertia_expected": list(expect),
