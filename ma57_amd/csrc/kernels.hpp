@@ -47,12 +47,20 @@ void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double*
 void launchSolveBegin(unsigned* gen, hipStream_t st);
 void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                      int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st);
+// own rows per sweep block: the sweeps apply the inverse of each block's
+// unit lower diagonal block (kSolveBlock^2 doubles, row-major, at
+// invOff[s] + kSolveBlock^2 * block), formed once per factorization
+constexpr int kSolveBlock = 32;
+void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
+                       double* inv, const int64_t* invOff, hipStream_t st);
 void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st);
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
+                    const int64_t* invOff, hipStream_t st);
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st);
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
+                    const int64_t* invOff, hipStream_t st);
 // fronts of at most kTinyFront rows: one wave per front, no dependency flags
 constexpr int kTinyFront = 64;
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,

@@ -321,7 +321,29 @@ void Plan::uploadLayout() {
   }
   // solve workspaces depend on the layout
   gOffNrhs_ = -1;
+  {
+    std::vector<int> lst, start(1, 0);
+    std::vector<int64_t> off(numSN_ + 1, 0);
+    for (int s = 0; s < numSN_; ++s) {
+      const int nb = hm_[s] > kTinyFront ? (hnE_[s] + kSolveBlock - 1) / kSolveBlock : 0;
+      off[s + 1] = off[s] + static_cast<int64_t>(nb) * kSolveBlock * kSolveBlock;
+      if (nb > 0) {
+        lst.push_back(s);
+        start.push_back(start.back() + nb);
+      }
+    }
+    invCount_ = static_cast<int>(lst.size());
+    invBlocks_ = start.back();
+    dinvList_.upload(lst, stream_);
+    dinvStart_.upload(start, stream_);
+    dinvOff_.upload(off, stream_);
+    dinv_.ensure(static_cast<size_t>(std::max<int64_t>(off[numSN_], 1)));
+  }
   hipCheck(hipStreamSynchronize(stream_), "uploadLayout");
+}
+
+void Plan::launchInverse() {
+  launchDiagInverse(dinvList_.p, invCount_, dinvStart_.p, invBlocks_, devFronts(), dinv_.p, dinvOff_.p, stream_);
 }
 
 void Plan::buildSchedule() {
@@ -604,6 +626,7 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
   }
   launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
   launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
+  launchInverse();
 }
 
 void Plan::collectStats(FactorInfo& info, bool& anyDelay) {
@@ -794,6 +817,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
   uploadLayout();
   fd = devFronts();
   launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
+  launchInverse();
   singular_ = false;
   singularColsOrig_.clear();
   std::vector<int> sing;
@@ -1033,14 +1057,16 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, dBp_.p, dG_.p,
                     dDl_.p, dgOff_.p, stream_);
     launchFwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_,
-                   nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+                   nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
+                   dinvOff_.p, stream_);
     launchFwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, stream_);
     profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
     profBegin(kProfBwd);
     launchBwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_,
-                   nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, stream_);
+                   nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
+                   dinvOff_.p, stream_);
     launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dZ_.p, dX_.p, stream_);
     profEnd(kProfBwd);
   }

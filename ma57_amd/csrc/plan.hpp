@@ -261,6 +261,14 @@ class Plan {
   DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;
   int gOffNrhs_ = -1;
+  // inverses of the sweeps' diagonal blocks: fronts above kTinyFront rows,
+  // their flat workgroup starts (one per kSolveBlock eligible rows) and
+  // per-front offsets into dinv_
+  DBuf<int> dinvList_, dinvStart_;
+  DBuf<int64_t> dinvOff_;
+  DBuf<double> dinv_;
+  int invCount_ = 0, invBlocks_ = 0;
+  void launchInverse();
   // per level: flat workgroup starts of the gather, forward and backward
   // sweep launches (offsets into the device arrays, and block totals)
   DBuf<int> dgatherStart_, dfwdStart_, dbwdStart_;

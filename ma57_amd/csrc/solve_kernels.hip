@@ -42,7 +42,7 @@ __global__ void k_permute_out(int n, int nrhs, const int* __restrict__ perm, con
 
 namespace {
 
-constexpr int SB = 32;  // own rows per sweep block; also the column tile of the sweeps
+constexpr int SB = kSolveBlock;  // own rows per sweep block; also the column tile of the sweeps
 constexpr int DT = 64;  // ancestor-row tile of the backward sweep
 constexpr int YW = 65;  // row stride of the right-hand-side tiles in LDS (64 columns + 1)
 // Polls of a dependency flag before a sweep block gives up (about a second).
@@ -185,6 +185,54 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
           Bp[fd.rowIdx[io + p] + static_cast<int64_t>(col) * n] - Gs[fd.perm[io + p] + static_cast<int64_t>(col) * m];
 }
 
+// Inverses of the unit lower diagonal blocks of L that the sweeps solve
+// with: block b of front s (own rows [32b, 32b + 32) of [0, nF)) gets
+// M = L_bb^{-1}, row-major 32 x 32 at inv + invOff[s] + 1024 b (FP64; rows
+// and columns past nF hold the identity).  One workgroup per block; thread c
+// forms column c by forward substitution.  Applying M is the substitution
+// reassociated (the inverted-diagonal-block form of the triangular solve).
+template <typename T>
+__global__ __launch_bounds__(64) void k_diag_inverse(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                     double* __restrict__ inv, const int64_t* __restrict__ invOff) {
+  __shared__ double Ls[SB][SB + 1];
+  __shared__ double Ms[SB][SB + 1];
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  if (b * SB >= nf) return;
+  const int r0 = b * SB;
+  const int nr = min(SB, nf - r0);
+  const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < SB * SB; e += 64) {
+    const int r = e % SB, c = e / SB;
+    Ls[r][c] = (r > c && r < nr) ? lval(F, m, pk, r0 + r, r0 + c) : 0.0;
+  }
+  __syncthreads();
+  if (tid < SB) {
+    const int c = tid;
+    for (int r = 0; r < SB; ++r) Ms[r][c] = (r == c) ? 1.0 : 0.0;
+    for (int r = c + 1; r < SB; ++r) {
+      double a0 = 0.0, a1 = 0.0;
+      int t = c;
+      for (; t + 2 <= r; t += 2) {
+        a0 = fma(Ls[r][t], Ms[t][c], a0);
+        a1 = fma(Ls[r][t + 1], Ms[t + 1][c], a1);
+      }
+      if (t < r) a0 = fma(Ls[r][t], Ms[t][c], a0);
+      Ms[r][c] = -(a0 + a1);
+    }
+  }
+  __syncthreads();
+  double* out = inv + invOff[s] + static_cast<int64_t>(b) * SB * SB;
+  for (int e = tid; e < SB * SB; e += 64) out[e] = Ms[e / SB][e % SB];
+}
+
 // Forward sweep of one tree level (solve.hpp:349-361 and :434-455).  The
 // workgroups of a front take, in order, the own-row blocks b = 0 .. nOwn-1
 // (SB rows each) and then the blocks of the rows [nF, m).  Own block b
@@ -200,13 +248,15 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
                                                     double* __restrict__ Z, double* __restrict__ Dl,
                                                     const int64_t* __restrict__ gOff, unsigned* flags,
                                                     const int64_t* __restrict__ flagOff, const unsigned* genp,
-                                                    int* err) {
+                                                    int* err, const double* __restrict__ inv,
+                                                    const int64_t* __restrict__ invOff) {
   using M = SweepMfma<T>;
   using E = typename M::E;
+  using MD = SweepMfma<double>;
   constexpr int NT = 256;
   __shared__ E Lt[SB][SB + 1];          // [row][col] operand tile of L
   __shared__ double ybuf[SB * YW];      // y_j operand tile (E, [q][rhs]); then this block's y (double)
-  __shared__ double Ld[SB][SB + 1];     // own block: its unit lower diagonal block
+  __shared__ double Ld[SB][SB + 1];     // own block: the inverse of its unit lower diagonal block
   __shared__ double dA[SB + 1], dB[SB + 1];  // own block: d_diag, d_off of positions r0 - 1 .. r0 + SB - 1
   __shared__ int pkA[SB + 1], rixA[SB + 1];  // and their pivot kinds and final-order rows
   E* yt = reinterpret_cast<E*>(ybuf);
@@ -239,10 +289,8 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   // What the own block's tail needs does not depend on other blocks: stage
   // it before the first wait (visible after the next barrier).
   if (own) {
-    for (int e = tid; e < SB * SB; e += NT) {
-      const int r = e % SB, j = e / SB;
-      Ld[r][j] = (r > j && r < nr) ? lval(F, m, pk, r0 + r, r0 + j) : 0.0;
-    }
+    const double* Mi = inv + invOff[s] + static_cast<int64_t>(b) * SB * SB;
+    for (int e = tid; e < SB * SB; e += NT) Ld[e / SB][e % SB] = Mi[e];
     for (int r = tid; r <= SB; r += NT) {
       const int p = r0 - 1 + r;
       const bool ok = p >= 0 && p < r1;
@@ -310,34 +358,35 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
         }
       continue;
     }
-    // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j)
+    // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j), the inverse applied
+    // as an FP64 tile product (rows past nr are zero so they add nothing)
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (r < nr && c < gw) ys[r][c] = Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[t][rr];
+        if (c < gw) ys[r][c] = (r < nr) ? Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[t][rr] : 0.0;
       }
     __syncthreads();
     {
-      // lane row of each half-wave owns one row; the half-waves of the
-      // workgroup take the right-hand sides in turn
-      const int row = lane & 31, half = lane >> 5;
-      double lrow[SB];
+      typename MD::v4 yv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lrow[j] = Ld[row][j];
-      for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
-        double y = (row < nr) ? ys[row][c] : 0.0;
-#pragma unroll
-        for (int j = 0; j < SB - 1; ++j) {
-          const double yj = __shfl(y, j + 32 * half, 64);
-          if (row > j) y = fma(-lrow[j], yj, y);
-        }
-        if (row < nr) {
-          ys[row][c] = y;
-          storeShared(Ds + (r0 + row) + static_cast<int64_t>(rg + c) * m, y);
-        }
+      for (int kk = 0; kk < SB; kk += 4) {
+        const double av = Ld[tr + lr][kk + lk];
+        yv[0] = MD::mma(av, (tc + lr < gw) ? ys[kk + lk][tc + lr] : 0.0, yv[0]);
+        yv[1] = MD::mma(av, (tc + 16 + lr < gw) ? ys[kk + lk][tc + 16 + lr] : 0.0, yv[1]);
       }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int r = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
+          if (r < nr && c < gw) {
+            ys[r][c] = yv[t][rr];
+            storeShared(Ds + (r0 + r) + static_cast<int64_t>(rg + c) * m, yv[t][rr]);
+          }
+        }
     }
     __syncthreads();
     // the later blocks need y only: release it before forming z
@@ -381,13 +430,15 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
                                                     double* __restrict__ X, double* __restrict__ Dl,
                                                     const int64_t* __restrict__ gOff, unsigned* flags,
                                                     const int64_t* __restrict__ flagOff, const unsigned* genp,
-                                                    int* err) {
+                                                    int* err, const double* __restrict__ inv,
+                                                    const int64_t* __restrict__ invOff) {
   using M = SweepMfma<T>;
   using E = typename M::E;
+  using MD = SweepMfma<double>;
   constexpr int NT = 256;
   __shared__ E Lt[DT][SB + 1];          // [row i][own column q] operand tile of L
   __shared__ double xbuf[DT * YW];      // x operand tile (E, [row i][rhs]); then this block's x (double)
-  __shared__ double Ld[SB][SB + 1];     // the block's unit lower diagonal block, [i][j] = L(r0 + i, r0 + j)
+  __shared__ double Ld[SB][SB + 1];     // the inverse of the block's unit lower diagonal block
   __shared__ int rixA[SB];              // final-order rows of the block
   E* xt = reinterpret_cast<E*>(xbuf);
   double (*xs)[YW] = reinterpret_cast<double (*)[YW]>(xbuf);
@@ -413,9 +464,9 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
   // staged before any wait (visible after the next barrier)
-  for (int e = tid; e < SB * SB; e += NT) {
-    const int i = e % SB, j = e / SB;
-    Ld[i][j] = (i > j && i < nr) ? lval(F, m, pk, r0 + i, r0 + j) : 0.0;
+  {
+    const double* Mi = inv + invOff[s] + static_cast<int64_t>(w) * SB * SB;
+    for (int e = tid; e < SB * SB; e += NT) Ld[e / SB][e % SB] = Mi[e];
   }
   for (int q = tid; q < SB; q += NT) rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
   // L tiles loaded one tile ahead: the ancestors' rows (DT x SB, eight
@@ -500,28 +551,29 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (qq < nr && c < gw) xs[qq][c] = Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr];
+        if (c < gw) xs[qq][c] = (qq < nr) ? Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr] : 0.0;
       }
     __syncthreads();
     {
-      // lane row of each half-wave owns one row (and column `row` of the
-      // block); the half-waves of the workgroup take the right-hand sides
-      const int row = lane & 31, half = lane >> 5;
-      double lcol[SB];
+      // x_w = L_ww^{-T} (z_w - ...): the transposed inverse as an FP64 tile product
+      typename MD::v4 xv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
-      for (int j = 0; j < SB; ++j) lcol[j] = (j > row) ? Ld[j][row] : 0.0;
-      for (int c = 2 * wv + half; c < gw; c += 2 * (NT / 64)) {
-        double x = (row < nr) ? xs[row][c] : 0.0;
-#pragma unroll
-        for (int j = SB - 1; j > 0; --j) {
-          const double xj = __shfl(x, j + 32 * half, 64);
-          if (row < j) x = fma(-lcol[j], xj, x);
-        }
-        if (row < nr) {
-          xs[row][c] = x;
-          storeShared(Ds + (r0 + row) + static_cast<int64_t>(rg + c) * m, x);
-        }
+      for (int kk = 0; kk < SB; kk += 4) {
+        const double av = Ld[kk + lk][tr + lr];
+        xv[0] = MD::mma(av, (tc + lr < gw) ? xs[kk + lk][tc + lr] : 0.0, xv[0]);
+        xv[1] = MD::mma(av, (tc + 16 + lr < gw) ? xs[kk + lk][tc + 16 + lr] : 0.0, xv[1]);
       }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int qq = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
+          if (qq < nr && c < gw) {
+            xs[qq][c] = xv[t][rr];
+            storeShared(Ds + (r0 + qq) + static_cast<int64_t>(rg + c) * m, xv[t][rr]);
+          }
+        }
     }
     __syncthreads();
     // the earlier blocks need x only through Ds: release it, then write X
@@ -810,28 +862,39 @@ void launchFwdGather(const int* list, int count, const int* blockStart, int bloc
   k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
+void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
+                       double* inv, const int64_t* invOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  if (fd.fp32)
+    k_diag_inverse<float><<<blocks, 64, 0, st>>>(list, count, blockStart, fd, inv, invOff);
+  else
+    k_diag_inverse<double><<<blocks, 64, 0, st>>>(list, count, blockStart, fd, inv, invOff);
+  hipCheck(hipGetLastError(), "launchDiagInverse");
+}
 void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
+                    const int64_t* invOff, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
   if (fd.fp32)
     k_fwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
-                                               gen, err);
+                                               gen, err, inv, invOff);
   else
     k_fwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
-                                                gen, err);
+                                                gen, err, inv, invOff);
   hipCheck(hipGetLastError(), "launchFwdSweep");
 }
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, hipStream_t st) {
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
+                    const int64_t* invOff, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
   if (fd.fp32)
     k_bwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                               gen, err);
+                                               gen, err, inv, invOff);
   else
     k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                                gen, err);
+                                                gen, err, inv, invOff);
   hipCheck(hipGetLastError(), "launchBwdSweep");
 }
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
