@@ -22,7 +22,10 @@ is better); `fp64_s` is the plain FP64 step.  `roofline` covers the dominant
 kernel of the FP32 variant (the FP32 MFMA trailing update) from HIP events
 recorded by the library around each of its launches on the plan's stream
 during the timed steps (`roofline_fp64`: the FP64 update of the FP64
-variant); `cpu_baseline` times the CPU oracle (a restatement of the
+variant).  Large fronts with big contribution blocks update those blocks
+once, after their last panel, as library SYRKs: that part is reported
+beside the roofline as `cb_update` (and `schur_total_tflops` covers both
+kinds of update); `cpu_baseline` times the CPU oracle (a restatement of the
 reference's OpenMP multifrontal path) on a bounded sample of the same
 workload.
 """
@@ -218,10 +221,17 @@ def main() -> int:
             "static_retries": info.retries_used,
             "device_factor_ms": fms.value,
             "kernel_ms_per_step": {k: getattr(prof, k + "_ms") / args.steps for k in
-                                   ("syrk", "panel", "panel_rows", "small", "assemble", "fwd", "bwd")},
+                                   ("syrk", "cb", "panel", "panel_rows", "small", "assemble", "fwd", "bwd")},
             "syrk": {"achieved": achieved, "launches_per_step": prof.syrk_launches / args.steps,
                      "avg_launch_ms": syrk_avg_ms, "flops_per_launch": flops_per_launch,
                      "schur_gflop_per_step": prof.schur_flops / args.steps / 1e9},
+            # deferred contribution-block updates: operand kernel + library SYRK per chunk
+            "cb_update": {"achieved": (prof.cb_flops / (prof.cb_ms * 1e-3) / 1e12) if prof.cb_ms > 0 else 0.0,
+                          "launches_per_step": prof.cb_launches / args.steps,
+                          "gflop_per_step": prof.cb_flops / args.steps / 1e9},
+            # every Schur-complement flop over the time of both kinds of update
+            "schur_total_tflops": ((prof.schur_flops + prof.cb_flops) / ((prof.syrk_ms + prof.cb_ms) * 1e-3) / 1e12
+                                   if prof.syrk_ms + prof.cb_ms > 0 else 0.0),
         }
 
     # the plain FP64 variant solves without refinement, except where the matrix needs it to reach the

@@ -197,6 +197,8 @@ typedef struct {
   double static_abs_floor;
   const int* expected_sign; /* static mode, size n or NULL */
   int exact_panel;          /* 1: no speculative panel step (pivot-by-pivot kernel only) */
+  int deferred_cb;          /* 1: rows and columns [n_eligible, n) take one rank-n_eligible update after the
+                               last panel instead of one per panel (the plan default for large blocks) */
 } symla_dense_opts;
 int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_dense_opts* opts, double* d_diag,
                        double* d_off, int* perm, int* pivot_start, int* pivot_kind, int* n_pivot_blocks,
@@ -221,6 +223,9 @@ typedef struct {
   int assemble_tile;
   int speculative_panel;    /* 1 (default): speculative blocked step on large fronts; 0: exact steps only */
   int speculative_min_rows; /* fronts with more eligible rows than this take it (default 2048) */
+  int deferred_cb_min_rows; /* large fronts with at least 256 eligible columns and a contribution block of at
+                               least this many rows update that block once, after their last panel, as
+                               library SYRKs (default 2048; -1: every block panel by panel) */
 } symla_tuning;
 typedef struct {
   symla_tuning tuning;
@@ -240,8 +245,9 @@ int symla_plan_timing(const symla_plan* p, double* factor_ms, double* solve_ms);
  * When enabled, every launch of the listed kernel classes is bracketed by
  * HIP events on the plan's stream; symla_plan_get_profile returns the sums
  * since the previous call and resets them.  schur_flops is the algorithmic
- * flop count p*r*(r+1) of the MFMA trailing updates (lower triangle, 2 flops
- * per multiply-add). */
+ * flop count p*r*(r+1) of the panels' MFMA trailing updates (lower triangle,
+ * 2 flops per multiply-add), cb_flops that of the deferred contribution-block
+ * updates. */
 typedef struct {
   double syrk_ms;
   long long syrk_launches;
@@ -258,6 +264,9 @@ typedef struct {
   long long fwd_launches;
   double bwd_ms;
   long long bwd_launches;
+  double cb_ms;        /* deferred contribution-block updates (operand kernel + library SYRK per chunk) */
+  long long cb_launches;
+  double cb_flops;     /* their algorithmic flops, 2 k r (r + 1) / 2 per chunk of k pivots and r rows */
 } symla_profile;
 int symla_plan_set_profiling(symla_plan* p, int enable);
 int symla_plan_get_profile(symla_plan* p, symla_profile* out);

@@ -59,12 +59,12 @@ class RefineInfo(C.Structure):
 class DenseOpts(C.Structure):
     _fields_ = [("zero_tolerance", C.c_double), ("relative_pivot_floor", C.c_double), ("static_mode", C.c_int),
                 ("static_delta", C.c_double), ("static_rel_floor", C.c_double), ("static_abs_floor", C.c_double),
-                ("expected_sign", i32p), ("exact_panel", C.c_int)]
+                ("expected_sign", i32p), ("exact_panel", C.c_int), ("deferred_cb", C.c_int)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("small_front_max", C.c_int), ("assemble_tile", C.c_int), ("speculative_panel", C.c_int),
-                ("speculative_min_rows", C.c_int)]
+                ("speculative_min_rows", C.c_int), ("deferred_cb_min_rows", C.c_int)]
 
 
 class TuningTrial(C.Structure):
@@ -76,7 +76,8 @@ class Profile(C.Structure):
                 ("panel_ms", C.c_double), ("panel_launches", C.c_longlong), ("panel_rows_ms", C.c_double),
                 ("panel_rows_launches", C.c_longlong), ("small_ms", C.c_double), ("small_launches", C.c_longlong),
                 ("assemble_ms", C.c_double), ("assemble_launches", C.c_longlong), ("fwd_ms", C.c_double),
-                ("fwd_launches", C.c_longlong), ("bwd_ms", C.c_double), ("bwd_launches", C.c_longlong)]
+                ("fwd_launches", C.c_longlong), ("bwd_ms", C.c_double), ("bwd_launches", C.c_longlong),
+                ("cb_ms", C.c_double), ("cb_launches", C.c_longlong), ("cb_flops", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/symla_hip.h

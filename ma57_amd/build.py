@@ -21,7 +21,9 @@ PRODUCT = ROOT / "ma57_amd" / "libsymla_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE = ORACLE_DIR / "build" / "libsymla_oracle.so"
 
-HIP_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "plan.cpp", "capi.cpp", "symbolic.cpp"]
+HIP_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "plan.cpp", "capi.cpp", "symbolic.cpp", "cb_update.cpp"]
+# rocBLAS runs the deferred contribution-block SYRKs (plain library products).
+LINK_LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
 # Packed FP32 ops are disabled for every HIP translation unit.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -62,7 +64,7 @@ def build_product(verbose: bool = False) -> Path:
         _run(cmd)
         objs.append(str(obj))
     tmp = PRODUCT.with_suffix(".so.tmp")
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, *LINK_LIBS])
     os.replace(tmp, PRODUCT)
     return PRODUCT
 

@@ -237,6 +237,7 @@ int symla_plan_get_tuning(const symla_plan* p, symla_tuning* out) {
   out->assemble_tile = p->plan->tuning().asmTile;
   out->speculative_panel = p->plan->tuning().specPanel;
   out->speculative_min_rows = p->plan->tuning().specMinRows;
+  out->deferred_cb_min_rows = p->plan->tuning().cbMinRows;
   return SYMLA_OK;
 }
 
@@ -246,7 +247,8 @@ int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t) {
     if (t->speculative_panel != 0 && t->speculative_panel != 1)
       throw std::invalid_argument("symla_plan_set_tuning: speculative_panel must be 0 or 1");
     p->plan->setTuning(
-        Plan::Tuning{t->small_front_max, t->assemble_tile, t->speculative_panel, t->speculative_min_rows});
+        Plan::Tuning{t->small_front_max, t->assemble_tile, t->speculative_panel, t->speculative_min_rows,
+                     t->deferred_cb_min_rows});
   });
 }
 
@@ -260,6 +262,7 @@ int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int pre
       chosen->assemble_tile = p->plan->tuning().asmTile;
       chosen->speculative_panel = p->plan->tuning().specPanel;
       chosen->speculative_min_rows = p->plan->tuning().specMinRows;
+      chosen->deferred_cb_min_rows = p->plan->tuning().cbMinRows;
     }
     if (ntrials) *ntrials = static_cast<int>(tr.size());
     for (int i = 0; trials && i < cap && i < static_cast<int>(tr.size()); ++i) {
@@ -267,6 +270,7 @@ int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int pre
       trials[i].tuning.assemble_tile = tr[i].t.asmTile;
       trials[i].tuning.speculative_panel = tr[i].t.specPanel;
       trials[i].tuning.speculative_min_rows = tr[i].t.specMinRows;
+      trials[i].tuning.deferred_cb_min_rows = tr[i].t.cbMinRows;
       trials[i].ms = tr[i].ms;
     }
   });
@@ -355,6 +359,9 @@ int symla_plan_get_profile(symla_plan* p, symla_profile* out) {
     out->fwd_launches = r.launches[Plan::kProfFwd];
     out->bwd_ms = r.ms[Plan::kProfBwd];
     out->bwd_launches = r.launches[Plan::kProfBwd];
+    out->cb_ms = r.ms[Plan::kProfCb];
+    out->cb_launches = r.launches[Plan::kProfCb];
+    out->cb_flops = r.cbFlops;
   });
 }
 
@@ -393,7 +400,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
                               hipMemcpyHostToDevice, st),
              "A upload");
     DBuf<int> dm, dnE, dnc, dfirst, dfi, drel, dcp, dcl, daRow, dperm, drow, dpk, dnF, dinert, dkcur, dpc, dstop,
-        dnpert, dlist, dsign, dcb;
+        dnpert, dlist, dsign, dcb, dcut;
     DBuf<int64_t> dio, dacp, daSrc;
     DBuf<double> ddd, ddo, dpert, dws;
     DBuf<void*> dF;
@@ -402,6 +409,9 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     upI(dnE, {nE});
     upI(dnc, {nE});
     upI(dfirst, {0});
+    const bool deferCb = opts && opts->deferred_cb && nE >= 1 && nE < n && n > kSmallFrontMax;
+    const std::vector<int> cutv{deferCb ? nE : n};
+    upI(dcut, cutv);
     std::vector<int> ident(n);
     for (int i = 0; i < n; ++i) ident[i] = i;
     upI(dfi, ident);
@@ -451,6 +461,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     f.acolPtr = dacp.p;
     f.aRow = daRow.p;
     f.aSrc = daSrc.p;
+    f.schurCut = dcut.p;
     f.perm = dperm.p;
     f.rowIdx = drow.p;
     f.dDiag = ddd.p;
@@ -484,13 +495,14 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     } else {
       std::vector<int> mv{n}, nev{nE};
       LargeSchedHost lh;
-      LargeSched ls = buildLargeSched({0}, mv, nev, lh, (opts && opts->exact_panel) ? -1 : 0);
-      DBuf<int> dl, dr, dt, dmw, dsp, derr;
+      LargeSched ls = buildLargeSched({0}, mv, nev, cutv, lh, (opts && opts->exact_panel) ? -1 : 0);
+      DBuf<int> dl, dr, dt, dtp, dmw, dsp, derr;
       DBuf<int64_t> dw;
       DBuf<MwSync> dsync;
       dl.upload(lh.lists, st);
       dr.upload(lh.rowStart, st);
       dt.upload(lh.tileStart, st);
+      dtp.upload(lh.tileTp, st);
       dmw.upload(lh.mwStart, st);
       dsp.upload(lh.specStart, st);
       dw.upload(std::vector<int64_t>{0}, st);
@@ -502,11 +514,28 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
       dv.lists = dl.p;
       dv.rowStart = dr.p;
       dv.tileStart = dt.p;
+      dv.tileTp = dtp.p;
       dv.mwStart = dmw.p;
       dv.specStart = dsp.p;
       dv.sync = dsync.p;
       dv.err = derr.p;
-      runLargeSched(ls, dv, f, prm, dws.p, dw.p, st, nullptr);
+      // library handle and workspace of the deferred contribution-block update
+      struct BlasGuard {
+        void* h = nullptr;
+        ~BlasGuard() { blasDestroy(h); }
+      } blas;
+      DBuf<double> dW;
+      const std::vector<void*> Fv{dA.p};
+      CbUpdate cb;
+      if (!ls.cb.empty()) {
+        blas.h = blasCreate(st);
+        dW.alloc(static_cast<size_t>(n - nE) * kCbChunk);
+        cb.blas = blas.h;
+        cb.w = dW.p;
+        cb.wElems = dW.n;
+        cb.F = &Fv;
+      }
+      runLargeSched(ls, dv, f, prm, dws.p, dw.p, st, &cb, nullptr);
       int herr = 0;
       hipCheck(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, st), "error word");
       hipCheck(hipStreamSynchronize(st), "dense large front");

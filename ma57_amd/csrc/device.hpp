@@ -33,6 +33,8 @@ struct DevFronts {
   const int64_t* acolPtr;  // A entries per final column (CSR over columns)
   const int* aRow;       // local pre-permutation row in the owning front
   const int64_t* aSrc;   // index into the caller's value array
+  const int* schurCut;   // the panels' trailing updates cover columns [0, schurCut); when it is nE < m the
+                         // contribution block [nE, m)^2 gets one rank-nE update after the last panel
   // outputs / state
   int* perm;             // local permutation: position -> pre-permutation position
   int* rowIdx;           // frontIdx[perm[i]]

@@ -2067,7 +2067,8 @@ struct SchurMfma<float> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur(const int* list, int count, const int* tileStart, DevFronts fd) {
+__global__ __launch_bounds__(256) void k_schur(const int* list, int count, const int* tileStart, const int* tileTp,
+                                               DevFronts fd) {
   using M = SchurMfma<T>;
   constexpr int TS = 64;
   constexpr int KP = kPanel + 4;  // padded K (a 2x2 at the panel end makes p = 65)
@@ -2085,11 +2086,12 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
   // Geometry of tile t.  A front's tiles are numbered column block by
   // column block (tj, then ti = tj .. Tp-1 down the column), so the
   // workgroups running at the same time read and write consecutive pieces of
-  // the same columns.  Tp is the planned tiles per side (the launch covers
-  // Tp (Tp + 1) / 2 tiles per front); the front's metadata is read again only
-  // when the walk enters another front.
+  // the same columns.  Tp is the planned tiles per side; the launch covers
+  // the column blocks that start before the front's cut (all Tp of them, or
+  // fewer when its contribution block is updated separately).  The front's
+  // metadata is read again only when the walk enters another front.
   struct Tile {
-    int f, s, p, m, k0, k1, Tp, lo, hi, ti, tj;
+    int f, s, p, m, k0, k1, Tp, cut, lo, hi, ti, tj;
     T* F;
   };
   auto tileOf = [&](int t, const Tile* prev) -> Tile {
@@ -2106,11 +2108,8 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
       x.k1 = fd.kcur[x.s];
       x.k0 = x.k1 - x.p;
       x.F = static_cast<T*>(fd.F[x.s]);
-      const int nt = x.hi - x.lo;
-      int Tp = static_cast<int>((sqrt(8.0 * nt + 1.0) - 1.0) * 0.5);
-      while ((Tp + 1) * (Tp + 2) / 2 <= nt) ++Tp;
-      while (Tp * (Tp + 1) / 2 > nt) --Tp;
-      x.Tp = Tp;
+      x.Tp = tileTp[x.f];
+      x.cut = fd.schurCut[x.s];
     }
     // column block tj starts at S(tj) = tj Tp - tj (tj - 1) / 2
     const int lt = t - x.lo;
@@ -2123,8 +2122,9 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
     x.ti = tj + (lt - S(tj));
     return x;
   };
-  // a tile does work when its front took pivots and the tile lies in the trailing block
-  auto live = [&](const Tile& x) { return x.p > 0 && x.k1 + x.ti * TS < x.m; };
+  // a tile does work when its front took pivots and the tile lies in the
+  // trailing block, left of the cut
+  auto live = [&](const Tile& x) { return x.p > 0 && x.k1 + x.ti * TS < x.m && x.k1 + x.tj * TS < x.cut; };
   T la[KU], lb[KU], cv[2][2][4];
   auto loadTile = [&](const Tile& x) {
     const int i0 = x.k1 + x.ti * TS, j0 = x.k1 + x.tj * TS;
@@ -2143,7 +2143,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
         for (int rr = 0; rr < 4; ++rr) {
           const int row = i0 + wr + bi * 16 + lr;
           const int col = j0 + wc + bj * 16 + M::col(lane, rr);
-          cv[bi][bj][rr] = (row < x.m && col < x.m && (x.ti != x.tj || row >= col))
+          cv[bi][bj][rr] = (row < x.m && col < x.cut && (x.ti != x.tj || row >= col))
                                ? x.F[row + static_cast<size_t>(col) * x.m]
                                : T(0);
         }
@@ -2165,13 +2165,12 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
           pkd[q] = fd.pkind[io + cur.k0 + q];
         }
         pivFront = cur.s;
-        if (cur.ti == 0 && cur.tj == 0 && tid == 0 && fd.schurFlops) {
-          const unsigned long long N = static_cast<unsigned long long>(cur.m - cur.k1);
-          atomicAdd(fd.schurFlops, static_cast<unsigned long long>(cur.p) * N * (N + 1));
-        }
-      } else if (cur.ti == 0 && cur.tj == 0 && tid == 0 && fd.schurFlops) {
+      }
+      if (cur.ti == 0 && cur.tj == 0 && tid == 0 && fd.schurFlops) {
+        // p r (r + 1) over the trailing lower triangle, less the deferred contribution block
         const unsigned long long N = static_cast<unsigned long long>(cur.m - cur.k1);
-        atomicAdd(fd.schurFlops, static_cast<unsigned long long>(cur.p) * N * (N + 1));
+        const unsigned long long C = static_cast<unsigned long long>(cur.m - max(cur.cut, cur.k1));
+        atomicAdd(fd.schurFlops, static_cast<unsigned long long>(cur.p) * (N * (N + 1) - C * (C + 1)));
       }
       // stage the operands: raw L rows, then the L D scaling (a 2x2 block
       // needs its neighbouring column, read back from LDS)
@@ -2243,7 +2242,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
           for (int rr = 0; rr < 4; ++rr) {
             const int row = i0 + wr + bi * 16 + lr;
             const int col = j0 + wc + bj * 16 + M::col(lane, rr);
-            if (row < cur.m && col < cur.m && (cur.ti != cur.tj || row >= col)) {
+            if (row < cur.m && col < cur.cut && (cur.ti != cur.tj || row >= col)) {
               const T v = static_cast<T>(cn[bi][bj][rr] - acc[bi][bj][rr]);
               cur.F[row + static_cast<size_t>(col) * cur.m] = v;
               mx = fmax(mx, fabs(static_cast<double>(v)));
@@ -2255,6 +2254,52 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
     }
     t = tn;
     if (t < total) cur = nxt;
+  }
+}
+
+// Operand of the deferred contribution-block update of front s (a front
+// whose panels' trailing updates stopped at column nE): for columns
+// q0 .. q0 + kc - 1, W = L(nE:m, q) D, the 2x2 blocks coupling neighbouring
+// columns as in the tile staging above; zero in the columns at or past the
+// pivots the front took, so the rank-kc product CB -= W L(nE:m, q0:q0+kc)^T
+// that follows adds nothing for them.  W is column-major with ld = m - nE.
+constexpr int kFormCols = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_form_w(DevFronts fd, int s, int q0, int kc, T* __restrict__ W) {
+  __shared__ double pdd[kFormCols + 2], pdo[kFormCols + 2];
+  __shared__ int pkd[kFormCols + 2];
+  const int m = fd.m[s], nE = fd.nE[s], nF = fd.nF[s];
+  const int N = m - nE;
+  const int c0 = q0 + blockIdx.y * kFormCols;
+  const int c1 = min(c0 + kFormCols, q0 + kc);
+  const int64_t io = fd.idxOff[s];
+  // slot t + 1 holds column c0 + t; slot 0 column c0 - 1
+  for (int t = threadIdx.x; t < kFormCols + 2; t += blockDim.x) {
+    const int q = c0 - 1 + t;
+    const bool ok = q >= 0 && q < nF;
+    pdd[t] = ok ? fd.dDiag[io + q] : 0.0;
+    pdo[t] = ok ? fd.dOff[io + q] : 0.0;
+    pkd[t] = ok ? fd.pkind[io + q] : -1;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const T* Lr = F + nE + i;  // row nE + i of the front, column q at Lr[q * m]
+  for (int q = c0; q < c1; ++q) {
+    const int t = q - c0 + 1;
+    const int kind = pkd[t];
+    double a = 0.0;
+    if (kind == 1) {
+      a = static_cast<double>(Lr[static_cast<size_t>(q) * m]) * pdd[t];
+    } else if (kind == 2) {
+      a = static_cast<double>(Lr[static_cast<size_t>(q) * m]) * pdd[t] +
+          static_cast<double>(Lr[static_cast<size_t>(q + 1) * m]) * pdo[t];
+    } else if (kind == 0) {
+      a = static_cast<double>(Lr[static_cast<size_t>(q - 1) * m]) * pdo[t - 1] +
+          static_cast<double>(Lr[static_cast<size_t>(q) * m]) * pdd[t];
+    }
+    W[i + static_cast<size_t>(q - q0) * N] = static_cast<T>(a);
   }
 }
 
@@ -2468,16 +2513,25 @@ void launchPanelRows(const int* list, int count, const int* blockStart, int tota
     k_panel_rows<double><<<totalBlocks, 256, 0, st>>>(list, count, blockStart, fd);
   hipCheck(hipGetLastError(), "launchPanelRows");
 }
-void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
-                hipStream_t st) {
+void launchSyrk(const int* list, int count, const int* tileStart, const int* tileTp, int totalTiles,
+                const DevFronts& fd, hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
   // persistent workgroups: two per CU (the operand tiles take 70 KB of LDS in FP64)
   const int blocks = std::min(totalTiles, 2 * 256);
   if (fd.fp32)
-    k_schur<float><<<blocks, 256, 0, st>>>(list, count, tileStart, fd);
+    k_schur<float><<<blocks, 256, 0, st>>>(list, count, tileStart, tileTp, fd);
   else
-    k_schur<double><<<blocks, 256, 0, st>>>(list, count, tileStart, fd);
+    k_schur<double><<<blocks, 256, 0, st>>>(list, count, tileStart, tileTp, fd);
   hipCheck(hipGetLastError(), "launchSyrk");
+}
+void launchFormW(const DevFronts& fd, int s, int rows, int q0, int kc, void* W, hipStream_t st) {
+  if (rows <= 0 || kc <= 0) return;
+  const dim3 grid((rows + 255) / 256, (kc + kFormCols - 1) / kFormCols);
+  if (fd.fp32)
+    k_form_w<float><<<grid, 256, 0, st>>>(fd, s, q0, kc, static_cast<float*>(W));
+  else
+    k_form_w<double><<<grid, 256, 0, st>>>(fd, s, q0, kc, static_cast<double*>(W));
+  hipCheck(hipGetLastError(), "launchFormW");
 }
 void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st) {
   if (count <= 0) return;

@@ -35,8 +35,12 @@ void launchSpecReset(const int* list, int count, const DevFronts& fd, double* ws
                      hipStream_t st);
 void launchPanelRows(const int* list, int count, const int* blockStart, int totalBlocks, const DevFronts& fd,
                      hipStream_t st);
-void launchSyrk(const int* list, int count, const int* tileStart, int totalTiles, const DevFronts& fd,
-                hipStream_t st);
+// tileTp: per listed front, the planned 64-row tiles per side of its trailing block
+void launchSyrk(const int* list, int count, const int* tileStart, const int* tileTp, int totalTiles,
+                const DevFronts& fd, hipStream_t st);
+// W = L(nE:m, q0:q0+kc) D of front s (rows = m - nE, column-major, ld = rows;
+// float for FP32 fronts), zero past the pivots taken
+void launchFormW(const DevFronts& fd, int s, int rows, int q0, int kc, void* W, hipStream_t st);
 void launchFrontFinish(const int* list, int count, const DevFronts& fd, hipStream_t st);
 void launchMakeRowIdx(const int* list, int count, const DevFronts& fd, hipStream_t st);
 void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* outD, hipStream_t st);

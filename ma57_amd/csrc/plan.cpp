@@ -169,6 +169,7 @@ Plan::~Plan() {
   for (void* p : dynAllocs_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  blasDestroy(blas_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -319,11 +320,14 @@ void Plan::uploadLayout() {
     ddDiag_.alloc(cap);
     ddOff_.alloc(cap);
   }
+  computeCut();
+  dschurCut_.upload(hcut_, stream_);
   // solve workspaces depend on the layout
   gOffNrhs_ = -1;
+  // the uploads below read these host vectors until the stream sync at the end
+  std::vector<int> lst, start(1, 0);
+  std::vector<int64_t> off(numSN_ + 1, 0);
   {
-    std::vector<int> lst, start(1, 0);
-    std::vector<int64_t> off(numSN_ + 1, 0);
     for (int s = 0; s < numSN_; ++s) {
       const int nb = hm_[s] > kTinyFront ? (hnE_[s] + kSolveBlock - 1) / kSolveBlock : 0;
       off[s + 1] = off[s] + static_cast<int64_t>(nb) * kSolveBlock * kSolveBlock;
@@ -340,6 +344,31 @@ void Plan::uploadLayout() {
     dinv_.ensure(static_cast<size_t>(std::max<int64_t>(off[numSN_], 1)));
   }
   hipCheck(hipStreamSynchronize(stream_), "uploadLayout");
+}
+
+void Plan::computeCut() {
+  hcut_.assign(numSN_, 0);
+  for (int s = 0; s < numSN_; ++s) {
+    const bool defer = tune_.cbMinRows >= 0 && hm_[s] > tune_.smallMax && hnE_[s] >= kCbMinCols &&
+                       hm_[s] - hnE_[s] >= std::max(tune_.cbMinRows, 1);
+    hcut_[s] = defer ? hnE_[s] : hm_[s];
+  }
+}
+
+CbUpdate Plan::cbUpdate(const std::vector<CbTask>& tasks) {
+  CbUpdate c;
+  c.F = &hF_;
+  c.fp32 = prec_ != 0;
+  if (!tasks.empty()) {
+    size_t need = 0;
+    for (const CbTask& t : tasks) need = std::max(need, static_cast<size_t>(t.m - t.nE) * kCbChunk);
+    if (!blas_) blas_ = blasCreate(stream_);
+    dcbW_.ensure(need);  // doubles: enough for either front type
+  }
+  c.blas = blas_;
+  c.w = dcbW_.p;
+  c.wElems = dcbW_.n * (c.fp32 ? 2 : 1);
+  return c;
 }
 
 void Plan::launchInverse() {
@@ -377,8 +406,11 @@ void Plan::buildSchedule() {
   LargeSchedHost lh;
   hwsOff_.assign(numSN_, 0);
   int64_t wsNeed = 2;
+  computeCut();
+  dschurCut_.upload(hcut_, stream_);
   for (auto& ls : sched_) {
-    ls.lsched = buildLargeSched(ls.large, hm_, hnE_, lh, tune_.specPanel ? tune_.specMinRows : -1);
+    ls.lsched = buildLargeSched(ls.large, hm_, hnE_, hcut_, lh, tune_.specPanel ? tune_.specMinRows : -1);
+    (void)cbUpdate(ls.lsched.cb);  // handle and workspace now: a recorded graph cannot allocate
     int64_t off = 0;
     for (int s : ls.large) {
       hwsOff_[s] = off;
@@ -391,6 +423,7 @@ void Plan::buildSchedule() {
   dlargeLists_.upload(lh.lists, stream_);
   dlargeRowStart_.upload(lh.rowStart, stream_);
   dlargeTileStart_.upload(lh.tileStart, stream_);
+  dlargeTileTp_.upload(lh.tileTp, stream_);
   dlargeMwStart_.upload(lh.mwStart, stream_);
   dlargeSpecStart_.upload(lh.specStart, stream_);
   dsmall_.upload(small, stream_);
@@ -455,6 +488,7 @@ DevFronts Plan::devFronts() {
   f.acolPtr = dacolPtr_.p;
   f.aRow = daRow_.p;
   f.aSrc = daSrc_.p;
+  f.schurCut = dschurCut_.p;
   f.perm = dperm_.p;
   f.rowIdx = drowIdx_.p;
   f.dDiag = ddDiag_.p;
@@ -489,7 +523,7 @@ FactorParams Plan::params(const FactorOptions& o, bool staticMode, double delta,
 }
 
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h, int specMinRows, bool allowMw) {
+                           const std::vector<int>& cut, LargeSchedHost& h, int specMinRows, bool allowMw) {
   LargeSched ls;
   ls.allOff = static_cast<int64_t>(h.lists.size());
   ls.allCount = static_cast<int>(fronts.size());
@@ -498,6 +532,7 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
     h.lists.push_back(s);
     ls.maxM = std::max(ls.maxM, m[s]);
     maxIters = std::max(maxIters, (nE[s] + kPanel - 1) / kPanel);
+    if (cut[s] < m[s]) ls.cb.push_back(CbTask{s, m[s], nE[s]});
   }
   for (int it = 0; it < maxIters; ++it) {
     LargeIter li;
@@ -518,8 +553,12 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
       // rows below this iteration's diagonal block (the block starts at or after it * kPanel)
       const int below = m[s] - it * kPanel - std::min(kPanel, nE[s] - it * kPanel);
       if (specMinRows >= 0 && nE[s] > specMinRows) sb += std::max(1, (below + kSpecRows - 1) / kSpecRows);
+      // trailing tiles: T per side from this iteration's panel start, column
+      // blocks up to the cut
       const int T = (m[s] - it * kPanel + 63) / 64;
-      tl += T * (T + 1) / 2;
+      const int Tc = std::min(T, (cut[s] - it * kPanel + 63) / 64);
+      h.tileTp.push_back(T);
+      tl += Tc * T - Tc * (Tc - 1) / 2;
       ++li.count;
       // eligible rows left at this iteration's first pivot (no delays: it * kPanel)
       const int rows = nE[s] - it * kPanel;
@@ -536,6 +575,7 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
     }
     h.rowStart.push_back(rb);
     h.tileStart.push_back(tl);
+    h.tileTp.push_back(0);  // keeps tileTp at the offsets of tileStart
     h.specStart.push_back(sb);
     li.specBlocks = sb;
     li.rowBlocks = rb;
@@ -560,7 +600,7 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
 }
 
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
-                   const int64_t* wsOff, hipStream_t st, Plan* prof) {
+                   const int64_t* wsOff, hipStream_t st, const CbUpdate* cb, Plan* prof) {
   if (ls.allCount == 0) return;
   launchFrontMaxAbs(dv.lists + ls.allOff, ls.allCount, ls.maxM, fd, st);
   if (prm.specPanel) launchSpecReset(dv.lists + ls.allOff, ls.allCount, fd, ws, wsOff, st);
@@ -581,15 +621,37 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
       if (prof) prof->profEnd(Plan::kProfPanelRows);
     }
     if (prof) prof->profBegin(Plan::kProfSyrk);
-    launchSyrk(lst, it.count, dv.tileStart + it.tileOff, it.tiles, fd, st);
+    launchSyrk(lst, it.count, dv.tileStart + it.tileOff, dv.tileTp + it.tileOff, it.tiles, fd, st);
     if (prof) prof->profEnd(Plan::kProfSyrk);
   }
   launchFrontFinish(dv.lists + ls.allOff, ls.allCount, fd, st);
+  if (ls.cb.empty()) return;
+  if (!cb || !cb->blas || !cb->F) throw std::logic_error("symla: deferred contribution blocks without a library handle");
+  const size_t esz = cb->fp32 ? sizeof(float) : sizeof(double);
+  for (const CbTask& t : ls.cb) {
+    const int N = t.m - t.nE;
+    if (static_cast<size_t>(N) * kCbChunk > cb->wElems) throw std::logic_error("symla: contribution-block workspace");
+    char* F = static_cast<char*>((*cb->F)[t.s]);
+    for (int q0 = 0; q0 < t.nE; q0 += kCbChunk) {
+      const int kc = std::min(kCbChunk, t.nE - q0);
+      // one profiled launch per chunk: the operand kernel and the library SYRK
+      if (prof) prof->profBegin(Plan::kProfCb);
+      launchFormW(fd, t.s, N, q0, kc, cb->w, st);
+      const void* B = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(q0) * t.m);
+      void* C = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(t.nE) * t.m);
+      blasSyrkxLower(cb->blas, cb->fp32, N, kc, cb->w, N, B, t.m, C, t.m);
+      if (prof) {
+        prof->profEnd(Plan::kProfCb);
+        prof->addCbFlops(static_cast<double>(kc) * N * (N + 1.0));
+      }
+    }
+  }
 }
 
-LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart,
-                        const int* specStart) {
+LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* tileTp,
+                        const int* mwStart, const int* specStart) {
   LargeDev d;
+  d.tileTp = tileTp;
   d.specStart = specStart;
   d.lists = lists;
   d.rowStart = rowStart;
@@ -619,10 +681,11 @@ void Plan::runStatic(const double* aValsDev, const FactorParams& prm) {
       launchFactorSmall(dsmall_.p + ls.smallOff, ls.smallCount, ls.smallMaxM, fd, prm, stream_);
       profEnd(kProfSmall);
     }
-    runLargeSched(ls.lsched, largeDev(dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, dlargeMwStart_.p,
-                                    dlargeSpecStart_.p), fd,
-                  prm, dws_.p, dwsOff_.p,
-                  stream_, profiling_ ? this : nullptr);
+    const CbUpdate cb = cbUpdate(ls.lsched.cb);
+    runLargeSched(ls.lsched,
+                  largeDev(dlargeLists_.p, dlargeRowStart_.p, dlargeTileStart_.p, dlargeTileTp_.p, dlargeMwStart_.p,
+                           dlargeSpecStart_.p),
+                  fd, prm, dws_.p, dwsOff_.p, stream_, &cb, profiling_ ? this : nullptr);
   }
   launchMakeRowIdx(dallFronts_.p, numSN_, fd, stream_);
   launchReduceStats(numSN_, fd, dstatI_.p, dstatD_.p, stream_);
@@ -781,7 +844,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
     launchFactorSmall(dlist.p + all.size(), static_cast<int>(small.size()), maxM, fd, prm, stream_);
     if (!large.empty()) {
       LargeSchedHost lh;
-      LargeSched lsd = buildLargeSched(large, hm_, hnE_, lh, tune_.specPanel ? tune_.specMinRows : -1);
+      LargeSched lsd = buildLargeSched(large, hm_, hnE_, hcut_, lh, tune_.specPanel ? tune_.specMinRows : -1);
       std::vector<int64_t> wsOffH(numSN_, 0);
       int64_t off = 0;
       for (int s : large) {
@@ -789,15 +852,18 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
         off += panelWorkspace(hnE_[s]);
       }
       dws_.ensure(static_cast<size_t>(std::max<int64_t>(off, 2)));
-      DBuf<int> dl, dr, dt, dmw, dsp;
+      DBuf<int> dl, dr, dt, dtp, dmw, dsp;
       DBuf<int64_t> dw;
       dl.upload(lh.lists, stream_);
       dr.upload(lh.rowStart, stream_);
       dt.upload(lh.tileStart, stream_);
+      dtp.upload(lh.tileTp, stream_);
       dmw.upload(lh.mwStart, stream_);
       dsp.upload(lh.specStart, stream_);
       dw.upload(wsOffH, stream_);
-      runLargeSched(lsd, largeDev(dl.p, dr.p, dt.p, dmw.p, dsp.p), fd, prm, dws_.p, dw.p, stream_, nullptr);
+      const CbUpdate cb = cbUpdate(lsd.cb);
+      runLargeSched(lsd, largeDev(dl.p, dr.p, dt.p, dtp.p, dmw.p, dsp.p), fd, prm, dws_.p, dw.p, stream_, &cb,
+                    nullptr);
       hipCheck(hipStreamSynchronize(stream_), "dynamic large fronts");
     }
     launchMakeRowIdx(dlist.p, static_cast<int>(all.size()), fd, stream_);
@@ -1377,7 +1443,7 @@ std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int p
   const int tiles[] = {32, 64, 128};
   for (int sm : smalls)
     for (int tw : tiles) {
-      setTuning(Tuning{sm, tw, keep.specPanel, keep.specMinRows});
+      setTuning(Tuning{sm, tw, keep.specPanel, keep.specMinRows, keep.cbMinRows});
       FactorInfo info;
       factorize(nullptr, dVals, o, info);  // warm-up: workspaces, kernels
       solve(dB.p, n_, nrhs, dXt.p, n_, true);

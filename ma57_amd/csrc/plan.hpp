@@ -42,10 +42,14 @@ struct DBuf {
   void ensure(size_t count) {
     if (count > n) alloc(count);
   }
+  // Returns once the copy is done: callers may free or reuse the host vector
+  // right away (set-up paths only; never under stream capture).
   void upload(const std::vector<T>& v, hipStream_t st) {
     ensure(v.size());
-    if (!v.empty())
+    if (!v.empty()) {
       hipCheck(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+      hipCheck(hipStreamSynchronize(st), "upload");
+    }
   }
 };
 
@@ -107,33 +111,64 @@ struct LargeIter {
   int64_t specOff = 0;  // speculative step: workgroup starts of the iteration's fronts
   int specBlocks = 0;
 };
+// Deferred contribution-block updates: a large front with at least
+// kCbMinCols eligible columns and a contribution block of at least the
+// tuned row count (Tuning::cbMinRows) stops its panels' trailing updates at
+// column nE; after its last panel the block [nE, m)^2 takes one rank-nE
+// update, kCbChunk pivot columns per library SYRK.
+constexpr int kCbMinCols = 256;
+constexpr int kCbMinRowsDefault = 2048;
+constexpr int kCbChunk = 2048;
+struct CbTask {
+  int s, m, nE;
+};
 struct LargeSched {
   int64_t allOff = 0;
   int allCount = 0;
   int maxM = 0;
   std::vector<LargeIter> iters;
+  std::vector<CbTask> cb;  // fronts whose contribution block is updated after their last panel
 };
 struct LargeSchedHost {
-  std::vector<int> lists, rowStart, tileStart, mwStart, specStart;
+  std::vector<int> lists, rowStart, tileStart, tileTp, mwStart, specStart;
 };
 // Device copies of a LargeSchedHost plus the cooperative-panel records.
 struct LargeDev {
   const int* lists = nullptr;
   const int* rowStart = nullptr;
   const int* tileStart = nullptr;
+  const int* tileTp = nullptr;
   const int* mwStart = nullptr;
   const int* specStart = nullptr;
   MwSync* sync = nullptr;
   int* err = nullptr;
 };
-// m and nE are indexed by front id; allowMw = false keeps every panel on one workgroup.
+// What the deferred contribution-block updates run with: the library
+// handle (on the plan's stream), a workspace of at least kCbChunk times the
+// largest contribution-block order (elements of the front type), and the
+// fronts' device pointers by front id.
+struct CbUpdate {
+  void* blas = nullptr;
+  void* w = nullptr;
+  size_t wElems = 0;
+  const std::vector<void*>* F = nullptr;
+  bool fp32 = false;
+};
+// m, nE and cut are indexed by front id (cut[s] < m[s]: the contribution
+// block is deferred); allowMw = false keeps every panel on one workgroup.
 // specMinRows < 0: no speculative step; else fronts with nE > specMinRows get its workgroups.
 LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int>& m, const std::vector<int>& nE,
-                           LargeSchedHost& h, int specMinRows, bool allowMw = true);
+                           const std::vector<int>& cut, LargeSchedHost& h, int specMinRows, bool allowMw = true);
 class Plan;
-// Launch sequence of a LargeSched; `prof` (may be null) brackets launches with events.
+// Launch sequence of a LargeSched; `prof` (may be null) brackets launches
+// with events; `cb` may be null only when no front defers its contribution block.
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
-                   const int64_t* wsOff, hipStream_t st, Plan* prof);
+                   const int64_t* wsOff, hipStream_t st, const CbUpdate* cb, Plan* prof);
+// cb_update.cpp
+void* blasCreate(hipStream_t st);
+void blasDestroy(void* h);
+void blasSyrkxLower(void* h, bool fp32, int n, int k, const void* W, int ldw, const void* B, int ldb, void* C,
+                    int ldc);
 
 class Plan {
  public:
@@ -174,6 +209,7 @@ class Plan {
     int asmTile = 64;
     int specPanel = 1;
     int specMinRows = kSpecMinRowsDefault;
+    int cbMinRows = kCbMinRowsDefault;  // < 0: every front updates its contribution block panel by panel
   };
   struct TuneTrial {
     Tuning t;
@@ -191,12 +227,24 @@ class Plan {
   std::string lastError;
 
   // per-kernel-class event profile
-  enum ProfCat { kProfSyrk = 0, kProfPanel, kProfPanelRows, kProfSmall, kProfAssemble, kProfFwd, kProfBwd, kNumCats };
+  enum ProfCat {
+    kProfSyrk = 0,
+    kProfPanel,
+    kProfPanelRows,
+    kProfSmall,
+    kProfAssemble,
+    kProfFwd,
+    kProfBwd,
+    kProfCb,
+    kNumCats
+  };
   struct Profile {
     double ms[kNumCats] = {};
     long long launches[kNumCats] = {};
-    double schurFlops = 0.0;
+    double schurFlops = 0.0;  // the panels' trailing updates (k_schur)
+    double cbFlops = 0.0;     // the deferred contribution-block updates (planned pivots)
   };
+  void addCbFlops(double f) { profAcc_.cbFlops += f; }
   void setProfiling(bool on);
   Profile takeProfile();
   void profBegin(int cat);
@@ -334,11 +382,20 @@ class Plan {
   FactorParams params(const FactorOptions& o, bool staticMode, double delta, const int* signDev) const;
   void runStatic(const double* aValsDev, const FactorParams& prm);
   void runDynamic(const double* aValsDev, const FactorParams& prm);
-  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_, dlargeMwStart_, dlargeSpecStart_;
+  DBuf<int> dlargeLists_, dlargeRowStart_, dlargeTileStart_, dlargeTileTp_, dlargeMwStart_, dlargeSpecStart_;
   DBuf<MwSync> dmwSync_;
   DBuf<int> dfactErr_;  // cooperative-panel wait errors
-  LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* mwStart,
-                    const int* specStart);
+  LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* tileTp,
+                    const int* mwStart, const int* specStart);
+  // deferred contribution-block updates: per-front trailing-update cut
+  // (host copy from the current sizes and tuning), library handle, workspace
+  std::vector<int> hcut_;
+  DBuf<int> dschurCut_;
+  void* blas_ = nullptr;
+  DBuf<double> dcbW_;
+  void computeCut();
+  // handle and workspace for the contribution blocks of `fronts` (outside any capture)
+  CbUpdate cbUpdate(const std::vector<CbTask>& tasks);
   DBuf<int64_t> dwsOff_;
   std::vector<int64_t> hwsOff_;
   void collectStats(FactorInfo& info, bool& anyDelay);

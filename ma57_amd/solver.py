@@ -403,18 +403,25 @@ class SymLDLT:
         _raise(rc, "symla_plan_get_tuning failed")
         return t
 
-    def setTuning(self, small_front_max: int, assemble_tile: int, speculative_panel=None, speculative_min_rows=None):
-        """Scheduling choices; speculative_* left as None keep their current values."""
+    def setTuning(self, small_front_max: int, assemble_tile: int, speculative_panel=None, speculative_min_rows=None,
+                  deferred_cb_min_rows=None):
+        """Scheduling choices; the optional ones left as None keep their current values."""
         cur = self._get_tuning()
         spec = cur.speculative_panel if speculative_panel is None else (1 if speculative_panel else 0)
         rows = cur.speculative_min_rows if speculative_min_rows is None else int(speculative_min_rows)
-        t = _lib.Tuning(small_front_max, assemble_tile, spec, rows)
+        cb = cur.deferred_cb_min_rows if deferred_cb_min_rows is None else int(deferred_cb_min_rows)
+        t = _lib.Tuning(small_front_max, assemble_tile, spec, rows, cb)
         rc = self._lib.symla_plan_set_tuning(self._plan, C.byref(t))
         _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
 
     def speculativePanel(self) -> bool:
         """Whether panels of large fronts start with the speculative blocked step."""
         return bool(self._get_tuning().speculative_panel)
+
+    def deferredCbMinRows(self) -> int:
+        """Contribution blocks of at least this many rows (large fronts, >= 256 eligible columns) are updated
+        once after the front's last panel; -1: every block panel by panel."""
+        return int(self._get_tuning().deferred_cb_min_rows)
 
     def setSpeculativePanel(self, on: bool, min_rows=None):
         """Switch the speculative panel step; min_rows: fronts with more eligible rows than this take it."""
@@ -513,7 +520,7 @@ class SymLDLT:
 def dense_factor(A: np.ndarray, n_eligible: int = -1, zero_tolerance: float = 1e-300,
                  relative_pivot_floor: float = 1e-12, static_mode: bool = False, static_delta: float = 0.0,
                  static_rel_floor: float = 1e-8, static_abs_floor: float = 1e-300, expected_sign=None,
-                 exact_panel: bool = False):
+                 exact_panel: bool = False, deferred_cb: bool = False):
     """DenseLDLT::factor / factorStatic (dense_kernel.hpp:393-837, :919-1059) on the device front kernels.
 
     Returns a dict with the factored matrix (lower triangle meaningful), D, perm, pivot blocks, n_factored,
@@ -522,7 +529,7 @@ def dense_factor(A: np.ndarray, n_eligible: int = -1, zero_tolerance: float = 1e
     A = np.array(A, dtype=np.float64, order="F", copy=True)
     n = A.shape[0]
     o = _lib.DenseOpts(zero_tolerance, relative_pivot_floor, 1 if static_mode else 0, static_delta,
-                       static_rel_floor, static_abs_floor, None, 1 if exact_panel else 0)
+                       static_rel_floor, static_abs_floor, None, 1 if exact_panel else 0, 1 if deferred_cb else 0)
     sg = None
     if expected_sign is not None:
         sg = as_i32(expected_sign)
