@@ -112,6 +112,9 @@ def main() -> int:
                     help="panels without the speculative blocked step (pivot-by-pivot kernel only)")
     ap.add_argument("--spec-min-rows", type=int, default=None,
                     help="fronts with more eligible rows than this take the speculative panel step")
+    ap.add_argument("--cb-min-rows", type=int, default=None,
+                    help="contribution blocks of at least this many rows are updated once after the front's last "
+                         "panel (-1: panel by panel)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,9 +148,9 @@ def main() -> int:
     n = A.shape[0]
     sf = solver.symbolicFactor()
     plan = solver._plan
-    if args.exact_panel or args.spec_min_rows is not None:
+    if args.exact_panel or args.spec_min_rows is not None or args.cb_min_rows is not None:
         small, tile = solver.tuning()
-        solver.setTuning(small, tile, not args.exact_panel, args.spec_min_rows)
+        solver.setTuning(small, tile, not args.exact_panel, args.spec_min_rows, args.cb_min_rows)
     tuning = solver._get_tuning()
     B = np.random.default_rng(SEED + int(args.config[1])).standard_normal((n, nrhs))
 
@@ -316,7 +319,8 @@ def main() -> int:
                         "supernodes": len(sf.supernodes),
                         "max_front": max(len(s.rowPattern) for s in sf.supernodes),
                         "speculative_panel": bool(tuning.speculative_panel),
-                        "speculative_min_rows": tuning.speculative_min_rows}, **extra),
+                        "speculative_min_rows": tuning.speculative_min_rows,
+                        "deferred_cb_min_rows": tuning.deferred_cb_min_rows}, **extra),
         "roofline": roofline("k_schur<float>", mixed, PEAK_FP32_MFMA_TFLOPS),
         "cpu_baseline": cpu,
         "fp32_ir": mixed,

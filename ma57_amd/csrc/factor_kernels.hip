@@ -2153,6 +2153,15 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
   if (t >= total) return;
   Tile cur = tileOf(t, nullptr);
   int pivFront = -1;  // front whose pivots are in pdd / pdo / pkd
+  // running max |F| of the updated entries: per lane over the tiles of one
+  // front, folded into the front's record when the walk leaves it
+  int mxFront = -1;
+  double lmx = 0.0;
+  auto flushMax = [&]() {
+    if (mxFront < 0) return;
+    const double wmx = __ockl_wfred_max_f64(lmx);
+    if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + mxFront, wmx);
+  };
   if (live(cur)) loadTile(cur);
   while (t < total) {
     const int tn = t + gridDim.x;
@@ -2233,7 +2242,11 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
         acc[1][1] = M::mma(b1, a1, acc[1][1]);
       }
       const int i0 = cur.k1 + cur.ti * TS, j0 = cur.k1 + cur.tj * TS;
-      double mx = 0.0;
+      if (cur.s != mxFront) {
+        flushMax();
+        mxFront = cur.s;
+        lmx = 0.0;
+      }
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -2245,16 +2258,15 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
             if (row < cur.m && col < cur.cut && (cur.ti != cur.tj || row >= col)) {
               const T v = static_cast<T>(cn[bi][bj][rr] - acc[bi][bj][rr]);
               cur.F[row + static_cast<size_t>(col) * cur.m] = v;
-              mx = fmax(mx, fabs(static_cast<double>(v)));
+              lmx = fmax(lmx, fabs(static_cast<double>(v)));
             }
           }
-      mx = __ockl_wfred_max_f64(mx);
-      if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + cur.s, mx);
       __syncthreads();  // the LDS tiles are restaged for the next tile
     }
     t = tn;
     if (t < total) cur = nxt;
   }
+  flushMax();
 }
 
 // Operand of the deferred contribution-block update of front s (a front
@@ -2516,8 +2528,9 @@ void launchPanelRows(const int* list, int count, const int* blockStart, int tota
 void launchSyrk(const int* list, int count, const int* tileStart, const int* tileTp, int totalTiles,
                 const DevFronts& fd, hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
-  // persistent workgroups: two per CU (the operand tiles take 70 KB of LDS in FP64)
-  const int blocks = std::min(totalTiles, 2 * 256);
+  // persistent workgroups: as many per CU as the operand tiles' LDS allows
+  // (70 KB in FP64: two; 35 KB in FP32: four)
+  const int blocks = std::min(totalTiles, (fd.fp32 ? 4 : 2) * 256);
   if (fd.fp32)
     k_schur<float><<<blocks, 256, 0, st>>>(list, count, tileStart, tileTp, fd);
   else

@@ -49,8 +49,11 @@ void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* 
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st);
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st);
 void launchSolveBegin(unsigned* gen, hipStream_t st);
+// forward gather: one thread per front position (blockStart: ceil(m / 256) workgroups per front);
+// mapPtr / mapSrc: per pre-permutation position, the (child, child position) entries summed into it
 void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st);
+                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
+                     const int2* mapSrc, hipStream_t st);
 // own rows per sweep block: the sweeps apply the inverse of each block's
 // unit lower diagonal block (kSolveBlock^2 doubles, row-major, at
 // invOff[s] + kSolveBlock^2 * block), formed once per factorization

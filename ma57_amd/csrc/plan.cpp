@@ -1028,9 +1028,32 @@ void Plan::ensureSolveWs(int nrhs) {
     dgOff_.upload(off, stream_);
     dG_.ensure(static_cast<size_t>(off[numSN_]));
     dDl_.ensure(static_cast<size_t>(off[numSN_]));
-    // Workgroups per front: the gather splits the right-hand sides over up
-    // to one workgroup per 64 rows; the sweeps take one workgroup per 32-row
-    // block (an upper bound: the own/ancestor split is known on the device).
+    // Gather map from the current layout and pivot counts (the static
+    // layout takes every eligible pivot; a repaired one has its own counts).
+    {
+      std::vector<int64_t> cnt(static_cast<size_t>(idxUsed_) + 1, 0);
+      auto forEach = [&](auto&& fn) {
+        for (int s = 0; s < numSN_; ++s)
+          for (int ci = hchildPtr_[s]; ci < hchildPtr_[s + 1]; ++ci) {
+            const int c = hchildList_[ci];
+            const int nfc = layoutDirty_ ? hnF_[c] : hnE_[c];
+            for (int a = nfc; a < hm_[c]; ++a) {
+              const int t = hrelmap_[hidxOff_[c] + a];
+              if (t >= 0) fn(hidxOff_[s] + t, c, a);
+            }
+          }
+      };
+      forEach([&](int64_t pos, int, int) { ++cnt[pos + 1]; });
+      for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+      std::vector<int2> src(static_cast<size_t>(std::max<int64_t>(cnt.back(), 1)));
+      forEach([&](int64_t pos, int c, int a) { src[fill[pos]++] = make_int2(c, a); });
+      dmapPtr_.upload(cnt, stream_);
+      dmapSrc_.upload(src, stream_);
+    }
+    // Workgroups per front: the gather takes one thread per front position;
+    // the sweeps take one workgroup per 32-row block (an upper bound: the
+    // own/ancestor split is known on the device).
     const int nl = static_cast<int>(levelFronts_.size());
     std::vector<int> gs, fs, bs;
     gatherOff_.assign(nl, 0);
@@ -1051,7 +1074,7 @@ void Plan::ensureSolveWs(int nrhs) {
       int ga = 0, fa = 0, ba = 0;
       for (int s : levelFronts_[L]) {
         gs.push_back(ga);
-        ga += std::max(1, std::min(nrhs, (hm_[s] + 63) / 64));
+        ga += std::max(1, (hm_[s] + 255) / 256);
       }
       sweepOff_[L] = static_cast<int64_t>(lists.size());
       for (int s : levelFronts_[L]) {
@@ -1121,7 +1144,7 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     const int* lst = dallFronts_.p + levelFrontOff_[L];
     profBegin(kProfFwd);
     launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, dBp_.p, dG_.p,
-                    dDl_.p, dgOff_.p, stream_);
+                    dDl_.p, dgOff_.p, dmapPtr_.p, dmapSrc_.p, stream_);
     launchFwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_,
                    nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
                    dinvOff_.p, stream_);

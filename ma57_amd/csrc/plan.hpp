@@ -309,6 +309,10 @@ class Plan {
   DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;
   int gOffNrhs_ = -1;
+  // forward gather map: per front position (idxOff-based), the children's
+  // delta entries (child, child position) that sum into it, in child order
+  DBuf<int64_t> dmapPtr_;
+  DBuf<int2> dmapSrc_;
   // inverses of the sweeps' diagonal blocks: fronts above kTinyFront rows,
   // their flat workgroup starts (one per kSolveBlock eligible rows) and
   // per-front offsets into dinv_

@@ -141,48 +141,56 @@ __global__ void k_solve_begin(unsigned* gen) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *gen = *gen + 1u;
 }
 
-// Forward step 1 (solve.hpp:327-347): sum the children's deltas into G
-// (pre-permutation positions, fixed child order, no atomics) and form
-// y0 = Bp - G on the own rows.  Workgroup q of a front handles the
-// right-hand sides q, q + nq, ...  G and Dl are m x nrhs per-front work
-// blocks at gOff[s]; Dl rows [0, nF) hold y, rows [nF, m) the delta handed
-// to the parent.
+// Forward step 1 (solve.hpp:327-347) through a gather map: for each
+// pre-permutation position t of front s, the children's delta entries that
+// land on it, in the children's order (mapPtr over idxOff positions, mapSrc
+// = (child, child position)).  Thread p of a front takes t = perm[p]: G[t]
+// is the sum of those entries in that order (the same sums as adding the
+// children one after another into a zeroed G) and, for an own row p < nF,
+// y0 = Bp - G.  G and Dl are m x nrhs per-front work blocks at gOff[s]; Dl
+// rows [0, nF) hold y, rows [nF, m) the delta handed to the parent.
+constexpr int kGatherCols = 8;
 __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
                                                      int n, int nrhs, const double* __restrict__ Bp,
                                                      double* __restrict__ G, double* __restrict__ Dl,
-                                                     const int64_t* __restrict__ gOff) {
+                                                     const int64_t* __restrict__ gOff,
+                                                     const int64_t* __restrict__ mapPtr,
+                                                     const int2* __restrict__ mapSrc) {
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf(blockStart, count, blockIdx.x);
   const int s = list[f];
-  const int q = blockIdx.x - blockStart[f];
-  const int nq = blockStart[f + 1] - blockStart[f];
+  const int p = (blockIdx.x - blockStart[f]) * blockDim.x + threadIdx.x;
   const int m = fd.m[s];
+  if (p >= m) return;
   const int nf = fd.nF[s];
   const int64_t io = fd.idxOff[s];
+  const int t = fd.perm[io + p];
+  const int64_t e0 = mapPtr[io + t], e1 = mapPtr[io + t + 1];
+  const bool own = p < nf;
+  const int rix = own ? fd.rowIdx[io + p] : 0;
   double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
-  const int tid = threadIdx.x, NT = blockDim.x;
-  for (int c = q; c < nrhs; c += nq)
-    for (int t = tid; t < m; t += NT) Gs[t + static_cast<int64_t>(c) * m] = 0.0;
-  __syncthreads();
-  for (int ci = fd.childPtr[s]; ci < fd.childPtr[s + 1]; ++ci) {
-    const int c = fd.childList[ci];
-    const int mc = fd.m[c];
-    const int nfc = fd.nF[c];
-    if (mc - nfc <= 0) continue;
-    const int* rel = fd.relmap + fd.idxOff[c];
-    const double* Dc = Dl + gOff[c];
-    for (int col = q; col < nrhs; col += nq)
-      for (int a = nfc + tid; a < mc; a += NT) {
-        const int t = rel[a];
-        if (t >= 0) Gs[t + static_cast<int64_t>(col) * m] += Dc[a + static_cast<int64_t>(col) * mc];
-      }
-    __syncthreads();
+  for (int c0 = 0; c0 < nrhs; c0 += kGatherCols) {
+    const int w = min(kGatherCols, nrhs - c0);
+    double g[kGatherCols];
+#pragma unroll
+    for (int j = 0; j < kGatherCols; ++j) g[j] = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const int2 src = mapSrc[e];
+      const int mc = fd.m[src.x];
+      const double* Dc = Dl + gOff[src.x] + src.y + static_cast<int64_t>(c0) * mc;
+#pragma unroll
+      for (int j = 0; j < kGatherCols; ++j)
+        if (j < w) g[j] += Dc[static_cast<int64_t>(j) * mc];
+    }
+#pragma unroll
+    for (int j = 0; j < kGatherCols; ++j) {
+      if (j >= w) break;
+      const int64_t col = c0 + j;
+      Gs[t + col * m] = g[j];
+      if (own) Ds[p + col * m] = Bp[rix + col * n] - g[j];
+    }
   }
-  for (int col = q; col < nrhs; col += nq)
-    for (int p = tid; p < nf; p += NT)
-      Ds[p + static_cast<int64_t>(col) * m] =
-          Bp[fd.rowIdx[io + p] + static_cast<int64_t>(col) * n] - Gs[fd.perm[io + p] + static_cast<int64_t>(col) * m];
 }
 
 // Inverses of the unit lower diagonal blocks of L that the sweeps solve
@@ -857,9 +865,10 @@ void launchSolveBegin(unsigned* gen, hipStream_t st) {
   hipCheck(hipGetLastError(), "launchSolveBegin");
 }
 void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, hipStream_t st) {
+                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
+                     const int2* mapSrc, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
-  k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff);
+  k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff, mapPtr, mapSrc);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
