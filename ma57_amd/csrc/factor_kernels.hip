@@ -190,7 +190,9 @@ __global__ void k_reset_perm(const int* list, int count, DevFronts fd) {
 // zero the lower triangle, add A's own-column entries, then extend-add every
 // child's contribution block in child order.  One workgroup owns each
 // (front, column tile), so every target entry has exactly one writer and the
-// summation order is fixed.
+// summation order is fixed.  The tile's max |F| then goes into the front's
+// orig_max (dense_kernel.hpp:413-415: the max over the assembled front is the
+// max over its tiles).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const int4* tasks, int ntasks, DevFronts fd,
@@ -262,6 +264,20 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* tasks, int ntasks,
       }
     }
     __syncthreads();
+  }
+
+  __shared__ double wmax[4];
+  double v = 0.0;
+  for (int j = c0 + wv; j < c1; j += nw) {
+    const T* col = F + static_cast<size_t>(j) * m;
+    for (int i = j + lane; i < m; i += 64) v = fmax(v, fabs(static_cast<double>(col[i])));
+  }
+  v = __ockl_wfred_max_f64(v);
+  if (lane == 0) wmax[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) v = fmax(v, wmax[w]);
+    atomicMaxNonNeg(fd.origMaxBits + s, v);
   }
 }
 

@@ -388,9 +388,9 @@ void Plan::buildSchedule() {
     for (int s : levelFronts_[L]) {
       all.push_back(s);
       // column tiles: a large front gets narrower tiles so that it alone
-      // spreads over at least 128 workgroups
+      // spreads over about 1024 workgroups (the top levels hold few fronts)
       int w = tune_.asmTile;
-      while (w > 8 && (hm_[s] + w - 1) / w < 128 && hm_[s] > 4 * tune_.asmTile) w >>= 1;
+      while (w > 8 && (hm_[s] + w - 1) / w < 1024 && hm_[s] > 4 * tune_.asmTile) w >>= 1;
       for (int c0 = 0; c0 < hm_[s]; c0 += w) tasks.push_back(make_int4(s, c0, std::min(c0 + w, hm_[s]), 0));
       if (hm_[s] <= tune_.smallMax) {
         small.push_back(s);
@@ -602,7 +602,7 @@ LargeSched buildLargeSched(const std::vector<int>& fronts, const std::vector<int
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
                    const int64_t* wsOff, hipStream_t st, const CbUpdate* cb, Plan* prof) {
   if (ls.allCount == 0) return;
-  launchFrontMaxAbs(dv.lists + ls.allOff, ls.allCount, ls.maxM, fd, st);
+  if (!dv.frontMaxDone) launchFrontMaxAbs(dv.lists + ls.allOff, ls.allCount, ls.maxM, fd, st);
   if (prm.specPanel) launchSpecReset(dv.lists + ls.allOff, ls.allCount, fd, ws, wsOff, st);
   for (size_t t = 0; t < ls.iters.size(); ++t) {
     const LargeIter& it = ls.iters[t];
@@ -651,6 +651,7 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
 LargeDev Plan::largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* tileTp,
                         const int* mwStart, const int* specStart) {
   LargeDev d;
+  d.frontMaxDone = true;  // k_assemble forms every front's max |F|
   d.tileTp = tileTp;
   d.specStart = specStart;
   d.lists = lists;

@@ -142,6 +142,7 @@ struct LargeDev {
   const int* specStart = nullptr;
   MwSync* sync = nullptr;
   int* err = nullptr;
+  bool frontMaxDone = false;  // the assembly already folded max |F| into origMaxBits
 };
 // What the deferred contribution-block updates run with: the library
 // handle (on the plan's stream), a workspace of at least kCbChunk times the

@@ -80,14 +80,18 @@ __device__ inline double loadShared(const double* p) {
 // stops waiting, so every workgroup still runs to its end.
 __device__ inline void waitBlock(const unsigned* f, unsigned gen, int* err) {
   if (threadIdx.x == 0) {
+    // one device-coherent load per poll; another workgroup's time-out and
+    // the spin limit are looked at every 32 polls
     unsigned spins = 0;
     while (!flagReached(f, gen)) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (++spins > kSpinLimit) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+      if ((++spins & 31u) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (spins > kSpinLimit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
   }
   __syncthreads();
