@@ -64,10 +64,14 @@ void launchFwdSweep(const int* list, int count, const int* blockStart, int block
                     int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
                     const int64_t* invOff, hipStream_t st);
+// the ancestors' x of a level's fronts into their work blocks (rows [nF, m)); same grid as the forward gather
+void launchBwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                     int nrhs, const double* X, double* Dl, const int64_t* gOff, hipStream_t st);
+// ancGathered: launchBwdGather ran for this level (the ancestors' x is read from Dl, else from X)
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, hipStream_t st);
+                    const int64_t* invOff, bool ancGathered, hipStream_t st);
 // fronts of at most kTinyFront rows: one wave per front, no dependency flags
 constexpr int kTinyFront = 64;
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,

@@ -1065,6 +1065,7 @@ void Plan::ensureSolveWs(int nrhs) {
     bwdBlocks_.assign(nl, 0);
     std::vector<int> lists;
     sweepOff_.assign(nl, 0);
+    bwdGather_.assign(nl, 0);
     tinyOff_.assign(nl, 0);
     sweepCnt_.assign(nl, 0);
     tinyCnt_.assign(nl, 0);
@@ -1084,6 +1085,7 @@ void Plan::ensureSolveWs(int nrhs) {
         lists.push_back(s);
         fs.push_back(fa);
         bs.push_back(ba);
+        if ((hnE_[s] + 31) / 32 >= kBwdGatherBlocks) bwdGather_[L] = 1;
         fa += (mm + 31) / 32 + 1;
         ba += (mm + 31) / 32;
       }
@@ -1154,9 +1156,13 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   }
   for (int L = nl - 1; L >= 0; --L) {
     profBegin(kProfBwd);
+    if (bwdGather_[L])
+      launchBwdGather(dallFronts_.p + levelFrontOff_[L],
+                      static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]), dgatherStart_.p + gatherOff_[L],
+                      gatherBlocks_[L], fd, n_, nrhs, dX_.p, dDl_.p, dgOff_.p, stream_);
     launchBwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_,
                    nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
-                   dinvOff_.p, stream_);
+                   dinvOff_.p, bwdGather_[L] != 0, stream_);
     launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dZ_.p, dX_.p, stream_);
     profEnd(kProfBwd);
   }

@@ -117,6 +117,9 @@ struct LargeIter {
 // column nE; after its last panel the block [nE, m)^2 takes one rank-nE
 // update, kCbChunk pivot columns per library SYRK.
 constexpr int kCbMinCols = 256;
+// levels with a front of at least this many 32-row own blocks gather the
+// ancestors' x once per front before the backward sweep
+constexpr int kBwdGatherBlocks = 16;
 constexpr int kCbMinRowsDefault = 2048;
 constexpr int kCbChunk = 2048;
 struct CbTask {
@@ -327,6 +330,10 @@ class Plan {
   DBuf<int> dgatherStart_, dfwdStart_, dbwdStart_;
   std::vector<int64_t> gatherOff_, fwdOff_, bwdOff_;
   std::vector<int> gatherBlocks_, fwdBlocks_, bwdBlocks_;
+  // per level: the backward sweep reads the ancestors' x from a per-front
+  // gather (levels with a front of at least kBwdGatherBlocks own-row blocks,
+  // where the sweep's workgroups would each gather it again)
+  std::vector<char> bwdGather_;
   // per level: fronts solved by the sweeps, then fronts of at most
   // kTinyFront rows solved one wave each (offsets into dsolveLists_)
   DBuf<int> dsolveLists_;

@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
                                                     const int64_t* __restrict__ gOff, unsigned* flags,
                                                     const int64_t* __restrict__ flagOff, const unsigned* genp,
                                                     int* err, const double* __restrict__ inv,
-                                                    const int64_t* __restrict__ invOff) {
+                                                    const int64_t* __restrict__ invOff, int ancGathered) {
   using M = SweepMfma<T>;
   using E = typename M::E;
   using MD = SweepMfma<double>;
@@ -481,6 +481,9 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
     for (int e = tid; e < SB * SB; e += NT) Ld[e / SB][e % SB] = Mi[e];
   }
   for (int q = tid; q < SB; q += NT) rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
+  // the block's tail reads rixA and Ld from every wave, and a block with no
+  // ancestors' rows and no later own blocks reaches it with no other barrier
+  __syncthreads();
   // L tiles loaded one tile ahead: the ancestors' rows (DT x SB, eight
   // elements per thread), then the own blocks below this one (SB x SB, four)
   E apre[8];
@@ -530,8 +533,11 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
       }
       for (int e = tid; e < DT * 64; e += NT) {
         const int i = e % DT, c = e / DT;
-        xt[i * YW + c] = (i < in && c < gw) ? static_cast<E>(X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n])
-                                            : E(0);
+        double v = 0.0;
+        if (i < in && c < gw)
+          v = ancGathered ? Ds[(i0 + i) + static_cast<int64_t>(rg + c) * m]
+                          : X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n];
+        xt[i * YW + c] = static_cast<E>(v);
       }
       if (i0 + DT < m) loadA(i0 + DT);
       __syncthreads();
@@ -596,6 +602,26 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
     }
     __syncthreads();
   }
+}
+
+// Backward step 0: the ancestors' x of every front of a level, gathered once
+// into the front's work block rows [nF, m) (free after the forward sweep), so
+// that the backward sweep's workgroups read it contiguously.  Workgroups per
+// front as the forward gather (blockStart: ceil(m / 256)); fronts of at most
+// kTinyFront rows are skipped (their kernel reads X itself).
+__global__ __launch_bounds__(256) void k_bwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                     int n, int nrhs, const double* __restrict__ X,
+                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int m = fd.m[s];
+  if (m <= kTinyFront) return;
+  const int i = fd.nF[s] + (blockIdx.x - blockStart[f]) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int r = fd.rowIdx[fd.idxOff[s] + i];
+  double* Ds = Dl + gOff[s];
+  for (int64_t c = 0; c < nrhs; ++c) Ds[i + c * m] = X[r + c * n];
 }
 
 // Fronts of at most 64 rows: one wave per front, lane i owning row i, and
@@ -897,17 +923,24 @@ void launchFwdSweep(const int* list, int count, const int* blockStart, int block
                                                 gen, err, inv, invOff);
   hipCheck(hipGetLastError(), "launchFwdSweep");
 }
+void launchBwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
+                     int nrhs, const double* X, double* Dl, const int64_t* gOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0) return;
+  k_bwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, X, Dl, gOff);
+  hipCheck(hipGetLastError(), "launchBwdGather");
+}
 void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                     int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
                     const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, hipStream_t st) {
+                    const int64_t* invOff, bool ancGathered, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
+  const int ag = ancGathered ? 1 : 0;
   if (fd.fp32)
     k_bwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                               gen, err, inv, invOff);
+                                               gen, err, inv, invOff, ag);
   else
     k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                                gen, err, inv, invOff);
+                                                gen, err, inv, invOff, ag);
   hipCheck(hipGetLastError(), "launchBwdSweep");
 }
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
