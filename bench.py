@@ -112,6 +112,8 @@ def main() -> int:
                     help="panels without the speculative blocked step (pivot-by-pivot kernel only)")
     ap.add_argument("--spec-min-rows", type=int, default=None,
                     help="fronts with more eligible rows than this take the speculative panel step")
+    ap.add_argument("--small-max", type=int, default=None,
+                    help="largest front order factored whole in LDS (at most 192)")
     ap.add_argument("--cb-min-rows", type=int, default=None,
                     help="contribution blocks of at least this many rows are updated once after the front's last "
                          "panel (-1: panel by panel)")
@@ -148,8 +150,10 @@ def main() -> int:
     n = A.shape[0]
     sf = solver.symbolicFactor()
     plan = solver._plan
-    if args.exact_panel or args.spec_min_rows is not None or args.cb_min_rows is not None:
+    if args.exact_panel or args.spec_min_rows is not None or args.cb_min_rows is not None or args.small_max is not None:
         small, tile = solver.tuning()
+        if args.small_max is not None:
+            small = args.small_max
         solver.setTuning(small, tile, not args.exact_panel, args.spec_min_rows, args.cb_min_rows)
     tuning = solver._get_tuning()
     B = np.random.default_rng(SEED + int(args.config[1])).standard_normal((n, nrhs))
@@ -318,6 +322,7 @@ def main() -> int:
                         "name": args.config, "n": n, "nrhs": nrhs, "nnz_L": int(sf.nnz_l),
                         "supernodes": len(sf.supernodes),
                         "max_front": max(len(s.rowPattern) for s in sf.supernodes),
+                        "small_front_max": tuning.small_front_max,
                         "speculative_panel": bool(tuning.speculative_panel),
                         "speculative_min_rows": tuning.speculative_min_rows,
                         "deferred_cb_min_rows": tuning.deferred_cb_min_rows}, **extra),

@@ -206,7 +206,8 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
 
 /* Scheduling choices of a plan (no counterpart in the reference, whose
  * front kernel is one CPU routine): fronts of order <= small_front_max
- * (at most 192) are factored whole in LDS, larger ones by the panel path;
+ * (at most 192, default 64) are factored whole in LDS, larger ones by the
+ * panel path;
  * assembly runs in column tiles of assemble_tile (a power of two in
  * [16, 1024]).  With speculative_panel set, each panel of a large front is
  * first factored as 1x1 pivots in order and the threshold tests checked for

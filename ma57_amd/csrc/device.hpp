@@ -9,6 +9,7 @@ namespace symla_hip {
 // Front dimensions above this use the panel path (global memory); at or below
 // it the whole packed lower triangle lives in LDS.
 constexpr int kSmallFrontMax = 192;
+constexpr int kSmallFrontDefault = 64;  // the plan's default bound (tunable up to kSmallFrontMax)
 // Panel width of the large-front path.
 constexpr int kPanel = 64;
 // Threads of the small-front and panel kernels.

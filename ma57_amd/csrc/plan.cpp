@@ -1469,7 +1469,7 @@ std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int p
   FactorOptions o;
   o.precision = precision;
   std::vector<TuneTrial> trials;
-  const int smalls[] = {64, 96, 128, 160, 192};
+  const int smalls[] = {0, 64, 96, 128, 192};
   const int tiles[] = {32, 64, 128};
   for (int sm : smalls)
     for (int tw : tiles) {

@@ -209,7 +209,10 @@ class Plan {
   // Scheduling choices: fronts of order <= smallMax take the LDS kernel,
   // larger ones the panel path; assembly runs in column tiles of asmTile.
   struct Tuning {
-    int smallMax = kSmallFrontMax;
+    // fronts up to this order are factored whole in LDS; larger ones (also the
+    // few mid-size fronts of the upper levels, where one workgroup per front
+    // is latency-bound) by the panel path
+    int smallMax = kSmallFrontDefault;
     int asmTile = 64;
     int specPanel = 1;
     int specMinRows = kSpecMinRowsDefault;
