@@ -1959,6 +1959,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   constexpr int P = kPanel + 1;  // a 2x2 pivot at the panel's last column makes it 65
   __shared__ double Lp[P][P + 1];
   __shared__ double Mi[P][P + 1];
+  __shared__ double Tt[P - 32][33];  // L21 M11 of the blocked inverse
   __shared__ double pdd[kPanel + 2], pdo[kPanel + 2];
   __shared__ int pkd[kPanel + 2];
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
@@ -1988,21 +1989,48 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
     Lp[r][c] = v;
   }
   __syncthreads();
-  // column c of M = L^{-1}: M[c][c] = 1, M[r][c] = -sum_{c <= t < r} L[r][t] M[t][c]
-  if (static_cast<int>(threadIdx.x) < p) {
-    const int c = threadIdx.x;
-    Mi[c][c] = 1.0;
-    for (int r = c + 1; r < p; ++r) {
-      double a0 = Lp[r][c], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  // M = L^{-1} by 2 x 2 blocks (the first H columns and the rest): both
+  // diagonal blocks are inverted at once, column by column (column c:
+  // M[c][c] = 1, M[r][c] = -sum_{c <= t < r} L[r][t] M[t][c]) by threads 0.. and
+  // 64.. (two waves), then M21 = -M22 (L21 M11).
+  constexpr int H = 32;
+  const int h = min(H, p), nb = p - h;
+  auto invCol = [&](int base, int nrow, int c) {
+    Mi[base + c][base + c] = 1.0;
+    for (int r = c + 1; r < nrow; ++r) {
+      double a0 = Lp[base + r][base + c], a1 = 0.0, a2 = 0.0, a3 = 0.0;
       int t = c + 1;
       for (; t + 4 <= r; t += 4) {
-        a0 = fma(Lp[r][t], Mi[t][c], a0);
-        a1 = fma(Lp[r][t + 1], Mi[t + 1][c], a1);
-        a2 = fma(Lp[r][t + 2], Mi[t + 2][c], a2);
-        a3 = fma(Lp[r][t + 3], Mi[t + 3][c], a3);
+        a0 = fma(Lp[base + r][base + t], Mi[base + t][base + c], a0);
+        a1 = fma(Lp[base + r][base + t + 1], Mi[base + t + 1][base + c], a1);
+        a2 = fma(Lp[base + r][base + t + 2], Mi[base + t + 2][base + c], a2);
+        a3 = fma(Lp[base + r][base + t + 3], Mi[base + t + 3][base + c], a3);
       }
-      for (; t < r; ++t) a0 = fma(Lp[r][t], Mi[t][c], a0);
-      Mi[r][c] = -((a0 + a1) + (a2 + a3));
+      for (; t < r; ++t) a0 = fma(Lp[base + r][base + t], Mi[base + t][base + c], a0);
+      Mi[base + r][base + c] = -((a0 + a1) + (a2 + a3));
+    }
+  };
+  const int tid = threadIdx.x;
+  if (tid < h)
+    invCol(0, h, tid);
+  else if (tid >= 64 && tid < 64 + nb)
+    invCol(h, nb, tid - 64);
+  __syncthreads();
+  if (nb > 0) {
+    // T = L21 M11 (M11 unit lower)
+    for (int e = tid; e < nb * h; e += blockDim.x) {
+      const int i = e % nb, j = e / nb;
+      double a = 0.0;
+      for (int k = j; k < h; ++k) a = fma(Lp[h + i][k], Mi[k][j], a);
+      Tt[i][j] = a;
+    }
+    __syncthreads();
+    // M21 = -(M22 T) (M22 unit lower)
+    for (int e = tid; e < nb * h; e += blockDim.x) {
+      const int i = e % nb, j = e / nb;
+      double a = Tt[i][j];
+      for (int k = 0; k < i; ++k) a = fma(Mi[h + i][h + k], Tt[k][j], a);
+      Mi[h + i][j] = -a;
     }
   }
   __syncthreads();

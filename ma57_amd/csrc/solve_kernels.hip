@@ -152,8 +152,9 @@ __global__ void k_solve_begin(unsigned* gen) {
 // is the sum of those entries in that order (the same sums as adding the
 // children one after another into a zeroed G) and, for an own row p < nF,
 // y0 = Bp - G.  G and Dl are m x nrhs per-front work blocks at gOff[s]; Dl
-// rows [0, nF) hold y, rows [nF, m) the delta handed to the parent.
-constexpr int kGatherCols = 8;
+// rows [0, nF) hold y, rows [nF, m) the delta handed to the parent.  The
+// right-hand sides are split over the grid's y dimension, kGatherCols each.
+constexpr int kGatherCols = 16;
 __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
                                                      int n, int nrhs, const double* __restrict__ Bp,
                                                      double* __restrict__ G, double* __restrict__ Dl,
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
   const int rix = own ? fd.rowIdx[io + p] : 0;
   double* Gs = G + gOff[s];
   double* Ds = Dl + gOff[s];
-  for (int c0 = 0; c0 < nrhs; c0 += kGatherCols) {
+  {
+    const int c0 = blockIdx.y * kGatherCols;
     const int w = min(kGatherCols, nrhs - c0);
     double g[kGatherCols];
 #pragma unroll
@@ -704,17 +706,26 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
   const T* F = static_cast<const T*>(fd.F[s]);
   const bool own = lane < nf;
   const int prow = own ? fd.rowIdx[io + lane] : 0;
+  // lane t holds the ancestors' x of the front's row nF + t (m <= 64)
+  const int ncb = m - nf;
+  const int crow = (lane < ncb) ? fd.rowIdx[io + nf + lane] : 0;
   for (int cg = 0; cg < nrhs; cg += 8) {
-    double x[8];
+    double x[8], xa[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) x[c] = (own && cg + c < nrhs) ? Z[prow + static_cast<int64_t>(cg + c) * n] : 0.0;
-    for (int i = nf; i < m; ++i) {
-      const double l = own ? static_cast<double>(F[i + static_cast<size_t>(lane) * m]) : 0.0;
-      const int arow = fd.rowIdx[io + i];
+    for (int c = 0; c < 8; ++c) {
+      x[c] = (own && cg + c < nrhs) ? Z[prow + static_cast<int64_t>(cg + c) * n] : 0.0;
+      xa[c] = (lane < ncb && cg + c < nrhs) ? X[crow + static_cast<int64_t>(cg + c) * n] : 0.0;
+    }
+    for (int i0 = 0; i0 < ncb; i0 += 8) {
+      double lv[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double xa = (cg + c < nrhs) ? X[arow + static_cast<int64_t>(cg + c) * n] : 0.0;
-        x[c] = fma(-l, xa, x[c]);
+      for (int u = 0; u < 8; ++u)
+        lv[u] = (own && i0 + u < ncb) ? static_cast<double>(F[(nf + i0 + u) + static_cast<size_t>(lane) * m]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (i0 + u >= ncb) break;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) x[c] = fma(-lv[u], __shfl(xa[c], i0 + u, 64), x[c]);
       }
     }
     for (int j = nf - 1; j > 0; --j) {
@@ -897,8 +908,9 @@ void launchSolveBegin(unsigned* gen, hipStream_t st) {
 void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
                      int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
                      const int2* mapSrc, hipStream_t st) {
-  if (count <= 0 || blocks <= 0) return;
-  k_fwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff, mapPtr, mapSrc);
+  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
+  const dim3 grid(blocks, (nrhs + kGatherCols - 1) / kGatherCols);
+  k_fwd_gather<<<grid, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff, mapPtr, mapSrc);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
