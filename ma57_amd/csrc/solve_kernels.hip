@@ -329,6 +329,16 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
     double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
+    // an own block's y0 (from the gather) does not depend on other blocks:
+    // load it before the waits, off the block chain's critical path
+    double y0p[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+        y0p[t][rr] = (own && r < nr && c < gw) ? Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] : 0.0;
+      }
     if (jEnd > 0) loadL(0);
     for (int j = 0; j < jEnd; ++j) {
       if (rg == 0) waitBlock(fl + j, gen, err);
@@ -379,7 +389,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (c < gw) ys[r][c] = (r < nr) ? Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] - acc[t][rr] : 0.0;
+        if (c < gw) ys[r][c] = (r < nr) ? y0p[t][rr] - acc[t][rr] : 0.0;
       }
     __syncthreads();
     {
@@ -511,6 +521,15 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
     double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
+    // the block's z does not depend on other blocks: load it before the waits
+    double z0p[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+        z0p[t][rr] = (qq < nr && c < gw) ? Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] : 0.0;
+      }
     // acc(q, c) += sum over rows i of the tile of L(i, r0 + q) x(i, c)
     auto tileProduct = [&](int kn) {
       typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -571,7 +590,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (c < gw) xs[qq][c] = (qq < nr) ? Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] - acc[t][rr] : 0.0;
+        if (c < gw) xs[qq][c] = (qq < nr) ? z0p[t][rr] - acc[t][rr] : 0.0;
       }
     __syncthreads();
     {
