@@ -789,11 +789,10 @@ __global__ void k_gather_vals(int64_t nnz, const int64_t* __restrict__ valSrc, c
 }
 
 // R = B - A X for column-major n x nrhs blocks (leading dimension n), A as
-// symmetric CSR (both triangles).  One thread per (row, chunk of kResChunk
-// right-hand sides), the chunks over the grid's y dimension, so the row's
-// indices and values are read once per chunk rather than once per column.
-// max |R| goes to *rmaxBits when given (the slot holds the bits of a
-// non-negative double).
+// symmetric CSR (both triangles).  One thread per row; the right-hand sides
+// go in chunks of kResChunk so the row's indices and values are read once
+// per chunk rather than once per column.  max |R| goes to *rmaxBits when
+// given (the slot holds the bits of a non-negative double).
 constexpr int kResChunk = 16;
 __global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t* __restrict__ rowPtr,
                                                   const int* __restrict__ colIdx, const double* __restrict__ aSym,
@@ -803,9 +802,7 @@ __global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t
   double mx = 0.0;
   if (i < n) {
     const int64_t t0 = rowPtr[i], t1 = rowPtr[i + 1];
-    // one chunk of kResChunk right-hand sides per grid row
-    {
-      const int c0 = blockIdx.y * kResChunk;
+    for (int c0 = 0; c0 < nrhs; c0 += kResChunk) {
       const int rc = min(kResChunk, nrhs - c0);
       const double* x = X + static_cast<int64_t>(c0) * n;
       double acc[kResChunk];
@@ -1000,8 +997,7 @@ void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
                     const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return;
-  const dim3 grid((n + 255) / 256, (nrhs + kResChunk - 1) / kResChunk);
-  k_residual<<<grid, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, X, R, rmaxBits);
+  k_residual<<<(n + 255) / 256, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, X, R, rmaxBits);
   hipCheck(hipGetLastError(), "launchResidual");
 }
 void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, double* aSym, hipStream_t st) {
