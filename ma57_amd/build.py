@@ -13,6 +13,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -52,7 +53,7 @@ def build_product(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     objdir = ROOT / "ma57_amd" / "_objs"
     objdir.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES:
         obj = objdir / (src + ".o")
         cmd = [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)]
@@ -61,8 +62,12 @@ def build_product(verbose: bool = False) -> Path:
             cmd.insert(2, "hip")
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # translation units are independent: compile them side by side
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1, 8)) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = PRODUCT.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, *LINK_LIBS])
     os.replace(tmp, PRODUCT)

@@ -519,15 +519,6 @@ std::vector<int> postorderForest(const std::vector<int>& parent) {
 // row-subtree traversal and supernode patterns built supernode-wise instead
 // of materializing every column's pattern).
 // ---------------------------------------------------------------------------
-namespace {
-
-int64_t storageNnzOf(int ncols, size_t m) {  // symbolic.hpp:297-301
-  const int64_t k = ncols;
-  const int64_t mm = static_cast<int64_t>(m);
-  return k * mm - k * (k - 1) / 2;
-}
-
-}  // namespace
 
 SymbolicResult analyze(int n, const int64_t* colptr, const int* rowidx, Ordering ordering,
                        const SymbolicOptions& opts, const int* userPerm) {
@@ -593,174 +584,168 @@ SymbolicResult analyze(int n, const int64_t* colptr, const int* rowidx, Ordering
     }
   }
 
-  // --- step 5: fundamental supernodes (symbolic.hpp:229-247) ---
-  struct FSN {
-    int first, ncols;
-    std::vector<int> pattern;  // stage-1 index space
-  };
-  std::vector<FSN> funda;
-  {
-    int j = 0;
-    while (j < n) {
-      const int start = j;
-      while (j + 1 < n && parent[j] == j + 1 && colCount[j] == colCount[j + 1] + 1) ++j;
-      funda.push_back({start, j - start + 1, {}});
-      ++j;
+  // --- step 5: chains of the postordered etree (symbolic.hpp:229-247) ---
+  // A column continues the chain of the column before it when it is that
+  // column's etree parent and its count is one less; a chain is a
+  // fundamental supernode.  chainOf[c] = chain id, chains numbered upwards.
+  std::vector<int> chainOf(n);
+  std::vector<int> chainLo, chainHi;  // [lo, hi] stage-1 columns of each chain
+  for (int c = 0; c < n; ++c) {
+    const bool extends = c > 0 && parent[c - 1] == c && colCount[c - 1] == colCount[c] + 1;
+    if (!extends) {
+      chainLo.push_back(c);
+      chainHi.push_back(c);
+    } else {
+      chainHi.back() = c;
     }
+    chainOf[c] = static_cast<int>(chainLo.size()) - 1;
   }
-  const int numFunda = static_cast<int>(funda.size());
-  std::vector<int> colToFunda(n);
-  for (int i = 0; i < numFunda; ++i)
-    for (int c = funda[i].first; c < funda[i].first + funda[i].ncols; ++c) colToFunda[c] = i;
-  std::vector<int> parentFunda(numFunda, -1);
-  std::vector<std::vector<int>> childrenFunda(numFunda);
-  for (int i = 0; i < numFunda; ++i) {
-    const int lastCol = funda[i].first + funda[i].ncols - 1;
-    const int p = parent[lastCol];
-    if (p != -1) {
-      parentFunda[i] = colToFunda[p];
-      childrenFunda[colToFunda[p]].push_back(i);
-    }
+  const int nch = static_cast<int>(chainLo.size());
+  // chain tree: the chain above is the one holding the etree parent of the top column
+  std::vector<int> chainUp(nch, -1);
+  for (int f = 0; f < nch; ++f) {
+    const int p = parent[chainHi[f]];
+    chainUp[f] = p < 0 ? -1 : chainOf[p];
   }
-  // Supernode patterns: own columns, A rows below, children's update rows.
+  // children of each chain in increasing id (CSR; counting sort keeps the order)
+  std::vector<int> kidPtr(nch + 1, 0), kids;
+  for (int f = 0; f < nch; ++f)
+    if (chainUp[f] >= 0) ++kidPtr[chainUp[f] + 1];
+  for (int f = 0; f < nch; ++f) kidPtr[f + 1] += kidPtr[f];
+  kids.resize(kidPtr[nch]);
   {
-    std::vector<int> mark(n, -1);
-    for (int i = 0; i < numFunda; ++i) {
-      auto& pat = funda[i].pattern;
-      const int f = funda[i].first, nc = funda[i].ncols;
-      for (int c = f; c < f + nc; ++c) {
-        mark[c] = i;
-        pat.push_back(c);
-      }
-      for (int c = f; c < f + nc; ++c) {
+    std::vector<int> fill(kidPtr.begin(), kidPtr.end() - 1);
+    for (int f = 0; f < nch; ++f)
+      if (chainUp[f] >= 0) kids[fill[chainUp[f]]++] = f;
+  }
+  // Rows of each chain below its own columns (stage-1 indices, sorted): the
+  // A entries below the chain's columns plus the children's rows above
+  // their own chains (symbolic elimination chain by chain).  The count must
+  // equal colCount of the chain's first column.
+  std::vector<std::vector<int>> below(nch);
+  {
+    std::vector<int> seen(n, -1);
+    for (int f = 0; f < nch; ++f) {
+      const int lo = chainLo[f], hi = chainHi[f];
+      auto& rows = below[f];
+      auto take = [&](int r) {
+        if (r > hi && seen[r] != f) {
+          seen[r] = f;
+          rows.push_back(r);
+        }
+      };
+      for (int c = lo; c <= hi; ++c) {
         const int v = finalPerm[c];
-        for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
-          const int r = inv1[g.adj[t]];
-          if (r > c && mark[r] != i) {
-            mark[r] = i;
-            pat.push_back(r);
-          }
-        }
+        for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) take(inv1[g.adj[t]]);
       }
-      for (int ch : childrenFunda[i]) {
-        const auto& cp = funda[ch].pattern;
-        for (size_t t = funda[ch].ncols; t < cp.size(); ++t) {
-          const int r = cp[t];
-          if (mark[r] != i) {
-            mark[r] = i;
-            pat.push_back(r);
-          }
-        }
-      }
-      std::sort(pat.begin(), pat.end());
-      if (static_cast<int>(pat.size()) != colCount[f])
+      for (int t = kidPtr[f]; t < kidPtr[f + 1]; ++t)
+        for (int r : below[kids[t]]) take(r);
+      std::sort(rows.begin(), rows.end());
+      if (static_cast<int>(rows.size()) + (hi - lo + 1) != colCount[lo])
         throw std::runtime_error("symla: supernode pattern does not match column count");
     }
   }
 
-  // --- step 6b: relaxed amalgamation (symbolic.hpp:269-329) ---
-  std::vector<int> dsu(numFunda);
-  std::iota(dsu.begin(), dsu.end(), 0);
-  auto find = [&](int x) {
-    while (dsu[x] != x) {
-      dsu[x] = dsu[dsu[x]];
-      x = dsu[x];
-    }
-    return x;
-  };
-  struct Agg {
-    int ncols;
-    std::vector<int> rowPattern;
-    std::vector<int> segments;
-  };
-  std::vector<Agg> agg(numFunda);
-  for (int i = 0; i < numFunda; ++i) {
-    agg[i].ncols = funda[i].ncols;
-    agg[i].rowPattern = funda[i].pattern;
-    agg[i].segments = {i};
-  }
-  std::vector<int> merged;
-  for (int i = 0; i < numFunda; ++i) {
-    for (int c : childrenFunda[i]) {
-      const int cr = find(c);
-      if (cr == i) continue;
-      const int mergedNcols = agg[i].ncols + agg[cr].ncols;
-      if (mergedNcols > opts.max_relax_size) continue;
-      merged.clear();
-      merged.reserve(agg[i].rowPattern.size() + agg[cr].rowPattern.size());
-      std::set_union(agg[i].rowPattern.begin(), agg[i].rowPattern.end(), agg[cr].rowPattern.begin(),
-                     agg[cr].rowPattern.end(), std::back_inserter(merged));
-      const int64_t mergedStorage = storageNnzOf(mergedNcols, merged.size());
-      const int64_t origStorage = storageNnzOf(agg[i].ncols, agg[i].rowPattern.size()) +
-                                  storageNnzOf(agg[cr].ncols, agg[cr].rowPattern.size());
-      const double extraFrac = static_cast<double>(mergedStorage - origStorage) / static_cast<double>(mergedStorage);
-      if (extraFrac > opts.max_relax_fill_fraction) continue;
-      dsu[cr] = i;
-      agg[i].ncols = mergedNcols;
-      agg[i].rowPattern.swap(merged);
-      agg[i].segments.insert(agg[i].segments.end() - 1, agg[cr].segments.begin(), agg[cr].segments.end());
-      std::vector<int>().swap(agg[cr].rowPattern);
+  // --- step 6: relaxed amalgamation (symbolic.hpp:269-329 rules) ---
+  // Chains are visited upwards; each tries to absorb its children in
+  // increasing id, the child bringing every chain it absorbed before.  The
+  // test needs only sizes: a child group's rows lie in its own columns plus
+  // the parent group's rows (etree nesting), and its columns are disjoint
+  // from the parent group's rows, so absorbing adds exactly the child
+  // group's columns to both the column count and the row count of the
+  // parent group.  Storage of a k-column, m-row block: k m - k (k - 1) / 2.
+  auto storage = [](int64_t k, int64_t m) { return k * m - k * (k - 1) / 2; };
+  std::vector<int> gCols(nch), gRows(nch);  // group totals, valid at group tops
+  std::vector<char> absorbed(nch, 0);       // chain f was absorbed by chainUp[f]
+  for (int f = 0; f < nch; ++f) {
+    gCols[f] = chainHi[f] - chainLo[f] + 1;
+    gRows[f] = colCount[chainLo[f]];
+    for (int t = kidPtr[f]; t < kidPtr[f + 1]; ++t) {
+      const int c = kids[t];
+      const int cols = gCols[f] + gCols[c];
+      if (cols > opts.max_relax_size) continue;
+      const int rows = gRows[f] + gCols[c];
+      const int64_t joint = storage(cols, rows);
+      const int64_t apart = storage(gCols[f], gRows[f]) + storage(gCols[c], gRows[c]);
+      if (static_cast<double>(joint - apart) / static_cast<double>(joint) > opts.max_relax_fill_fraction) continue;
+      absorbed[c] = 1;
+      gCols[f] = cols;
+      gRows[f] = rows;
     }
   }
+  // group top of every chain: parents have larger ids, so sweep downwards
+  std::vector<int> top(nch);
+  for (int f = nch - 1; f >= 0; --f) top[f] = absorbed[f] ? top[chainUp[f]] : f;
 
-  // --- step 6c/6d: group postorder and relabeling (symbolic.hpp:335-366) ---
-  std::vector<int> finalRoots;
-  for (int i = 0; i < numFunda; ++i)
-    if (find(i) == i) finalRoots.push_back(i);
-  const int numFinal = static_cast<int>(finalRoots.size());
-  std::vector<int> compactOf(numFunda, -1);
-  for (int k = 0; k < numFinal; ++k) compactOf[finalRoots[k]] = k;
-  std::vector<int> finalParentCompact(numFinal);
-  for (int k = 0; k < numFinal; ++k) {
-    const int pf = parentFunda[finalRoots[k]];
-    finalParentCompact[k] = (pf == -1) ? -1 : compactOf[find(pf)];
+  // --- step 6': relabel so every group is contiguous (symbolic.hpp:335-366 rules) ---
+  // Groups in postorder of the group forest (children and roots by
+  // increasing top id); inside a group, its chains in postorder of the
+  // absorbed edges (children by increasing id, the top last).
+  std::vector<int> tops;
+  std::vector<int> slot(nch, -1);
+  for (int f = 0; f < nch; ++f)
+    if (!absorbed[f]) {
+      slot[f] = static_cast<int>(tops.size());
+      tops.push_back(f);
+    }
+  const int ngroups = static_cast<int>(tops.size());
+  std::vector<int> groupUp(ngroups, -1);
+  for (int q = 0; q < ngroups; ++q) {
+    const int u = chainUp[tops[q]];
+    groupUp[q] = u < 0 ? -1 : slot[top[u]];
   }
-  std::vector<int> groupPostorder = postorderForest(finalParentCompact);
-  std::vector<int> newOrder;
-  newOrder.reserve(n);
-  sr.supernodes.reserve(numFinal);
-  std::vector<int> groupOfSupernode;
-  for (int gk : groupPostorder) {
-    const int r = finalRoots[gk];
-    Supernode sn;
-    sn.firstCol = static_cast<int>(newOrder.size());
-    sn.ncols = agg[r].ncols;
-    for (int fi : agg[r].segments)
-      for (int c = funda[fi].first; c < funda[fi].first + funda[fi].ncols; ++c) newOrder.push_back(c);
-    sr.supernodes.push_back(std::move(sn));
-    groupOfSupernode.push_back(r);
-  }
+  const std::vector<int> groupOrder = postorderForest(groupUp);
   std::vector<int> inv2(n);  // stage-1 index -> final index
-  for (int k = 0; k < n; ++k) inv2[newOrder[k]] = k;
+  std::vector<int> groupTopOf;
+  groupTopOf.reserve(ngroups);
+  sr.supernodes.reserve(ngroups);
+  {
+    int next = 0;
+    std::vector<std::pair<int, int>> dfs;  // (chain, next child slot)
+    for (int q : groupOrder) {
+      const int f0 = tops[q];
+      Supernode sn;
+      sn.firstCol = next;
+      sn.ncols = gCols[f0];
+      dfs.assign(1, {f0, kidPtr[f0]});
+      while (!dfs.empty()) {
+        auto& [f, t] = dfs.back();
+        while (t < kidPtr[f + 1] && !absorbed[kids[t]]) ++t;
+        if (t < kidPtr[f + 1]) {
+          const int c = kids[t++];
+          dfs.push_back({c, kidPtr[c]});
+          continue;
+        }
+        for (int c = chainLo[f]; c <= chainHi[f]; ++c) inv2[c] = next++;
+        dfs.pop_back();
+      }
+      sr.supernodes.push_back(std::move(sn));
+      groupTopOf.push_back(f0);
+    }
+  }
 
-  // --- step 6e/6f: final perm, etree and patterns ---
+  // --- final permutation, etree, counts and supernode row patterns ---
+  // A group's rows are its own (now contiguous) columns followed by the
+  // rows of its top chain above that chain.
   sr.perm.resize(n);
-  for (int k = 0; k < n; ++k) sr.perm[k] = finalPerm[newOrder[k]];
+  for (int c = 0; c < n; ++c) sr.perm[inv2[c]] = finalPerm[c];
   sr.etreeParent.assign(n, -1);
   sr.colCount.assign(n, 0);
   for (int j = 0; j < n; ++j) {
     sr.etreeParent[inv2[j]] = parent[j] == -1 ? -1 : inv2[parent[j]];
     sr.colCount[inv2[j]] = colCount[j];
   }
-  {
-    std::vector<int> mark(n, -1);
-    for (size_t s = 0; s < sr.supernodes.size(); ++s) {
-      auto& sn = sr.supernodes[s];
-      const int r = groupOfSupernode[s];
-      std::vector<int> pat;
-      for (int fi : agg[r].segments)
-        for (int x : funda[fi].pattern) {
-          const int y = inv2[x];
-          if (mark[y] != static_cast<int>(s)) {
-            mark[y] = static_cast<int>(s);
-            pat.push_back(y);
-          }
-        }
-      std::sort(pat.begin(), pat.end());
-      for (int t = 0; t < sn.ncols; ++t)
-        if (pat[t] != sn.firstCol + t) throw std::runtime_error("symla: supernode columns are not contiguous");
-      sn.rowPattern.swap(pat);
-    }
+  for (size_t s = 0; s < sr.supernodes.size(); ++s) {
+    auto& sn = sr.supernodes[s];
+    const auto& up = below[groupTopOf[s]];
+    std::vector<int> rows(sn.ncols);
+    std::iota(rows.begin(), rows.end(), sn.firstCol);
+    const size_t own = rows.size();
+    for (int r : up) rows.push_back(inv2[r]);
+    std::sort(rows.begin() + own, rows.end());
+    if (own < rows.size() && rows[own] < sn.firstCol + sn.ncols)
+      throw std::runtime_error("symla: supernode columns are not contiguous");
+    sn.rowPattern.swap(rows);
   }
   sr.nnzL = 0;
   for (int c : colCount) sr.nnzL += c;

@@ -7,9 +7,9 @@
 // supernodes whose own columns are contiguous and lead their sorted row
 // pattern.  The supernode construction keeps the reference's rules
 // (postorder with children in increasing index order, the fundamental
-// supernode test, union-find amalgamation with a size cap and a relative
-// fill cap, group postorder relabeling) so that, for the same ordering, the
-// supernode partition is the same.  The orderings themselves are this
+// supernode test, bottom-up absorption of children under a size cap and a
+// relative fill cap, relabeling in a postorder of the merged groups) so
+// that, for the same ordering, the supernode partition is the same.  The orderings themselves are this
 // project's own (SuiteSparse AMD and METIS are not available).
 #pragma once
 

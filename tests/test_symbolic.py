@@ -113,3 +113,26 @@ def test_invalid_inputs_raise():
         analyze(A, 7)
     with pytest.raises(InvalidArgument):
         analyze(A, 3, user_perm=[0, 0, 1])
+
+
+@pytest.mark.parametrize("case", ["stencil27_7", "laplace3d_9", "batched", "random400"])
+@pytest.mark.parametrize("relax", [(64, 0.25), (16, 0.1), (128, 0.6)])
+def test_larger_partitions_match_restatement(case, relax):
+    """Bushier trees (3D stencils, random couplings) with the product's own
+    orders fed back: the amalgamation decisions and the relabeling must equal
+    the restatement's on many merges, not only on small examples."""
+    if case == "stencil27_7":
+        A, order = P.stencil27(7, 0.2), 1
+    elif case == "laplace3d_9":
+        A, order = P.laplace3d(9, 9, 9, 0.5), 1
+    elif case == "batched":
+        A, order = P.batched_system(1234), 0
+    else:
+        A, order = P.random_sparse_indefinite(400, 5, 11), 0
+    sf0 = _product(A, order, relax)
+    nn, cp, ri = _arrays(A)
+    sf = _product(A, 3, relax, user_perm=sf0.perm)
+    ref = R.analyze(nn, cp, ri, list(sf0.perm), *relax)
+    _same(sf, ref)
+    # the case really amalgamates: fewer supernodes than the fundamental ones
+    assert len(ref[2]) < len(R.analyze(nn, cp, ri, list(sf0.perm), 1, 0.0)[2])
