@@ -271,6 +271,7 @@ void Plan::buildStaticLayout() {
   maxNE_ = 0;
   for (int s = 0; s < numSN_; ++s) maxNE_ = std::max(maxNE_, hnE_[s]);
   dws_.ensure(2 * static_cast<size_t>(std::max(maxNE_, 1)));
+  if (!dynAllocs_.empty()) ++g_devAllocEpoch;
   for (void* p : dynAllocs_) (void)hipFree(p);
   dynAllocs_.clear();
 }
@@ -287,6 +288,7 @@ void Plan::ensureIdxCapacity(int64_t need) {
       hipCheck(hipMemcpyAsync(np, buf.p, idxUsed_ * sizeof(T), hipMemcpyDeviceToDevice, stream_), "grow copy");
     hipCheck(hipStreamSynchronize(stream_), "grow sync");
     if (buf.p) (void)hipFree(buf.p);
+    ++g_devAllocEpoch;
     buf.p = np;
     buf.n = ncap;
   };
@@ -322,8 +324,8 @@ void Plan::uploadLayout() {
   }
   computeCut();
   dschurCut_.upload(hcut_, stream_);
-  // solve workspaces depend on the layout
-  gOffNrhs_ = -1;
+  // the solve's gather map, launch shapes and flags depend on the layout
+  solveLayoutReady_ = false;
   // the uploads below read these host vectors until the stream sync at the end
   std::vector<int> lst, start(1, 0);
   std::vector<int64_t> off(numSN_ + 1, 0);
@@ -777,6 +779,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
         hnE_[s] = ncols + nExtra;
         void* np = nullptr;
         hipCheck(hipMalloc(&np, elemSize() * static_cast<size_t>(m) * m), "hipMalloc front");
+        ++g_devAllocEpoch;
         dynAllocs_.push_back(np);
         hF_[s] = np;
         // own-column A entries: local rows shift past the extras
@@ -930,6 +933,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     factorMs_ = ms;
+    ensureSolveLayout();  // a no-op unless the layout changed
     factorized_ = true;
     return;
   }
@@ -990,6 +994,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     factorMs_ = ms;
+    ensureSolveLayout();
     factorized_ = true;
   };
   const int maxAttempts = std::max(1, opts.maxRetries + 1);
@@ -1019,16 +1024,22 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
 }
 
 void Plan::ensureSolveWs(int nrhs) {
+  ensureSolveLayout();
   const size_t nn = static_cast<size_t>(n_) * nrhs;
   dBp_.ensure(nn);
   dZ_.ensure(nn);
   dX_.ensure(nn);
-  if (gOffNrhs_ != nrhs) {
+  dG_.ensure(static_cast<size_t>(sumM_) * nrhs);
+  dDl_.ensure(static_cast<size_t>(sumM_) * nrhs);
+}
+
+void Plan::ensureSolveLayout() {
+  if (!solveLayoutReady_) {
+    // per-front work blocks of m x nrhs start at off[s] * nrhs
     std::vector<int64_t> off(numSN_ + 1, 0);
-    for (int s = 0; s < numSN_; ++s) off[s + 1] = off[s] + static_cast<int64_t>(hm_[s]) * nrhs;
+    for (int s = 0; s < numSN_; ++s) off[s + 1] = off[s] + hm_[s];
+    sumM_ = off[numSN_];
     dgOff_.upload(off, stream_);
-    dG_.ensure(static_cast<size_t>(off[numSN_]));
-    dDl_.ensure(static_cast<size_t>(off[numSN_]));
     // Gather map from the current layout and pivot counts (the static
     // layout takes every eligible pivot; a repaired one has its own counts).
     {
@@ -1121,7 +1132,7 @@ void Plan::ensureSolveWs(int nrhs) {
     launchZeroU32(reinterpret_cast<unsigned*>(dsolveErr_.p), 1, stream_);
     // the uploads above read host vectors local to this block
     hipCheck(hipStreamSynchronize(stream_), "solve workspace");
-    gOffNrhs_ = nrhs;
+    solveLayoutReady_ = true;
   }
 }
 
@@ -1136,22 +1147,29 @@ void Plan::checkSolveError() {
   }
 }
 
-void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
-  ensureSolveWs(nrhs);
+void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, const SolveWs* ext) {
+  SolveWs w;
+  if (ext) {
+    if (!solveLayoutReady_) throw std::logic_error("symla: the solve layout is not built (factorize first)");
+    w = *ext;
+  } else {
+    ensureSolveWs(nrhs);
+    w = SolveWs{dBp_.p, dZ_.p, dX_.p, dG_.p, dDl_.p};
+  }
   DevFronts fd = devFronts();
   launchSolveBegin(dgen_.p, stream_);
-  launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, dBp_.p, stream_);
+  launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, w.Bp, stream_);
   const int nl = static_cast<int>(levelFronts_.size());
   for (int L = 0; L < nl; ++L) {
     const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
     const int* lst = dallFronts_.p + levelFrontOff_[L];
     profBegin(kProfFwd);
-    launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, dBp_.p, dG_.p,
-                    dDl_.p, dgOff_.p, dmapPtr_.p, dmapSrc_.p, stream_);
+    launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, w.Bp, w.G,
+                    w.Dl, dgOff_.p, dmapPtr_.p, dmapSrc_.p, stream_);
     launchFwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_,
-                   nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
+                   nrhs, w.G, w.Z, w.Dl, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
                    dinvOff_.p, stream_);
-    launchFwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dG_.p, dZ_.p, dDl_.p, dgOff_.p, stream_);
+    launchFwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, w.G, w.Z, w.Dl, dgOff_.p, stream_);
     profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
@@ -1159,14 +1177,14 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     if (bwdGather_[L])
       launchBwdGather(dallFronts_.p + levelFrontOff_[L],
                       static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]), dgatherStart_.p + gatherOff_[L],
-                      gatherBlocks_[L], fd, n_, nrhs, dX_.p, dDl_.p, dgOff_.p, stream_);
+                      gatherBlocks_[L], fd, n_, nrhs, w.X, w.Dl, dgOff_.p, stream_);
     launchBwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_,
-                   nrhs, dZ_.p, dX_.p, dDl_.p, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
+                   nrhs, w.Z, w.X, w.Dl, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
                    dinvOff_.p, bwdGather_[L] != 0, stream_);
-    launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, dZ_.p, dX_.p, stream_);
+    launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, w.Z, w.X, stream_);
     profEnd(kProfBwd);
   }
-  launchPermuteOut(n_, nrhs, dpermVec_.p, dX_.p, dX, ldx, stream_);
+  launchPermuteOut(n_, nrhs, dpermVec_.p, w.X, dX, ldx, stream_);
 }
 
 void Plan::solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo) {
@@ -1255,6 +1273,9 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
              "norms");
     hipCheck(hipStreamSynchronize(stream_), "norms");
     const double nR = bitsToDoubleHost(h2[0]), nX = bitsToDoubleHost(h2[1]);
+    // the device maps NaN entries to +inf: a non-finite residual or solution
+    // is an infinite relative residual, never a converged one
+    if (!std::isfinite(nR) || !std::isfinite(nX)) return std::numeric_limits<double>::infinity();
     const double denom = normA * nX + normB;
     return denom > 0.0 ? nR / denom : nR;
   };
@@ -1294,7 +1315,10 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
       const double nr = bitsToDoubleHost(h[3 * c]), nx = bitsToDoubleHost(h[3 * c + 1]),
                    nb = bitsToDoubleHost(h[3 * c + 2]);
       const double den = normAsym * nx + nb;
-      be = std::max(be, den > 0.0 ? nr / den : nr);
+      const double bc = (!std::isfinite(nr) || !std::isfinite(nx)) ? std::numeric_limits<double>::infinity()
+                        : den > 0.0                                 ? nr / den
+                                                                    : nr;
+      be = std::max(be, bc);
     }
     res.backwardError = be;
   }
@@ -1327,22 +1351,6 @@ void Plan::groupInertia(int ngroups, const int* groupStart, int* inertia3, int* 
     std::fill(singular, singular + ngroups, 0);
     for (int c : singularColsOrig_) ++singular[groupOf(c)];
   }
-}
-
-std::vector<const void*> Plan::graphFingerprint() const {
-  std::vector<const void*> v = {pool_.p,   dF_.p,      dm_.p,       dnE_.p,      dncols_.p,   dfirst_.p,
-                                dchildPtr_.p, dchildList_.p, dfrontIdx_.p, drelmap_.p, daRow_.p, didxOff_.p,
-                                dacolPtr_.p, daSrc_.p,  dperm_.p,    drowIdx_.p,  dpkind_.p,   dnF_.p,
-                                dinertia_.p, dkcur_.p,  dpcount_.p,  dstopped_.p, dnpert_.p,   ddDiag_.p,
-                                ddOff_.p,   dpert_.p,   dorigMax_.p, drunMax_.p,  dstatI_.p,   dstatD_.p,
-                                dws_.p,     dsmall_.p,  dtasks_.p,   dallFronts_.p, dlargeLists_.p,
-                                dlargeRowStart_.p, dlargeTileStart_.p, dwsOff_.p, dpermVec_.p, dBp_.p, dZ_.p,
-                                dX_.p,      dG_.p,      dDl_.p,      dgOff_.p,    dgatherStart_.p, dfwdStart_.p,
-                                dbwdStart_.p, dflagOff_.p, dflagsF_.p, dflagsB_.p, dgen_.p, dsolveErr_.p,
-                                dschur_.p,  dsolveLists_.p, dlargeMwStart_.p, dmwSync_.p, dfactErr_.p,
-                                dlargeSpecStart_.p, dcbDone_.p};
-  for (void* f : hF_) v.push_back(f);
-  return v;
 }
 
 void Plan::graphRecord() {
@@ -1378,7 +1386,8 @@ void Plan::graphRecord() {
   hipCheck(hipStreamEndCapture(stream_, &graph_), "end capture");
   profiling_ = prof;
   hipCheck(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "instantiate graph");
-  graphPrint_ = graphFingerprint();
+  graphEpoch_ = g_devAllocEpoch.load();
+  graphStale_ = false;
 }
 
 void Plan::graphCapture(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
@@ -1396,18 +1405,19 @@ void Plan::graphRelease() {
   if (graph_) (void)hipGraphDestroy(graph_);
   graphExec_ = nullptr;
   graph_ = nullptr;
-  graphPrint_.clear();
+  graphStale_ = true;
 }
 
 void Plan::graphReplay(FactorInfo& info) {
   if (!graphExec_) throw std::logic_error("symla: no recorded graph (call symla_graph_capture first)");
   factorized_ = false;
-  // a repair pass or another solve size may have moved what the graph refers to
-  if (layoutDirty_ || graphArgs_.opts.precision != prec_ || graphFingerprint() != graphPrint_) {
+  // a repair pass, another solve size or a new tuning may have moved or
+  // changed what the graph refers to
+  if (layoutDirty_ || graphArgs_.opts.precision != prec_ || !graphCurrent()) {
     graphRecord();
   } else {
     ensureSolveWs(graphArgs_.nrhs);
-    if (graphFingerprint() != graphPrint_) graphRecord();
+    if (!graphCurrent()) graphRecord();
   }
   info = FactorInfo{};
   singular_ = false;
@@ -1449,7 +1459,7 @@ void Plan::setTuning(const Tuning& t) {
   if (t.specMinRows < 0) throw std::invalid_argument("symla: speculative-panel row bound must be >= 0");
   tune_ = t;
   buildSchedule();
-  graphPrint_.clear();  // a recorded graph holds the old schedule: re-record on replay
+  graphStale_ = true;  // a recorded graph holds the old schedule: re-record on replay
 }
 
 std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int precision, int reps) {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -19,6 +20,13 @@ struct DeviceError : std::runtime_error {
 
 void hipCheck(hipError_t e, const char* what);
 
+// Moves on every allocation or release of plan device memory (DBuf, the
+// per-front blocks of a repair pass, grown index arrays).  A recorded graph
+// refers to device addresses, so it is stale once this has moved since its
+// recording.  Process-wide: another plan's allocations also count, which
+// only costs an extra re-record.
+inline std::atomic<unsigned long long> g_devAllocEpoch{0};
+
 // Device buffer with RAII free.
 template <typename T>
 struct DBuf {
@@ -29,7 +37,10 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      ++g_devAllocEpoch;
+    }
     p = nullptr;
     n = 0;
   }
@@ -37,6 +48,7 @@ struct DBuf {
     release();
     if (count == 0) count = 1;
     hipCheck(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+    ++g_devAllocEpoch;
     n = count;
   }
   void ensure(size_t count) {
@@ -314,8 +326,9 @@ class Plan {
   // solve
   DBuf<int> dpermVec_;
   DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
-  DBuf<int64_t> dgOff_;
-  int gOffNrhs_ = -1;
+  DBuf<int64_t> dgOff_;  // running sum of the front orders: front s's work blocks start at dgOff_[s] * nrhs
+  int64_t sumM_ = 0;
+  bool solveLayoutReady_ = false;
   // forward gather map: per front position (idxOff-based), the children's
   // delta entries (child, child position) that sum into it, in child order
   DBuf<int64_t> dmapPtr_;
@@ -375,9 +388,12 @@ class Plan {
     int64_t ldx = 0;
     FactorOptions opts;
   } graphArgs_;
-  std::vector<const void*> graphPrint_;
+  // g_devAllocEpoch when the graph was recorded; graphStale_ forces a
+  // re-record (new schedule)
+  unsigned long long graphEpoch_ = 0;
+  bool graphStale_ = true;
   Tuning tune_;
-  std::vector<const void*> graphFingerprint() const;
+  bool graphCurrent() const { return !graphStale_ && graphEpoch_ == g_devAllocEpoch.load(); }
   void graphRecord();
   bool profiling_ = false;
   struct ProfEvents {
@@ -415,8 +431,17 @@ class Plan {
   std::vector<int64_t> hwsOff_;
   void collectStats(FactorInfo& info, bool& anyDelay);
   void finishSingular();
+  // nrhs-independent solve set-up (gather map, launch shapes, flags) for
+  // the current layout; built after each factorization
+  void ensureSolveLayout();
   void ensureSolveWs(int nrhs);
-  void solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  // per-call solve workspace: permuted B, Z, X (n x nrhs each) and the
+  // per-front blocks G, Dl (sum of front orders x nrhs each)
+  struct SolveWs {
+    double *Bp = nullptr, *Z = nullptr, *X = nullptr, *G = nullptr, *Dl = nullptr;
+  };
+  // ext: caller-provided workspace (nothing is allocated); null: the plan's own
+  void solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, const SolveWs* ext = nullptr);
   void ensureIdxCapacity(int64_t need);
 };
 

@@ -151,7 +151,8 @@ __global__ void k_solve_begin(unsigned* gen) {
 // = (child, child position)).  Thread p of a front takes t = perm[p]: G[t]
 // is the sum of those entries in that order (the same sums as adding the
 // children one after another into a zeroed G) and, for an own row p < nF,
-// y0 = Bp - G.  G and Dl are m x nrhs per-front work blocks at gOff[s]; Dl
+// y0 = Bp - G.  G and Dl are m x nrhs per-front work blocks at gOff[s] *
+// nrhs (gOff: running sum of the front orders, independent of nrhs); Dl
 // rows [0, nF) hold y, rows [nF, m) the delta handed to the parent.  The
 // right-hand sides are split over the grid's y dimension, kGatherCols each.
 constexpr int kGatherCols = 16;
@@ -173,8 +174,8 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
   const int64_t e0 = mapPtr[io + t], e1 = mapPtr[io + t + 1];
   const bool own = p < nf;
   const int rix = own ? fd.rowIdx[io + p] : 0;
-  double* Gs = G + gOff[s];
-  double* Ds = Dl + gOff[s];
+  double* Gs = G + gOff[s] * nrhs;
+  double* Ds = Dl + gOff[s] * nrhs;
   {
     const int c0 = blockIdx.y * kGatherCols;
     const int w = min(kGatherCols, nrhs - c0);
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
     for (int64_t e = e0; e < e1; ++e) {
       const int2 src = mapSrc[e];
       const int mc = fd.m[src.x];
-      const double* Dc = Dl + gOff[src.x] + src.y + static_cast<int64_t>(c0) * mc;
+      const double* Dc = Dl + gOff[src.x] * nrhs + src.y + static_cast<int64_t>(c0) * mc;
 #pragma unroll
       for (int j = 0; j < kGatherCols; ++j)
         if (j < w) g[j] += Dc[static_cast<int64_t>(j) * mc];
@@ -293,8 +294,8 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, c
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
-  const double* Gs = G + gOff[s];
-  double* Ds = Dl + gOff[s];
+  const double* Gs = G + gOff[s] * nrhs;
+  double* Ds = Dl + gOff[s] * nrhs;
   unsigned* fl = flags + flagOff[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // wave wv owns output rows tr .. tr + 15 and right-hand sides tc .. tc + 31
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
-  double* Ds = Dl + gOff[s];
+  double* Ds = Dl + gOff[s] * nrhs;
   unsigned* fl = flags + flagOff[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // wave wv owns own columns tr .. tr + 15 and right-hand sides tc .. tc + 31
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const int* list, int count, 
   const int i = fd.nF[s] + (blockIdx.x - blockStart[f]) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int r = fd.rowIdx[fd.idxOff[s] + i];
-  double* Ds = Dl + gOff[s];
+  double* Ds = Dl + gOff[s] * nrhs;
   for (int64_t c = 0; c < nrhs; ++c) Ds[i + c * m] = X[r + c * n];
 }
 
@@ -662,8 +663,8 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
-  const double* Gs = G + gOff[s];
-  double* Ds = Dl + gOff[s];
+  const double* Gs = G + gOff[s] * nrhs;
+  double* Ds = Dl + gOff[s] * nrhs;
   const bool row = lane < m;
   const bool own = lane < nf;
   const int kind = own ? pk[lane] : 0;
@@ -765,6 +766,11 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
 // Refinement support (refine.hpp:69-144).
 // ---------------------------------------------------------------------------
 
+// |v| for the max reductions of the norms, with NaN mapped to +inf: fmax
+// drops a NaN operand, and a NaN residual or solution must not read as a
+// small norm (refinement would then stop as converged on garbage).
+__device__ inline double normAbs(double v) { return v != v ? static_cast<double>(INFINITY) : fabs(v); }
+
 // Max of v over a 256-thread workgroup into *bits (bits of a non-negative
 // double) with one atomic per workgroup; every thread of the group calls it.
 __device__ inline void groupAtomicMax(double v, unsigned long long* bits) {
@@ -830,7 +836,7 @@ __global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t
           const int64_t e = static_cast<int64_t>(c0 + u) * n + i;
           const double r = B[e] - acc[u];
           R[e] = r;
-          mx = fmax(mx, fabs(r));
+          mx = fmax(mx, normAbs(r));
         }
     }
   }
@@ -845,7 +851,7 @@ __global__ void k_axpy(int64_t len, const double* __restrict__ x, const double* 
        e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const double v = x[e] + dx[e];
     out[e] = v;
-    mx = fmax(mx, fabs(v));
+    mx = fmax(mx, normAbs(v));
   }
   if (maxBits) groupAtomicMax(mx, maxBits);
 }
@@ -857,7 +863,7 @@ __global__ void k_absmax(int n, int nrhs, const double* __restrict__ v, int64_t 
   for (int c = 0; c < nrhs; ++c) {
     const double* col = v + static_cast<int64_t>(c) * ld;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-      mx = fmax(mx, fabs(col[i]));
+      mx = fmax(mx, normAbs(col[i]));
   }
   groupAtomicMax(mx, outBits);
 }
@@ -871,7 +877,7 @@ __global__ void k_absmax_cols(int n, int nrhs, const double* __restrict__ v, int
   if (c >= nrhs) return;
   const double* col = v + static_cast<int64_t>(c) * ld;
   double mx = 0.0;
-  for (int i = part * blockDim.x + threadIdx.x; i < n; i += parts * blockDim.x) mx = fmax(mx, fabs(col[i]));
+  for (int i = part * blockDim.x + threadIdx.x; i < n; i += parts * blockDim.x) mx = fmax(mx, normAbs(col[i]));
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
   if ((threadIdx.x & 63) == 0 && mx > 0.0)
     atomicMax(outBits + static_cast<int64_t>(c) * stride, static_cast<unsigned long long>(__double_as_longlong(mx)));
@@ -886,7 +892,7 @@ __global__ void k_rowsum_max(int n, const int64_t* __restrict__ rowPtr, const in
   double mx = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double sacc = 0.0;
-    for (int64_t t = rowPtr[i]; t < rowPtr[i + 1]; ++t) sacc += fabs(aVals[valSrc[t]]);
+    for (int64_t t = rowPtr[i]; t < rowPtr[i + 1]; ++t) sacc += normAbs(aVals[valSrc[t]]);
     mx = fmax(mx, sacc);
   }
   groupAtomicMax(mx, outBits);

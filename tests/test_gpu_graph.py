@@ -104,3 +104,43 @@ def test_graph_rejects_static_mode():
     d_B = torch.zeros((1, 36), dtype=torch.float64, device=dev)
     with pytest.raises(InvalidArgument):
         s.captureGraph(d_vals.data_ptr(), d_B.data_ptr(), 1, d_B.data_ptr())
+
+
+def test_graph_replay_with_delays_on_large_fronts_interleaved_with_eager_work():
+    """Threshold-mode SQD KKT (fronts well above 64 rows delay pivots): every
+    replay is repaired.  Between replays the same plan runs an eager
+    factorize + solve (which lays the plan out again) and a repaired replay
+    reallocates per-front storage; the next replay must notice that the
+    recorded graph refers to moved buffers and record it again rather than
+    run on freed memory.  Inertia is Sylvester's (n2, n1, 0) for every
+    positive scaling of the matrix."""
+    K, n1, n2 = P.sqd_kkt(nh=16, m2=1200, seed=77)
+    n = K.shape[0]
+    base = csc_arrays(K)[3].copy()
+    dev = torch.device("cuda:0")
+    s = _solver(K, OrderingType.Metis, Precision.FP64)
+    d_vals = torch.from_numpy(base.copy()).to(dev)
+    d_B = torch.zeros((2, n), dtype=torch.float64, device=dev)
+    d_X = torch.zeros_like(d_B)
+    torch.cuda.synchronize()
+    s.captureGraph(d_vals.data_ptr(), d_B.data_ptr(), 2, d_X.data_ptr())
+    rng = np.random.default_rng(21)
+    for step, scale in enumerate([1.0, 0.7, 1.3, 0.9, 1.1]):
+        Ks = K * scale
+        B = rng.standard_normal((n, 2))
+        d_vals.copy_(torch.from_numpy(base * scale))
+        d_B.copy_(torch.from_numpy(np.ascontiguousarray(B.T)))
+        torch.cuda.synchronize()
+        info = s.replayGraph()
+        assert info.repaired == 1 and info.singular == 0, step
+        assert (info.n_pos, info.n_neg, info.n_zero) == (n2, n1, 0), step
+        X = d_X.cpu().numpy().T
+        assert P.backward_error(Ks, X, B) <= 1e-10, step
+        if step % 2 == 0:
+            # eager work on the same plan between replays
+            s.factorize(Ks)
+            inr = s.inertia()
+            assert (inr.n_pos, inr.n_neg, inr.n_zero) == (n2, n1, 0)
+            Xe = s.solve(B)
+            assert P.backward_error(Ks, Xe, B) <= 1e-10
+        torch.cuda.synchronize()
