@@ -22,9 +22,8 @@ PRODUCT = ROOT / "ma57_amd" / "libsymla_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE = ORACLE_DIR / "build" / "libsymla_oracle.so"
 
-HIP_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "plan.cpp", "capi.cpp", "symbolic.cpp", "cb_update.cpp"]
-# rocBLAS runs the deferred contribution-block SYRKs (plain library products).
-LINK_LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
+HIP_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "cb_syrk.hip", "plan.cpp", "capi.cpp", "symbolic.cpp"]
+LINK_LIBS: list[str] = []
 # Packed FP32 ops are disabled for every HIP translation unit.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -155,6 +155,13 @@ int symla_symbolic_get(const symla_symbolic* s, int* perm, int* etree_parent, in
   });
 }
 
+int symla_symbolic_colcounts(const symla_symbolic* s, int* col_count) {
+  return guarded([&] {
+    if (!s || (!col_count && s->r.n > 0)) throw std::invalid_argument("symla_symbolic_colcounts: null argument");
+    std::copy(s->r.colCount.begin(), s->r.colCount.end(), col_count);
+  });
+}
+
 void symla_symbolic_free(symla_symbolic* s) { delete s; }
 
 int symla_plan_create(const symla_symbolic* sym, const int64_t* a_colptr, const int* a_rowidx, symla_plan** out) {
@@ -519,18 +526,12 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
       dv.specStart = dsp.p;
       dv.sync = dsync.p;
       dv.err = derr.p;
-      // library handle and workspace of the deferred contribution-block update
-      struct BlasGuard {
-        void* h = nullptr;
-        ~BlasGuard() { blasDestroy(h); }
-      } blas;
+      // operand workspace of the deferred contribution-block update
       DBuf<double> dW;
       const std::vector<void*> Fv{dA.p};
       CbUpdate cb;
       if (!ls.cb.empty()) {
-        blas.h = blasCreate(st);
         dW.alloc(static_cast<size_t>(n - nE) * kCbChunk);
-        cb.blas = blas.h;
         cb.w = dW.p;
         cb.wElems = dW.n;
         cb.F = &Fv;

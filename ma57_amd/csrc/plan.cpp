@@ -169,7 +169,6 @@ Plan::~Plan() {
   for (void* p : dynAllocs_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
-  blasDestroy(blas_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -364,10 +363,8 @@ CbUpdate Plan::cbUpdate(const std::vector<CbTask>& tasks) {
   if (!tasks.empty()) {
     size_t need = 0;
     for (const CbTask& t : tasks) need = std::max(need, static_cast<size_t>(t.m - t.nE) * kCbChunk);
-    if (!blas_) blas_ = blasCreate(stream_);
     dcbW_.ensure(need);  // doubles: enough for either front type
   }
-  c.blas = blas_;
   c.w = dcbW_.p;
   c.wElems = dcbW_.n * (c.fp32 ? 2 : 1);
   return c;
@@ -628,7 +625,7 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
   }
   launchFrontFinish(dv.lists + ls.allOff, ls.allCount, fd, st);
   if (ls.cb.empty()) return;
-  if (!cb || !cb->blas || !cb->F) throw std::logic_error("symla: deferred contribution blocks without a library handle");
+  if (!cb || !cb->w || !cb->F) throw std::logic_error("symla: deferred contribution blocks without a workspace");
   const size_t esz = cb->fp32 ? sizeof(float) : sizeof(double);
   for (const CbTask& t : ls.cb) {
     const int N = t.m - t.nE;
@@ -636,12 +633,12 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
     char* F = static_cast<char*>((*cb->F)[t.s]);
     for (int q0 = 0; q0 < t.nE; q0 += kCbChunk) {
       const int kc = std::min(kCbChunk, t.nE - q0);
-      // one profiled launch per chunk: the operand kernel and the library SYRK
+      // one profiled launch per chunk: the operand kernel and the MFMA SYRK
       if (prof) prof->profBegin(Plan::kProfCb);
       launchFormW(fd, t.s, N, q0, kc, cb->w, st);
       const void* B = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(q0) * t.m);
       void* C = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(t.nE) * t.m);
-      blasSyrkxLower(cb->blas, cb->fp32, N, kc, cb->w, N, B, t.m, C, t.m);
+      launchCbSyrk(cb->fp32, N, kc, cb->w, N, B, t.m, C, t.m, st);
       if (prof) {
         prof->profEnd(Plan::kProfCb);
         prof->addCbFlops(static_cast<double>(kc) * N * (N + 1.0));

@@ -127,7 +127,7 @@ struct LargeIter {
 // kCbMinCols eligible columns and a contribution block of at least the
 // tuned row count (Tuning::cbMinRows) stops its panels' trailing updates at
 // column nE; after its last panel the block [nE, m)^2 takes one rank-nE
-// update, kCbChunk pivot columns per library SYRK.
+// update, kCbChunk pivot columns per MFMA SYRK launch (cb_syrk.hip).
 constexpr int kCbMinCols = 256;
 // levels with a front of at least this many 32-row own blocks gather the
 // ancestors' x once per front before the backward sweep
@@ -159,12 +159,10 @@ struct LargeDev {
   int* err = nullptr;
   bool frontMaxDone = false;  // the assembly already folded max |F| into origMaxBits
 };
-// What the deferred contribution-block updates run with: the library
-// handle (on the plan's stream), a workspace of at least kCbChunk times the
-// largest contribution-block order (elements of the front type), and the
-// fronts' device pointers by front id.
+// What the deferred contribution-block updates run with: a workspace of at
+// least kCbChunk times the largest contribution-block order (elements of the
+// front type) and the fronts' device pointers by front id.
 struct CbUpdate {
-  void* blas = nullptr;
   void* w = nullptr;
   size_t wElems = 0;
   const std::vector<void*>* F = nullptr;
@@ -180,11 +178,10 @@ class Plan;
 // with events; `cb` may be null only when no front defers its contribution block.
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
                    const int64_t* wsOff, hipStream_t st, const CbUpdate* cb, Plan* prof);
-// cb_update.cpp
-void* blasCreate(hipStream_t st);
-void blasDestroy(void* h);
-void blasSyrkxLower(void* h, bool fp32, int n, int k, const void* W, int ldw, const void* B, int ldb, void* C,
-                    int ldc);
+// cb_syrk.hip: C -= W B^T over the lower triangle of the N x N block C
+// (column-major; W, B: N x K) on the matrix cores, in the front precision
+void launchCbSyrk(bool fp32, int N, int K, const void* W, int ldw, const void* B, int ldb, void* C, int ldc,
+                  hipStream_t st);
 
 class Plan {
  public:
@@ -419,13 +416,12 @@ class Plan {
   LargeDev largeDev(const int* lists, const int* rowStart, const int* tileStart, const int* tileTp,
                     const int* mwStart, const int* specStart);
   // deferred contribution-block updates: per-front trailing-update cut
-  // (host copy from the current sizes and tuning), library handle, workspace
+  // (host copy from the current sizes and tuning), operand workspace
   std::vector<int> hcut_;
   DBuf<int> dschurCut_;
-  void* blas_ = nullptr;
   DBuf<double> dcbW_;
   void computeCut();
-  // handle and workspace for the contribution blocks of `fronts` (outside any capture)
+  // workspace for the contribution blocks of `fronts` (outside any capture)
   CbUpdate cbUpdate(const std::vector<CbTask>& tasks);
   DBuf<int64_t> dwsOff_;
   std::vector<int64_t> hwsOff_;

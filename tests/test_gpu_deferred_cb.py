@@ -36,7 +36,8 @@ def _front(n, ne, seed, pairs=0, delays=0):
     return A
 
 
-@pytest.mark.parametrize("n,ne,pairs,delays", [(600, 300, 3, 0), (3000, 1200, 8, 0), (2500, 900, 4, 7)])
+@pytest.mark.parametrize("n,ne,pairs,delays", [(600, 300, 3, 0), (3000, 1200, 8, 0), (2500, 900, 4, 7),
+                                                 (5000, 2600, 6, 0)])
 def test_dense_deferred_cb_matches_panel_updates(n, ne, pairs, delays):
     A = _front(n, ne, n + ne, pairs, delays)
     g = dense_factor(A, n_eligible=ne, deferred_cb=True)
