@@ -260,14 +260,13 @@ std::vector<int> minimumDegreeOrder(const Graph& g) {
 }
 
 // ---------------------------------------------------------------------------
-// Nested dissection by recursive level-structure bisection.
+// Nested dissection.
 //
-// For each connected piece: find a pseudo-peripheral vertex by repeated BFS,
-// build the level structure, and take as separator the smallest level whose
-// two sides are both at most 65% of the piece (thinned to the vertices that
-// actually touch the next level).  Pieces at or below the leaf size are
-// ordered by minimum degree on the induced subgraph.  Order = [side A, side
-// B, separator].
+// Each connected piece above the leaf size is split by a level-structure
+// vertex separator (levelSplit below) into two sides and the separator; the
+// sides are dissected in turn and the separator is ordered after them.  Pieces at or
+// below the leaf size, and separators, are ordered by minimum degree on their
+// induced subgraph.  Order = [side A, side B, separator].
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -275,13 +274,14 @@ struct NDState {
   const Graph& g;
   std::vector<int>& out;
   std::vector<int> where;   // piece id of each vertex (-1 = ordered)
-  std::vector<int> level;   // BFS level scratch
+  std::vector<int> mark;    // BFS scratch
   std::vector<int> local;   // global -> local index scratch
   int nextPiece = 1;
   NDState(const Graph& gg, std::vector<int>& o)
-      : g(gg), out(o), where(gg.n, 0), level(gg.n, -1), local(gg.n, -1) {}
+      : g(gg), out(o), where(gg.n, 0), mark(gg.n, -1), local(gg.n, -1) {}
 
-  void orderLeaf(const std::vector<int>& V) {
+  // Subgraph induced by the vertices V of one piece, in V's local numbering.
+  Graph induced(const std::vector<int>& V) {
     Graph sub;
     sub.n = static_cast<int>(V.size());
     for (int t = 0; t < sub.n; ++t) local[V[t]] = t;
@@ -295,31 +295,135 @@ struct NDState {
       }
       sub.ptr[t + 1] = static_cast<int64_t>(sub.adj.size());
     }
-    std::vector<int> o = minimumDegreeOrder(sub);
-    for (int t : o) out.push_back(V[t]);
-    for (int v : V) {
-      local[v] = -1;
-      where[v] = -1;
-    }
+    for (int v : V) local[v] = -1;
+    return sub;
   }
 
-  // BFS inside piece `pid` from `src`; returns vertices in BFS order and sets level[].
-  int bfs(int src, int pid, std::vector<int>& q) {
-    q.clear();
-    q.push_back(src);
-    level[src] = 0;
-    int maxLev = 0;
-    for (size_t h = 0; h < q.size(); ++h) {
-      const int v = q[h];
-      for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
-        const int u = g.adj[t];
-        if (where[u] != pid || level[u] >= 0) continue;
-        level[u] = level[v] + 1;
-        maxLev = std::max(maxLev, level[u]);
-        q.push_back(u);
+  void orderLeaf(const std::vector<int>& V) {
+    std::vector<int> o = minimumDegreeOrder(induced(V));
+    for (int t : o) out.push_back(V[t]);
+    for (int v : V) where[v] = -1;
+  }
+
+  // Connected components of piece `pid` among V (breadth-first).
+  std::vector<std::vector<int>> components(const std::vector<int>& V, int pid) {
+    std::vector<std::vector<int>> comps;
+    for (int v : V) {
+      if (mark[v] >= 0) continue;
+      std::vector<int> q{v};
+      mark[v] = 1;
+      for (size_t h = 0; h < q.size(); ++h) {
+        const int x = q[h];
+        for (int64_t t = g.ptr[x]; t < g.ptr[x + 1]; ++t) {
+          const int u = g.adj[t];
+          if (where[u] == pid && mark[u] < 0) {
+            mark[u] = 1;
+            q.push_back(u);
+          }
+        }
+      }
+      comps.push_back(std::move(q));
+    }
+    for (int v : V) mark[v] = -1;
+    return comps;
+  }
+
+  // Level-structure split of a connected local graph.  Breadth-first levels
+  // from a root; the separator is the smallest level leaving both sides at
+  // most 70% of the piece, thinned to the vertices that touch the next
+  // level.  The root matters: from a corner of a box the levels are shells,
+  // from the middle of a face they become planes once they span the face.
+  // So several roots are tried -- a pseudo-peripheral vertex and vertices
+  // spread over the farthest level seen from it and from its own farthest
+  // vertex -- and the smallest separator wins (ties: the better balance).
+  // part[v]: 0 / 1 = the sides, 2 = separator; empty when no level splits.
+  static std::vector<int> levelSplit(const Graph& s) {
+    const int n = s.n;
+    std::vector<int> lev(n, -1), q;
+    q.reserve(n);
+    auto bfs = [&](int src) {
+      std::fill(lev.begin(), lev.end(), -1);
+      q.clear();
+      q.push_back(src);
+      lev[src] = 0;
+      for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h];
+        for (int64_t t = s.ptr[v]; t < s.ptr[v + 1]; ++t)
+          if (lev[s.adj[t]] < 0) {
+            lev[s.adj[t]] = lev[v] + 1;
+            q.push_back(s.adj[t]);
+          }
+      }
+      return lev[q.back()];
+    };
+    // vertices of the farthest level of the last search, up to k spread evenly
+    auto farLevel = [&](int k) {
+      const int ecc = lev[q.back()];
+      size_t first = q.size();
+      while (first > 0 && lev[q[first - 1]] == ecc) --first;
+      std::vector<int> out;
+      const size_t cnt = q.size() - first;
+      for (int i = 0; i < k && static_cast<size_t>(i) < cnt; ++i) out.push_back(q[first + (cnt * i) / k]);
+      return out;
+    };
+    constexpr int kRoots = 8;
+    std::vector<int> roots;
+    bfs(0);
+    const int p1 = farLevel(1)[0];
+    roots.push_back(p1);
+    bfs(p1);
+    const std::vector<int> f1 = farLevel(kRoots);
+    roots.insert(roots.end(), f1.begin(), f1.end());
+    bfs(f1[0]);
+    const std::vector<int> f2 = farLevel(kRoots);
+    roots.insert(roots.end(), f2.begin(), f2.end());
+
+    int bestRoot = -1, bestL = -1, bestS = 0, bestBig = 0;
+    int fbRoot = -1, fbL = -1, fbBig = n + 1;
+    std::vector<int> cnt;
+    for (int r : roots) {
+      const int ecc = bfs(r);
+      cnt.assign(ecc + 2, 0);
+      for (int v = 0; v < n; ++v) ++cnt[lev[v]];
+      int before = 0;
+      for (int L = 0; L <= ecc; ++L) {
+        const int a = before, sz = cnt[L], b = n - before - sz;
+        before += sz;
+        if (a == 0 || b == 0) continue;
+        const int big = std::max(a, b);
+        if (big < fbBig) {
+          fbBig = big;
+          fbL = L;
+          fbRoot = r;
+        }
+        if (10 * static_cast<int64_t>(big) > 7 * static_cast<int64_t>(n)) continue;
+        if (bestL < 0 || sz < bestS || (sz == bestS && big < bestBig)) {
+          bestL = L;
+          bestS = sz;
+          bestBig = big;
+          bestRoot = r;
+        }
       }
     }
-    return maxLev;
+    if (bestL < 0) {
+      bestL = fbL;
+      bestRoot = fbRoot;
+    }
+    if (bestL < 0) return {};
+    bfs(bestRoot);
+    std::vector<int> part(n);
+    for (int v = 0; v < n; ++v) {
+      if (lev[v] < bestL) {
+        part[v] = 0;
+      } else if (lev[v] > bestL) {
+        part[v] = 1;
+      } else {
+        bool touchesNext = false;
+        for (int64_t t = s.ptr[v]; t < s.ptr[v + 1] && !touchesNext; ++t) touchesNext = lev[s.adj[t]] == bestL + 1;
+        part[v] = touchesNext ? 2 : 0;
+      }
+    }
+    return part;
   }
 
   void dissect(std::vector<int> V) {
@@ -330,16 +434,7 @@ struct NDState {
       return;
     }
     const int pid = where[V[0]];
-    // connected components of the piece
-    std::vector<int> q;
-    std::vector<std::vector<int>> comps;
-    for (int v : V) level[v] = -1;
-    for (int v : V) {
-      if (level[v] >= 0) continue;
-      bfs(v, pid, q);
-      comps.push_back(q);
-    }
-    for (int v : V) level[v] = -1;
+    std::vector<std::vector<int>> comps = components(V, pid);
     if (comps.size() > 1) {
       for (auto& c : comps) {
         const int np = nextPiece++;
@@ -348,92 +443,24 @@ struct NDState {
       }
       return;
     }
-    // pseudo-peripheral vertex
-    int src = V[0];
-    int ecc = bfs(src, pid, q);
-    for (int iter = 0; iter < 8; ++iter) {
-      // farthest vertex of minimum degree in the last level
-      int best = -1, bestDeg = 1 << 30;
-      for (size_t t = q.size(); t-- > 0;) {
-        const int v = q[t];
-        if (level[v] != ecc) break;
-        int d = 0;
-        for (int64_t s = g.ptr[v]; s < g.ptr[v + 1]; ++s) d += where[g.adj[s]] == pid;
-        if (d < bestDeg) {
-          bestDeg = d;
-          best = v;
-        }
-      }
-      for (int v : q) level[v] = -1;
-      const int e2 = bfs(best, pid, q);
-      if (e2 <= ecc) {
-        for (int v : q) level[v] = -1;
-        ecc = bfs(best, pid, q);
-        src = best;
-        break;
-      }
-      ecc = e2;
-      src = best;
-    }
-    const int total = static_cast<int>(q.size());
-    std::vector<int> levCount(ecc + 2, 0);
-    for (int v : q) ++levCount[level[v]];
-    // choose the separator level: the smallest level leaving both sides at
-    // most 70% of the piece; if none does, the most balanced level
-    int bestL = -1, bestS = 0, bestBig = 0;
-    int fallbackL = -1, fallbackBig = total + 1;
-    int before = 0;
-    for (int L = 0; L <= ecc; ++L) {
-      const int a = before, s = levCount[L], b = total - before - s;
-      before += s;
-      if (a == 0 || b == 0) continue;
-      const int big = std::max(a, b);
-      if (big < fallbackBig) {
-        fallbackBig = big;
-        fallbackL = L;
-      }
-      if (10 * static_cast<int64_t>(big) > 7 * static_cast<int64_t>(total)) continue;
-      if (bestL < 0 || s < bestS || (s == bestS && big < bestBig)) {
-        bestL = L;
-        bestS = s;
-        bestBig = big;
-      }
-    }
-    if (bestL < 0) bestL = fallbackL;
-    if (bestL < 0) {
-      for (int v : q) level[v] = -1;
+    const Graph sub = induced(V);
+    std::vector<int> part = levelSplit(sub);
+    if (part.empty()) {
       orderLeaf(V);
       return;
     }
     std::vector<int> A, B, S;
-    for (int v : q) {
-      if (level[v] < bestL)
-        A.push_back(v);
-      else if (level[v] > bestL)
-        B.push_back(v);
-      else {
-        bool touchesNext = false;
-        for (int64_t t = g.ptr[v]; t < g.ptr[v + 1]; ++t) {
-          const int u = g.adj[t];
-          if (where[u] == pid && level[u] == bestL + 1) {
-            touchesNext = true;
-            break;
-          }
-        }
-        if (touchesNext)
-          S.push_back(v);
-        else
-          A.push_back(v);
-      }
+    for (size_t t = 0; t < V.size(); ++t) (part[t] == 0 ? A : part[t] == 1 ? B : S).push_back(V[t]);
+    if (A.empty() || B.empty()) {  // no useful split: one leaf
+      orderLeaf(V);
+      return;
     }
-    for (int v : q) level[v] = -1;
     const int pa = nextPiece++, pb = nextPiece++, ps = nextPiece++;
     for (int v : A) where[v] = pa;
     for (int v : B) where[v] = pb;
     for (int v : S) where[v] = ps;
     dissect(std::move(A));
     dissect(std::move(B));
-    // separator last, ordered by minimum degree on its induced subgraph
     if (!S.empty()) orderLeaf(S);
   }
 };

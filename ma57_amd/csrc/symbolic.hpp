@@ -55,9 +55,10 @@ Graph buildSymmetricGraph(int n, const int64_t* colptr, const int* rowidx);
 // vertex).
 std::vector<int> minimumDegreeOrder(const Graph& g);
 
-// Nested dissection: multilevel bisection for vertex separators, minimum
-// degree on small leaves.  Returns perm (position -> vertex).
+// Nested dissection: level-structure vertex separators from several
+// breadth-first roots, minimum degree on small leaves.  Returns perm (position -> vertex).
 std::vector<int> nestedDissectionOrder(const Graph& g);
+
 
 // Full pipeline.  `userPerm` is used when ordering == User.
 SymbolicResult analyze(int n, const int64_t* colptr, const int* rowidx, Ordering ordering,
