@@ -36,7 +36,7 @@ class DenseResult(C.Structure):
 
 class MfOpts(C.Structure):
     _fields_ = [("parallel", C.c_int), ("num_threads", C.c_int), ("task_cutoff", C.c_longlong),
-                ("static_pivoting", C.c_int), ("dense", DenseOpts), ("stat", StaticOpts)]
+                ("static_pivoting", C.c_int), ("dense", DenseOpts), ("stat", StaticOpts), ("sn_limit", C.c_int)]
 
 
 class MfInfo(C.Structure):
@@ -236,12 +236,13 @@ class Factor:
 
 
 def mf_factorize(sym: Symbolic, A, parallel=False, threads=0, static_pivoting=False, expected_sign_final=None,
-                 delta=0.0, static_rel_floor=1e-8, static_abs_floor=1e-300):
-    """MultifrontalFactorizer::factorize (multifrontal.hpp:299-805)."""
+                 delta=0.0, static_rel_floor=1e-8, static_abs_floor=1e-300, sn_limit=0):
+    """MultifrontalFactorizer::factorize (multifrontal.hpp:299-805).  sn_limit > 0 factors only the first
+    sn_limit supernodes in postorder (complete subtrees): a CPU timing sample, not a factorization."""
     lib = load()
     cp, ri, vals = _csc(A)
     o = MfOpts(int(parallel), threads, -1, int(static_pivoting), dense_opts(),
-               StaticOpts(delta, static_rel_floor, static_abs_floor, 64))
+               StaticOpts(delta, static_rel_floor, static_abs_floor, 64), int(sn_limit))
     sg = None if expected_sign_final is None else np.ascontiguousarray(expected_sign_final, np.int32)
     h = lib.oracle_mf_factorize(*sym.args(), _p(cp, i64p), _p(ri, i32p), _p(vals, f64p),
                                 _p(sg, i32p) if sg is not None else None, C.byref(o))

@@ -620,6 +620,7 @@ struct MfOpts {
   DenseOpts dense;
   StaticOpts stat;
   std::vector<int> expectedSign;  // final order
+  int snLimit = 0;                // > 0: supernodes [0, snLimit) only (timing sample)
 };
 
 // parallel::runTaskDag (task_graph.hpp:48-89)
@@ -705,11 +706,14 @@ static NumericFactor mfFactorize(const Symbolic& sf, const Csc& A, const MfOpts&
   nf.n = n;
   if (n == 0) return nf;
   const auto& supernodes = sf.supernodes;
-  const int numSN = static_cast<int>(supernodes.size());
+  const int numSNAll = static_cast<int>(supernodes.size());
+  // a timing sample stops after the first snLimit supernodes in postorder;
+  // every loop below runs over [0, numSN)
+  const int numSN = (mfOptions.snLimit > 0 && mfOptions.snLimit < numSNAll) ? mfOptions.snLimit : numSNAll;
 
   // supernode tree (:312-332)
   std::vector<int> colToSupernode(n, -1);
-  for (int si = 0; si < numSN; ++si)
+  for (int si = 0; si < numSNAll; ++si)
     for (int c = supernodes[si].firstCol; c < supernodes[si].firstCol + supernodes[si].ncols; ++c)
       colToSupernode[c] = si;
   std::vector<int> parentSN(numSN, -1);
@@ -720,6 +724,7 @@ static NumericFactor mfFactorize(const Symbolic& sf, const Csc& A, const MfOpts&
     const int p = sf.etreeParent[lastCol];
     if (p != -1) {
       const int psn = colToSupernode[p];
+      if (psn >= numSN) continue;  // outside a timing sample: the front is a sample root
       parentSN[si] = psn;
       childrenSN[psn].push_back(si);
     }
@@ -1376,6 +1381,7 @@ static MfOpts toMf(const oracle_mf_opts* o) {
     m.static_pivoting = o->static_pivoting != 0;
     m.dense = toDense(&o->dense);
     m.stat = toStatic(&o->stat);
+    m.snLimit = o->sn_limit;
   }
   return m;
 }

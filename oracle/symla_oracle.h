@@ -70,6 +70,10 @@ typedef struct {
   int static_pivoting; /* use factorStatic per front */
   oracle_dense_opts dense;
   oracle_static_opts stat;
+  /* > 0: factor only supernodes [0, sn_limit) -- in postorder a union of
+   * complete subtrees; the contribution blocks of their topmost fronts are
+   * left unconsumed.  A timing sample (benchmarks), not a factorization. */
+  int sn_limit;
 } oracle_mf_opts;
 
 /* Symbolic input (the caller's analysis): perm[n], etree_parent[n] (final

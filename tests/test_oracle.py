@@ -294,6 +294,22 @@ def test_3d_laplacian_closed_form_nd(oracle):
     assert f.inertia() == P.closed_form_inertia(P.laplace_eigs_3d(12, 12, 12), 0.5)[:3]
 
 
+
+@pytest.mark.parametrize("parallel", [False, True])
+def test_timing_sample_fronts_equal_full_factorization(parallel, oracle):
+    """The CPU baseline's timing sample (the first K supernodes in postorder, complete subtrees) factors those
+    fronts exactly as the full factorization does: same rows, pivots, D and L."""
+    A = P.laplace3d(12, 12, 12, 0.5)
+    sym = oracle.Symbolic.from_product(_analyze_sparse(A, 1))
+    full = oracle.mf_factorize(sym, A)
+    K = len(sym.first) // 3
+    samp = oracle.mf_factorize(sym, A, parallel=parallel, threads=4, sn_limit=K)
+    assert samp.info().nsn == K
+    for si in range(K):
+        a, b = full.front(si), samp.front(si)
+        assert np.array_equal(a["rows"], b["rows"]) and a["pivots"] == b["pivots"]
+        assert np.array_equal(a["d_diag"], b["d_diag"]) and np.array_equal(a["L"], b["L"])
+
 def _kkt(n1, n2, seed):
     E = P.random_sparse_spd(n1, 4, seed)
     F = P.random_sparse_spd(n2, 4, seed + 17)
