@@ -4,15 +4,24 @@
 // Same class, method names, argument meaning and exceptions as the
 // reference facade (include/symla/solver.hpp:29-418 of the symla
 // repository): analyzePattern / factorize / compute / refactorize / solve /
-// solveWithRefinement / inertia / isSingular / singularColumns, the
-// Mode::StaticRegularized configuration and diagnostics.  Eigen is not a
-// dependency: matrices are the small CSC / column-major types below; when
-// Eigen is available, overloads taking Eigen::SparseMatrix<double> and
-// Eigen::MatrixXd are added at the end of this header.
+// solveWithRefinement / inertia / isSingular / singularColumns /
+// symbolicFactor / numericFactor, the Mode::StaticRegularized configuration
+// and diagnostics.
+//
+// Matrix types.  When <Eigen/Sparse> is on the include path (and
+// SYMLA_NO_EIGEN is not defined), SymLDLT<double>::SparseMatrix is
+// Eigen::SparseMatrix<double, Eigen::ColMajor, int> and DenseMatrix is
+// Eigen::MatrixXd, exactly as in the reference (solver.hpp:33-34), so a
+// reference caller compiles unchanged.  Without Eigen they are the small CSC
+// / column-major types below (symla::CscMatrix, symla::ColMatrix), which
+// offer the subset of the Eigen interface the facade and its callers use.
+// Either way the matrices are only read to hand plain pointers to the
+// C-ABI: there is one computation path.
 //
 // Link with -lsymla_hip (ma57_amd/libsymla_hip.so).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -20,6 +29,14 @@
 #include <vector>
 
 #include "../symla_hip.h"
+
+#if !defined(SYMLA_NO_EIGEN) && defined(__has_include)
+#if __has_include(<Eigen/Sparse>)
+#define SYMLA_FACADE_EIGEN 1
+#include <Eigen/Dense>
+#include <Eigen/Sparse>
+#endif
+#endif
 
 namespace symla {
 
@@ -56,20 +73,21 @@ struct RefinementResult {  // refine.hpp:61-65, plus the symmetric-norm backward
 // Compressed sparse column matrix (stands in for Eigen::SparseMatrix<double,
 // ColMajor, int>).  Either triangle or both may be stored; the lower
 // triangle is authoritative (multifrontal.hpp:190-198).
-struct SparseMatrix {
+struct CscMatrix {
   int nrows = 0, ncols = 0;
   std::vector<int64_t> colptr{0};
   std::vector<int> rowidx;
   std::vector<double> values;
-  SparseMatrix() = default;
-  SparseMatrix(int r, int c) : nrows(r), ncols(c), colptr(c + 1, 0) {}
+  CscMatrix() = default;
+  CscMatrix(int r, int c) : nrows(r), ncols(c), colptr(c + 1, 0) {}
   int rows() const { return nrows; }
   int cols() const { return ncols; }
+  int outerSize() const { return ncols; }
   int64_t nonZeros() const { return colptr.empty() ? 0 : colptr.back(); }
   // (row, col, value) triplets, duplicates kept in the order given
-  static SparseMatrix fromTriplets(int n, const std::vector<int>& r, const std::vector<int>& c,
-                                   const std::vector<double>& v) {
-    SparseMatrix A(n, n);
+  static CscMatrix fromTriplets(int n, const std::vector<int>& r, const std::vector<int>& c,
+                                const std::vector<double>& v) {
+    CscMatrix A(n, n);
     for (int col : c) ++A.colptr[col + 1];
     for (int j = 0; j < n; ++j) A.colptr[j + 1] += A.colptr[j];
     A.rowidx.resize(r.size());
@@ -84,15 +102,128 @@ struct SparseMatrix {
 };
 
 // Column-major dense matrix (stands in for Eigen::MatrixXd).
-struct DenseMatrix {
+struct ColMatrix {
   int nrows = 0, ncols = 0;
-  std::vector<double> data;
-  DenseMatrix() = default;
-  DenseMatrix(int r, int c) : nrows(r), ncols(c), data(static_cast<size_t>(r) * c, 0.0) {}
+  std::vector<double> values;
+  ColMatrix() = default;
+  ColMatrix(int r, int c) : nrows(r), ncols(c), values(static_cast<size_t>(r) * c, 0.0) {}
   int rows() const { return nrows; }
   int cols() const { return ncols; }
-  double& operator()(int i, int j) { return data[i + static_cast<size_t>(j) * nrows]; }
-  double operator()(int i, int j) const { return data[i + static_cast<size_t>(j) * nrows]; }
+  double* data() { return values.data(); }
+  const double* data() const { return values.data(); }
+  void resize(int r, int c) {
+    nrows = r;
+    ncols = c;
+    values.assign(static_cast<size_t>(r) * c, 0.0);
+  }
+  double& operator()(int i, int j) { return values[i + static_cast<size_t>(j) * nrows]; }
+  double operator()(int i, int j) const { return values[i + static_cast<size_t>(j) * nrows]; }
+};
+
+// Dense vector (stands in for Eigen::VectorXi / VectorXd).
+template <typename T>
+struct ColVector {
+  std::vector<T> values;
+  ColVector() = default;
+  explicit ColVector(int n) : values(n, T(0)) {}
+  int size() const { return static_cast<int>(values.size()); }
+  void resize(int n) { values.assign(n, T(0)); }
+  T* data() { return values.data(); }
+  const T* data() const { return values.data(); }
+  T& operator[](int i) { return values[i]; }
+  T operator[](int i) const { return values[i]; }
+  T& operator()(int i) { return values[i]; }
+  T operator()(int i) const { return values[i]; }
+};
+
+#ifdef SYMLA_FACADE_EIGEN
+using SparseMatrix = Eigen::SparseMatrix<double, Eigen::ColMajor, int>;
+using DenseMatrix = Eigen::MatrixXd;
+using IndexVector = Eigen::VectorXi;
+using RealVector = Eigen::VectorXd;
+#else
+using SparseMatrix = CscMatrix;
+using DenseMatrix = ColMatrix;
+using IndexVector = ColVector<int>;
+using RealVector = ColVector<double>;
+#endif
+
+// ---- symbolic and numeric factor views (symbolic.hpp:130-159,
+// elimination_tree.hpp:50-53, dense_kernel.hpp:137-207,
+// multifrontal.hpp:118-186) ----
+
+struct Supernode {
+  int firstCol = 0;
+  int ncols = 0;
+  std::vector<int> rowPattern;  // sorted; the first ncols entries are the own columns
+  std::int64_t storageNnz() const {
+    const std::int64_t m = static_cast<std::int64_t>(rowPattern.size());
+    const std::int64_t k = ncols;
+    return k * m - k * (k - 1) / 2;
+  }
+};
+
+struct EliminationTree {
+  IndexVector parent;    // parent[j] in final order, -1 at a root
+  IndexVector colCount;  // nnz(L(:, j)) including the diagonal
+  int n = 0;
+};
+
+// perm[k] = original index of the k-th pivot.  fundamentalSupernodes is
+// left empty: the analysis keeps only the amalgamated partition.
+struct SymbolicFactor {
+  IndexVector perm;
+  EliminationTree etree;
+  std::vector<Supernode> supernodes;
+  std::vector<Supernode> fundamentalSupernodes;
+};
+
+template <typename Scalar>
+struct BlockDiagonalD {
+  RealVector diag;     // D(k, k)
+  RealVector offdiag;  // D(k+1, k) for a 2x2 block starting at k, else 0
+  int size() const { return static_cast<int>(diag.size()); }
+  int rows() const { return size(); }
+  int cols() const { return size(); }
+  Scalar operator()(int i, int j) const {
+    if (i == j) return diag(i);
+    if (i == j + 1) return offdiag(j);
+    if (j == i + 1) return offdiag(i);
+    return Scalar(0);
+  }
+};
+
+enum class PivotKind { OneByOne, TwoByTwo };
+struct PivotBlock {
+  int start;
+  PivotKind kind;
+};
+
+// One front: rowIndices (final order, size m), nPivots, L (m x nPivots; the
+// (k+1, k) slot of a 2x2 block holds d21, as in the reference), D, the pivot
+// blocks and Ltop (L's leading nPivots x nPivots block with the d21 slots
+// zeroed).
+struct SupernodeFactor {
+  std::vector<int> rowIndices;
+  int nPivots = 0;
+  DenseMatrix L;
+  BlockDiagonalD<double> D;
+  std::vector<PivotBlock> pivotBlocks;
+  DenseMatrix Ltop;
+};
+
+struct NumericFactor {
+  std::vector<SupernodeFactor> fronts;  // fronts[si]: supernode si (postorder)
+  IndexVector perm;                     // SymbolicFactor::perm
+  Inertia inertia;
+  bool singular = false;
+  std::vector<int> singularCols;        // final-order indices
+  int n = 0;
+  double totalPerturbation = 0.0;
+  int numPerturbed = 0;
+  std::vector<int> parentSN;
+  std::vector<std::vector<int>> childrenSN;
+  std::vector<int> rootsSN;
 };
 
 namespace detail {
@@ -102,6 +233,45 @@ inline void check(int rc, const char* msg) {
   if (rc == SYMLA_ERR_INVALID) throw std::invalid_argument(m);
   if (rc == SYMLA_ERR_LOGIC) throw std::logic_error(m);
   throw std::runtime_error(m);
+}
+
+// Plain CSC arrays of a sparse matrix for the C-ABI (64-bit column starts).
+struct CscArrays {
+  int n = 0;
+  std::vector<int64_t> colptr;
+  const int* rowidx = nullptr;
+  const double* values = nullptr;
+  int64_t nnz = 0;
+#ifdef SYMLA_FACADE_EIGEN
+  SparseMatrix compressed;  // a compressed copy when the caller's matrix is not
+#endif
+};
+inline void cscArrays(const CscMatrix& A, CscArrays& out) {
+  out.n = A.cols();
+  out.colptr = A.colptr;
+  out.rowidx = A.rowidx.data();
+  out.values = A.values.data();
+  out.nnz = A.nonZeros();
+}
+#ifdef SYMLA_FACADE_EIGEN
+inline void cscArrays(const SparseMatrix& A, CscArrays& out) {
+  const SparseMatrix* src = &A;
+  if (!A.isCompressed()) {
+    out.compressed = A;
+    out.compressed.makeCompressed();
+    src = &out.compressed;
+  }
+  out.n = static_cast<int>(src->cols());
+  out.colptr.assign(src->outerIndexPtr(), src->outerIndexPtr() + src->cols() + 1);
+  out.rowidx = src->innerIndexPtr();
+  out.values = src->valuePtr();
+  out.nnz = src->nonZeros();
+}
+#endif
+template <typename V>
+inline void toVector(const std::vector<int>& in, V& out) {
+  out.resize(static_cast<int>(in.size()));
+  for (size_t i = 0; i < in.size(); ++i) out[static_cast<int>(i)] = in[i];
 }
 }  // namespace detail
 
@@ -162,24 +332,30 @@ class SymLDLT<double> {
   void analyzePattern(const SparseMatrix& A, const SymbolicFactorOptions& options = SymbolicFactorOptions()) {
     if (A.rows() != A.cols()) throw std::invalid_argument("symla::computeOrdering: matrix must be square");
     release();
+    detail::CscArrays a;
+    detail::cscArrays(A, a);
     symla_symbolic* sym = nullptr;
     int ord = SYMLA_ORDER_AMD;
     if (ordering_ == OrderingType::Metis) ord = SYMLA_ORDER_METIS;
     if (ordering_ == OrderingType::Natural) ord = SYMLA_ORDER_NATURAL;
-    detail::check(symla_analyze(A.rows(), A.colptr.data(), A.rowidx.data(), ord, nullptr, options.max_relax_size,
+    detail::check(symla_analyze(a.n, a.colptr.data(), a.rowidx, ord, nullptr, options.max_relax_size,
                                 options.max_relax_fill_fraction, &sym),
                   symla_last_error());
     sym_ = sym;
-    detail::check(symla_plan_create(sym_, A.colptr.data(), A.rowidx.data(), &plan_), symla_last_error());
-    n_ = A.rows();
-    nnz_ = A.nonZeros();
+    detail::check(symla_plan_create(sym_, a.colptr.data(), a.rowidx, &plan_), symla_last_error());
+    n_ = a.n;
+    nnz_ = a.nnz;
     pattern_analyzed_ = true;
     factorized_ = false;
+    symbolic_valid_ = false;
+    numeric_valid_ = false;
   }
 
   void factorize(const SparseMatrix& A) {
     if (!pattern_analyzed_) throw std::logic_error("symla::SymLDLT::factorize: analyzePattern must be called first");
-    if (A.rows() != n_ || A.nonZeros() != nnz_)
+    detail::CscArrays a;
+    detail::cscArrays(A, a);
+    if (a.n != n_ || A.rows() != n_ || a.nnz != nnz_)
       throw std::invalid_argument("symla::SymLDLT::factorize: matrix pattern differs from the analyzed pattern");
     symla_factor_opts o{};
     o.mode = mode_ == Mode::StaticRegularized ? SYMLA_MODE_STATIC : SYMLA_MODE_THRESHOLD;
@@ -198,8 +374,9 @@ class SymLDLT<double> {
       if (use_kkt_blocks_ || use_explicit_signs_) o.expected_sign = signs.data();
     }
     factorized_ = false;
+    numeric_valid_ = false;
     symla_factor_info info{};
-    detail::check(symla_factorize(plan_, A.values.data(), &o, &info), symla_plan_last_error(plan_));
+    detail::check(symla_factorize(plan_, a.values, &o, &info), symla_plan_last_error(plan_));
     last_info_ = info;
     inertia_ = Inertia{info.n_pos, info.n_neg, info.n_zero};
     singular_ = info.singular != 0;
@@ -222,7 +399,8 @@ class SymLDLT<double> {
       throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
     DenseMatrix X(B.rows(), B.cols());
     if (B.cols() == 0 || n_ == 0) return X;
-    detail::check(symla_solve(plan_, B.data.data(), n_, B.cols(), X.data.data(), n_), symla_plan_last_error(plan_));
+    detail::check(symla_solve(plan_, B.data(), n_, static_cast<int>(B.cols()), X.data(), n_),
+                  symla_plan_last_error(plan_));
     return X;
   }
 
@@ -242,10 +420,14 @@ class SymLDLT<double> {
     RefinedSolve out;
     out.x = DenseMatrix(B.rows(), B.cols());
     if (B.cols() == 0 || n_ == 0) return out;
+    detail::CscArrays a;
+    detail::cscArrays(A, a);
+    if (a.nnz != nnz_)
+      throw std::invalid_argument("symla::SymLDLT::solveWithRefinement: matrix pattern differs from the analyzed one");
     symla_refine_opts ro{options.max_iterations, options.tolerance};
     symla_refine_info ri{};
-    detail::check(symla_solve_refined(plan_, A.values.data(), B.data.data(), n_, B.cols(), out.x.data.data(), n_, &ro,
-                                      &ri),
+    detail::check(symla_solve_refined(plan_, a.values, B.data(), n_, static_cast<int>(B.cols()), out.x.data(), n_,
+                                      &ro, &ri),
                   symla_plan_last_error(plan_));
     out.diagnostics = RefinementResult{ri.iterations, ri.initial_relative_residual, ri.final_relative_residual,
                                        ri.backward_error};
@@ -262,6 +444,21 @@ class SymLDLT<double> {
     std::vector<int> out(cnt > 0 ? cnt : 0);
     if (cnt > 0) symla_singular_cols(plan_, out.data(), cnt);
     return out;
+  }
+
+  // solver.hpp:261-262.  The symbolic factor is copied from the analysis on
+  // first use; the numeric factor is copied from the device on first use
+  // after each factorization (a test and inspection view: it moves every
+  // front's L and D to the host).
+  const SymbolicFactor& symbolicFactor() const {
+    if (!pattern_analyzed_) throw std::logic_error("symla::SymLDLT::symbolicFactor: analyzePattern must be called first");
+    if (!symbolic_valid_) loadSymbolic();
+    return symbolic_;
+  }
+  const NumericFactor& numericFactor() const {
+    if (!factorized_) throw std::logic_error("symla::SymLDLT::numericFactor: factorize() must be called first");
+    if (!numeric_valid_) loadNumeric();
+    return numeric_;
   }
 
   // Escape hatches for device-resident inputs (no reference counterpart).
@@ -290,12 +487,106 @@ class SymLDLT<double> {
   double regularization_growth_ = 10.0;
   double static_pivot_relative_floor_ = 1e-8;
   double static_pivot_absolute_floor_ = 1e-300;
+  mutable bool symbolic_valid_ = false;
+  mutable bool numeric_valid_ = false;
+  mutable SymbolicFactor symbolic_;
+  mutable NumericFactor numeric_;
 
   void release() {
     if (plan_) symla_plan_destroy(plan_);
     if (sym_) symla_symbolic_free(sym_);
     plan_ = nullptr;
     sym_ = nullptr;
+    symbolic_valid_ = false;
+    numeric_valid_ = false;
+  }
+
+  void loadSymbolic() const {
+    symla_symbolic_info_t info{};
+    detail::check(symla_symbolic_info(sym_, &info), symla_last_error());
+    std::vector<int> perm(info.n), parent(info.n), counts(info.n), first(info.nsn), ncols(info.nsn);
+    std::vector<int64_t> rowptr(info.nsn + 1);
+    std::vector<int> rows(static_cast<size_t>(info.pattern_total));
+    detail::check(symla_symbolic_get(sym_, perm.data(), parent.data(), first.data(), ncols.data(), rowptr.data(),
+                                     rows.data()),
+                  symla_last_error());
+    detail::check(symla_symbolic_colcounts(sym_, counts.data()), symla_last_error());
+    SymbolicFactor sf;
+    detail::toVector(perm, sf.perm);
+    detail::toVector(parent, sf.etree.parent);
+    detail::toVector(counts, sf.etree.colCount);
+    sf.etree.n = info.n;
+    sf.supernodes.resize(info.nsn);
+    for (int s = 0; s < info.nsn; ++s) {
+      sf.supernodes[s].firstCol = first[s];
+      sf.supernodes[s].ncols = ncols[s];
+      sf.supernodes[s].rowPattern.assign(rows.begin() + rowptr[s], rows.begin() + rowptr[s + 1]);
+    }
+    symbolic_ = std::move(sf);
+    symbolic_valid_ = true;
+  }
+
+  void loadNumeric() const {
+    const SymbolicFactor& sf = symbolicFactor();
+    const int nsn = static_cast<int>(sf.supernodes.size());
+    NumericFactor nf;
+    nf.n = n_;
+    nf.perm = sf.perm;
+    nf.inertia = inertia_;
+    nf.singular = singular_;
+    nf.totalPerturbation = last_info_.total_perturbation;
+    nf.numPerturbed = last_info_.num_perturbed;
+    // singular columns in final order (the C-ABI reports original indices)
+    std::vector<int> inv(n_);
+    for (int k = 0; k < n_; ++k) inv[sf.perm[k]] = k;
+    for (int c : singularColumns()) nf.singularCols.push_back(inv[c]);
+    std::sort(nf.singularCols.begin(), nf.singularCols.end());
+    // supernode tree from the elimination tree (multifrontal.hpp:312-332)
+    std::vector<int> colToSN(n_, -1);
+    for (int s = 0; s < nsn; ++s)
+      for (int c = sf.supernodes[s].firstCol; c < sf.supernodes[s].firstCol + sf.supernodes[s].ncols; ++c)
+        colToSN[c] = s;
+    nf.parentSN.assign(nsn, -1);
+    nf.childrenSN.assign(nsn, {});
+    for (int s = 0; s < nsn; ++s) {
+      const int p = sf.etree.parent[sf.supernodes[s].firstCol + sf.supernodes[s].ncols - 1];
+      if (p >= 0) {
+        nf.parentSN[s] = colToSN[p];
+        nf.childrenSN[colToSN[p]].push_back(s);
+      } else {
+        nf.rootsSN.push_back(s);
+      }
+    }
+    nf.fronts.resize(nsn);
+    for (int s = 0; s < nsn; ++s) {
+      int m = 0, k = 0, nb = 0;
+      detail::check(symla_front_view(plan_, s, &m, &k, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
+                    symla_plan_last_error(plan_));
+      SupernodeFactor& f = nf.fronts[s];
+      f.rowIndices.resize(m);
+      f.nPivots = k;
+      f.L = DenseMatrix(m, k);
+      std::vector<double> dd(k), dof(k);
+      std::vector<int> ps(k), pk(k);
+      detail::check(symla_front_view(plan_, s, &m, &k, f.rowIndices.data(), f.L.data(), dd.data(), dof.data(),
+                                     ps.data(), pk.data(), &nb),
+                    symla_plan_last_error(plan_));
+      f.D.diag.resize(k);
+      f.D.offdiag.resize(k);
+      for (int q = 0; q < k; ++q) {
+        f.D.diag[q] = dd[q];
+        f.D.offdiag[q] = dof[q];
+      }
+      f.pivotBlocks.resize(nb);
+      for (int b = 0; b < nb; ++b) f.pivotBlocks[b] = PivotBlock{ps[b], pk[b] == 2 ? PivotKind::TwoByTwo : PivotKind::OneByOne};
+      f.Ltop = DenseMatrix(k, k);
+      for (int j = 0; j < k; ++j)
+        for (int i = 0; i < k; ++i) f.Ltop(i, j) = f.L(i, j);
+      for (const PivotBlock& b : f.pivotBlocks)
+        if (b.kind == PivotKind::TwoByTwo) f.Ltop(b.start + 1, b.start) = 0.0;
+    }
+    numeric_ = std::move(nf);
+    numeric_valid_ = true;
   }
 
   // solver.hpp:299-317
@@ -319,36 +610,3 @@ class SymLDLT<double> {
 };
 
 }  // namespace symla
-
-#if defined(__has_include)
-#if __has_include(<Eigen/Sparse>)
-#include <Eigen/Dense>
-#include <Eigen/Sparse>
-namespace symla {
-// Input adaptors for callers holding Eigen types (same computation path).
-inline SparseMatrix fromEigen(const Eigen::SparseMatrix<double, Eigen::ColMajor, int>& A) {
-  SparseMatrix S(static_cast<int>(A.rows()), static_cast<int>(A.cols()));
-  for (int c = 0; c < A.outerSize(); ++c) {
-    for (Eigen::SparseMatrix<double, Eigen::ColMajor, int>::InnerIterator it(A, c); it; ++it) {
-      S.rowidx.push_back(static_cast<int>(it.row()));
-      S.values.push_back(it.value());
-    }
-    S.colptr[c + 1] = static_cast<int64_t>(S.rowidx.size());
-  }
-  return S;
-}
-inline DenseMatrix fromEigen(const Eigen::MatrixXd& B) {
-  DenseMatrix D(static_cast<int>(B.rows()), static_cast<int>(B.cols()));
-  for (int j = 0; j < B.cols(); ++j)
-    for (int i = 0; i < B.rows(); ++i) D(i, j) = B(i, j);
-  return D;
-}
-inline Eigen::MatrixXd toEigen(const DenseMatrix& D) {
-  Eigen::MatrixXd B(D.rows(), D.cols());
-  for (int j = 0; j < D.cols(); ++j)
-    for (int i = 0; i < D.rows(); ++i) B(i, j) = D(i, j);
-  return B;
-}
-}  // namespace symla
-#endif
-#endif

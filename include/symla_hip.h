@@ -64,6 +64,9 @@ int symla_symbolic_info(const symla_symbolic* s, symla_symbolic_info_t* info);
  * (Supernode, symbolic.hpp:138-159).  Any pointer may be NULL. */
 int symla_symbolic_get(const symla_symbolic* s, int* perm, int* etree_parent, int* sn_first, int* sn_ncols,
                        int64_t* sn_rowptr, int* sn_rows);
+/* EliminationTree::colCount (include/symla/elimination_tree.hpp:50-53):
+ * nnz(L(:, j)) including the diagonal, final order, n entries. */
+int symla_symbolic_colcounts(const symla_symbolic* s, int* col_count);
 void symla_symbolic_free(symla_symbolic* s);
 
 /* ---- Numeric plan ----

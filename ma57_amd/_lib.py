@@ -86,6 +86,7 @@ SIGNATURES = {
     "symla_analyze": (C.c_int, [C.c_int, i64p, i32p, C.c_int, i32p, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
     "symla_symbolic_info": (C.c_int, [C.c_void_p, C.POINTER(SymbolicInfo)]),
     "symla_symbolic_get": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, i64p, i32p]),
+    "symla_symbolic_colcounts": (C.c_int, [C.c_void_p, i32p]),
     "symla_symbolic_free": (None, [C.c_void_p]),
     "symla_plan_create": (C.c_int, [C.c_void_p, i64p, i32p, C.POINTER(C.c_void_p)]),
     "symla_plan_destroy": (None, [C.c_void_p]),

@@ -113,7 +113,10 @@ def test_graph_replay_with_delays_on_large_fronts_interleaved_with_eager_work():
     reallocates per-front storage; the next replay must notice that the
     recorded graph refers to moved buffers and record it again rather than
     run on freed memory.  Inertia is Sylvester's (n2, n1, 0) for every
-    positive scaling of the matrix."""
+    positive scaling of the matrix.  Without refinement the reference's
+    algorithm reaches backward error 1.4e-9 / 9.1e-10 on this matrix at scales
+    1.0 / 0.7 (the CPU oracle, same ordering); the bar is the reference's own
+    KKT residual bar, 1e-8 (test/correctness/kkt_test.cpp:134)."""
     K, n1, n2 = P.sqd_kkt(nh=16, m2=1200, seed=77)
     n = K.shape[0]
     base = csc_arrays(K)[3].copy()
@@ -135,12 +138,12 @@ def test_graph_replay_with_delays_on_large_fronts_interleaved_with_eager_work():
         assert info.repaired == 1 and info.singular == 0, step
         assert (info.n_pos, info.n_neg, info.n_zero) == (n2, n1, 0), step
         X = d_X.cpu().numpy().T
-        assert P.backward_error(Ks, X, B) <= 1e-10, step
+        assert P.backward_error(Ks, X, B) <= 1e-8, step
         if step % 2 == 0:
             # eager work on the same plan between replays
             s.factorize(Ks)
             inr = s.inertia()
             assert (inr.n_pos, inr.n_neg, inr.n_zero) == (n2, n1, 0)
             Xe = s.solve(B)
-            assert P.backward_error(Ks, Xe, B) <= 1e-10
+            assert P.backward_error(Ks, Xe, B) <= 1e-8
         torch.cuda.synchronize()
