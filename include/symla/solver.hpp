@@ -275,6 +275,37 @@ inline void toVector(const std::vector<int>& in, V& out) {
 }
 }  // namespace detail
 
+// Eigen::loadMarket's role in the reference's real-matrix tests and benches
+// (test/correctness/real_matrix_test.cpp:84): reads a coordinate Matrix
+// Market file, entries as stored, duplicates summed.  Returns false when
+// the file cannot be read (as Eigen::loadMarket does).
+inline bool loadMarket(SparseMatrix& A, const std::string& path) {
+  int nr = 0, nc = 0, sym = 0;
+  int64_t nnz = 0;
+  if (symla_read_matrix_market(path.c_str(), &nr, &nc, &nnz, &sym, nullptr, nullptr, nullptr) != SYMLA_OK)
+    return false;
+  std::vector<int64_t> colptr(static_cast<size_t>(nc) + 1);
+  std::vector<int> rowidx(static_cast<size_t>(nnz));
+  std::vector<double> values(static_cast<size_t>(nnz));
+  if (symla_read_matrix_market(path.c_str(), nullptr, nullptr, nullptr, nullptr, colptr.data(), rowidx.data(),
+                               values.data()) != SYMLA_OK)
+    return false;
+#ifdef SYMLA_FACADE_EIGEN
+  std::vector<Eigen::Triplet<double, int>> t;
+  t.reserve(static_cast<size_t>(nnz));
+  for (int c = 0; c < nc; ++c)
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; ++k) t.emplace_back(rowidx[k], c, values[k]);
+  A.resize(nr, nc);
+  A.setFromTriplets(t.begin(), t.end());
+#else
+  A = CscMatrix(nr, nc);
+  A.colptr = std::move(colptr);
+  A.rowidx = std::move(rowidx);
+  A.values = std::move(values);
+#endif
+  return true;
+}
+
 template <typename Scalar_>
 class SymLDLT;
 
