@@ -69,6 +69,17 @@ int symla_symbolic_get(const symla_symbolic* s, int* perm, int* etree_parent, in
 int symla_symbolic_colcounts(const symla_symbolic* s, int* col_count);
 void symla_symbolic_free(symla_symbolic* s);
 
+/* Matrix Market coordinate reader: the role of Eigen::loadMarket in the
+ * reference's real-matrix tests and benches (test/correctness/
+ * real_matrix_test.cpp:84, bench/tier1_timing_main.cpp:24).  Real, integer
+ * or pattern fields; general, symmetric or skew-symmetric storage, entries
+ * kept as stored (a symmetric file's one triangle), duplicates summed, rows
+ * sorted per column.  *symmetry = 0 general, 1 symmetric, 2 skew.  Call
+ * once with NULL colptr/rowidx/values for the sizes, then with arrays of
+ * ncols+1 / nnz / nnz entries. */
+int symla_read_matrix_market(const char* path, int* nrows, int* ncols, int64_t* nnz, int* symmetry, int64_t* colptr,
+                             int* rowidx, double* values);
+
 /* ---- Numeric plan ----
  * Takes the setup part of MultifrontalFactorizer::factorize
  * (include/symla/multifrontal.hpp:299-429): supernode tree, value
@@ -176,6 +187,21 @@ int symla_solve_refined_device(symla_plan* p, const double* d_a_vals, const doub
 int symla_graph_capture(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs, double* dX,
                         int64_t ldx, const symla_factor_opts* opts);
 int symla_graph_replay(symla_plan* p, symla_factor_info* info);
+/* The same with FP64 iterative refinement of the recorded solve
+ * (SymLDLT::solveWithRefinement, solver.hpp:234-240; iterativeRefine,
+ * refine.hpp:97-144): symla_graph_capture_refined records the
+ * factorization, the norms of A and B, the first solve and its residual as
+ * one graph, and one refinement step (correction solve, update, residual) as
+ * two more (ping-pong buffers).  symla_graph_replay_refined launches the
+ * first, then one step graph per iteration while refine.hpp's tests (stop at
+ * ropts->tolerance, when the residual no longer decreases, or after
+ * ropts->max_iterations) ask for another, and fills rinfo like
+ * symla_solve_refined.  A replay that delays pivots is repaired and refined
+ * on the regular path. */
+int symla_graph_capture_refined(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
+                                double* dX, int64_t ldx, const symla_factor_opts* opts,
+                                const symla_refine_opts* ropts);
+int symla_graph_replay_refined(symla_plan* p, symla_factor_info* info, symla_refine_info* rinfo);
 int symla_graph_release(symla_plan* p);
 
 /* NumericFactor::fronts (multifrontal.hpp:118-186), copied to the host on

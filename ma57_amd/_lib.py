@@ -87,6 +87,7 @@ SIGNATURES = {
     "symla_symbolic_info": (C.c_int, [C.c_void_p, C.POINTER(SymbolicInfo)]),
     "symla_symbolic_get": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, i64p, i32p]),
     "symla_symbolic_colcounts": (C.c_int, [C.c_void_p, i32p]),
+    "symla_read_matrix_market": (C.c_int, [C.c_char_p, i32p, i32p, i64p, i32p, i64p, i32p, f64p]),
     "symla_symbolic_free": (None, [C.c_void_p]),
     "symla_plan_create": (C.c_int, [C.c_void_p, i64p, i32p, C.POINTER(C.c_void_p)]),
     "symla_plan_destroy": (None, [C.c_void_p]),
@@ -103,6 +104,9 @@ SIGNATURES = {
                                              C.c_int64, C.POINTER(RefineOpts), C.POINTER(RefineInfo)]),
     "symla_graph_capture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
                                       C.POINTER(FactorOpts)]),
+    "symla_graph_capture_refined": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
+                                              C.c_int64, C.POINTER(FactorOpts), C.POINTER(RefineOpts)]),
+    "symla_graph_replay_refined": (C.c_int, [C.c_void_p, C.POINTER(FactorInfo), C.POINTER(RefineInfo)]),
     "symla_graph_replay": (C.c_int, [C.c_void_p, C.POINTER(FactorInfo)]),
     "symla_graph_release": (C.c_int, [C.c_void_p]),
     "symla_plan_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),

@@ -9,6 +9,7 @@
 
 #include "../../include/symla_hip.h"
 #include "kernels.hpp"
+#include "mmio.hpp"
 #include "plan.hpp"
 #include "symbolic.hpp"
 
@@ -164,6 +165,21 @@ int symla_symbolic_colcounts(const symla_symbolic* s, int* col_count) {
 
 void symla_symbolic_free(symla_symbolic* s) { delete s; }
 
+int symla_read_matrix_market(const char* path, int* nrows, int* ncols, int64_t* nnz, int* symmetry, int64_t* colptr,
+                             int* rowidx, double* values) {
+  return guarded([&] {
+    if (!path) throw std::invalid_argument("symla_read_matrix_market: null path");
+    const MarketMatrix M = readMatrixMarket(path);
+    if (nrows) *nrows = M.rows;
+    if (ncols) *ncols = M.cols;
+    if (nnz) *nnz = static_cast<int64_t>(M.rowidx.size());
+    if (symmetry) *symmetry = M.symmetry;
+    if (colptr) std::copy(M.colptr.begin(), M.colptr.end(), colptr);
+    if (rowidx) std::copy(M.rowidx.begin(), M.rowidx.end(), rowidx);
+    if (values) std::copy(M.values.begin(), M.values.end(), values);
+  });
+}
+
 int symla_plan_create(const symla_symbolic* sym, const int64_t* a_colptr, const int* a_rowidx, symla_plan** out) {
   return guarded([&] {
     if (!sym || !out) throw std::invalid_argument("symla_plan_create: null argument");
@@ -228,6 +244,32 @@ int symla_graph_replay(symla_plan* p, symla_factor_info* info) {
     FactorInfo fi;
     p->plan->graphReplay(fi);
     fillInfo(fi, info);
+  });
+}
+
+int symla_graph_capture_refined(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
+                                double* dX, int64_t ldx, const symla_factor_opts* opts,
+                                const symla_refine_opts* ropts) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_graph_capture_refined: null plan");
+    p->plan->graphCaptureRefined(d_a_vals, dB, ldb, nrhs, dX, ldx, toFactorOptions(opts, p->plan->n()),
+                                 ropts ? ropts->max_iterations : 10, ropts ? ropts->tolerance : 1e-14);
+  });
+}
+
+int symla_graph_replay_refined(symla_plan* p, symla_factor_info* info, symla_refine_info* rinfo) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_graph_replay_refined: null plan");
+    FactorInfo fi;
+    RefineResult r;
+    p->plan->graphReplay(fi, &r);
+    fillInfo(fi, info);
+    if (rinfo) {
+      rinfo->iterations = r.iterations;
+      rinfo->initial_relative_residual = r.initialRel;
+      rinfo->final_relative_residual = r.finalRel;
+      rinfo->backward_error = r.backwardError;
+    }
   });
 }
 

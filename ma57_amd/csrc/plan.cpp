@@ -1215,6 +1215,87 @@ void Plan::solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx,
   checkSolveError();
 }
 
+// Refinement in pieces, shared by the eager loop and the recorded graphs:
+// refineSetup (norms of A and B, symmetric values, first solve, residual)
+// and refineStep (one correction); the host reads the norm slots between
+// them.  Norm slots of dnorms_: 0 the reference's ||A|| (refine.hpp:106), 1
+// the symmetric ||A||inf, 2 ||B||, 3 ||R||, 4 ||X||, 8.. per-column norms
+// of the backward error.
+void Plan::ensureRefineWs(int nrhs) {
+  const size_t nn = static_cast<size_t>(n_) * nrhs;
+  dBr_.ensure(nn);
+  dR_.ensure(nn);
+  dDx_.ensure(nn);
+  dXn_.ensure(nn);
+  dRn_.ensure(nn);
+  dXout_.ensure(nn);
+  dnorms_.ensure(8 + 3 * static_cast<size_t>(nrhs));
+  dsymVal_.ensure(static_cast<size_t>(nnzSym_));
+}
+
+void Plan::refineSetup(const double* vals, int nrhs) {
+  launchZeroU64(dnorms_.p, 8, stream_);
+  launchRowSumMax(n_, dstPtr_.p, dstSrc_.p, vals, dnorms_.p + 0, stream_);
+  launchRowSumMax(n_, dsymPtr_.p, dsymSrc_.p, vals, dnorms_.p + 1, stream_);
+  launchAbsMax(n_, nrhs, dBr_.p, n_, dnorms_.p + 2, stream_);
+  launchGatherVals(nnzSym_, dsymSrc_.p, vals, dsymVal_.p, stream_);
+  solveCore(dBr_.p, n_, nrhs, dXout_.p, n_);
+  // max |R| and max |X| accumulate into slots 3 and 4
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXout_.p, dR_.p, dnorms_.p + 3, stream_);
+  launchAbsMax(n_, nrhs, dXout_.p, n_, dnorms_.p + 4, stream_);
+}
+
+void Plan::refineStep(const double* Xc, const double* Rc, double* Xn, double* Rn, int nrhs) {
+  solveCore(Rc, n_, nrhs, dDx_.p, n_);
+  launchZeroU64(dnorms_.p + 3, 2, stream_);
+  launchAxpy(static_cast<int64_t>(n_) * nrhs, Xc, dDx_.p, Xn, dnorms_.p + 4, stream_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xn, Rn, dnorms_.p + 3, stream_);
+}
+
+Plan::RefineNorms Plan::readNorms() {
+  unsigned long long h[5];
+  hipCheck(hipMemcpyAsync(h, dnorms_.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "norms");
+  hipCheck(hipStreamSynchronize(stream_), "norms");
+  RefineNorms r;
+  r.normA = bitsToDoubleHost(h[0]);
+  r.normAsym = bitsToDoubleHost(h[1]);
+  r.normB = bitsToDoubleHost(h[2]);
+  const double nR = bitsToDoubleHost(h[3]), nX = bitsToDoubleHost(h[4]);
+  // the device maps NaN entries to +inf: a non-finite residual or solution
+  // is an infinite relative residual, never a converged one
+  if (!std::isfinite(nR) || !std::isfinite(nX)) {
+    r.rel = std::numeric_limits<double>::infinity();
+  } else {
+    const double denom = r.normA * nX + r.normB;
+    r.rel = denom > 0.0 ? nR / denom : nR;
+  }
+  return r;
+}
+
+double Plan::backwardError(const double* R, const double* X, int nrhs, double normAsym) {
+  // per column with the symmetric infinity norm
+  launchZeroU64(dnorms_.p + 8, 3 * nrhs, stream_);
+  launchAbsMaxCols(n_, nrhs, R, n_, dnorms_.p + 8, 3, stream_);
+  launchAbsMaxCols(n_, nrhs, X, n_, dnorms_.p + 9, 3, stream_);
+  launchAbsMaxCols(n_, nrhs, dBr_.p, n_, dnorms_.p + 10, 3, stream_);
+  std::vector<unsigned long long> h(3 * static_cast<size_t>(nrhs));
+  hipCheck(hipMemcpyAsync(h.data(), dnorms_.p + 8, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          stream_),
+           "be");
+  hipCheck(hipStreamSynchronize(stream_), "be");
+  double be = 0.0;
+  for (int c = 0; c < nrhs; ++c) {
+    const double nr = bitsToDoubleHost(h[3 * c]), nx = bitsToDoubleHost(h[3 * c + 1]),
+                 nb = bitsToDoubleHost(h[3 * c + 2]);
+    const double den = normAsym * nx + nb;
+    const double bc = (!std::isfinite(nr) || !std::isfinite(nx)) ? std::numeric_limits<double>::infinity()
+                      : den > 0.0                                 ? nr / den
+                                                                  : nr;
+    be = std::max(be, bc);
+  }
+  return be;
+}
+
 RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const double* B, int64_t ldb, int nrhs,
                                 double* X, int64_t ldx, bool deviceIo, int maxIt, double tol) {
   if (!factorized_) throw std::logic_error("symla::SymLDLT::solve: factorize() must be called before solve()");
@@ -1229,102 +1310,48 @@ RefineResult Plan::solveRefined(const double* aHost, const double* aDev, const d
     hipCheck(hipMemcpyAsync(dvals_.p, aHost, sizeof(double) * nnzA_, hipMemcpyHostToDevice, stream_), "values");
     vals = dvals_.p;
   }
-  const size_t nn = static_cast<size_t>(n_) * nrhs;
-  dBr_.ensure(nn);
-  dR_.ensure(nn);
-  dDx_.ensure(nn);
-  dXn_.ensure(nn);
-  dRn_.ensure(nn);
-  dXout_.ensure(nn);
+  ensureRefineWs(nrhs);
   hipCheck(hipMemcpy2DAsync(dBr_.p, sizeof(double) * n_, B, sizeof(double) * ldb, sizeof(double) * n_, nrhs,
                             deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_),
            "B copy");
-  double* Xc = dXout_.p;
-  double* Rc = dR_.p;
-  double* Xn = dXn_.p;
-  double* Rn = dRn_.p;
-  dnorms_.ensure(8 + 3 * static_cast<size_t>(nrhs));
-  auto norm = [&](const double* v, int nc, int64_t ld, unsigned long long* slot) {
-    launchAbsMax(n_, nc, v, ld, slot, stream_);
-  };
-  // normA (refine.hpp:106) and the symmetric infinity norm
-  launchZeroU64(dnorms_.p, 8, stream_);
-  launchRowSumMax(n_, dstPtr_.p, dstSrc_.p, vals, dnorms_.p + 0, stream_);
-  launchRowSumMax(n_, dsymPtr_.p, dsymSrc_.p, vals, dnorms_.p + 1, stream_);
-  norm(dBr_.p, nrhs, n_, dnorms_.p + 2);
-  unsigned long long hb[8];
-  hipCheck(hipMemcpyAsync(hb, dnorms_.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "norms");
-  hipCheck(hipStreamSynchronize(stream_), "norms");
-  const double normA = bitsToDoubleHost(hb[0]);
-  const double normAsym = bitsToDoubleHost(hb[1]);
-  const double normB = bitsToDoubleHost(hb[2]);
-
-  dsymVal_.ensure(static_cast<size_t>(nnzSym_));
-  launchGatherVals(nnzSym_, dsymSrc_.p, vals, dsymVal_.p, stream_);
-  solveCore(dBr_.p, n_, nrhs, Xc, n_);
-  // max |R| and max |X| were accumulated into slots 3 and 4 by the residual
-  // and the update that produced them
-  auto relRes = [&]() {
-    unsigned long long h2[2];
-    hipCheck(hipMemcpyAsync(h2, dnorms_.p + 3, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_),
-             "norms");
-    hipCheck(hipStreamSynchronize(stream_), "norms");
-    const double nR = bitsToDoubleHost(h2[0]), nX = bitsToDoubleHost(h2[1]);
-    // the device maps NaN entries to +inf: a non-finite residual or solution
-    // is an infinite relative residual, never a converged one
-    if (!std::isfinite(nR) || !std::isfinite(nX)) return std::numeric_limits<double>::infinity();
-    const double denom = normA * nX + normB;
-    return denom > 0.0 ? nR / denom : nR;
-  };
-  launchZeroU64(dnorms_.p + 3, 2, stream_);
-  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xc, Rc, dnorms_.p + 3, stream_);
-  norm(Xc, nrhs, n_, dnorms_.p + 4);
-  double r0 = relRes();
-  res.initialRel = r0;
-  res.finalRel = r0;
-  for (int it = 0; it < maxIt; ++it) {  // refine.hpp:121-141
-    if (r0 <= tol) break;
-    solveCore(Rc, n_, nrhs, dDx_.p, n_);
-    launchZeroU64(dnorms_.p + 3, 2, stream_);
-    launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, Xn, dnorms_.p + 4, stream_);
-    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xn, Rn, dnorms_.p + 3, stream_);
-    const double rNew = relRes();
-    if (!(rNew < r0)) break;
-    std::swap(Xc, Xn);
-    std::swap(Rc, Rn);
-    r0 = rNew;
-    res.iterations = it + 1;
-    res.finalRel = r0;
-  }
-  // backward error per column with the symmetric infinity norm
-  {
-    launchZeroU64(dnorms_.p + 8, 3 * nrhs, stream_);
-    launchAbsMaxCols(n_, nrhs, Rc, n_, dnorms_.p + 8, 3, stream_);
-    launchAbsMaxCols(n_, nrhs, Xc, n_, dnorms_.p + 9, 3, stream_);
-    launchAbsMaxCols(n_, nrhs, dBr_.p, n_, dnorms_.p + 10, 3, stream_);
-    std::vector<unsigned long long> h(3 * static_cast<size_t>(nrhs));
-    hipCheck(hipMemcpyAsync(h.data(), dnorms_.p + 8, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                            stream_),
-             "be");
-    hipCheck(hipStreamSynchronize(stream_), "be");
-    double be = 0.0;
-    for (int c = 0; c < nrhs; ++c) {
-      const double nr = bitsToDoubleHost(h[3 * c]), nx = bitsToDoubleHost(h[3 * c + 1]),
-                   nb = bitsToDoubleHost(h[3 * c + 2]);
-      const double den = normAsym * nx + nb;
-      const double bc = (!std::isfinite(nr) || !std::isfinite(nx)) ? std::numeric_limits<double>::infinity()
-                        : den > 0.0                                 ? nr / den
-                                                                    : nr;
-      be = std::max(be, bc);
-    }
-    res.backwardError = be;
-  }
+  refineSetup(vals, nrhs);
+  const RefineNorms nm = readNorms();
+  const double* Xc = refineLoop(nm, nrhs, maxIt, tol, nullptr, res);
   hipCheck(hipMemcpy2DAsync(X, sizeof(double) * ldx, Xc, sizeof(double) * n_, sizeof(double) * n_, nrhs,
                             deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_),
            "X out");
   hipCheck(hipStreamSynchronize(stream_), "refine");
   checkSolveError();
   return res;
+}
+
+const double* Plan::refineLoop(const RefineNorms& nm, int nrhs, int maxIt, double tol, hipGraphExec_t* steps,
+                               RefineResult& res) {
+  double* Xc = dXout_.p;
+  double* Rc = dR_.p;
+  double* Xn = dXn_.p;
+  double* Rn = dRn_.p;
+  double r0 = nm.rel;
+  res.initialRel = r0;
+  res.finalRel = r0;
+  int parity = 0;
+  for (int it = 0; it < maxIt; ++it) {  // refine.hpp:121-141
+    if (r0 <= tol) break;
+    if (steps)
+      hipCheck(hipGraphLaunch(steps[parity], stream_), "graph launch");
+    else
+      refineStep(Xc, Rc, Xn, Rn, nrhs);
+    const double rNew = readNorms().rel;
+    if (!(rNew < r0)) break;
+    std::swap(Xc, Xn);
+    std::swap(Rc, Rn);
+    parity ^= 1;
+    r0 = rNew;
+    res.iterations = it + 1;
+    res.finalRel = r0;
+  }
+  res.backwardError = backwardError(Rc, Xc, nrhs, nm.normAsym);
+  return Xc;
 }
 
 void Plan::groupInertia(int ngroups, const int* groupStart, int* inertia3, int* singular) {
@@ -1355,6 +1382,10 @@ void Plan::graphRecord() {
   if (graph_) (void)hipGraphDestroy(graph_);
   graphExec_ = nullptr;
   graph_ = nullptr;
+  for (auto& e : stepExec_) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
   const GraphArgs& g = graphArgs_;
   if (g.opts.precision != prec_) {
     prec_ = g.opts.precision;
@@ -1366,23 +1397,49 @@ void Plan::graphRecord() {
     layoutDirty_ = false;
   }
   ensureSolveWs(g.nrhs);
+  if (g.refine) ensureRefineWs(g.nrhs);
   const bool prof = profiling_;
   profiling_ = false;  // no event records inside the graph
   FactorParams prm = params(g.opts, false, 0.0, nullptr);
-  hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-  try {
-    runStatic(g.vals, prm);
-    solveCore(g.B, g.ldb, g.nrhs, g.X, g.ldx);
-  } catch (...) {
-    hipGraph_t tmp = nullptr;
-    (void)hipStreamEndCapture(stream_, &tmp);
-    if (tmp) (void)hipGraphDestroy(tmp);
-    profiling_ = prof;
-    throw;
+  // records what `body` launches on the plan's stream as one graph
+  auto record = [&](auto&& body, hipGraph_t* graphOut, hipGraphExec_t* execOut) {
+    hipGraph_t gr = nullptr;
+    hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+      body();
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(stream_, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      profiling_ = prof;
+      throw;
+    }
+    hipCheck(hipStreamEndCapture(stream_, &gr), "end capture");
+    const hipError_t e = hipGraphInstantiate(execOut, gr, nullptr, nullptr, 0);
+    if (graphOut) {
+      *graphOut = gr;
+    } else {
+      (void)hipGraphDestroy(gr);
+    }
+    hipCheck(e, "instantiate graph");
+  };
+  if (!g.refine) {
+    record([&] {
+      runStatic(g.vals, prm);
+      solveCore(g.B, g.ldb, g.nrhs, g.X, g.ldx);
+    }, &graph_, &graphExec_);
+  } else {
+    record([&] {
+      runStatic(g.vals, prm);
+      hipCheck(hipMemcpy2DAsync(dBr_.p, sizeof(double) * n_, g.B, sizeof(double) * g.ldb, sizeof(double) * n_,
+                                g.nrhs, hipMemcpyDeviceToDevice, stream_),
+               "B copy");
+      refineSetup(g.vals, g.nrhs);
+    }, &graph_, &graphExec_);
+    record([&] { refineStep(dXout_.p, dR_.p, dXn_.p, dRn_.p, g.nrhs); }, nullptr, &stepExec_[0]);
+    record([&] { refineStep(dXn_.p, dRn_.p, dXout_.p, dR_.p, g.nrhs); }, nullptr, &stepExec_[1]);
   }
-  hipCheck(hipStreamEndCapture(stream_, &graph_), "end capture");
   profiling_ = prof;
-  hipCheck(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "instantiate graph");
   graphEpoch_ = g_devAllocEpoch.load();
   graphStale_ = false;
 }
@@ -1397,16 +1454,33 @@ void Plan::graphCapture(const double* dVals, const double* dB, int64_t ldb, int 
   graphRecord();
 }
 
+void Plan::graphCaptureRefined(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                               const FactorOptions& opts, int maxIt, double tol) {
+  if (opts.mode != 0) throw std::invalid_argument("symla: graph capture covers threshold-pivoting factorizations");
+  if (!dVals || !dB || !dX) throw std::invalid_argument("symla: graph capture needs device buffers");
+  if (nrhs <= 0 || ldb < n_ || ldx < n_) throw std::invalid_argument("symla: bad right-hand-side layout");
+  if (opts.precision != 0 && opts.precision != 1) throw std::invalid_argument("symla: unknown precision");
+  if (maxIt < 0) throw std::invalid_argument("symla: max_iterations must be non-negative");
+  graphArgs_ = GraphArgs{dVals, dB, ldb, nrhs, dX, ldx, opts, true, maxIt, tol};
+  graphRecord();
+}
+
 void Plan::graphRelease() {
   if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   graphExec_ = nullptr;
   graph_ = nullptr;
+  for (auto& e : stepExec_) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
   graphStale_ = true;
 }
 
-void Plan::graphReplay(FactorInfo& info) {
+void Plan::graphReplay(FactorInfo& info, RefineResult* rres) {
   if (!graphExec_) throw std::logic_error("symla: no recorded graph (call symla_graph_capture first)");
+  if (rres && !graphArgs_.refine)
+    throw std::logic_error("symla: the recorded graph has no refinement (use symla_graph_capture_refined)");
   factorized_ = false;
   // a repair pass, another solve size or a new tuning may have moved or
   // changed what the graph refers to
@@ -1414,6 +1488,7 @@ void Plan::graphReplay(FactorInfo& info) {
     graphRecord();
   } else {
     ensureSolveWs(graphArgs_.nrhs);
+    if (graphArgs_.refine) ensureRefineWs(graphArgs_.nrhs);
     if (!graphCurrent()) graphRecord();
   }
   info = FactorInfo{};
@@ -1433,11 +1508,26 @@ void Plan::graphReplay(FactorInfo& info) {
     info.repaired = true;
     info.singular = singular_;
     info.nSingular = static_cast<int>(singularColsOrig_.size());
-    if (!singular_) {
+    if (!singular_ && !graphArgs_.refine) {
       solveCore(graphArgs_.B, graphArgs_.ldb, graphArgs_.nrhs, graphArgs_.X, graphArgs_.ldx);
       hipCheck(hipStreamSynchronize(stream_), "repaired solve");
       checkSolveError();
+    } else if (!singular_) {
+      factorized_ = true;
+      RefineResult r = solveRefined(nullptr, graphArgs_.vals, graphArgs_.B, graphArgs_.ldb, graphArgs_.nrhs,
+                                    graphArgs_.X, graphArgs_.ldx, true, graphArgs_.maxIt, graphArgs_.tol);
+      if (rres) *rres = r;
     }
+  } else if (graphArgs_.refine && !singular_) {
+    // the recorded steps, while refine.hpp's tests ask for more
+    RefineResult r;
+    const double* Xc = refineLoop(readNorms(), graphArgs_.nrhs, graphArgs_.maxIt, graphArgs_.tol, stepExec_, r);
+    hipCheck(hipMemcpy2DAsync(graphArgs_.X, sizeof(double) * graphArgs_.ldx, Xc, sizeof(double) * n_,
+                              sizeof(double) * n_, graphArgs_.nrhs, hipMemcpyDeviceToDevice, stream_),
+             "X out");
+    hipCheck(hipStreamSynchronize(stream_), "refine");
+    checkSolveError();
+    if (rres) *rres = r;
   }
   hipCheck(hipEventRecord(ev1_, stream_), "event");
   hipCheck(hipEventSynchronize(ev1_), "event");

@@ -211,7 +211,13 @@ class Plan {
   // buffer it refers to has moved.
   void graphCapture(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
                     const FactorOptions& opts);
-  void graphReplay(FactorInfo& info);
+  // The same with FP64 iterative refinement (refine.hpp:97-144): the graph
+  // holds the factorization, the norms, the first solve and its residual;
+  // two more graphs hold one refinement step each (ping-pong buffers), and
+  // replay launches them while the host's refine.hpp tests ask for more.
+  void graphCaptureRefined(const double* dVals, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                           const FactorOptions& opts, int maxIt, double tol);
+  void graphReplay(FactorInfo& info, RefineResult* rres = nullptr);
   void graphRelease();
   bool hasGraph() const { return graphExec_ != nullptr; }
 
@@ -384,7 +390,11 @@ class Plan {
     double* X = nullptr;
     int64_t ldx = 0;
     FactorOptions opts;
+    bool refine = false;
+    int maxIt = 10;
+    double tol = 1e-14;
   } graphArgs_;
+  hipGraphExec_t stepExec_[2] = {nullptr, nullptr};  // recorded refinement steps
   // g_devAllocEpoch when the graph was recorded; graphStale_ forces a
   // re-record (new schedule)
   unsigned long long graphEpoch_ = 0;
@@ -438,6 +448,21 @@ class Plan {
   };
   // ext: caller-provided workspace (nothing is allocated); null: the plan's own
   void solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, const SolveWs* ext = nullptr);
+  // refinement pieces (shared by the eager loop and the recorded graphs)
+  struct RefineNorms {
+    double normA = 0.0, normAsym = 0.0, normB = 0.0, rel = 0.0;
+  };
+  void ensureRefineWs(int nrhs);
+  void refineSetup(const double* vals, int nrhs);
+  void refineStep(const double* Xc, const double* Rc, double* Xn, double* Rn, int nrhs);
+  RefineNorms readNorms();
+  double backwardError(const double* R, const double* X, int nrhs, double normAsym);
+  // iterations of refine.hpp:121-141 from the state refineSetup left (X in
+  // dXout_, R in dR_); steps: two recorded refinement steps (X/R from
+  // dXout_/dR_ into dXn_/dRn_ and back), or null to launch them eagerly.
+  // Returns the buffer holding the accepted solution.
+  const double* refineLoop(const RefineNorms& nm, int nrhs, int maxIt, double tol, hipGraphExec_t* steps,
+                           RefineResult& res);
   void ensureIdxCapacity(int64_t need);
 };
 

@@ -445,20 +445,34 @@ class SymLDLT:
                  for i in range(min(nt.value, 64))])
 
     # --- recorded refactorize + solve (no reference counterpart) ---
-    def captureGraph(self, d_vals: int, d_B: int, nrhs: int, d_X: int):
-        """Record factorize(values at device address d_vals) + solve(d_B -> d_X), n x nrhs column-major."""
+    def captureGraph(self, d_vals: int, d_B: int, nrhs: int, d_X: int, refine: Optional[RefinementOptions] = None):
+        """Record factorize(values at device address d_vals) + solve(d_B -> d_X), n x nrhs column-major; with
+        `refine`, the solve is refined in FP64 (solveWithRefinement semantics) by recorded refinement steps."""
         if not self._pattern_analyzed:
             raise LogicError("symla::SymLDLT::factorize: analyzePattern must be called first")
         o = self._opts()
-        rc = self._lib.symla_graph_capture(self._plan, C.c_void_p(d_vals), C.c_void_p(d_B), self._n, nrhs,
-                                           C.c_void_p(d_X), self._n, C.byref(o))
+        if refine is None:
+            rc = self._lib.symla_graph_capture(self._plan, C.c_void_p(d_vals), C.c_void_p(d_B), self._n, nrhs,
+                                               C.c_void_p(d_X), self._n, C.byref(o))
+        else:
+            ro = _lib.RefineOpts(refine.max_iterations, refine.tolerance)
+            rc = self._lib.symla_graph_capture_refined(self._plan, C.c_void_p(d_vals), C.c_void_p(d_B), self._n,
+                                                       nrhs, C.c_void_p(d_X), self._n, C.byref(o), C.byref(ro))
+        self._graph_refined = refine is not None
         _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
 
     def replayGraph(self):
-        """Re-run the recorded factorize + solve on the current buffer contents; returns the factor info."""
+        """Re-run the recorded factorize + solve on the current buffer contents; returns the factor info (and,
+        for a graph recorded with refinement, `last_refinement` holds the RefinementResult)."""
         info = _lib.FactorInfo()
         self._factorized = False
-        rc = self._lib.symla_graph_replay(self._plan, C.byref(info))
+        if getattr(self, "_graph_refined", False):
+            ri = _lib.RefineInfo()
+            rc = self._lib.symla_graph_replay_refined(self._plan, C.byref(info), C.byref(ri))
+            self.last_refinement = RefinementResult(ri.iterations, ri.initial_relative_residual,
+                                                    ri.final_relative_residual, ri.backward_error)
+        else:
+            rc = self._lib.symla_graph_replay(self._plan, C.byref(info))
         _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
         self.last_info = info
         self._inertia = Inertia(info.n_pos, info.n_neg, info.n_zero)

@@ -147,3 +147,71 @@ def test_graph_replay_with_delays_on_large_fronts_interleaved_with_eager_work():
             Xe = s.solve(B)
             assert P.backward_error(Ks, Xe, B) <= 1e-8
         torch.cuda.synchronize()
+
+
+def test_refined_graph_replays_match_eager_refinement():
+    """Recorded factorize + FP64-refined solve (FP32 fronts): every replay on new values reaches the 1e-12
+    backward-error bar with the closed-form inertia and equals the eager factorize + solveWithRefinement of
+    the same values bit for bit (same kernels, same order), with eager work between the replays."""
+    from ma57_amd.solver import RefinementOptions, csc_arrays
+    G = 20
+    shifts = [0.5, 0.37, 0.61]
+    n = G ** 3
+    dev = torch.device("cuda:0")
+    s = _solver(P.laplace3d(G, G, G, shifts[0]), OrderingType.Metis, Precision.FP32)
+    eager = _solver(P.laplace3d(G, G, G, shifts[0]), OrderingType.Metis, Precision.FP32)
+    d_vals = torch.from_numpy(csc_arrays(P.laplace3d(G, G, G, shifts[0]))[3].copy()).to(dev)
+    d_B = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    d_X = torch.zeros_like(d_B)
+    torch.cuda.synchronize()
+    opts = RefinementOptions(max_iterations=10, tolerance=1e-12)
+    s.captureGraph(d_vals.data_ptr(), d_B.data_ptr(), 3, d_X.data_ptr(), refine=opts)
+    rng = np.random.default_rng(33)
+    for rep in range(2):
+        for sh in shifts:
+            A = P.laplace3d(G, G, G, sh)
+            B = rng.standard_normal((n, 3))
+            d_vals.copy_(torch.from_numpy(csc_arrays(A)[3]))
+            d_B.copy_(torch.from_numpy(np.ascontiguousarray(B.T)))
+            torch.cuda.synchronize()
+            info = s.replayGraph()
+            r = s.last_refinement
+            X = d_X.cpu().numpy().T
+            expect = tuple(P.closed_form_inertia(P.laplace_eigs_3d(G, G, G), sh)[:3])
+            assert (info.n_pos, info.n_neg, info.n_zero) == expect and info.repaired == 0
+            assert r.iterations >= 1 and r.backward_error <= 1e-12
+            assert P.backward_error(A, X, B) <= 1e-12
+            eager.factorize(A)
+            re = eager.solveWithRefinement(A, B, opts)
+            assert re.diagnostics.iterations == r.iterations
+            assert np.array_equal(re.x, X)
+            junk = torch.randn(4096, device=dev, dtype=torch.float64)
+            assert torch.isfinite(junk.sum()).item()
+            torch.cuda.synchronize()
+
+
+def test_refined_graph_replay_with_delays_is_repaired_and_refined():
+    """Threshold-mode SQD KKT: every replay delays pivots, is repaired on the regular path and refined there;
+    refinement brings the backward error from the unrefined ~1e-9 to the 1e-12 bar."""
+    from ma57_amd.solver import RefinementOptions, csc_arrays
+    K, n1, n2 = P.sqd_kkt(nh=16, m2=1200, seed=77)
+    n = K.shape[0]
+    base = csc_arrays(K)[3].copy()
+    dev = torch.device("cuda:0")
+    s = _solver(K, OrderingType.Metis, Precision.FP64)
+    d_vals = torch.from_numpy(base.copy()).to(dev)
+    d_B = torch.zeros((2, n), dtype=torch.float64, device=dev)
+    d_X = torch.zeros_like(d_B)
+    torch.cuda.synchronize()
+    s.captureGraph(d_vals.data_ptr(), d_B.data_ptr(), 2, d_X.data_ptr(),
+                   refine=RefinementOptions(max_iterations=10, tolerance=1e-14))
+    rng = np.random.default_rng(8)
+    for scale in (1.0, 0.8):
+        B = rng.standard_normal((n, 2))
+        d_vals.copy_(torch.from_numpy(base * scale))
+        d_B.copy_(torch.from_numpy(np.ascontiguousarray(B.T)))
+        torch.cuda.synchronize()
+        info = s.replayGraph()
+        assert info.repaired == 1 and (info.n_pos, info.n_neg, info.n_zero) == (n2, n1, 0)
+        assert s.last_refinement.iterations >= 1
+        assert P.backward_error(K * scale, d_X.cpu().numpy().T, B) <= 1e-12
