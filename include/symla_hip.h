@@ -155,6 +155,19 @@ int symla_group_inertia(symla_plan* p, int ngroups, const int* group_start, int*
 int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx);
 int symla_solve_device(symla_plan* p, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
 
+/* No counterpart in the reference: symla_solve_device for callers that
+ * cannot allocate device memory (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and results as
+ * symla_solve_device; the per-call workspace, (3 n + 2 sum of front orders)
+ * x nrhs doubles, comes from a static 128 MiB device array of the library
+ * instead of plan buffers, and the call makes no hipMalloc / hipFree.  A
+ * solve needing more returns SYMLA_ERR_INVALID (nothing runs).  Call
+ * symla_factorize first, on the current device (the plan's).  The array of
+ * a device is set up by the first symla_plan_create on it and held until
+ * the process ends.  All calls on one GPU share that GPU's array: calls that
+ * overlap on one GPU race. */
+int symla_solve_device_noalloc(symla_plan* p, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+
 /* SymLDLT::solveWithRefinement (solver.hpp:234-240) with iterativeRefine
  * (refine.hpp:97-144).  a_vals must be the values last factorized. */
 typedef struct {

@@ -88,6 +88,7 @@ SIGNATURES = {
     "symla_symbolic_get": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, i64p, i32p]),
     "symla_symbolic_colcounts": (C.c_int, [C.c_void_p, i32p]),
     "symla_read_matrix_market": (C.c_int, [C.c_char_p, i32p, i32p, i64p, i32p, i64p, i32p, f64p]),
+    "symla_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64]),
     "symla_symbolic_free": (None, [C.c_void_p]),
     "symla_plan_create": (C.c_int, [C.c_void_p, i64p, i32p, C.POINTER(C.c_void_p)]),
     "symla_plan_destroy": (None, [C.c_void_p]),

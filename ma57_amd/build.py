@@ -23,7 +23,7 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE = ORACLE_DIR / "build" / "libsymla_oracle.so"
 
 HIP_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "cb_syrk.hip", "plan.cpp", "capi.cpp", "symbolic.cpp",
-               "mmio.cpp"]
+               "noalloc.hip", "mmio.cpp"]
 LINK_LIBS: list[str] = []
 # Packed FP32 ops are disabled for every HIP translation unit.
 HIP_FLAGS = [

@@ -187,6 +187,12 @@ int symla_plan_create(const symla_symbolic* sym, const int64_t* a_colptr, const 
     if (sym->r.n > 0 && !a_colptr) throw std::invalid_argument("symla_plan_create: null pattern");
     auto p = std::make_unique<symla_plan>();
     p->plan = std::make_unique<Plan>(sym->r, a_colptr, a_rowidx);
+    // set-up of the allocation-free solve: the static scratch of this device
+    // is loaded and its address looked up now, so that entry point's calls
+    // take no device memory at all
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    (void)noAllocScratch(dev);
     *out = p.release();
   });
 }
@@ -340,6 +346,17 @@ int symla_solve_device(symla_plan* p, const double* dB, int64_t ldb, int nrhs, d
     if (ldb < p->plan->n() || ldx < p->plan->n())
       throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
     p->plan->solve(dB, ldb, nrhs, dX, ldx, true);
+  });
+}
+
+int symla_solve_device_noalloc(symla_plan* p, const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_solve_device_noalloc: null plan");
+    if (ldb < p->plan->n() || ldx < p->plan->n())
+      throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    p->plan->solveNoAlloc(dB, ldb, nrhs, dX, ldx, noAllocScratch(dev), kNoAllocScratchBytes / sizeof(double));
   });
 }
 

@@ -1215,6 +1215,39 @@ void Plan::solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx,
   checkSolveError();
 }
 
+void Plan::solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, double* scratch,
+                        size_t capacity) {
+  if (!factorized_) throw std::logic_error("symla::SymLDLT::solve: factorize() must be called before solve()");
+  if (singular_)
+    throw std::logic_error(
+        "symla::SymLDLT::solve: factorization is numerically singular (see isSingular()/singularColumns()); refusing "
+        "to solve");
+  if (nrhs < 0) throw std::invalid_argument("symla: nrhs must be non-negative");
+  if (nrhs == 0 || n_ == 0) return;
+  if (!solveLayoutReady_) throw std::logic_error("symla: the solve layout is not built (factorize first)");
+  const size_t nn = static_cast<size_t>(n_) * nrhs;
+  const size_t gm = static_cast<size_t>(sumM_) * nrhs;
+  const size_t need = 3 * nn + 2 * gm;
+  if (need > capacity)
+    throw std::invalid_argument("symla_solve_device_noalloc: this solve needs " + std::to_string(need * 8 >> 20) +
+                                " MiB of workspace ((3 n + 2 sum of front orders) x nrhs doubles), above the " +
+                                std::to_string(capacity * 8 >> 20) +
+                                " MiB static scratch; use fewer right-hand sides per call or symla_solve_device");
+  const SolveWs w{scratch, scratch + nn, scratch + 2 * nn, scratch + 3 * nn, scratch + 3 * nn + gm};
+  // the event profile would create events on first use: off for this call
+  const bool prof = profiling_;
+  profiling_ = false;
+  hipCheck(hipEventRecord(ev0_, stream_), "event");
+  solveCore(dB, ldb, nrhs, dX, ldx, &w);
+  hipCheck(hipEventRecord(ev1_, stream_), "event");
+  profiling_ = prof;
+  hipCheck(hipEventSynchronize(ev1_), "solve");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  solveMs_ = ms;
+  checkSolveError();
+}
+
 // Refinement in pieces, shared by the eager loop and the recorded graphs:
 // refineSetup (norms of A and B, symmetric values, first solve, residual)
 // and refineStep (one correction); the host reads the norm slots between

@@ -178,6 +178,10 @@ class Plan;
 // with events; `cb` may be null only when no front defers its contribution block.
 void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd, const FactorParams& prm, double* ws,
                    const int64_t* wsOff, hipStream_t st, const CbUpdate* cb, Plan* prof);
+// noalloc.hip: the static solve scratch of `device` (its device address,
+// looked up once per device); kNoAllocScratchBytes is its size.
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+double* noAllocScratch(int device);
 // cb_syrk.hip: C -= W B^T over the lower triangle of the N x N block C
 // (column-major; W, B: N x K) on the matrix cores, in the front precision
 void launchCbSyrk(bool fp32, int N, int K, const void* W, int ldw, const void* B, int ldb, void* C, int ldc,
@@ -191,6 +195,12 @@ class Plan {
   // values: device pointer (already resident) after `setValuesDevice`.
   void factorize(const double* aValsHostOrNull, const double* aValsDev, const FactorOptions& opts, FactorInfo& info);
   void solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo);
+  // The device solve with its per-call workspace in `scratch` (capacity in
+  // doubles) instead of plan buffers: allocates nothing.  Throws
+  // std::invalid_argument when (3 n + 2 sum of front orders) x nrhs doubles
+  // exceed the capacity.
+  void solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, double* scratch,
+                    size_t capacity);
   RefineResult solveRefined(const double* aHost, const double* aDev, const double* B, int64_t ldb, int nrhs,
                             double* X, int64_t ldx, bool deviceIo, int maxIt, double tol);
 
