@@ -85,9 +85,10 @@ def build_workload(cfg, args, P, OrderingType, Mode, SymLDLT, BatchedLDLT):
         systems = [P.batched_system(SEED + 4 + 1000 * i) for i in range(args.systems)]
         b = BatchedLDLT(OrderingType.AMD)
         b.analyzePattern(systems)
+        args.batch = b
         return (f"{args.systems} independent 2D 5-point systems + 2 random couplings per row, n_i ~ U[200, 5000], "
-                f"- 0.3 I, one block-diagonal plan", b.matrix, b.solver, None, args.nrhs or 1,
-                {"systems": args.systems, "sum_n": int(b.matrix.shape[0])})
+                f"- 0.3 I, one block-diagonal plan (every launch covers all systems' fronts at one tree level)",
+                b.matrix, b.solver, None, args.nrhs or 1, {"systems": args.systems, "sum_n": int(b.matrix.shape[0])})
     if cfg == "c5":
         N = args.grid or 80
         A = P.stencil27(N, 0.2)
@@ -264,7 +265,7 @@ def main() -> int:
     ap.add_argument("--config", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--grid", type=int, default=0, help="grid side (c2: 40, c5: 80)")
     ap.add_argument("--nrhs", type=int, default=0, help="right-hand sides (default per config)")
-    ap.add_argument("--systems", type=int, default=256, help="c4: number of independent systems")
+    ap.add_argument("--systems", type=int, default=2048, help="c4: number of independent systems")
     ap.add_argument("--replays", type=int, default=10, help="c5: recorded refactorize + solve replays")
     ap.add_argument("--legs", default="both", choices=["both", "fp32", "fp64"],
                     help="variants to time (profiling runs take one)")
@@ -341,6 +342,8 @@ def main() -> int:
             opts.expected_sign = signs.ctypes.data_as(_lib.i32p)
         info = _lib.FactorInfo()
         rinfo = _lib.RefineInfo()
+        batch = getattr(args, "batch", None)
+        sys_infos = (_lib.RefineInfo * len(batch.sizes))() if batch is not None else None
         # refinement stops once the relative residual (the reference's stored-entry norm of A, which is no
         # larger than the symmetric one) reaches the 1e-12 bar; the reported backward error is recomputed
         # per column with the symmetric norm
@@ -350,7 +353,14 @@ def main() -> int:
             rc = lib.symla_factorize_device(plan, C.c_void_p(d_vals.data_ptr()), C.byref(opts), C.byref(info))
             if rc:
                 raise RuntimeError(lib.symla_plan_last_error(plan).decode())
-            if refine:
+            if refine and batch is not None:
+                # refine.hpp's loop for every system on its own (own norms, stopping tests, iteration count)
+                rc = lib.symla_batch_solve_refined_device(plan, len(batch.sizes), _lib.ptr(batch.starts, _lib.i32p),
+                                                          C.c_void_p(d_vals.data_ptr()), C.c_void_p(d_B.data_ptr()),
+                                                          n, nrhs, C.c_void_p(d_X.data_ptr()), n, C.byref(ropts),
+                                                          sys_infos)
+                rinfo.iterations = max(i.iterations for i in sys_infos)
+            elif refine:
                 rc = lib.symla_solve_refined_device(plan, C.c_void_p(d_vals.data_ptr()), C.c_void_p(d_B.data_ptr()),
                                                     n, nrhs, C.c_void_p(d_X.data_ptr()), n, C.byref(ropts),
                                                     C.byref(rinfo))
@@ -409,6 +419,13 @@ def main() -> int:
                                    if prof.syrk_ms + prof.cb_ms > 0 else 0.0),
         }
         res["rooflines"] = rooflines(res, work, 4 if precision == Precision.FP32 else 8)
+        if batch is not None:
+            # per-system refinement: iterations and backward error of every system
+            if refine:
+                its = [i.iterations for i in sys_infos]
+                bes = [i.backward_error for i in sys_infos]
+                res["per_system"] = {"ir_iterations_min": min(its), "ir_iterations_max": max(its),
+                                     "ir_iterations_mean": float(np.mean(its)), "backward_error_max": max(bes)}
         print(f"[bench] {'FP32+IR' if precision == Precision.FP32 else 'FP64'}: {res['s'] * 1e3:.1f} ms/step, "
               f"BE {res['backward_error']:.2e}, IR {res['ir_iterations']}", file=sys.stderr, flush=True)
         return res

@@ -148,6 +148,26 @@ int symla_singular_cols(const symla_plan* p, int* out, int cap);
  * zero}; singular (may be NULL): singular columns per group. */
 int symla_group_inertia(symla_plan* p, int ngroups, const int* group_start, int* inertia3, int* singular);
 
+/* ---- Batches of independent systems (no counterpart in the reference,
+ * which factors one system per SymLDLT, include/symla/solver.hpp:29-418).
+ * System s is an n[s] x n[s] CSC pattern (colptr[s], rowidx[s]).  The
+ * batch is factored and solved as the block-diagonal matrix of its systems:
+ * every launch of the numeric phase covers the fronts of all systems at one
+ * tree level.
+ * symla_batch_analyze runs symla_analyze on each system (on host threads)
+ * and joins the results as the analysis of the block-diagonal matrix;
+ * symla_batch_plan_create builds its plan from the per-system patterns.
+ * Then symla_factorize / symla_solve take the systems' values and
+ * right-hand sides concatenated (system s's values after system s-1's, its
+ * rows after system s-1's rows), symla_group_inertia returns per-system
+ * inertia (group_start = running sums of n), and symla_batch_solve_refined
+ * runs iterativeRefine (refine.hpp:97-144) for each system on its own: its
+ * own norms and stopping tests, per_system[s] its iterations and backward
+ * error (sys_start: nsys + 1 running sums of n). */
+int symla_batch_analyze(int nsys, const int* n, const int64_t* const* colptr, const int* const* rowidx, int ordering,
+                        int max_relax_size, double max_relax_fill, symla_symbolic** out);
+int symla_batch_plan_create(const symla_symbolic* sym, int nsys, const int* n, const int64_t* const* colptr,
+                            const int* const* rowidx, symla_plan** out);
 /* SymLDLT::solve (solver.hpp:198-218) / MultifrontalSolver::solve
  * (solve.hpp:159-431).  B, X: n x nrhs column-major.  Host pointers
  * (symla_solve) or device pointers (symla_solve_device).  SYMLA_ERR_LOGIC
@@ -184,6 +204,14 @@ int symla_solve_refined(symla_plan* p, const double* a_vals, const double* B, in
                         int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info);
 int symla_solve_refined_device(symla_plan* p, const double* d_a_vals, const double* dB, int64_t ldb, int nrhs,
                                double* dX, int64_t ldx, const symla_refine_opts* opts, symla_refine_info* info);
+
+/* Per-system refinement of a batch (see symla_batch_analyze). */
+int symla_batch_solve_refined(symla_plan* p, int nsys, const int* sys_start, const double* a_vals, const double* B,
+                              int64_t ldb, int nrhs, double* X, int64_t ldx, const symla_refine_opts* opts,
+                              symla_refine_info* per_system);
+int symla_batch_solve_refined_device(symla_plan* p, int nsys, const int* sys_start, const double* d_a_vals,
+                                     const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                                     const symla_refine_opts* opts, symla_refine_info* per_system);
 
 /* Recorded refactorize + solve (no counterpart in the reference; it serves
  * callers that refactorize one pattern many times, e.g. inside a time or

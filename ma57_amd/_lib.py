@@ -87,6 +87,15 @@ SIGNATURES = {
     "symla_symbolic_info": (C.c_int, [C.c_void_p, C.POINTER(SymbolicInfo)]),
     "symla_symbolic_get": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, i64p, i32p]),
     "symla_symbolic_colcounts": (C.c_int, [C.c_void_p, i32p]),
+    "symla_batch_analyze": (C.c_int, [C.c_int, i32p, C.POINTER(i64p), C.POINTER(i32p), C.c_int, C.c_int, C.c_double,
+                                      C.POINTER(C.c_void_p)]),
+    "symla_batch_plan_create": (C.c_int, [C.c_void_p, C.c_int, i32p, C.POINTER(i64p), C.POINTER(i32p),
+                                          C.POINTER(C.c_void_p)]),
+    "symla_batch_solve_refined": (C.c_int, [C.c_void_p, C.c_int, i32p, f64p, f64p, C.c_int64, C.c_int, f64p,
+                                            C.c_int64, C.POINTER(RefineOpts), C.POINTER(RefineInfo)]),
+    "symla_batch_solve_refined_device": (C.c_int, [C.c_void_p, C.c_int, i32p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                   C.c_int, C.c_void_p, C.c_int64, C.POINTER(RefineOpts),
+                                                   C.POINTER(RefineInfo)]),
     "symla_read_matrix_market": (C.c_int, [C.c_char_p, i32p, i32p, i64p, i32p, i64p, i32p, f64p]),
     "symla_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64]),
     "symla_symbolic_free": (None, [C.c_void_p]),

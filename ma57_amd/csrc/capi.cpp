@@ -1,7 +1,9 @@
 // capi.cpp -- extern "C" entry points of libsymla_hip.so (include/symla_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <thread>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -119,6 +121,45 @@ int symla_analyze(int n, const int64_t* colptr, const int* rowidx, int ordering,
   });
 }
 
+int symla_batch_analyze(int nsys, const int* n, const int64_t* const* colptr, const int* const* rowidx, int ordering,
+                        int max_relax_size, double max_relax_fill, symla_symbolic** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("symla_batch_analyze: out is null");
+    *out = nullptr;
+    if (nsys <= 0 || !n || !colptr || !rowidx) throw std::invalid_argument("symla_batch_analyze: empty batch");
+    if (ordering < 0 || ordering > 2) throw std::invalid_argument("symla::computeOrdering: unknown OrderingType");
+    for (int s = 0; s < nsys; ++s) {
+      if (n[s] < 0) throw std::invalid_argument("symla_batch_analyze: n must be non-negative");
+      if (n[s] > 0 && (!colptr[s] || (colptr[s][n[s]] > 0 && !rowidx[s])))
+        throw std::invalid_argument("symla_batch_analyze: null pattern");
+    }
+    SymbolicOptions so;
+    so.max_relax_size = max_relax_size;
+    so.max_relax_fill_fraction = max_relax_fill;
+    // the systems are independent: analyze them on host threads
+    std::vector<SymbolicResult> parts(nsys);
+    std::vector<std::string> errs(nsys);
+    const int nthreads = std::max(1, std::min<int>({nsys, 16, static_cast<int>(std::thread::hardware_concurrency())}));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&, t] {
+        for (int s = t; s < nsys; s += nthreads) {
+          try {
+            parts[s] = analyze(n[s], colptr[s], rowidx[s], static_cast<Ordering>(ordering), so, nullptr);
+          } catch (const std::exception& e) {
+            errs[s] = e.what();
+          }
+        }
+      });
+    for (auto& th : pool) th.join();
+    for (int s = 0; s < nsys; ++s)
+      if (!errs[s].empty()) throw std::invalid_argument("symla_batch_analyze: system " + std::to_string(s) + ": " + errs[s]);
+    auto sym = std::make_unique<symla_symbolic>();
+    sym->r = concatenateAnalyses(parts);
+    *out = sym.release();
+  });
+}
+
 int symla_symbolic_info(const symla_symbolic* s, symla_symbolic_info_t* info) {
   return guarded([&] {
     if (!s || !info) throw std::invalid_argument("symla_symbolic_info: null argument");
@@ -219,6 +260,79 @@ int symla_factorize(symla_plan* p, const double* a_vals, const symla_factor_opts
 int symla_factorize_device(symla_plan* p, const double* d_a_vals, const symla_factor_opts* opts,
                            symla_factor_info* info) {
   return factorizeImpl(p, nullptr, d_a_vals, opts, info);
+}
+
+int symla_batch_plan_create(const symla_symbolic* sym, int nsys, const int* n, const int64_t* const* colptr,
+                            const int* const* rowidx, symla_plan** out) {
+  return guarded([&] {
+    if (!sym || !out || nsys <= 0 || !n || !colptr || !rowidx)
+      throw std::invalid_argument("symla_batch_plan_create: null argument");
+    *out = nullptr;
+    // the block-diagonal pattern: system s's columns follow system s-1's, its rows shifted by the same offset
+    int total = 0;
+    int64_t nnz = 0;
+    for (int s = 0; s < nsys; ++s) {
+      total += n[s];
+      nnz += n[s] > 0 ? colptr[s][n[s]] : 0;
+    }
+    if (total != sym->r.n) throw std::invalid_argument("symla_batch_plan_create: sizes differ from the analysis");
+    std::vector<int64_t> cp;
+    cp.reserve(static_cast<size_t>(total) + 1);
+    cp.push_back(0);
+    std::vector<int> ri;
+    ri.reserve(static_cast<size_t>(nnz));
+    int off = 0;
+    for (int s = 0; s < nsys; ++s) {
+      for (int c = 0; c < n[s]; ++c) {
+        for (int64_t t = colptr[s][c]; t < colptr[s][c + 1]; ++t) {
+          const int r = rowidx[s][t];
+          if (r < 0 || r >= n[s]) throw std::invalid_argument("symla: row index out of range");
+          ri.push_back(r + off);
+        }
+        cp.push_back(static_cast<int64_t>(ri.size()));
+      }
+      off += n[s];
+    }
+    auto p = std::make_unique<symla_plan>();
+    p->plan = std::make_unique<Plan>(sym->r, cp.data(), ri.data());
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    (void)noAllocScratch(dev);
+    *out = p.release();
+  });
+}
+
+static int batchRefinedImpl(symla_plan* p, int nsys, const int* sys_start, const double* aHost, const double* aDev,
+                            const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool dev,
+                            const symla_refine_opts* opts, symla_refine_info* per_system) {
+  return planGuarded(p, [&] {
+    if (!p) throw std::invalid_argument("symla_batch_solve_refined: null plan");
+    if (ldb < p->plan->n() || ldx < p->plan->n())
+      throw std::invalid_argument("symla::SymLDLT::solve: B.rows() must match the factorized matrix size");
+    const int maxIt = opts ? opts->max_iterations : 10;
+    const double tol = opts ? opts->tolerance : 1e-14;
+    const std::vector<RefineResult> r =
+        p->plan->solveRefinedGroups(aHost, aDev, B, ldb, nrhs, X, ldx, dev, maxIt, tol, nsys, sys_start);
+    if (per_system)
+      for (int s = 0; s < nsys; ++s) {
+        per_system[s].iterations = r[s].iterations;
+        per_system[s].initial_relative_residual = r[s].initialRel;
+        per_system[s].final_relative_residual = r[s].finalRel;
+        per_system[s].backward_error = r[s].backwardError;
+      }
+  });
+}
+
+int symla_batch_solve_refined(symla_plan* p, int nsys, const int* sys_start, const double* a_vals, const double* B,
+                              int64_t ldb, int nrhs, double* X, int64_t ldx, const symla_refine_opts* opts,
+                              symla_refine_info* per_system) {
+  return batchRefinedImpl(p, nsys, sys_start, a_vals, nullptr, B, ldb, nrhs, X, ldx, false, opts, per_system);
+}
+
+int symla_batch_solve_refined_device(symla_plan* p, int nsys, const int* sys_start, const double* d_a_vals,
+                                     const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx,
+                                     const symla_refine_opts* opts, symla_refine_info* per_system) {
+  return batchRefinedImpl(p, nsys, sys_start, nullptr, d_a_vals, dB, ldb, nrhs, dX, ldx, true, opts, per_system);
 }
 
 int symla_singular_cols(const symla_plan* p, int* out, int cap) {

@@ -91,6 +91,18 @@ void launchAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, unsigned lon
                       hipStream_t st);
 void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
                      unsigned long long* outBits, hipStream_t st);
+// per-system norms of a batch (systems own consecutive rows; rowGroup[i] =
+// system of row i): outputs at out[g * stride] (+ c * colStride per column),
+// bits of non-negative doubles folded with atomic max (start them at 0)
+void launchGroupAbsMax(int n, int nrhs, const double* v, int64_t ld, const int* rowGroup, unsigned long long* out,
+                       int stride, hipStream_t st);
+void launchGroupAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, const int* rowGroup,
+                           unsigned long long* out, int stride, int colStride, hipStream_t st);
+void launchGroupRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
+                          const int* rowGroup, unsigned long long* out, int stride, hipStream_t st);
+// rows of systems with accept[g] != 0 copied from src to dst (n x nrhs, leading dimension ld)
+void launchGroupSelect(int n, int nrhs, const int* rowGroup, const int* accept, const double* src, double* dst,
+                       int64_t ld, hipStream_t st);
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st);
 void launchZeroU32(unsigned* p, int64_t count, hipStream_t st);
 

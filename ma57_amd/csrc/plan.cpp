@@ -1387,6 +1387,145 @@ const double* Plan::refineLoop(const RefineNorms& nm, int nrhs, int maxIt, doubl
   return Xc;
 }
 
+std::vector<RefineResult> Plan::solveRefinedGroups(const double* aHost, const double* aDev, const double* B,
+                                                   int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo,
+                                                   int maxIt, double tol, int ngroups, const int* groupStart) {
+  if (!factorized_) throw std::logic_error("symla::SymLDLT::solve: factorize() must be called before solve()");
+  if (singular_)
+    throw std::logic_error(
+        "symla::SymLDLT::solve: factorization is numerically singular (see isSingular()/singularColumns()); refusing "
+        "to solve");
+  if (ngroups <= 0 || !groupStart || groupStart[0] != 0 || groupStart[ngroups] != n_)
+    throw std::invalid_argument("symla: systems must cover the original indices 0..n-1");
+  for (int g = 0; g < ngroups; ++g)
+    if (groupStart[g + 1] <= groupStart[g]) throw std::invalid_argument("symla: every system needs at least one row");
+  std::vector<RefineResult> res(ngroups);
+  if (nrhs <= 0 || n_ == 0) return res;
+  const double* vals = aDev;
+  if (aHost) {
+    hipCheck(hipMemcpyAsync(dvals_.p, aHost, sizeof(double) * nnzA_, hipMemcpyHostToDevice, stream_), "values");
+    vals = dvals_.p;
+  }
+  ensureRefineWs(nrhs);
+  if (hgroupStart_.size() != static_cast<size_t>(ngroups) + 1 ||
+      !std::equal(hgroupStart_.begin(), hgroupStart_.end(), groupStart)) {
+    hgroupStart_.assign(groupStart, groupStart + ngroups + 1);
+    std::vector<int> rg(n_);
+    for (int g = 0; g < ngroups; ++g)
+      std::fill(rg.begin() + groupStart[g], rg.begin() + groupStart[g + 1], g);
+    drowGroup_.upload(rg, stream_);
+  }
+  dgAccept_.ensure(ngroups);
+  const size_t G = static_cast<size_t>(ngroups);
+  dgNorms_.ensure((5 + 3 * static_cast<size_t>(nrhs)) * G);
+  unsigned long long* slot = dgNorms_.p;  // slot k of system g at slot[k * G + g]
+  const size_t nn = static_cast<size_t>(n_) * nrhs;
+  hipCheck(hipMemcpy2DAsync(dBr_.p, sizeof(double) * n_, B, sizeof(double) * ldb, sizeof(double) * n_, nrhs,
+                            deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_),
+           "B copy");
+  launchZeroU64(slot, static_cast<int>(5 * G), stream_);
+  launchGroupRowSumMax(n_, dstPtr_.p, dstSrc_.p, vals, drowGroup_.p, slot + 0 * G, 1, stream_);
+  launchGroupRowSumMax(n_, dsymPtr_.p, dsymSrc_.p, vals, drowGroup_.p, slot + 1 * G, 1, stream_);
+  launchGroupAbsMax(n_, nrhs, dBr_.p, n_, drowGroup_.p, slot + 2 * G, 1, stream_);
+  launchGatherVals(nnzSym_, dsymSrc_.p, vals, dsymVal_.p, stream_);
+  double* Xc = dXout_.p;
+  double* Rc = dR_.p;
+  solveCore(dBr_.p, n_, nrhs, Xc, n_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xc, Rc, nullptr, stream_);
+  launchGroupAbsMax(n_, nrhs, Rc, n_, drowGroup_.p, slot + 3 * G, 1, stream_);
+  launchGroupAbsMax(n_, nrhs, Xc, n_, drowGroup_.p, slot + 4 * G, 1, stream_);
+  std::vector<unsigned long long> h(5 * G);
+  auto readSlots = [&](int first, int count) {
+    hipCheck(hipMemcpyAsync(h.data() + first * G, slot + first * G, count * G * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stream_),
+             "norms");
+    hipCheck(hipStreamSynchronize(stream_), "norms");
+  };
+  readSlots(0, 5);
+  std::vector<double> normA(G), normAsym(G), normB(G);
+  for (size_t g = 0; g < G; ++g) {
+    normA[g] = bitsToDoubleHost(h[g]);
+    normAsym[g] = bitsToDoubleHost(h[G + g]);
+    normB[g] = bitsToDoubleHost(h[2 * G + g]);
+  }
+  auto rel = [&](size_t g) {  // refine.hpp's relative residual of system g from slots 3 and 4
+    const double nR = bitsToDoubleHost(h[3 * G + g]), nX = bitsToDoubleHost(h[4 * G + g]);
+    if (!std::isfinite(nR) || !std::isfinite(nX)) return std::numeric_limits<double>::infinity();
+    const double denom = normA[g] * nX + normB[g];
+    return denom > 0.0 ? nR / denom : nR;
+  };
+  std::vector<double> r0(G);
+  std::vector<char> active(G);
+  size_t nActive = 0;
+  for (size_t g = 0; g < G; ++g) {
+    r0[g] = rel(g);
+    res[g].initialRel = res[g].finalRel = r0[g];
+    active[g] = r0[g] > tol;
+    nActive += active[g];
+  }
+  std::vector<int> accept(G, 0);
+  for (int it = 0; it < maxIt && nActive > 0; ++it) {  // refine.hpp:121-141, per system
+    solveCore(Rc, n_, nrhs, dDx_.p, n_);
+    launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, dXn_.p, nullptr, stream_);
+    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXn_.p, dRn_.p, nullptr, stream_);
+    launchZeroU64(slot + 3 * G, static_cast<int>(2 * G), stream_);
+    launchGroupAbsMax(n_, nrhs, dRn_.p, n_, drowGroup_.p, slot + 3 * G, 1, stream_);
+    launchGroupAbsMax(n_, nrhs, dXn_.p, n_, drowGroup_.p, slot + 4 * G, 1, stream_);
+    readSlots(3, 2);
+    for (size_t g = 0; g < G; ++g) {
+      accept[g] = 0;
+      if (!active[g]) continue;
+      const double rNew = rel(g);
+      if (rNew < r0[g]) {
+        accept[g] = 1;
+        r0[g] = rNew;
+        res[g].iterations = it + 1;
+        res[g].finalRel = rNew;
+        if (rNew <= tol) active[g] = 0;
+      } else {
+        active[g] = 0;  // no improvement: this system keeps its current solution
+      }
+    }
+    nActive = 0;
+    for (size_t g = 0; g < G; ++g) nActive += active[g];
+    hipCheck(hipMemcpyAsync(dgAccept_.p, accept.data(), G * sizeof(int), hipMemcpyHostToDevice, stream_), "accept");
+    launchGroupSelect(n_, nrhs, drowGroup_.p, dgAccept_.p, dXn_.p, Xc, n_, stream_);
+    launchGroupSelect(n_, nrhs, drowGroup_.p, dgAccept_.p, dRn_.p, Rc, n_, stream_);
+    hipCheck(hipStreamSynchronize(stream_), "accept");  // `accept` is rewritten next iteration
+  }
+  // backward error per system: max over its columns with its symmetric infinity norm
+  unsigned long long* cR = slot + 5 * G;
+  unsigned long long* cX = cR + static_cast<size_t>(nrhs) * G;
+  unsigned long long* cB = cX + static_cast<size_t>(nrhs) * G;
+  launchZeroU64(cR, static_cast<int>(3 * nrhs * G), stream_);
+  launchGroupAbsMaxCols(n_, nrhs, Rc, n_, drowGroup_.p, cR, 1, ngroups, stream_);
+  launchGroupAbsMaxCols(n_, nrhs, Xc, n_, drowGroup_.p, cX, 1, ngroups, stream_);
+  launchGroupAbsMaxCols(n_, nrhs, dBr_.p, n_, drowGroup_.p, cB, 1, ngroups, stream_);
+  std::vector<unsigned long long> hc(3 * static_cast<size_t>(nrhs) * G);
+  hipCheck(hipMemcpyAsync(hc.data(), cR, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_),
+           "be");
+  hipCheck(hipMemcpy2DAsync(X, sizeof(double) * ldx, Xc, sizeof(double) * n_, sizeof(double) * n_, nrhs,
+                            deviceIo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_),
+           "X out");
+  hipCheck(hipStreamSynchronize(stream_), "refine");
+  for (size_t g = 0; g < G; ++g) {
+    double be = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+      const size_t k = static_cast<size_t>(c) * G + g;
+      const double nr = bitsToDoubleHost(hc[k]), nx = bitsToDoubleHost(hc[nrhs * G + k]),
+                   nb = bitsToDoubleHost(hc[2 * nrhs * G + k]);
+      const double den = normAsym[g] * nx + nb;
+      const double bc = (!std::isfinite(nr) || !std::isfinite(nx)) ? std::numeric_limits<double>::infinity()
+                        : den > 0.0                                 ? nr / den
+                                                                    : nr;
+      be = std::max(be, bc);
+    }
+    res[g].backwardError = be;
+  }
+  checkSolveError();
+  return res;
+}
+
 void Plan::groupInertia(int ngroups, const int* groupStart, int* inertia3, int* singular) {
   if (!factorized_) throw std::logic_error("symla: no factorization");
   if (ngroups <= 0 || !groupStart || !inertia3) throw std::invalid_argument("symla: bad group arguments");

@@ -195,6 +195,14 @@ class Plan {
   // values: device pointer (already resident) after `setValuesDevice`.
   void factorize(const double* aValsHostOrNull, const double* aValsDev, const FactorOptions& opts, FactorInfo& info);
   void solve(const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx, bool deviceIo);
+  // Refinement of a batch of independent systems (a block-diagonal matrix
+  // whose system g owns the original indices [groupStart[g],
+  // groupStart[g+1])): refine.hpp's loop applied to each system on its own --
+  // own norms, own stopping tests, own iteration count -- while the
+  // correction solves run on the whole batch.  Returns one result per system.
+  std::vector<RefineResult> solveRefinedGroups(const double* aHost, const double* aDev, const double* B, int64_t ldb,
+                                               int nrhs, double* X, int64_t ldx, bool deviceIo, int maxIt,
+                                               double tol, int ngroups, const int* groupStart);
   // The device solve with its per-call workspace in `scratch` (capacity in
   // doubles) instead of plan buffers: allocates nothing.  Throws
   // std::invalid_argument when (3 n + 2 sum of front orders) x nrhs doubles
@@ -380,6 +388,11 @@ class Plan {
   DBuf<double> dsymVal_;  // symmetric CSR values of the matrix being refined against
   DBuf<double> dR_, dDx_, dXn_, dRn_, dBr_;
   DBuf<unsigned long long> dnorms_;
+  // per-system refinement of a batch: row -> system map (for the group
+  // boundaries in hgroupStart_), per-system norm slots, accept flags
+  std::vector<int> hgroupStart_;
+  DBuf<int> drowGroup_, dgAccept_;
+  DBuf<unsigned long long> dgNorms_;
   // state
   bool factorized_ = false;
   bool singular_ = false;

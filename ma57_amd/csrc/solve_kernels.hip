@@ -898,6 +898,77 @@ __global__ void k_rowsum_max(int n, const int64_t* __restrict__ rowPtr, const in
   groupAtomicMax(mx, outBits);
 }
 
+// ---- per-system norms of a batch of independent systems ----
+// The systems own consecutive row ranges; rowGroup[i] is the system of row
+// i.  A wave whose first and last rows belong to one system reduces before
+// its single atomic; a wave straddling a boundary falls back to one atomic
+// per lane.  Loops step by whole workgroups so every lane reaches every
+// shuffle.
+__device__ inline void segmentMax(double v, int g, unsigned long long* out, int stride) {
+  const int g0 = __shfl(g, 0, 64), g1 = __shfl(g, 63, 64);
+  if (g0 == g1) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && g0 >= 0 && v > 0.0)
+      atomicMax(out + static_cast<int64_t>(g0) * stride, static_cast<unsigned long long>(__double_as_longlong(v)));
+  } else if (g >= 0 && v > 0.0) {
+    atomicMax(out + static_cast<int64_t>(g) * stride, static_cast<unsigned long long>(__double_as_longlong(v)));
+  }
+}
+
+// out[g * stride] = max |v(i, c)| over the rows i of system g and all columns
+__global__ void k_group_absmax(int n, int nrhs, const double* __restrict__ v, int64_t ld,
+                               const int* __restrict__ rowGroup, unsigned long long* out, int stride) {
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    double mx = 0.0;
+    int g = -1;
+    if (i < n) {
+      g = rowGroup[i];
+      for (int c = 0; c < nrhs; ++c) mx = fmax(mx, normAbs(v[i + static_cast<int64_t>(c) * ld]));
+    }
+    segmentMax(mx, g, out, stride);
+  }
+}
+
+// out[g * stride + c * colStride] = max |v(i, c)| over the rows i of system g
+__global__ void k_group_absmax_cols(int n, int nrhs, const double* __restrict__ v, int64_t ld,
+                                    const int* __restrict__ rowGroup, unsigned long long* out, int stride,
+                                    int colStride) {
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    const int g = i < n ? rowGroup[i] : -1;
+    for (int c = 0; c < nrhs; ++c) {
+      const double a = i < n ? normAbs(v[i + static_cast<int64_t>(c) * ld]) : 0.0;
+      segmentMax(a, g, out + static_cast<int64_t>(c) * colStride, stride);
+    }
+  }
+}
+
+// out[g * stride] = max over the rows i of system g of sum |a| over row i's entries
+__global__ void k_group_rowsum_max(int n, const int64_t* __restrict__ rowPtr, const int64_t* __restrict__ valSrc,
+                                   const double* __restrict__ aVals, const int* __restrict__ rowGroup,
+                                   unsigned long long* out, int stride) {
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    double sacc = 0.0;
+    int g = -1;
+    if (i < n) {
+      g = rowGroup[i];
+      for (int64_t t = rowPtr[i]; t < rowPtr[i + 1]; ++t) sacc += normAbs(aVals[valSrc[t]]);
+    }
+    segmentMax(sacc, g, out, stride);
+  }
+}
+
+// dst(i, :) = src(i, :) for the rows of systems with accept[g] != 0
+__global__ void k_group_select(int n, int nrhs, const int* __restrict__ rowGroup, const int* __restrict__ accept,
+                               const double* __restrict__ src, double* __restrict__ dst, int64_t ld) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (!accept[rowGroup[i]]) continue;
+    for (int c = 0; c < nrhs; ++c) dst[i + static_cast<int64_t>(c) * ld] = src[i + static_cast<int64_t>(c) * ld];
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0ull;
@@ -1034,6 +1105,32 @@ void launchRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const 
                      unsigned long long* outBits, hipStream_t st) {
   k_rowsum_max<<<std::min(gridFor(n, 256), 2048), 256, 0, st>>>(n, rowPtr, valSrc, aVals, outBits);
   hipCheck(hipGetLastError(), "launchRowSumMax");
+}
+void launchGroupAbsMax(int n, int nrhs, const double* v, int64_t ld, const int* rowGroup, unsigned long long* out,
+                       int stride, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return;
+  k_group_absmax<<<std::min(gridFor(n, 256), 4096), 256, 0, st>>>(n, nrhs, v, ld, rowGroup, out, stride);
+  hipCheck(hipGetLastError(), "launchGroupAbsMax");
+}
+void launchGroupAbsMaxCols(int n, int nrhs, const double* v, int64_t ld, const int* rowGroup,
+                           unsigned long long* out, int stride, int colStride, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return;
+  k_group_absmax_cols<<<std::min(gridFor(n, 256), 4096), 256, 0, st>>>(n, nrhs, v, ld, rowGroup, out, stride,
+                                                                       colStride);
+  hipCheck(hipGetLastError(), "launchGroupAbsMaxCols");
+}
+void launchGroupRowSumMax(int n, const int64_t* rowPtr, const int64_t* valSrc, const double* aVals,
+                          const int* rowGroup, unsigned long long* out, int stride, hipStream_t st) {
+  if (n <= 0) return;
+  k_group_rowsum_max<<<std::min(gridFor(n, 256), 4096), 256, 0, st>>>(n, rowPtr, valSrc, aVals, rowGroup, out,
+                                                                      stride);
+  hipCheck(hipGetLastError(), "launchGroupRowSumMax");
+}
+void launchGroupSelect(int n, int nrhs, const int* rowGroup, const int* accept, const double* src, double* dst,
+                       int64_t ld, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return;
+  k_group_select<<<std::min(gridFor(n, 256), 4096), 256, 0, st>>>(n, nrhs, rowGroup, accept, src, dst, ld);
+  hipCheck(hipGetLastError(), "launchGroupSelect");
 }
 void launchZeroU64(unsigned long long* p, int count, hipStream_t st) {
   if (count <= 0) return;

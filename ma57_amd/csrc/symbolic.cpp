@@ -479,6 +479,34 @@ std::vector<int> nestedDissectionOrder(const Graph& g) {
   return out;
 }
 
+SymbolicResult concatenateAnalyses(const std::vector<SymbolicResult>& parts) {
+  SymbolicResult r;
+  for (const auto& p : parts) r.n += p.n;
+  r.perm.reserve(r.n);
+  r.etreeParent.reserve(r.n);
+  r.colCount.reserve(r.n);
+  size_t nsn = 0;
+  for (const auto& p : parts) nsn += p.supernodes.size();
+  r.supernodes.reserve(nsn);
+  int off = 0;
+  for (const auto& p : parts) {
+    for (int v : p.perm) r.perm.push_back(v + off);
+    for (int v : p.etreeParent) r.etreeParent.push_back(v < 0 ? -1 : v + off);
+    r.colCount.insert(r.colCount.end(), p.colCount.begin(), p.colCount.end());
+    for (const Supernode& s : p.supernodes) {
+      Supernode t;
+      t.firstCol = s.firstCol + off;
+      t.ncols = s.ncols;
+      t.rowPattern.reserve(s.rowPattern.size());
+      for (int v : s.rowPattern) t.rowPattern.push_back(v + off);
+      r.supernodes.push_back(std::move(t));
+    }
+    r.nnzL += p.nnzL;
+    off += p.n;
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Elimination tree and postorder.
 // ---------------------------------------------------------------------------

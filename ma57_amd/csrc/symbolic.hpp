@@ -64,6 +64,12 @@ std::vector<int> nestedDissectionOrder(const Graph& g);
 SymbolicResult analyze(int n, const int64_t* colptr, const int* rowidx, Ordering ordering,
                        const SymbolicOptions& opts, const int* userPerm = nullptr);
 
+// Block-diagonal union of independent analyses: system s's indices shift by
+// the sizes of the systems before it.  Concatenated postordered forests stay
+// postordered, so the result is the analysis of the block-diagonal matrix
+// (each system analyzed on its own: orderings never mix systems).
+SymbolicResult concatenateAnalyses(const std::vector<SymbolicResult>& parts);
+
 // Elimination tree (Liu, path compression) of the graph under permutation
 // `perm`; returns parent in permuted index space.
 std::vector<int> eliminationTree(const Graph& g, const std::vector<int>& perm, const std::vector<int>& inv);

@@ -149,8 +149,15 @@ def analyze(A, ordering: int = OrderingType.AMD, options: Optional[SymbolicFacto
     rc = lib.symla_analyze(n, ptr(cp, i64p), ptr(ri, i32p), ordering, ptr(up, i32p) if up is not None else None,
                            opts.max_relax_size, opts.max_relax_fill_fraction, C.byref(h))
     _raise(rc, lib.symla_last_error().decode())
+    return h, symbolic_factor(h)
+
+
+def symbolic_factor(h) -> "SymbolicFactor":
+    """The SymbolicFactor fields of an analysis handle (symla_symbolic_get)."""
+    lib = _lib.load()
     info = _lib.SymbolicInfo()
     _raise(lib.symla_symbolic_info(h, C.byref(info)), lib.symla_last_error().decode())
+    n = info.n
     perm = np.empty(n, np.int32)
     par = np.empty(n, np.int32)
     first = np.empty(info.nsn, np.int32)
@@ -160,7 +167,7 @@ def analyze(A, ordering: int = OrderingType.AMD, options: Optional[SymbolicFacto
     _raise(lib.symla_symbolic_get(h, ptr(perm, i32p), ptr(par, i32p), ptr(first, i32p), ptr(nc, i32p),
                                   ptr(rp, i64p), ptr(rows, i32p)), lib.symla_last_error().decode())
     sns = [Supernode(int(first[i]), int(nc[i]), rows[rp[i]:rp[i + 1]].copy()) for i in range(info.nsn)]
-    return h, SymbolicFactor(perm, par, sns, int(info.nnz_l))
+    return SymbolicFactor(perm, par, sns, int(info.nnz_l))
 
 
 class SymLDLT:
