@@ -449,8 +449,8 @@ def main() -> int:
                       "max_front": max(len(s.rowPattern) for s in s2.symbolicFactor().supernodes)}
         del s2
         print(f"[bench] cross-check inertia {expect}", file=sys.stderr, flush=True)
-    if expect is None and fp64 is not None:
-        expect = tuple(fp64["inertia"])
+    if expect is None:  # single-variant (profiling) runs compare against their own inertia
+        expect = tuple((fp64 if fp64 is not None else mixed)["inertia"])
 
     def traffic(kernel):
         """HBM bytes per dispatch from the committed counter passes of the same workload, or None."""
