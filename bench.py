@@ -146,16 +146,22 @@ def algorithmic_work(sf, nnz_a, nrhs):
     }
 
 
-def rooflines(r, work, es):
-    """Per-kernel-class rooflines of one variant (es = bytes per front entry: 4 FP32, 8 FP64)."""
+PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def rooflines(r, work, es, bf16x3=False):
+    """Per-kernel-class rooflines of one variant (es = bytes per front entry: 4 FP32, 8 FP64).  With BF16x3
+    contribution-block updates their flops count once (FP32-equivalent) against a third of the BF16 peak
+    (three BF16 products per FP32-equivalent product)."""
     ms = r["kernel_ms_per_step"]
     peak_mfma = PEAK_FP32_MFMA_TFLOPS if es == 4 else PEAK_FP64_MFMA_TFLOPS
     tname = "float" if es == 4 else "double"
     out = {}
 
-    def mfma(name, tflops, ms_step, gflop):
-        out[name] = {"bound": "mfma", "achieved": tflops, "peak": peak_mfma, "unit": "TFLOP/s",
-                     "frac": tflops / peak_mfma, "ms_per_step": ms_step, "gflop_per_step": gflop}
+    def mfma(name, tflops, ms_step, gflop, peak=None):
+        pk = peak or peak_mfma
+        out[name] = {"bound": "mfma", "achieved": tflops, "peak": pk, "unit": "TFLOP/s",
+                     "frac": tflops / pk, "ms_per_step": ms_step, "gflop_per_step": gflop}
 
     def hbm(name, nbytes, ms_step, note=None):
         gbs = nbytes / (ms_step * 1e-3) / 1e9 if ms_step > 0 else 0.0
@@ -166,7 +172,12 @@ def rooflines(r, work, es):
 
     mfma(f"k_schur<{tname}>", r["syrk"]["achieved"], ms["syrk"], r["syrk"]["schur_gflop_per_step"])
     if r["cb_update"]["launches_per_step"] > 0:
-        mfma(f"k_form_w + k_cb_syrk<{tname}>", r["cb_update"]["achieved"], ms["cb"], r["cb_update"]["gflop_per_step"])
+        if bf16x3:
+            mfma("k_form_w + k_cb_syrk_bf16x3", r["cb_update"]["achieved"], ms["cb"], r["cb_update"]["gflop_per_step"],
+                 PEAK_BF16_MFMA_TFLOPS / 3.0)
+        else:
+            mfma(f"k_form_w + k_cb_syrk<{tname}>", r["cb_update"]["achieved"], ms["cb"],
+                 r["cb_update"]["gflop_per_step"])
     hbm("k_assemble", work["front_entries"] * es + work["cb_entries"] * es + work["cb_map_bytes"] + work["a_bytes"],
         ms["assemble"])
     nsol = 1 + r["ir_iterations"]
@@ -267,7 +278,7 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=0, help="right-hand sides (default per config)")
     ap.add_argument("--systems", type=int, default=2048, help="c4: number of independent systems")
     ap.add_argument("--replays", type=int, default=10, help="c5: recorded refactorize + solve replays")
-    ap.add_argument("--legs", default="both", choices=["both", "fp32", "fp64"],
+    ap.add_argument("--legs", default="both", choices=["both", "fp32", "fp64", "bf16x3"],
                     help="variants to time (profiling runs take one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-crosscheck", action="store_true", help="skip the second-ordering FP64 inertia check")
@@ -418,7 +429,8 @@ def main() -> int:
             "schur_total_tflops": ((prof.schur_flops + prof.cb_flops) / ((prof.syrk_ms + prof.cb_ms) * 1e-3) / 1e12
                                    if prof.syrk_ms + prof.cb_ms > 0 else 0.0),
         }
-        res["rooflines"] = rooflines(res, work, 4 if precision == Precision.FP32 else 8)
+        res["rooflines"] = rooflines(res, work, 8 if precision == Precision.FP64 else 4,
+                                     bf16x3=precision == Precision.BF16X3)
         if batch is not None:
             # per-system refinement: iterations and backward error of every system
             if refine:
@@ -426,14 +438,18 @@ def main() -> int:
                 bes = [i.backward_error for i in sys_infos]
                 res["per_system"] = {"ir_iterations_min": min(its), "ir_iterations_max": max(its),
                                      "ir_iterations_mean": float(np.mean(its)), "backward_error_max": max(bes)}
-        print(f"[bench] {'FP32+IR' if precision == Precision.FP32 else 'FP64'}: {res['s'] * 1e3:.1f} ms/step, "
+        label = {Precision.FP32: "FP32+IR", Precision.FP64: "FP64", Precision.BF16X3: "BF16x3+IR"}[precision]
+        print(f"[bench] {label}: {res['s'] * 1e3:.1f} ms/step, "
               f"BE {res['backward_error']:.2e}, IR {res['ir_iterations']}", file=sys.stderr, flush=True)
         return res
 
     # the plain FP64 variant solves without refinement, except where the matrix needs it to reach the
     # 1e-12 bar (static perturbations on c3; the 27-point system's conditioning on c5)
-    fp64 = leg(Precision.FP64, refine=args.config in ("c3", "c5")) if args.legs != "fp32" else None
-    mixed = leg(Precision.FP32, refine=True) if args.legs != "fp64" else None
+    fp64 = leg(Precision.FP64, refine=args.config in ("c3", "c5")) if args.legs in ("both", "fp64") else None
+    mixed = leg(Precision.FP32, refine=True) if args.legs in ("both", "fp32") else None
+    # FP32 fronts whose deferred contribution-block updates (the largest fronts) run as BF16x3 split products
+    bf16x3 = leg(Precision.BF16X3, refine=True) if (args.legs == "bf16x3" or
+                                                    (args.legs == "both" and args.config == "c5")) else None
 
     crosscheck = None
     if expect is None and fp64 is not None and not args.no_crosscheck and args.config == "c5":
@@ -450,7 +466,7 @@ def main() -> int:
         del s2
         print(f"[bench] cross-check inertia {expect}", file=sys.stderr, flush=True)
     if expect is None:  # single-variant (profiling) runs compare against their own inertia
-        expect = tuple((fp64 if fp64 is not None else mixed)["inertia"])
+        expect = tuple(next(r for r in (fp64, mixed, bf16x3) if r is not None)["inertia"])
 
     def traffic(kernel):
         """HBM bytes per dispatch from the committed counter passes of the same workload, or None."""
@@ -519,7 +535,8 @@ def main() -> int:
     ok_graph = graph is None or ok_graph
     ok64 = fp64 is None or (tuple(fp64["inertia"]) == tuple(expect) and fp64["backward_error"] <= BE_BAR)
     ok32 = mixed is None or (tuple(mixed["inertia"]) == tuple(expect) and mixed["backward_error"] <= BE_BAR)
-    head = mixed if mixed is not None else fp64
+    okbf = bf16x3 is None or (tuple(bf16x3["inertia"]) == tuple(expect) and bf16x3["backward_error"] <= BE_BAR)
+    head = next(r for r in (mixed, fp64, bf16x3) if r is not None)
     out = {
         "metric": "factorize+solve s, FP32+IR vs FP64 at backward err <=1e-12; Schur TFLOP/s",
         "value": head["s"],
@@ -549,6 +566,8 @@ def main() -> int:
         "fp64_s": fp64["s"] if fp64 else None,
         "speedup_fp32ir_vs_fp64": (fp64["s"] / mixed["s"]) if (fp64 and mixed) else None,
         "roofline_fp64": main_roofline(fp64, 8) if fp64 else None,
+        "bf16x3_ir": bf16x3,
+        "bf16x3_s": bf16x3["s"] if bf16x3 else None,
         "backward_error": head["backward_error"],
         "inertia": head["inertia"],
         "inertia_expected": list(expect) if expect is not None else None,
@@ -559,7 +578,7 @@ def main() -> int:
         "graph": graph,
     }
     print(json.dumps(out))
-    return 0 if (ok64 and ok32 and ok_graph) else 1
+    return 0 if (ok64 and ok32 and okbf and ok_graph) else 1
 
 
 if __name__ == "__main__":

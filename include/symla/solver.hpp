@@ -44,8 +44,10 @@ enum class Mode { ThresholdPivot, StaticRegularized };       // solver.hpp:20-23
 enum class OrderingType { AMD, Metis, Natural };              // ordering.hpp:31-35
 // Front storage precision (no counterpart in the reference): FP32 keeps the
 // fronts, multipliers and Schur updates in FP32 with FP64 pivots, D and
-// inertia; solveWithRefinement then restores FP64 accuracy.
-enum class Precision { FP64, FP32 };
+// inertia; solveWithRefinement then restores FP64 accuracy.  BF16x3: FP32
+// fronts whose deferred contribution-block updates run as BF16x3 split
+// products accumulated in FP32.
+enum class Precision { FP64, FP32, BF16x3 };
 
 struct Inertia {  // inertia.hpp:14-18
   int n_pos = 0;
@@ -357,7 +359,9 @@ class SymLDLT<double> {
   // Front precision of the last factorization (FP32 static-regularized
   // factorizations may fall back to FP64; no reference counterpart).
   Precision precisionUsed() const {
-    return last_info_.precision_used == SYMLA_PREC_FP32 ? Precision::FP32 : Precision::FP64;
+    return last_info_.precision_used == SYMLA_PREC_FP32    ? Precision::FP32
+           : last_info_.precision_used == SYMLA_PREC_BF16X3 ? Precision::BF16x3
+                                                             : Precision::FP64;
   }
 
   void analyzePattern(const SparseMatrix& A, const SymbolicFactorOptions& options = SymbolicFactorOptions()) {
@@ -398,7 +402,9 @@ class SymLDLT<double> {
     o.growth = regularization_growth_;
     o.static_rel_floor = static_pivot_relative_floor_;
     o.static_abs_floor = static_pivot_absolute_floor_;
-    o.precision = precision_ == Precision::FP32 ? SYMLA_PREC_FP32 : SYMLA_PREC_FP64;
+    o.precision = precision_ == Precision::FP32    ? SYMLA_PREC_FP32
+                  : precision_ == Precision::BF16x3 ? SYMLA_PREC_BF16X3
+                                                    : SYMLA_PREC_FP64;
     std::vector<int> signs;
     if (mode_ == Mode::StaticRegularized) {
       signs = buildExpectedSignOriginal(n_);

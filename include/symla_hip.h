@@ -33,8 +33,10 @@ enum { SYMLA_ORDER_AMD = 0, SYMLA_ORDER_METIS = 1, SYMLA_ORDER_NATURAL = 2, SYML
 /* Mode (include/symla/solver.hpp:20-23). */
 enum { SYMLA_MODE_THRESHOLD = 0, SYMLA_MODE_STATIC = 1 };
 
-/* Front storage precision (symla_factor_opts.precision). */
-enum { SYMLA_PREC_FP64 = 0, SYMLA_PREC_FP32 = 1 };
+/* Front storage precision (symla_factor_opts.precision).  SYMLA_PREC_BF16X3:
+ * FP32 fronts whose deferred contribution-block updates (the largest fronts'
+ * Schur complements) run as BF16x3 split products accumulated in FP32. */
+enum { SYMLA_PREC_FP64 = 0, SYMLA_PREC_FP32 = 1, SYMLA_PREC_BF16X3 = 2 };
 
 typedef struct symla_symbolic symla_symbolic;
 typedef struct symla_plan symla_plan;
@@ -105,7 +107,10 @@ typedef struct {
   /* Front storage precision (no counterpart in the reference, which factors
    * in FP64): SYMLA_PREC_FP64, or SYMLA_PREC_FP32 = fronts, multipliers and
    * Schur updates in FP32 (FP32 MFMA), with pivot tests, 2x2 inversion, D
-   * and inertia in FP64; pair with symla_solve_refined for FP64 accuracy. */
+   * and inertia in FP64; pair with symla_solve_refined for FP64 accuracy.
+   * SYMLA_PREC_BF16X3: as FP32, with the deferred contribution-block updates
+   * as BF16x3 split products (hi = bf16(x), lo = bf16(x - hi); a b ~ hi hi +
+   * hi lo + lo hi, FP32 accumulation). */
   int precision;
 } symla_factor_opts;
 

@@ -21,7 +21,7 @@ SYMLA_ERR_DEVICE = 4
 
 ORDER_AMD, ORDER_METIS, ORDER_NATURAL, ORDER_USER = 0, 1, 2, 3
 MODE_THRESHOLD, MODE_STATIC = 0, 1
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 
 i32p = C.POINTER(C.c_int)
 i64p = C.POINTER(C.c_int64)

@@ -78,7 +78,13 @@ FactorOptions toFactorOptions(const symla_factor_opts* o, int n) {
   f.growth = o->growth;
   f.staticRelFloor = o->static_rel_floor;
   f.staticAbsFloor = o->static_abs_floor;
-  f.precision = o->precision;
+  // SYMLA_PREC_BF16X3 keeps FP32 fronts; only the deferred contribution-block updates change
+  if (o->precision == SYMLA_PREC_BF16X3) {
+    f.precision = SYMLA_PREC_FP32;
+    f.bf16x3 = true;
+  } else {
+    f.precision = o->precision;
+  }
   return f;
 }
 

@@ -194,7 +194,134 @@ __global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restri
   }
 }
 
+// BF16x3 split products (Henry, Tang and Heinecke, "Leveraging the bfloat16
+// artificial intelligence datatype for higher-precision computations", ARITH
+// 2019): each FP32 operand x is split as hi = bf16(x), lo = bf16(x - hi), and
+// a b is formed as hi_a hi_b + hi_a lo_b + lo_a hi_b (the lo_a lo_b term,
+// below 2^-16 of |a b|, is dropped) on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation: three bf16 products of 32 k each against eight FP32
+// 16x16x4 products.  Same tiling, super-tile order and transposed product as
+// k_cb_syrk; the operands are split while they are staged into LDS (hi and
+// lo planes, [row][k], 16-byte row pad).  Layout of the instruction on gfx950
+// (measured): A[i][k] from lane i + 16 (k / 8) element k % 8, B[k][j] from
+// lane j + 16 (k / 8) element k % 8, D[i][j] in lane j + 16 (i / 4) register
+// i % 4 -- the D layout of v_mfma_f32_16x16x4f32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_cb_syrk_bf16x3(int N, int K, const float* __restrict__ W, int ldw,
+                                                        const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                        int ldc, int nt, int per) {
+  constexpr int TS = kSyTile;
+  constexpr int KT = 32;       // k per step = one bf16 instruction
+  constexpr int KH = KT / 2;   // staged k columns per thread
+  constexpr int LDK = KT + 8;  // 16-byte row pad
+  __shared__ __attribute__((aligned(16))) __bf16 Wh[2][TS][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Wl[2][TS][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][TS][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][TS][LDK];
+
+  const int total = nt * (nt + 1) / 2;
+  const int t = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (t >= total) return;
+  int ti, tj;
+  syrkTile(t, nt, ti, tj);
+  const int I0 = ti * TS, J0 = tj * TS;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int sr = tid & (TS - 1), sh = tid >> 7;
+  const int gi = I0 + sr, gj = J0 + sr;
+  const bool wIn = gi < N, bIn = gj < N;
+  const float* Wr = W + gi;
+  const float* Br = B + gj;
+
+  float wreg[KH], breg[KH];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < KH; ++u) {
+      const int k = k0 + sh * KH + u;
+      const bool kin = k < K;
+      wreg[u] = (wIn && kin) ? Wr[static_cast<size_t>(k) * ldw] : 0.0f;
+      breg[u] = (bIn && kin) ? Br[static_cast<size_t>(k) * ldb] : 0.0f;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < KH; ++u) {
+      const __bf16 wh = static_cast<__bf16>(wreg[u]);
+      const __bf16 bh = static_cast<__bf16>(breg[u]);
+      Wh[buf][sr][sh * KH + u] = wh;
+      Wl[buf][sr][sh * KH + u] = static_cast<__bf16>(wreg[u] - static_cast<float>(wh));
+      Bh[buf][sr][sh * KH + u] = bh;
+      Bl[buf][sr][sh * KH + u] = static_cast<__bf16>(breg[u] - static_cast<float>(bh));
+    }
+  };
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f32x4{0, 0, 0, 0};
+
+  const int nk = (K + KT - 1) / KT;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * KT);
+    bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      ah[bj] = *reinterpret_cast<const bf16x8*>(&Bh[buf][wc + bj * 16 + lr][lk * 8]);
+      al[bj] = *reinterpret_cast<const bf16x8*>(&Bl[buf][wc + bj * 16 + lr][lk * 8]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      bh[bi] = *reinterpret_cast<const bf16x8*>(&Wh[buf][wr + bi * 16 + lr][lk * 8]);
+      bl[bi] = *reinterpret_cast<const bf16x8*>(&Wl[buf][wr + bi * 16 + lr][lk * 8]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bj], bh[bi], acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bj], bl[bi], acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bj], bh[bi], acc[bi][bj], 0, 0, 0);
+      }
+    if (kt + 1 < nk) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int row = I0 + wr + bi * 16 + lr;
+    if (row >= N) continue;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = J0 + wc + bj * 16 + 4 * lk + r;
+        if (col < N && (ti != tj || row >= col)) {
+          float* c = C + row + static_cast<size_t>(col) * ldc;
+          *c = *c - acc[bi][bj][r];
+        }
+      }
+  }
+}
+
 }  // namespace
+
+void launchCbSyrkBf16x3(int N, int K, const float* W, int ldw, const float* B, int ldb, float* C, int ldc,
+                        hipStream_t st) {
+  if (N <= 0 || K <= 0) return;
+  const int nt = (N + kSyTile - 1) / kSyTile;
+  const int total = nt * (nt + 1) / 2;
+  const int per = (total + kXcds - 1) / kXcds;
+  k_cb_syrk_bf16x3<<<dim3(per * kXcds), 256, 0, st>>>(N, K, W, ldw, B, ldb, C, ldc, nt, per);
+  hipCheck(hipGetLastError(), "launchCbSyrkBf16x3");
+}
 
 void launchCbSyrk(bool fp32, int N, int K, const void* W, int ldw, const void* B, int ldb, void* C, int ldc,
                   hipStream_t st) {

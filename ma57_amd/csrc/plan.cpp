@@ -360,6 +360,7 @@ CbUpdate Plan::cbUpdate(const std::vector<CbTask>& tasks) {
   CbUpdate c;
   c.F = &hF_;
   c.fp32 = prec_ != 0;
+  c.bf16x3 = prec_ != 0 && cbBf16x3_;
   if (!tasks.empty()) {
     size_t need = 0;
     for (const CbTask& t : tasks) need = std::max(need, static_cast<size_t>(t.m - t.nE) * kCbChunk);
@@ -638,7 +639,11 @@ void runLargeSched(const LargeSched& ls, const LargeDev& dv, const DevFronts& fd
       launchFormW(fd, t.s, N, q0, kc, cb->w, st);
       const void* B = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(q0) * t.m);
       void* C = F + esz * (static_cast<size_t>(t.nE) + static_cast<size_t>(t.nE) * t.m);
-      launchCbSyrk(cb->fp32, N, kc, cb->w, N, B, t.m, C, t.m, st);
+      if (cb->bf16x3)
+        launchCbSyrkBf16x3(N, kc, static_cast<const float*>(cb->w), N, static_cast<const float*>(B), t.m,
+                           static_cast<float*>(C), t.m, st);
+      else
+        launchCbSyrk(cb->fp32, N, kc, cb->w, N, B, t.m, C, t.m, st);
       if (prof) {
         prof->profEnd(Plan::kProfCb);
         prof->addCbFlops(static_cast<double>(kc) * N * (N + 1.0));
@@ -904,6 +909,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
   }
   info = FactorInfo{};
   if (opts.precision != 0 && opts.precision != 1) throw std::invalid_argument("symla: unknown precision");
+  cbBf16x3_ = opts.bf16x3 && opts.precision == 1;
   if (opts.precision != prec_) {
     // the front storage changes element size: re-plan the pool
     prec_ = opts.precision;
@@ -924,7 +930,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
     }
     info.singular = singular_;
     info.nSingular = static_cast<int>(singularColsOrig_.size());
-    info.precisionUsed = prec_;
+    info.precisionUsed = precisionCode();
     hipCheck(hipEventRecord(ev1_, stream_), "event");
     hipCheck(hipEventSynchronize(ev1_), "event");
     float ms = 0.f;
@@ -977,7 +983,7 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
       collectStats(info, anyDelay);
       info.lastDelta = delta;
       info.retriesUsed = attempt;
-      info.precisionUsed = prec_;
+      info.precisionUsed = precisionCode();
       const bool ok = !haveExp || (info.zero == 0 && !singular_ &&
                                    (!fullCoverage || (info.pos == expectedPos && info.neg == expectedNeg)));
       if (ok) return true;
@@ -1559,6 +1565,7 @@ void Plan::graphRecord() {
     e = nullptr;
   }
   const GraphArgs& g = graphArgs_;
+  cbBf16x3_ = g.opts.bf16x3 && g.opts.precision == 1;
   if (g.opts.precision != prec_) {
     prec_ = g.opts.precision;
     layoutDirty_ = true;
@@ -1670,7 +1677,7 @@ void Plan::graphReplay(FactorInfo& info, RefineResult* rres) {
   hipCheck(hipGraphLaunch(graphExec_, stream_), "graph launch");
   bool anyDelay = false;
   collectStats(info, anyDelay);
-  info.precisionUsed = prec_;
+  info.precisionUsed = precisionCode();
   checkSolveError();
   if (anyDelay) {
     // the recorded schedule assumed no delays: repair and solve again
@@ -1736,7 +1743,8 @@ std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int p
   dXt.alloc(hb.size());
   hipCheck(hipMemcpy(dB.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice), "autotune rhs");
   FactorOptions o;
-  o.precision = precision;
+  o.precision = precision == 2 ? 1 : precision;  // SYMLA_PREC_BF16X3: FP32 fronts, BF16x3 contribution blocks
+  o.bf16x3 = precision == 2;
   std::vector<TuneTrial> trials;
   const int smalls[] = {0, 64, 96, 128, 192};
   const int tiles[] = {32, 64, 128};

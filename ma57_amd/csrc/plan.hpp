@@ -68,6 +68,7 @@ struct DBuf {
 struct FactorOptions {
   int mode = 0;
   int precision = 0;  // 0: FP64 fronts; 1: FP32 fronts (FP64 pivots, D and refinement)
+  bool bf16x3 = false;  // FP32 fronts whose deferred contribution-block updates use BF16x3 split products
   double zeroTol = 1e-300;
   double relFloor = 1e-12;
   std::vector<int> expectedSignOrig;  // empty = none
@@ -167,6 +168,7 @@ struct CbUpdate {
   size_t wElems = 0;
   const std::vector<void*>* F = nullptr;
   bool fp32 = false;
+  bool bf16x3 = false;  // FP32 fronts: BF16x3 split products instead of FP32 MFMA
 };
 // m, nE and cut are indexed by front id (cut[s] < m[s]: the contribution
 // block is deferred); allowMw = false keeps every panel on one workgroup.
@@ -186,6 +188,9 @@ double* noAllocScratch(int device);
 // (column-major; W, B: N x K) on the matrix cores, in the front precision
 void launchCbSyrk(bool fp32, int N, int K, const void* W, int ldw, const void* B, int ldb, void* C, int ldc,
                   hipStream_t st);
+// the same on FP32 fronts as BF16x3 split products accumulated in FP32
+void launchCbSyrkBf16x3(int N, int K, const float* W, int ldw, const float* B, int ldb, float* C, int ldc,
+                        hipStream_t st);
 
 class Plan {
  public:
@@ -399,6 +404,9 @@ class Plan {
   std::vector<int> singularColsOrig_;
   bool layoutDirty_ = false;  // dynamic mode changed the layout
   int prec_ = 0;              // precision of the current front storage
+  bool cbBf16x3_ = false;     // FP32 fronts: deferred contribution-block updates as BF16x3 products
+  // SYMLA_PREC_* of the current factorization (2: FP32 fronts with BF16x3 contribution-block updates)
+  int precisionCode() const { return prec_ == 0 ? 0 : (cbBf16x3_ ? 2 : 1); }
   size_t elemSize() const { return prec_ ? sizeof(float) : sizeof(double); }
   double factorMs_ = 0.0, solveMs_ = 0.0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;

@@ -63,6 +63,7 @@ class OrderingType:
 class Precision:
     FP64 = _lib.PREC_FP64
     FP32 = _lib.PREC_FP32
+    BF16X3 = _lib.PREC_BF16X3  # FP32 fronts; deferred contribution-block updates as BF16x3 split products
 
 
 @dataclass
@@ -207,7 +208,7 @@ class SymLDLT:
     def setPrecision(self, precision):
         """Front storage precision (no counterpart in the reference): Precision.FP64, or Precision.FP32
         (FP32 fronts and Schur updates, FP64 pivots/D/inertia; use solveWithRefinement for FP64 accuracy)."""
-        if precision not in (Precision.FP64, Precision.FP32):
+        if precision not in (Precision.FP64, Precision.FP32, Precision.BF16X3):
             raise InvalidArgument("symla: unknown precision")
         self.precision = precision
 

@@ -134,3 +134,25 @@ def test_graph_replays_with_deferred_cb_match_eager():
             junk = torch.randn(4096, device=dev, dtype=torch.float64)
             assert torch.isfinite(junk.sum()).item()
             torch.cuda.synchronize()
+
+
+def test_bf16x3_deferred_cb_refines_to_fp64_accuracy():
+    """BF16x3 split products for the deferred contribution-block updates (FP32 fronts): the same inertia as
+    FP64, FP64 refinement reaches the 1e-12 bar, and the unrefined solution sits between the FP32 one and a
+    plain-BF16 update's (the dropped lo*lo term is below 2^-16 of each product, against 2^-24 for FP32 and
+    2^-8 for one bf16 product)."""
+    A = P.laplace3d(32, 32, 32, 0.5)
+    b = np.random.default_rng(7).standard_normal((A.shape[0], 2))
+    expect = tuple(P.closed_form_inertia(P.laplace_eigs_3d(32, 32, 32), 0.5)[:3])
+    unrefined = {}
+    for prec in (Precision.FP32, Precision.BF16X3):
+        s = _sparse(A, 64, prec)
+        inr = s.inertia()
+        assert (inr.n_pos, inr.n_neg, inr.n_zero) == expect
+        assert s.last_info.precision_used == prec
+        unrefined[prec] = P.backward_error(A, s.solve(b), b)
+        r = s.solveWithRefinement(A, b)
+        assert P.backward_error(A, r.x, b) <= 1e-12
+        assert r.diagnostics.iterations >= 1
+    assert unrefined[Precision.BF16X3] < 1e-3
+    assert unrefined[Precision.BF16X3] <= 200.0 * unrefined[Precision.FP32], unrefined
