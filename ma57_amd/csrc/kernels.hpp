@@ -78,9 +78,11 @@ void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
                    double* Dl, const int64_t* gOff, hipStream_t st);
 void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
                    hipStream_t st);
-// R = B - A X over symmetric CSR values aSym; max |R| into *rmaxBits if non-null
+// R = B - A X over symmetric CSR values aSym (X, B, R: n x nrhs column-major, ld n); max |R| into
+// *rmaxBits if non-null.  Xt (n x nrhs doubles, may be null): workspace for a row-major copy of X, used
+// from 16 right-hand sides on
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
-                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st);
+                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st, double* Xt = nullptr);
 void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, double* aSym, hipStream_t st);
 // out = x + dx; max |out| into *maxBits if non-null
 void launchAxpy(int64_t len, const double* x, const double* dx, double* out, unsigned long long* maxBits,

@@ -1268,6 +1268,7 @@ void Plan::ensureRefineWs(int nrhs) {
   dXn_.ensure(nn);
   dRn_.ensure(nn);
   dXout_.ensure(nn);
+  if (nrhs >= 16) dXt_.ensure(nn);  // row-major copy of X for the many-right-hand-side residual
   dnorms_.ensure(8 + 3 * static_cast<size_t>(nrhs));
   dsymVal_.ensure(static_cast<size_t>(nnzSym_));
 }
@@ -1280,7 +1281,8 @@ void Plan::refineSetup(const double* vals, int nrhs) {
   launchGatherVals(nnzSym_, dsymSrc_.p, vals, dsymVal_.p, stream_);
   solveCore(dBr_.p, n_, nrhs, dXout_.p, n_);
   // max |R| and max |X| accumulate into slots 3 and 4
-  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXout_.p, dR_.p, dnorms_.p + 3, stream_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXout_.p, dR_.p, dnorms_.p + 3, stream_,
+                 dXt_.p);
   launchAbsMax(n_, nrhs, dXout_.p, n_, dnorms_.p + 4, stream_);
 }
 
@@ -1288,7 +1290,7 @@ void Plan::refineStep(const double* Xc, const double* Rc, double* Xn, double* Rn
   solveCore(Rc, n_, nrhs, dDx_.p, n_);
   launchZeroU64(dnorms_.p + 3, 2, stream_);
   launchAxpy(static_cast<int64_t>(n_) * nrhs, Xc, dDx_.p, Xn, dnorms_.p + 4, stream_);
-  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xn, Rn, dnorms_.p + 3, stream_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xn, Rn, dnorms_.p + 3, stream_, dXt_.p);
 }
 
 Plan::RefineNorms Plan::readNorms() {
@@ -1437,7 +1439,7 @@ std::vector<RefineResult> Plan::solveRefinedGroups(const double* aHost, const do
   double* Xc = dXout_.p;
   double* Rc = dR_.p;
   solveCore(dBr_.p, n_, nrhs, Xc, n_);
-  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xc, Rc, nullptr, stream_);
+  launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, Xc, Rc, nullptr, stream_, dXt_.p);
   launchGroupAbsMax(n_, nrhs, Rc, n_, drowGroup_.p, slot + 3 * G, 1, stream_);
   launchGroupAbsMax(n_, nrhs, Xc, n_, drowGroup_.p, slot + 4 * G, 1, stream_);
   std::vector<unsigned long long> h(5 * G);
@@ -1473,7 +1475,8 @@ std::vector<RefineResult> Plan::solveRefinedGroups(const double* aHost, const do
   for (int it = 0; it < maxIt && nActive > 0; ++it) {  // refine.hpp:121-141, per system
     solveCore(Rc, n_, nrhs, dDx_.p, n_);
     launchAxpy(static_cast<int64_t>(nn), Xc, dDx_.p, dXn_.p, nullptr, stream_);
-    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXn_.p, dRn_.p, nullptr, stream_);
+    launchResidual(n_, nrhs, dsymPtr_.p, dsymCol_.p, dsymVal_.p, dBr_.p, dXn_.p, dRn_.p, nullptr, stream_,
+                   dXt_.p);
     launchZeroU64(slot + 3 * G, static_cast<int>(2 * G), stream_);
     launchGroupAbsMax(n_, nrhs, dRn_.p, n_, drowGroup_.p, slot + 3 * G, 1, stream_);
     launchGroupAbsMax(n_, nrhs, dXn_.p, n_, drowGroup_.p, slot + 4 * G, 1, stream_);

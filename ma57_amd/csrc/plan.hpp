@@ -391,7 +391,7 @@ class Plan {
   DBuf<int64_t> dsymPtr_, dsymSrc_, dstPtr_, dstSrc_;
   DBuf<int> dsymCol_;
   DBuf<double> dsymVal_;  // symmetric CSR values of the matrix being refined against
-  DBuf<double> dR_, dDx_, dXn_, dRn_, dBr_;
+  DBuf<double> dR_, dDx_, dXn_, dRn_, dBr_, dXt_;
   DBuf<unsigned long long> dnorms_;
   // per-system refinement of a batch: row -> system map (for the group
   // boundaries in hgroupStart_), per-system norm slots, accept flags

@@ -843,6 +843,89 @@ __global__ __launch_bounds__(256) void k_residual(int n, int nrhs, const int64_t
   if (rmaxBits) groupAtomicMax(mx, rmaxBits);
 }
 
+// Many right-hand sides (nrhs >= kResWide).  Column-major X makes the
+// product's gather read one cache line per right-hand side for every matrix
+// entry, so X is first transposed into Xt (row i's right-hand sides
+// contiguous, k_transpose_rows) and the residual takes a wave per row with a
+// lane per right-hand side: each entry's gather becomes one contiguous
+// nrhs-word read.  A workgroup covers 64 rows x 64 right-hand sides and
+// moves B in and R out through LDS so both go to memory in 512-byte column
+// segments.
+constexpr int kResWide = 16;
+__global__ __launch_bounds__(256) void k_transpose_rows(int n, int nrhs, const double* __restrict__ X, int64_t ldx,
+                                                        double* __restrict__ Xt) {
+  __shared__ double tile[64][65];
+  const int i0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 4 rows of 64 lanes
+  for (int cc = ty; cc < 64; cc += 4) {  // read columns c0 + cc, rows i0 + tx (coalesced)
+    const int c = c0 + cc, i = i0 + tx;
+    tile[cc][tx] = (c < nrhs && i < n) ? X[i + static_cast<int64_t>(c) * ldx] : 0.0;
+  }
+  __syncthreads();
+  for (int ii = ty; ii < 64; ii += 4) {  // write rows i0 + ii, right-hand sides c0 + tx (coalesced)
+    const int i = i0 + ii, c = c0 + tx;
+    if (i < n && c < nrhs) Xt[static_cast<int64_t>(i) * nrhs + c] = tile[tx][ii];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_residual_wide(int n, int nrhs, const int64_t* __restrict__ rowPtr,
+                                                       const int* __restrict__ colIdx,
+                                                       const double* __restrict__ aSym,
+                                                       const double* __restrict__ B, const double* __restrict__ Xt,
+                                                       double* __restrict__ R, unsigned long long* rmaxBits) {
+  __shared__ double tile[64][65];  // [right-hand side][row]
+  const int i0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int cc = w; cc < 64; cc += 4) {  // B tile in, column segments
+    const int c = c0 + cc, i = i0 + lane;
+    tile[cc][lane] = (c < nrhs && i < n) ? B[i + static_cast<int64_t>(c) * n] : 0.0;
+  }
+  __syncthreads();
+  const int c = c0 + lane;
+  const int cl = c < nrhs ? c : nrhs - 1;  // lanes past nrhs read a valid word and drop it
+  for (int rr = w; rr < 64; rr += 4) {  // wave w: rows i0 + w, + 4, ...
+    const int i = i0 + rr;
+    if (i >= n) break;
+    double acc = 0.0;
+    const int64_t t0 = rowPtr[i], t1 = rowPtr[i + 1];
+    // the row's entries are read once per 64 (a lane each) and broadcast by shuffles, so the gathers of
+    // X rows for consecutive entries have their addresses at hand and go out back to back
+    for (int64_t base = t0; base < t1; base += 64) {
+      const int cnt = static_cast<int>(min<int64_t>(64, t1 - base));
+      const int64_t tl = base + lane;
+      const int colL = lane < cnt ? colIdx[tl] : 0;
+      const double aL = lane < cnt ? aSym[tl] : 0.0;
+      int j = 0;
+      for (; j + 4 <= cnt; j += 4) {
+        const int cA = __shfl(colL, j, 64), cB = __shfl(colL, j + 1, 64), cC = __shfl(colL, j + 2, 64),
+                  cD = __shfl(colL, j + 3, 64);
+        const double xA = Xt[static_cast<int64_t>(cA) * nrhs + cl];
+        const double xB = Xt[static_cast<int64_t>(cB) * nrhs + cl];
+        const double xC = Xt[static_cast<int64_t>(cC) * nrhs + cl];
+        const double xD = Xt[static_cast<int64_t>(cD) * nrhs + cl];
+        acc = fma(__shfl(aL, j, 64), xA, acc);
+        acc = fma(__shfl(aL, j + 1, 64), xB, acc);
+        acc = fma(__shfl(aL, j + 2, 64), xC, acc);
+        acc = fma(__shfl(aL, j + 3, 64), xD, acc);
+      }
+      for (; j < cnt; ++j)
+        acc = fma(__shfl(aL, j, 64), Xt[static_cast<int64_t>(__shfl(colL, j, 64)) * nrhs + cl], acc);
+    }
+    tile[lane][rr] -= acc;
+  }
+  __syncthreads();
+  double mx = 0.0;
+  for (int cc = w; cc < 64; cc += 4) {  // R tile out, column segments
+    const int cR = c0 + cc, i = i0 + lane;
+    if (cR < nrhs && i < n) {
+      const double r = tile[cc][lane];
+      R[i + static_cast<int64_t>(cR) * n] = r;
+      mx = fmax(mx, normAbs(r));
+    }
+  }
+  if (rmaxBits) groupAtomicMax(mx, rmaxBits);
+}
+
 // out = x + dx, and max |out| into *maxBits when given.
 __global__ void k_axpy(int64_t len, const double* __restrict__ x, const double* __restrict__ dx,
                        double* __restrict__ out, unsigned long long* maxBits) {
@@ -1072,9 +1155,15 @@ void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int n
   hipCheck(hipGetLastError(), "launchBwdTiny");
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
-                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st) {
+                    const double* X, double* R, unsigned long long* rmaxBits, hipStream_t st, double* Xt) {
   if (n <= 0 || nrhs <= 0) return;
-  k_residual<<<(n + 255) / 256, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, X, R, rmaxBits);
+  if (Xt && nrhs >= kResWide) {
+    const dim3 grid((n + 63) / 64, (nrhs + 63) / 64);
+    k_transpose_rows<<<grid, 256, 0, st>>>(n, nrhs, X, n, Xt);
+    k_residual_wide<<<grid, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, Xt, R, rmaxBits);
+  } else {
+    k_residual<<<(n + 255) / 256, 256, 0, st>>>(n, nrhs, rowPtr, colIdx, aSym, B, X, R, rmaxBits);
+  }
   hipCheck(hipGetLastError(), "launchResidual");
 }
 void launchGatherVals(int64_t nnz, const int64_t* valSrc, const double* aVals, double* aSym, hipStream_t st) {
