@@ -282,18 +282,26 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
 /* Scheduling choices of a plan (no counterpart in the reference, whose
  * front kernel is one CPU routine): fronts of order <= small_front_max
  * (at most 192, default 64) are factored whole in LDS, larger ones by the
- * panel path;
- * assembly runs in column tiles of assemble_tile (a power of two in
- * [16, 1024]).  With speculative_panel set, each panel of a large front is
- * first factored as 1x1 pivots in order and the threshold tests checked for
- * all its columns at once; the exact pivot-by-pivot kernel continues from
- * the first column that fails (the same pivot decisions, up to rounding of
- * the tested values).  symla_plan_autotune times factorize + solve of the values at
- * d_a_vals (device) with nrhs fixed pseudo-random right-hand sides, best of
- * reps, over a grid of choices in the given precision, keeps the fastest
- * (returned in *chosen) and reports every trial (up to cap records,
- * *ntrials = grid size).  The plan then holds a factorization of those
- * values. */
+ * panel path; assembly runs in column tiles of assemble_tile (a power of two
+ * in [16, 1024]).  With speculative_panel set, each panel of a front with
+ * more than speculative_min_rows eligible rows is first factored as 1x1
+ * pivots in order and the threshold tests checked for all its columns at
+ * once; the exact pivot-by-pivot kernel continues from the first column that
+ * fails (the same pivot decisions, up to rounding of the tested values).
+ * symla_plan_autotune times factorize + solve of the values at d_a_vals
+ * (device) with nrhs fixed pseudo-random right-hand sides, best of reps, in
+ * the given precision, by coordinate descent from the plan's current
+ * tuning: each choice in turn over its candidates (small_front_max 0..192,
+ * assemble_tile 32/64/128, the speculative step off or from 0/1,024/2,048/
+ * 4,096 rows, the deferred contribution-block bound off or 1,024/2,048/
+ * 4,096) with the others fixed, the fastest kept, until a sweep changes
+ * nothing.  It keeps the fastest (returned in *chosen) and reports every
+ * trial (up to cap records, *ntrials = number timed).  The plan then holds a
+ * factorization of those values.  symla_plan_autotune_cached first looks up
+ * cache_path (a text file, one line per result) for the same key -- GPU
+ * architecture, precision, nrhs and histogram of front orders -- and applies
+ * a stored tuning without timing (*from_cache = 1, *ntrials = 0); otherwise it
+ * tunes and appends the result. */
 typedef struct {
   int small_front_max;
   int assemble_tile;
@@ -311,6 +319,9 @@ int symla_plan_get_tuning(const symla_plan* p, symla_tuning* out);
 int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t);
 int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
                         symla_tuning* chosen, symla_tuning_trial* trials, int cap, int* ntrials);
+int symla_plan_autotune_cached(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                               const char* cache_path, symla_tuning* chosen, symla_tuning_trial* trials, int cap,
+                               int* ntrials, int* from_cache);
 
 /* Device-side timing of the last factorize / solve (milliseconds, HIP
  * events on the plan's stream). */

@@ -130,6 +130,9 @@ SIGNATURES = {
     "symla_plan_timing": (C.c_int, [C.c_void_p, f64p, f64p]),
     "symla_plan_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "symla_plan_get_profile": (C.c_int, [C.c_void_p, C.POINTER(Profile)]),
+    "symla_plan_autotune_cached": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                             C.POINTER(Tuning), C.POINTER(TuningTrial), C.c_int,
+                                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -427,11 +427,14 @@ int symla_plan_set_tuning(symla_plan* p, const symla_tuning* t) {
   });
 }
 
-int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
-                        symla_tuning* chosen, symla_tuning_trial* trials, int cap, int* ntrials) {
+static int autotuneImpl(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                        const char* cache_path, symla_tuning* chosen, symla_tuning_trial* trials, int cap,
+                        int* ntrials, int* from_cache) {
   return planGuarded(p, [&] {
     if (!p) throw std::invalid_argument("symla_plan_autotune: null plan");
-    const auto tr = p->plan->autotune(d_a_vals, nrhs, precision, reps);
+    bool cached = false;
+    const auto tr = p->plan->autotune(d_a_vals, nrhs, precision, reps, cache_path, &cached);
+    if (from_cache) *from_cache = cached ? 1 : 0;
     if (chosen) {
       chosen->small_front_max = p->plan->tuning().smallMax;
       chosen->assemble_tile = p->plan->tuning().asmTile;
@@ -449,6 +452,18 @@ int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int pre
       trials[i].ms = tr[i].ms;
     }
   });
+}
+
+int symla_plan_autotune(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                        symla_tuning* chosen, symla_tuning_trial* trials, int cap, int* ntrials) {
+  return autotuneImpl(p, d_a_vals, nrhs, precision, reps, nullptr, chosen, trials, cap, ntrials, nullptr);
+}
+
+int symla_plan_autotune_cached(symla_plan* p, const double* d_a_vals, int nrhs, int precision, int reps,
+                               const char* cache_path, symla_tuning* chosen, symla_tuning_trial* trials, int cap,
+                               int* ntrials, int* from_cache) {
+  if (!cache_path) return fail(SYMLA_ERR_INVALID, "symla_plan_autotune_cached: null cache path");
+  return autotuneImpl(p, d_a_vals, nrhs, precision, reps, cache_path, chosen, trials, cap, ntrials, from_cache);
 }
 
 int symla_solve(symla_plan* p, const double* B, int64_t ldb, int nrhs, double* X, int64_t ldx) {

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <numeric>
 #include <sstream>
@@ -1731,10 +1732,46 @@ void Plan::setTuning(const Tuning& t) {
   graphStale_ = true;  // a recorded graph holds the old schedule: re-record on replay
 }
 
-std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int precision, int reps) {
+std::string Plan::tuneKey(int nrhs, int precision) const {
+  hipDeviceProp_t prop{};
+  int dev = 0;
+  hipCheck(hipGetDevice(&dev), "hipGetDevice");
+  hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  // front-order histogram in powers of two: plans with the same histogram share a tuning
+  int hist[32] = {};
+  for (int s = 0; s < numSN_; ++s) {
+    int b = 0;
+    while ((2 << b) <= hm_[s] && b < 31) ++b;
+    ++hist[b];
+  }
+  std::string key = std::string(prop.gcnArchName) + "|p" + std::to_string(precision) + "|r" + std::to_string(nrhs) +
+                    "|h";
+  for (int b = 0; b < 32; ++b)
+    if (hist[b]) key += std::to_string(b) + ":" + std::to_string(hist[b]) + ",";
+  return key;
+}
+
+std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int precision, int reps,
+                                            const char* cachePath, bool* fromCache) {
   if (!dVals) throw std::invalid_argument("symla: autotune needs device values");
   if (nrhs <= 0 || reps <= 0) throw std::invalid_argument("symla: autotune needs nrhs > 0 and reps > 0");
-  const Tuning keep = tune_;
+  if (precision < 0 || precision > 2) throw std::invalid_argument("symla: unknown precision");
+  if (fromCache) *fromCache = false;
+  const std::string key = cachePath ? tuneKey(nrhs, precision) : std::string();
+  if (cachePath) {  // one line per entry: key, then the five tuning fields
+    std::ifstream in(cachePath);
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string k;
+      Tuning t;
+      if (ls >> k >> t.smallMax >> t.asmTile >> t.specPanel >> t.specMinRows >> t.cbMinRows && k == key) {
+        setTuning(t);
+        if (fromCache) *fromCache = true;
+        return {};
+      }
+    }
+  }
   std::vector<double> hb(static_cast<size_t>(n_) * nrhs);
   uint64_t st = 0x9E3779B97F4A7C15ull;
   for (double& v : hb) {  // fixed pseudo-random right-hand sides in [-1, 1)
@@ -1749,32 +1786,67 @@ std::vector<Plan::TuneTrial> Plan::autotune(const double* dVals, int nrhs, int p
   o.precision = precision == 2 ? 1 : precision;  // SYMLA_PREC_BF16X3: FP32 fronts, BF16x3 contribution blocks
   o.bf16x3 = precision == 2;
   std::vector<TuneTrial> trials;
-  const int smalls[] = {0, 64, 96, 128, 192};
-  const int tiles[] = {32, 64, 128};
-  for (int sm : smalls)
-    for (int tw : tiles) {
-      setTuning(Tuning{sm, tw, keep.specPanel, keep.specMinRows, keep.cbMinRows});
-      FactorInfo info;
-      factorize(nullptr, dVals, o, info);  // warm-up: workspaces, kernels
+  auto same = [](const Tuning& a, const Tuning& b) {
+    return a.smallMax == b.smallMax && a.asmTile == b.asmTile && a.specPanel == b.specPanel &&
+           a.specMinRows == b.specMinRows && a.cbMinRows == b.cbMinRows;
+  };
+  auto timeOf = [&](const Tuning& t) {
+    for (const auto& tr : trials)
+      if (same(tr.t, t)) return tr.ms;
+    setTuning(t);
+    FactorInfo info;
+    factorize(nullptr, dVals, o, info);  // warm-up: workspaces, kernels
+    solve(dB.p, n_, nrhs, dXt.p, n_, true);
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      factorize(nullptr, dVals, o, info);
       solve(dB.p, n_, nrhs, dXt.p, n_, true);
-      double best = 1e300;
-      for (int r = 0; r < reps; ++r) {
-        const auto t0 = std::chrono::steady_clock::now();
-        factorize(nullptr, dVals, o, info);
-        solve(dB.p, n_, nrhs, dXt.p, n_, true);
-        const auto t1 = std::chrono::steady_clock::now();
-        best = std::min(best, std::chrono::duration<double, std::milli>(t1 - t0).count());
+      const auto t1 = std::chrono::steady_clock::now();
+      best = std::min(best, std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    trials.push_back(TuneTrial{t, best});
+    return best;
+  };
+  // coordinate descent from the current tuning: each knob in turn over its
+  // candidates with the others fixed, the fastest kept; repeated until a
+  // sweep changes nothing (at most three sweeps)
+  Tuning cur = tune_;
+  double curMs = timeOf(cur);
+  const int smalls[] = {0, 32, 64, 96, 128, 192};
+  const int tiles[] = {32, 64, 128};
+  const int specs[] = {-1, 0, 1024, 2048, 4096};  // -1: no speculative step
+  const int cbs[] = {-1, 1024, 2048, 4096};
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    const Tuning before = cur;
+    auto tryKnob = [&](auto&& apply, const int* vals, int nv) {
+      for (int i = 0; i < nv; ++i) {
+        Tuning t = cur;
+        apply(t, vals[i]);
+        const double ms = timeOf(t);
+        if (ms < curMs) {
+          curMs = ms;
+          cur = t;
+        }
       }
-      trials.push_back(TuneTrial{tune_, best});
-    }
-  Tuning bestT = keep;
-  double bestMs = 1e300;
-  for (const auto& tr : trials)
-    if (tr.ms < bestMs) {
-      bestMs = tr.ms;
-      bestT = tr.t;
-    }
-  setTuning(bestT);
+    };
+    tryKnob([](Tuning& t, int v) { t.smallMax = v; }, smalls, 6);
+    tryKnob([](Tuning& t, int v) { t.asmTile = v; }, tiles, 3);
+    tryKnob(
+        [](Tuning& t, int v) {
+          t.specPanel = v >= 0;
+          if (v >= 0) t.specMinRows = v;
+        },
+        specs, 5);
+    tryKnob([](Tuning& t, int v) { t.cbMinRows = v; }, cbs, 4);
+    if (same(before, cur)) break;
+  }
+  setTuning(cur);
+  if (cachePath) {
+    std::ofstream out(cachePath, std::ios::app);
+    out << key << ' ' << cur.smallMax << ' ' << cur.asmTile << ' ' << cur.specPanel << ' ' << cur.specMinRows << ' '
+        << cur.cbMinRows << '\n';
+  }
   return trials;
 }
 

@@ -262,9 +262,17 @@ class Plan {
   };
   const Tuning& tuning() const { return tune_; }
   void setTuning(const Tuning& t);
-  // Times factorize + solve (values at dVals, nrhs random right-hand sides)
-  // for every candidate tuning, keeps the fastest and returns all trials.
-  std::vector<TuneTrial> autotune(const double* dVals, int nrhs, int precision, int reps);
+  // Times factorize + solve (values at dVals, nrhs pseudo-random right-hand
+  // sides) by coordinate descent over the scheduling knobs (small-front
+  // bound, assembly tile, speculative panel row bound or off, deferred
+  // contribution-block row bound or off), keeps the fastest and returns every
+  // trial.  With cachePath, a tuning stored there for the same key (GPU
+  // architecture, precision, nrhs, histogram of front orders) is applied
+  // without timing (*fromCache = true, no trials), and a new result is
+  // appended to the file.
+  std::vector<TuneTrial> autotune(const double* dVals, int nrhs, int precision, int reps,
+                                  const char* cachePath = nullptr, bool* fromCache = nullptr);
+  std::string tuneKey(int nrhs, int precision) const;
   void frontView(int si, int* m, int* npiv, int* rows, double* L, double* dd, double* dof, int* ps, int* pk,
                  int* nblocks);
   double lastFactorMs() const { return factorMs_; }

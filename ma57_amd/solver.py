@@ -436,21 +436,30 @@ class SymLDLT:
         small, tile = self.tuning()
         self.setTuning(small, tile, on, min_rows)
 
-    def autotune(self, d_vals: int, nrhs: int = 1, reps: int = 2):
-        """Time factorize + solve over the tuning grid on device values; keep the fastest.  Returns
-        (chosen (small_front_max, assemble_tile), [((small_front_max, assemble_tile), ms), ...])."""
+    def autotune(self, d_vals: int, nrhs: int = 1, reps: int = 2, cache: Optional[str] = None):
+        """Time factorize + solve by coordinate descent over the scheduling choices on device values; keep the
+        fastest.  Returns (chosen, [(tuning, ms), ...], from_cache) with each tuning a tuple (small_front_max,
+        assemble_tile, speculative_panel, speculative_min_rows, deferred_cb_min_rows).  With `cache` (a file
+        path), a stored result for the same GPU, precision, nrhs and front-order histogram is applied without
+        timing."""
         if not self._pattern_analyzed:
             raise LogicError("symla::SymLDLT::factorize: analyzePattern must be called first")
         chosen = _lib.Tuning()
-        trials = (_lib.TuningTrial * 64)()
+        trials = (_lib.TuningTrial * 128)()
         nt = C.c_int(0)
-        rc = self._lib.symla_plan_autotune(self._plan, C.c_void_p(d_vals), nrhs, self.precision, reps,
-                                           C.byref(chosen), trials, 64, C.byref(nt))
+        fc = C.c_int(0)
+        if cache is None:
+            rc = self._lib.symla_plan_autotune(self._plan, C.c_void_p(d_vals), nrhs, self.precision, reps,
+                                               C.byref(chosen), trials, 128, C.byref(nt))
+        else:
+            rc = self._lib.symla_plan_autotune_cached(self._plan, C.c_void_p(d_vals), nrhs, self.precision, reps,
+                                                      str(cache).encode(), C.byref(chosen), trials, 128,
+                                                      C.byref(nt), C.byref(fc))
         _raise(rc, self._lib.symla_plan_last_error(self._plan).decode())
         self._factorized = False  # the trials factored the tuning values: factorize() before solving
-        return ((chosen.small_front_max, chosen.assemble_tile),
-                [((trials[i].tuning.small_front_max, trials[i].tuning.assemble_tile), trials[i].ms)
-                 for i in range(min(nt.value, 64))])
+        tup = lambda t: (t.small_front_max, t.assemble_tile, t.speculative_panel, t.speculative_min_rows,
+                         t.deferred_cb_min_rows)
+        return tup(chosen), [(tup(trials[i].tuning), trials[i].ms) for i in range(min(nt.value, 128))], bool(fc.value)
 
     # --- recorded refactorize + solve (no reference counterpart) ---
     def captureGraph(self, d_vals: int, d_B: int, nrhs: int, d_X: int, refine: Optional[RefinementOptions] = None):
