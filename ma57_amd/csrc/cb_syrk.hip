@@ -40,6 +40,7 @@ template <>
 struct SyrkMfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   static constexpr int KT = 16;  // k columns per step
+  static constexpr int kWaves = 8;  // 32 x 64 per wave: half the accumulators, two workgroups per CU
   static __device__ inline v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -50,6 +51,7 @@ template <>
 struct SyrkMfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   static constexpr int KT = 32;
+  static constexpr int kWaves = 4;  // 64 x 64 per wave
   static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   static __device__ inline int col(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
@@ -96,14 +98,17 @@ __device__ inline void syrkTile(int t, int nt, int& ti, int& tj) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restrict__ W, int ldw,
-                                                 const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc,
-                                                 int nt, int per) {
+__global__ __launch_bounds__(64 * SyrkMfma<T>::kWaves) void k_cb_syrk(int N, int K, const T* __restrict__ W, int ldw,
+                                                                      const T* __restrict__ B, int ldb,
+                                                                      T* __restrict__ C, int ldc, int nt, int per) {
   using M = SyrkMfma<T>;
   using v4 = typename M::v4;
   constexpr int TS = kSyTile;
   constexpr int KT = M::KT;
-  constexpr int KH = KT / 2;              // staged k columns per thread (two threads per row)
+  constexpr int NW = M::kWaves;           // 4: waves of 64 x 64; 8: waves of 32 x 64
+  constexpr int BI = NW == 4 ? 4 : 2;     // 16-row MFMA blocks per wave
+  constexpr int SPLIT = NW * 64 / TS;     // threads per staged row
+  constexpr int KH = KT / SPLIT;          // staged k columns per thread
   constexpr int KQ = KT / 4;              // k values per lane per step
   constexpr int LDK = KT + 16 / sizeof(T);  // 16-byte row pad: conflict-free 16-row vector reads
   __shared__ __attribute__((aligned(16))) T Ws[2][TS][LDK];
@@ -117,9 +122,9 @@ __global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restri
   const int I0 = ti * TS, J0 = tj * TS;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int wr = (w >> 1) * (16 * BI), wc = (w & 1) * 64;
   const int lr = lane & 15, lk = lane >> 4;
-  const int sr = tid & (TS - 1), sh = tid >> 7;  // staging: row sr, k columns sh * KH .. + KH
+  const int sr = tid & (TS - 1), sh = tid / TS;  // staging: row sr, k columns sh * KH .. + KH
   const int gi = I0 + sr, gj = J0 + sr;
   const bool wIn = gi < N, bIn = gj < N;
   const T* Wr = W + gi;
@@ -143,9 +148,9 @@ __global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restri
     }
   };
 
-  v4 acc[4][4];
+  v4 acc[BI][4];
 #pragma unroll
-  for (int bi = 0; bi < 4; ++bi)
+  for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4{0, 0, 0, 0};
 
@@ -156,19 +161,19 @@ __global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restri
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * KT);
-    T a[4][KQ], b[4][KQ];
+    T a[4][KQ], b[BI][KQ];
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int s = 0; s < KQ; ++s) a[bj][s] = Bs[buf][wc + bj * 16 + lr][lk * KQ + s];
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
+    for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
       for (int s = 0; s < KQ; ++s) b[bi][s] = Ws[buf][wr + bi * 16 + lr][lk * KQ + s];
 #pragma unroll
     for (int s = 0; s < KQ; ++s)
 #pragma unroll
-      for (int bi = 0; bi < 4; ++bi)
+      for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::mma(a[bj][s], b[bi][s], acc[bi][bj]);
     if (kt + 1 < nk) lstore(buf ^ 1);
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(256) void k_cb_syrk(int N, int K, const T* __restri
   // C(row, col) -= acc: register r of block (bi, bj) holds row lr of the
   // block and column M::col(lane, r)
 #pragma unroll
-  for (int bi = 0; bi < 4; ++bi) {
+  for (int bi = 0; bi < BI; ++bi) {
     const int row = I0 + wr + bi * 16 + lr;
     if (row >= N) continue;
 #pragma unroll
@@ -331,12 +336,13 @@ void launchCbSyrk(bool fp32, int N, int K, const void* W, int ldw, const void* B
   const int per = (total + kXcds - 1) / kXcds;
   const dim3 grid(per * kXcds);
   if (fp32)
-    k_cb_syrk<float><<<grid, 256, 0, st>>>(N, K, static_cast<const float*>(W), ldw, static_cast<const float*>(B), ldb,
-                                           static_cast<float*>(C), ldc, nt, per);
+    k_cb_syrk<float><<<grid, 64 * SyrkMfma<float>::kWaves, 0, st>>>(N, K, static_cast<const float*>(W), ldw,
+                                                                     static_cast<const float*>(B), ldb,
+                                                                     static_cast<float*>(C), ldc, nt, per);
   else
-    k_cb_syrk<double><<<grid, 256, 0, st>>>(N, K, static_cast<const double*>(W), ldw,
-                                            static_cast<const double*>(B), ldb, static_cast<double*>(C), ldc, nt,
-                                            per);
+    k_cb_syrk<double><<<grid, 64 * SyrkMfma<double>::kWaves, 0, st>>>(N, K, static_cast<const double*>(W), ldw,
+                                                                       static_cast<const double*>(B), ldb,
+                                                                       static_cast<double*>(C), ldc, nt, per);
   hipCheck(hipGetLastError(), "launchCbSyrk");
 }
 
