@@ -49,34 +49,43 @@ void launchReduceStats(int numSN, const DevFronts& fd, long long* outI, double* 
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st);
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st);
 void launchSolveBegin(unsigned* gen, hipStream_t st);
-// forward gather: one thread per front position (blockStart: ceil(m / 256) workgroups per front);
-// mapPtr / mapSrc: per pre-permutation position, the (child, child position) entries summed into it
-void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
+// Inside a solve, Bp, Z and X are n x nrhs row-major (ld nrhs) and a front's
+// work block Dl is m x nrhs row-major at gOff[s] * nrhs.
+// forward gather over every front of a level: posStart[f] is the running sum
+// of the fronts' orders (positions = posStart[count]); mapPtr / mapSrc: per
+// pre-permutation position, the (child, child position) entries summed into it
+void launchFwdGather(const int* list, int count, const int64_t* posStart, int64_t positions, const DevFronts& fd,
+                     int nrhs, const double* Bp, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
                      const int2* mapSrc, hipStream_t st);
-// own rows per sweep block: the sweeps apply the inverse of each block's
+// own rows per solve block: the solve applies the inverse of each block's
 // unit lower diagonal block (kSolveBlock^2 doubles, row-major, at
 // invOff[s] + kSolveBlock^2 * block), formed once per factorization
 constexpr int kSolveBlock = 32;
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
                        double* inv, const int64_t* invOff, hipStream_t st);
-void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                    int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, hipStream_t st);
-// the ancestors' x of a level's fronts into their work blocks (rows [nF, m)); same grid as the forward gather
-void launchBwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* X, double* Dl, const int64_t* gOff, hipStream_t st);
-// ancGathered: launchBwdGather ran for this level (the ancestors' x is read from Dl, else from X)
-void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                    int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, bool ancGathered, hipStream_t st);
+// own rows, one workgroup per front (a short chain of own blocks)
+void launchFwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,
+                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st);
+// own rows, one workgroup per own block (blockStart: ceil(nF / kSolveBlock) per front), dependency flags
+void launchFwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* Z, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
+                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st);
+// rows [nF, m) after the own rows (blockStart: ceil((m - nF) / 64) per front)
+void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* Dl, const int64_t* gOff, int* err, hipStream_t st);
+// P = z - L(anc, own)^T x_anc into the own work rows (blockStart: ceil(nF / kSolveBlock) per front)
+void launchBwdAnc(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                  const double* Z, const double* X, double* Dl, const int64_t* gOff, int* err, hipStream_t st);
+void launchBwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* X, double* Dl,
+                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st);
+void launchBwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* X, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
+                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st);
 // fronts of at most kTinyFront rows: one wave per front, no dependency flags
 constexpr int kTinyFront = 64;
-void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
-                   double* Dl, const int64_t* gOff, hipStream_t st);
-void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
+void launchFwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,
+                   const int64_t* gOff, hipStream_t st);
+void launchBwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, const double* Z, double* X,
                    hipStream_t st);
 // R = B - A X over symmetric CSR values aSym (X, B, R: n x nrhs column-major, ld n); max |R| into
 // *rmaxBits if non-null.  Xt (n x nrhs doubles, may be null): workspace for a row-major copy of X, used

@@ -1033,7 +1033,6 @@ void Plan::ensureSolveWs(int nrhs) {
   dBp_.ensure(nn);
   dZ_.ensure(nn);
   dX_.ensure(nn);
-  dG_.ensure(static_cast<size_t>(sumM_) * nrhs);
   dDl_.ensure(static_cast<size_t>(sumM_) * nrhs);
 }
 
@@ -1067,59 +1066,64 @@ void Plan::ensureSolveLayout() {
       dmapPtr_.upload(cnt, stream_);
       dmapSrc_.upload(src, stream_);
     }
-    // Workgroups per front: the gather takes one thread per front position;
-    // the sweeps take one workgroup per 32-row block (an upper bound: the
-    // own/ancestor split is known on the device).
+    // Per level launch lists (see SolveLevel); the pivots each front took
+    // are known here (the static layout takes every eligible pivot).
     const int nl = static_cast<int>(levelFronts_.size());
-    std::vector<int> gs, fs, bs;
-    gatherOff_.assign(nl, 0);
-    fwdOff_.assign(nl, 0);
-    bwdOff_.assign(nl, 0);
-    gatherBlocks_.assign(nl, 0);
-    fwdBlocks_.assign(nl, 0);
-    bwdBlocks_.assign(nl, 0);
-    std::vector<int> lists;
-    sweepOff_.assign(nl, 0);
-    bwdGather_.assign(nl, 0);
-    tinyOff_.assign(nl, 0);
-    sweepCnt_.assign(nl, 0);
-    tinyCnt_.assign(nl, 0);
-    for (int L = 0; L < nl; ++L) {
-      gatherOff_[L] = static_cast<int64_t>(gs.size());
-      fwdOff_[L] = static_cast<int64_t>(fs.size());
-      bwdOff_[L] = static_cast<int64_t>(bs.size());
-      int ga = 0, fa = 0, ba = 0;
-      for (int s : levelFronts_[L]) {
-        gs.push_back(ga);
-        ga += std::max(1, (hm_[s] + 255) / 256);
-      }
-      sweepOff_[L] = static_cast<int64_t>(lists.size());
-      for (int s : levelFronts_[L]) {
-        const int mm = hm_[s];
-        if (mm <= kTinyFront) continue;
+    solveLv_.assign(nl, {});
+    std::vector<int> lists, starts;
+    std::vector<int64_t> pos;
+    auto addLaunch = [&](SolveLaunch& ln, const std::vector<int>& fronts, auto&& blocksOf) {
+      ln.list = static_cast<int64_t>(lists.size());
+      ln.start = static_cast<int64_t>(starts.size());
+      ln.count = static_cast<int>(fronts.size());
+      int acc = 0;
+      for (int s : fronts) {
         lists.push_back(s);
-        fs.push_back(fa);
-        bs.push_back(ba);
-        if ((hnE_[s] + 31) / 32 >= kBwdGatherBlocks) bwdGather_[L] = 1;
-        fa += (mm + 31) / 32 + 1;
-        ba += (mm + 31) / 32;
+        starts.push_back(acc);
+        acc += blocksOf(s);
       }
-      sweepCnt_[L] = static_cast<int>(lists.size() - sweepOff_[L]);
-      tinyOff_[L] = static_cast<int64_t>(lists.size());
-      for (int s : levelFronts_[L])
-        if (hm_[s] <= kTinyFront) lists.push_back(s);
-      tinyCnt_[L] = static_cast<int>(lists.size() - tinyOff_[L]);
-      gs.push_back(ga);
-      fs.push_back(fa);
-      bs.push_back(ba);
-      gatherBlocks_[L] = ga;
-      fwdBlocks_[L] = fa;
-      bwdBlocks_[L] = ba;
+      starts.push_back(acc);
+      ln.blocks = acc;
+    };
+    for (int L = 0; L < nl; ++L) {
+      SolveLevel& lv = solveLv_[L];
+      const auto& fr = levelFronts_[L];
+      lv.gatherList = static_cast<int64_t>(lists.size());
+      lv.gatherCount = static_cast<int>(fr.size());
+      lv.posOff = static_cast<int64_t>(pos.size());
+      int64_t p = 0;
+      for (int s : fr) {
+        lists.push_back(s);
+        pos.push_back(p);
+        p += hm_[s];
+      }
+      pos.push_back(p);
+      lv.positions = p;
+      std::vector<int> own, chain, delta, anc, tiny;
+      for (int s : fr) {
+        const int nfs = layoutDirty_ ? hnF_[s] : hnE_[s];
+        if (hm_[s] <= kTinyFront) {
+          tiny.push_back(s);
+          continue;
+        }
+        anc.push_back(s);
+        if ((nfs + kSolveBlock - 1) / kSolveBlock <= kOwnChainBlocks)
+          own.push_back(s);
+        else
+          chain.push_back(s);
+        if (hm_[s] > nfs) delta.push_back(s);
+      }
+      auto nfOf = [&](int s) { return layoutDirty_ ? hnF_[s] : hnE_[s]; };
+      auto ownBlocks = [&](int s) { return (nfOf(s) + kSolveBlock - 1) / kSolveBlock; };
+      addLaunch(lv.own, own, [](int) { return 1; });
+      addLaunch(lv.chain, chain, ownBlocks);
+      addLaunch(lv.delta, delta, [&](int s) { return (hm_[s] - nfOf(s) + 63) / 64; });
+      addLaunch(lv.anc, anc, ownBlocks);
+      addLaunch(lv.tiny, tiny, [](int) { return 1; });
     }
     dsolveLists_.upload(lists, stream_);
-    dgatherStart_.upload(gs, stream_);
-    dfwdStart_.upload(fs, stream_);
-    dbwdStart_.upload(bs, stream_);
+    dsolveStarts_.upload(starts, stream_);
+    dposStart_.upload(pos, stream_);
     std::vector<int64_t> fo(numSN_ + 1, 0);
     for (int s = 0; s < numSN_; ++s) fo[s + 1] = fo[s] + (hm_[s] + 31) / 32;
     dflagOff_.upload(fo, stream_);
@@ -1158,34 +1162,37 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     w = *ext;
   } else {
     ensureSolveWs(nrhs);
-    w = SolveWs{dBp_.p, dZ_.p, dX_.p, dG_.p, dDl_.p};
+    w = SolveWs{dBp_.p, dZ_.p, dX_.p, dDl_.p};
   }
   DevFronts fd = devFronts();
   launchSolveBegin(dgen_.p, stream_);
   launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, w.Bp, stream_);
-  const int nl = static_cast<int>(levelFronts_.size());
+  const int* lists = dsolveLists_.p;
+  const int* starts = dsolveStarts_.p;
+  int* err = dsolveErr_.p;
+  const int nl = static_cast<int>(solveLv_.size());
   for (int L = 0; L < nl; ++L) {
-    const int cnt = static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]);
-    const int* lst = dallFronts_.p + levelFrontOff_[L];
+    const SolveLevel& lv = solveLv_[L];
     profBegin(kProfFwd);
-    launchFwdGather(lst, cnt, dgatherStart_.p + gatherOff_[L], gatherBlocks_[L], fd, n_, nrhs, w.Bp, w.G,
+    launchFwdGather(lists + lv.gatherList, lv.gatherCount, dposStart_.p + lv.posOff, lv.positions, fd, nrhs, w.Bp,
                     w.Dl, dgOff_.p, dmapPtr_.p, dmapSrc_.p, stream_);
-    launchFwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dfwdStart_.p + fwdOff_[L], fwdBlocks_[L], fd, n_,
-                   nrhs, w.G, w.Z, w.Dl, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
-                   dinvOff_.p, stream_);
-    launchFwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, w.G, w.Z, w.Dl, dgOff_.p, stream_);
+    launchFwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.Z, w.Dl, dgOff_.p, dinv_.p, dinvOff_.p, stream_);
+    launchFwdChain(lists + lv.chain.list, lv.chain.count, starts + lv.chain.start, lv.chain.blocks, fd, nrhs, w.Z,
+                   w.Dl, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p, stream_);
+    launchFwdDelta(lists + lv.delta.list, lv.delta.count, starts + lv.delta.start, lv.delta.blocks, fd, nrhs, w.Dl,
+                   dgOff_.p, err, stream_);
+    launchFwdTiny(lists + lv.tiny.list, lv.tiny.count, fd, nrhs, w.Z, w.Dl, dgOff_.p, stream_);
     profEnd(kProfFwd);
   }
   for (int L = nl - 1; L >= 0; --L) {
+    const SolveLevel& lv = solveLv_[L];
     profBegin(kProfBwd);
-    if (bwdGather_[L])
-      launchBwdGather(dallFronts_.p + levelFrontOff_[L],
-                      static_cast<int>(levelFrontOff_[L + 1] - levelFrontOff_[L]), dgatherStart_.p + gatherOff_[L],
-                      gatherBlocks_[L], fd, n_, nrhs, w.X, w.Dl, dgOff_.p, stream_);
-    launchBwdSweep(dsolveLists_.p + sweepOff_[L], sweepCnt_[L], dbwdStart_.p + bwdOff_[L], bwdBlocks_[L], fd, n_,
-                   nrhs, w.Z, w.X, w.Dl, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, dsolveErr_.p, dinv_.p,
-                   dinvOff_.p, bwdGather_[L] != 0, stream_);
-    launchBwdTiny(dsolveLists_.p + tinyOff_[L], tinyCnt_[L], fd, n_, nrhs, w.Z, w.X, stream_);
+    launchBwdAnc(lists + lv.anc.list, lv.anc.count, starts + lv.anc.start, lv.anc.blocks, fd, nrhs, w.Z, w.X, w.Dl,
+                 dgOff_.p, err, stream_);
+    launchBwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.X, w.Dl, dgOff_.p, dinv_.p, dinvOff_.p, stream_);
+    launchBwdChain(lists + lv.chain.list, lv.chain.count, starts + lv.chain.start, lv.chain.blocks, fd, nrhs, w.X,
+                   w.Dl, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p, stream_);
+    launchBwdTiny(lists + lv.tiny.list, lv.tiny.count, fd, nrhs, w.Z, w.X, stream_);
     profEnd(kProfBwd);
   }
   launchPermuteOut(n_, nrhs, dpermVec_.p, w.X, dX, ldx, stream_);
@@ -1234,13 +1241,13 @@ void Plan::solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int
   if (!solveLayoutReady_) throw std::logic_error("symla: the solve layout is not built (factorize first)");
   const size_t nn = static_cast<size_t>(n_) * nrhs;
   const size_t gm = static_cast<size_t>(sumM_) * nrhs;
-  const size_t need = 3 * nn + 2 * gm;
+  const size_t need = 3 * nn + gm;
   if (need > capacity)
     throw std::invalid_argument("symla_solve_device_noalloc: this solve needs " + std::to_string(need * 8 >> 20) +
-                                " MiB of workspace ((3 n + 2 sum of front orders) x nrhs doubles), above the " +
+                                " MiB of workspace ((3 n + sum of front orders) x nrhs doubles), above the " +
                                 std::to_string(capacity * 8 >> 20) +
                                 " MiB static scratch; use fewer right-hand sides per call or symla_solve_device");
-  const SolveWs w{scratch, scratch + nn, scratch + 2 * nn, scratch + 3 * nn, scratch + 3 * nn + gm};
+  const SolveWs w{scratch, scratch + nn, scratch + 2 * nn, scratch + 3 * nn};
   // the event profile would create events on first use: off for this call
   const bool prof = profiling_;
   profiling_ = false;

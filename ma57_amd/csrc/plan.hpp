@@ -130,11 +130,11 @@ struct LargeIter {
 // column nE; after its last panel the block [nE, m)^2 takes one rank-nE
 // update, kCbChunk pivot columns per MFMA SYRK launch (cb_syrk.hip).
 constexpr int kCbMinCols = 256;
-// levels with a front of at least this many 32-row own blocks gather the
-// ancestors' x once per front before the backward sweep
-constexpr int kBwdGatherBlocks = 16;
 constexpr int kCbMinRowsDefault = 2048;
 constexpr int kCbChunk = 2048;
+// fronts with at most this many 32-row own blocks solve their own rows on
+// one workgroup; longer chains take one workgroup per block
+constexpr int kOwnChainBlocks = 10;
 struct CbTask {
   int s, m, nE;
 };
@@ -210,7 +210,7 @@ class Plan {
                                                double tol, int ngroups, const int* groupStart);
   // The device solve with its per-call workspace in `scratch` (capacity in
   // doubles) instead of plan buffers: allocates nothing.  Throws
-  // std::invalid_argument when (3 n + 2 sum of front orders) x nrhs doubles
+  // std::invalid_argument when (3 n + sum of front orders) x nrhs doubles
   // exceed the capacity.
   void solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, double* scratch,
                     size_t capacity);
@@ -359,7 +359,7 @@ class Plan {
   int maxNE_ = 0;
   // solve
   DBuf<int> dpermVec_;
-  DBuf<double> dBp_, dZ_, dX_, dG_, dDl_, dBin_, dXout_;
+  DBuf<double> dBp_, dZ_, dX_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;  // running sum of the front orders: front s's work blocks start at dgOff_[s] * nrhs
   int64_t sumM_ = 0;
   bool solveLayoutReady_ = false;
@@ -375,20 +375,25 @@ class Plan {
   DBuf<double> dinv_;
   int invCount_ = 0, invBlocks_ = 0;
   void launchInverse();
-  // per level: flat workgroup starts of the gather, forward and backward
-  // sweep launches (offsets into the device arrays, and block totals)
-  DBuf<int> dgatherStart_, dfwdStart_, dbwdStart_;
-  std::vector<int64_t> gatherOff_, fwdOff_, bwdOff_;
-  std::vector<int> gatherBlocks_, fwdBlocks_, bwdBlocks_;
-  // per level: the backward sweep reads the ancestors' x from a per-front
-  // gather (levels with a front of at least kBwdGatherBlocks own-row blocks,
-  // where the sweep's workgroups would each gather it again)
-  std::vector<char> bwdGather_;
-  // per level: fronts solved by the sweeps, then fronts of at most
-  // kTinyFront rows solved one wave each (offsets into dsolveLists_)
-  DBuf<int> dsolveLists_;
-  std::vector<int64_t> sweepOff_, tinyOff_;
-  std::vector<int> sweepCnt_, tinyCnt_;
+  // Per level, the solve's launches: the gather over every front (fronts in
+  // solveLists_ at posOff's list offset, positions numbered front by front),
+  // fronts whose own rows take one workgroup (own), fronts whose own rows
+  // take one workgroup per block (chain), fronts with rows past their own
+  // (delta), every front above kTinyFront rows (anc: the backward ancestors'
+  // part), and fronts of at most kTinyFront rows (tiny).  Lists are offsets
+  // into dsolveLists_, workgroup starts offsets into dsolveStarts_.
+  struct SolveLaunch {
+    int64_t list = 0, start = 0;
+    int count = 0, blocks = 0;
+  };
+  struct SolveLevel {
+    int64_t gatherList = 0, posOff = 0, positions = 0;
+    int gatherCount = 0;
+    SolveLaunch own, chain, delta, anc, tiny;
+  };
+  std::vector<SolveLevel> solveLv_;
+  DBuf<int> dsolveLists_, dsolveStarts_;
+  DBuf<int64_t> dposStart_;
   // dependency flags of the sweeps (one per own-row block), the solve
   // generation they are compared with, and the bounded-wait error word
   DBuf<int64_t> dflagOff_;
@@ -480,10 +485,10 @@ class Plan {
   // the current layout; built after each factorization
   void ensureSolveLayout();
   void ensureSolveWs(int nrhs);
-  // per-call solve workspace: permuted B, Z, X (n x nrhs each) and the
-  // per-front blocks G, Dl (sum of front orders x nrhs each)
+  // per-call solve workspace: permuted B, Z, X (n x nrhs each, row-major) and
+  // the per-front work blocks Dl (sum of front orders x nrhs, row-major)
   struct SolveWs {
-    double *Bp = nullptr, *Z = nullptr, *X = nullptr, *G = nullptr, *Dl = nullptr;
+    double *Bp = nullptr, *Z = nullptr, *X = nullptr, *Dl = nullptr;
   };
   // ext: caller-provided workspace (nothing is allocated); null: the plan's own
   void solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, const SolveWs* ext = nullptr);

@@ -7,10 +7,13 @@
 // contributions (fixed child order, no atomics), solves its own rows, and
 // hands a private delta for its remaining rows to its parent.  The backward
 // pass is the reference's per-front step (solve.hpp:364-383) run top-down.
-// Within one tree level, the rows of a front are split into 32-row blocks,
-// each solved by its own workgroup; a block waits on a per-block flag for
-// the blocks it depends on, so a large front is solved by many workgroups in
-// one launch.
+// Per tree level, the forward pass gathers the children's deltas, solves the
+// fronts' own rows in 32-row blocks (one workgroup per front when its chain
+// of blocks is short; else one workgroup per block, a block waiting on a
+// per-block flag for the blocks it depends on), then forms the deltas of the
+// remaining rows with no waits.  The backward pass first takes the
+// ancestors' part of every own block, then runs the own-row chains the same
+// two ways.
 #include <algorithm>
 
 #include "device.hpp"
@@ -18,38 +21,59 @@
 
 namespace symla_hip {
 
-__global__ void k_permute_in(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ B,
-                             int64_t ldb, double* __restrict__ Bp) {
-  const int64_t total = static_cast<int64_t>(n) * nrhs;
-  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int t = static_cast<int>(e % n);
-    const int64_t c = e / n;
-    Bp[e] = B[perm[t] + c * ldb];
+// Right-hand sides inside a solve are row-major: Bp, Z and X are n x nrhs
+// with row r's right-hand sides contiguous at r * nrhs, and a front's work
+// block is m x nrhs, row i at (gOff[s] + i) * nrhs.  A row reached through a
+// front's index map is then one contiguous read of nrhs words.
+
+// Caller's column-major B (ld ldb) into the solve's row order through a
+// 64 x 64 tile in LDS: read with lanes over rows (the gather through perm
+// stays inside one column), write with lanes over right-hand sides.
+__global__ __launch_bounds__(256) void k_permute_in(int n, int nrhs, const int* __restrict__ perm,
+                                                    const double* __restrict__ B, int64_t ldb,
+                                                    double* __restrict__ Bp) {
+  __shared__ double tile[64][65];
+  const int t0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tn = min(64, n - t0), cn = min(64, nrhs - c0);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int t = e & 63, c = e >> 6;
+    if (t < tn && c < cn) tile[t][c] = B[perm[t0 + t] + static_cast<int64_t>(c0 + c) * ldb];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e & 63, t = e >> 6;
+    if (t < tn && c < cn) Bp[static_cast<int64_t>(t0 + t) * nrhs + c0 + c] = tile[t][c];
   }
 }
 
-__global__ void k_permute_out(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ X,
-                              double* __restrict__ Xo, int64_t ldx) {
-  const int64_t total = static_cast<int64_t>(n) * nrhs;
-  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int t = static_cast<int>(e % n);
-    const int64_t c = e / n;
-    Xo[perm[t] + c * ldx] = X[e];
+__global__ __launch_bounds__(256) void k_permute_out(int n, int nrhs, const int* __restrict__ perm,
+                                                     const double* __restrict__ X, double* __restrict__ Xo,
+                                                     int64_t ldx) {
+  __shared__ double tile[64][65];
+  const int t0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tn = min(64, n - t0), cn = min(64, nrhs - c0);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e & 63, t = e >> 6;
+    if (t < tn && c < cn) tile[t][c] = X[static_cast<int64_t>(t0 + t) * nrhs + c0 + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int t = e & 63, c = e >> 6;
+    if (t < tn && c < cn) Xo[perm[t0 + t] + static_cast<int64_t>(c0 + c) * ldx] = tile[t][c];
   }
 }
 
 namespace {
 
-constexpr int SB = kSolveBlock;  // own rows per sweep block; also the column tile of the sweeps
-constexpr int DT = 64;  // ancestor-row tile of the backward sweep
-constexpr int YW = 65;  // row stride of the right-hand-side tiles in LDS (64 columns + 1)
-// Polls of a dependency flag before a sweep block gives up (about a second).
+constexpr int SB = kSolveBlock;  // own rows per solve block; also the column tile of the own-row products
+constexpr int DT = 64;           // row tile of the delta and ancestor products
+constexpr int YW = 65;           // row stride of the right-hand-side tiles in LDS (64 columns + 1)
+// Polls of a dependency flag before a chain block gives up (about a second).
 constexpr unsigned kSpinLimit = 1u << 22;
 
-// Front owning flat block b: start[f] <= b < start[f+1].
-__device__ inline int segmentOf(const int* start, int count, int b) {
+// Front owning flat item b: start[f] <= b < start[f+1].
+template <typename I>
+__device__ inline int segmentOf(const I* start, int count, I b) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -66,13 +90,17 @@ __device__ inline bool flagReached(const unsigned* f, unsigned gen) {
 }
 
 // Values handed between workgroups of one launch (the solved rows of a
-// block) are written and read with device-coherent accesses, so publishing a
-// block needs no cache write-back and waiting for one no cache invalidation.
+// chain block) are written and read with device-coherent accesses, so
+// publishing a block needs no cache write-back and waiting for one no cache
+// invalidation.
 __device__ inline void storeShared(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline double loadShared(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void raiseError(int* err, int code) {
+  __hip_atomic_store(err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wait until another workgroup has published block f of this solve.  Thread 0
@@ -87,7 +115,7 @@ __device__ inline void waitBlock(const unsigned* f, unsigned gen, int* err) {
       if ((++spins & 31u) == 0) {
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (spins > kSpinLimit) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          raiseError(err, 1);
           break;
         }
       }
@@ -113,7 +141,7 @@ __device__ inline double lval(const T* F, int m, const int* pk, int row, int col
   return (row == col + 1 && pk[col] == 2) ? 0.0 : v;
 }
 
-// The sweeps' tile products on the matrix cores, in the storage type of the
+// The solve's tile products on the matrix cores, in the storage type of the
 // factor: v_mfma_f64_16x16x4f64 for FP64 fronts, v_mfma_f32_16x16x4f32 for
 // FP32 fronts (operands rounded to FP32).  A[i][k] comes from lane i + 16k,
 // B[k][j] from lane j + 16k; row(lane, r) is the row i of the result held in
@@ -137,35 +165,323 @@ struct SweepMfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// LDS of one own-row block step: the L operand tile, the right-hand-side
+// operand tile (reused for the block's rows in FP64 once the products are
+// done), the inverse of its unit diagonal block and its pivot data.
+template <typename E>
+struct BlockLds {
+  E Lt[SB][SB + 1];
+  union {
+    E yt[SB][YW];       // the right-hand-side operand tile of the products
+    double ys[SB][YW];  // then the block's own rows (after the products)
+  };
+  double Ld[SB][SB + 1];
+  double dA[SB + 1], dB[SB + 1];  // d_diag, d_off of positions r0 - 1 .. r0 + SB - 1
+  int pkA[SB + 1], rixA[SB + 1];  // their pivot kinds and final-order rows
+};
+
+// Work block rows of other blocks: read back by the workgroup that wrote
+// them (kShared false: ordered by its barriers) or published by another
+// workgroup of the launch (kShared true: device-coherent).
+template <bool kShared>
+__device__ inline double readRow(const double* p) {
+  if constexpr (kShared)
+    return loadShared(p);
+  else
+    return *p;
+}
+template <bool kShared>
+__device__ inline void writeRow(double* p, double v) {
+  if constexpr (kShared)
+    storeShared(p, v);
+  else
+    *p = v;
+}
+
+// Forward step of own block b of front s for right-hand sides rg .. rg + gw
+// (solve.hpp:349-361, :434-455): y_b = Ltop(b, b)^{-1} (y0_b - sum_{j<b}
+// L(b, j) y_j), y0_b in the block's work rows (from the gather), y_b written
+// over it, and z = D^{-1} y of the pivot blocks that end in this block into
+// Z.  With kShared the y_j come from other workgroups, waited for by flag
+// (on the first right-hand-side group only), and y_b is published when done.
+template <typename T, bool kShared>
+__device__ void fwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFronts& fd, int s, int b, int rg,
+                            int gw, int nrhs, double* __restrict__ Z, double* Ds, const double* __restrict__ Mi,
+                            unsigned* fl, unsigned gen, int* err, bool publish) {
+  using M = SweepMfma<T>;
+  using E = typename M::E;
+  using MD = SweepMfma<double>;
+  constexpr int NT = 256;
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const int r0 = b * SB, nr = min(SB, nf - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // wave wv owns output rows tr .. tr + 15 and right-hand sides tc .. tc + 31
+  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int e = tid; e < SB * SB; e += NT) S.Ld[e / SB][e % SB] = Mi[e];
+  for (int r = tid; r <= SB; r += NT) {
+    const int p = r0 - 1 + r;
+    const bool ok = p >= 0 && p < r0 + nr;
+    S.dA[r] = ok ? fd.dDiag[io + p] : 0.0;
+    S.dB[r] = ok ? fd.dOff[io + p] : 0.0;
+    S.pkA[r] = ok ? pk[p] : 0;
+    S.rixA[r] = ok ? fd.rowIdx[io + p] : 0;
+  }
+  double y0p[2][4];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+      y0p[t][rr] = (r < nr && c < gw) ? Ds[static_cast<int64_t>(r0 + r) * nrhs + rg + c] : 0.0;
+    }
+  // L(b, j) tiles four elements per thread, loaded one tile ahead
+  E lpre[4];
+  auto loadL = [&](int j) {
+    const int c0 = j * SB, cn = min(SB, nf - c0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      const int r = e % SB, qq = e / SB;
+      lpre[u] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
+    }
+  };
+  double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
+  if (b > 0) loadL(0);
+  for (int j = 0; j < b; ++j) {
+    if (kShared && rg == 0) waitBlock(fl + j, gen, err);
+    const int c0 = j * SB, cn = min(SB, nf - c0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      S.Lt[e % SB][e / SB] = lpre[u];
+    }
+    for (int e = tid; e < SB * 64; e += NT) {
+      const int c = e & 63, qq = e >> 6;
+      S.yt[qq][c] = (qq < cn && c < gw)
+                        ? static_cast<E>(readRow<kShared>(Ds + static_cast<int64_t>(c0 + qq) * nrhs + rg + c))
+                        : E(0);
+    }
+    if (j + 1 < b) loadL(j + 1);
+    __syncthreads();
+    typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kk = 0; kk < SB; kk += 4) {
+      const E av = S.Lt[tr + lr][kk + lk];
+      part[0] = M::mma(av, S.yt[kk + lk][tc + lr], part[0]);
+      part[1] = M::mma(av, S.yt[kk + lk][tc + 16 + lr], part[1]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
+    __syncthreads();
+  }
+  // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j), the inverse applied as
+  // an FP64 tile product (rows past nr are zero so they add nothing)
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+      S.ys[r][c] = (r < nr && c < gw) ? y0p[t][rr] - acc[t][rr] : 0.0;
+    }
+  __syncthreads();
+  {
+    typename MD::v4 yv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kk = 0; kk < SB; kk += 4) {
+      const double av = S.Ld[tr + lr][kk + lk];
+      yv[0] = MD::mma(av, S.ys[kk + lk][tc + lr], yv[0]);
+      yv[1] = MD::mma(av, S.ys[kk + lk][tc + 16 + lr], yv[1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int r = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
+        if (r < nr && c < gw) {
+          S.ys[r][c] = yv[t][rr];
+          writeRow<kShared>(Ds + static_cast<int64_t>(r0 + r) * nrhs + rg + c, yv[t][rr]);
+        }
+      }
+  }
+  __syncthreads();
+  // the later blocks need y only: release it before forming z
+  if (kShared && publish) publishBlock(fl + b, gen);
+  // z = D^{-1} y for the pivot blocks that end in this block (slot r + 1 of
+  // the staged pivot data is position r0 + r)
+  for (int e = tid; e < nr * gw; e += NT) {
+    const int c = e % gw, r = e / gw;
+    const int p = r0 + r;
+    const int64_t cc = rg + c;
+    const int kind = S.pkA[r + 1];
+    if (kind == 1) {
+      Z[static_cast<int64_t>(S.rixA[r + 1]) * nrhs + cc] = S.ys[r][c] / S.dA[r + 1];
+    } else if (kind == 2 && r + 1 < nr) {
+      const double d11 = S.dA[r + 1], d21 = S.dB[r + 1], d22 = S.dA[r + 2];
+      const double det = d11 * d22 - d21 * d21;
+      const double y0 = S.ys[r][c], y1 = S.ys[r + 1][c];
+      Z[static_cast<int64_t>(S.rixA[r + 1]) * nrhs + cc] = (d22 * y0 - d21 * y1) / det;
+      Z[static_cast<int64_t>(S.rixA[r + 2]) * nrhs + cc] = (-d21 * y0 + d11 * y1) / det;
+    } else if (kind == 0 && r == 0 && p > 0) {
+      // second row of a 2x2 block that starts in the previous block
+      const double d11 = S.dA[0], d21 = S.dB[0], d22 = S.dA[1];
+      const double det = d11 * d22 - d21 * d21;
+      const double y0 = readRow<kShared>(Ds + static_cast<int64_t>(p - 1) * nrhs + cc), y1 = S.ys[0][c];
+      Z[static_cast<int64_t>(S.rixA[0]) * nrhs + cc] = (d22 * y0 - d21 * y1) / det;
+      Z[static_cast<int64_t>(S.rixA[1]) * nrhs + cc] = (-d21 * y0 + d11 * y1) / det;
+    }
+  }
+  __syncthreads();
+}
+
+// Backward step of own block w of front s (solve.hpp:364-383): x_w =
+// Ltop(w, w)^{-T} (P_w - sum_{j>w} L(j, w)^T x_j), P_w (z less the
+// ancestors' part) in the block's work rows, x_w written over it and into X.
+template <typename T, bool kShared>
+__device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFronts& fd, int s, int w, int rg,
+                            int gw, int nrhs, double* __restrict__ X, double* Ds, const double* __restrict__ Mi,
+                            unsigned* fl, unsigned gen, int* err, bool publish) {
+  using M = SweepMfma<T>;
+  using E = typename M::E;
+  using MD = SweepMfma<double>;
+  constexpr int NT = 256;
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int nOwn = (nf + SB - 1) / SB;
+  const int64_t io = fd.idxOff[s];
+  const int* pk = fd.pkind + io;
+  const T* F = static_cast<const T*>(fd.F[s]);
+  const int r0 = w * SB, nr = min(SB, nf - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // wave wv owns own columns tr .. tr + 15 and right-hand sides tc .. tc + 31
+  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int e = tid; e < SB * SB; e += NT) S.Ld[e / SB][e % SB] = Mi[e];
+  for (int q = tid; q < SB; q += NT) S.rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
+  double p0[2][4];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+      p0[t][rr] = (qq < nr && c < gw) ? Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] : 0.0;
+    }
+  E opre[4];
+  auto loadO = [&](int j) {
+    const int i0 = j * SB, in = min(SB, nf - i0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      const int i = e % SB, qq = e / SB;
+      opre[u] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
+    }
+  };
+  double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  if (nOwn - 1 > w) loadO(nOwn - 1);
+  for (int j = nOwn - 1; j > w; --j) {
+    if (kShared && rg == 0) waitBlock(fl + j, gen, err);
+    const int i0 = j * SB, in = min(SB, nf - i0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * NT;
+      S.Lt[e % SB][e / SB] = opre[u];
+    }
+    for (int e = tid; e < SB * 64; e += NT) {
+      const int c = e & 63, i = e >> 6;
+      S.yt[i][c] = (i < in && c < gw)
+                       ? static_cast<E>(readRow<kShared>(Ds + static_cast<int64_t>(i0 + i) * nrhs + rg + c))
+                       : E(0);
+    }
+    if (j - 1 > w) loadO(j - 1);
+    __syncthreads();
+    typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kk = 0; kk < SB; kk += 4) {
+      const E av = S.Lt[kk + lk][tr + lr];
+      part[0] = M::mma(av, S.yt[kk + lk][tc + lr], part[0]);
+      part[1] = M::mma(av, S.yt[kk + lk][tc + 16 + lr], part[1]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
+      S.ys[qq][c] = (qq < nr && c < gw) ? p0[t][rr] - acc[t][rr] : 0.0;
+    }
+  __syncthreads();
+  {
+    // x_w = L_ww^{-T} (...): the transposed inverse as an FP64 tile product
+    typename MD::v4 xv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kk = 0; kk < SB; kk += 4) {
+      const double av = S.Ld[kk + lk][tr + lr];
+      xv[0] = MD::mma(av, S.ys[kk + lk][tc + lr], xv[0]);
+      xv[1] = MD::mma(av, S.ys[kk + lk][tc + 16 + lr], xv[1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int qq = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
+        if (qq < nr && c < gw) {
+          S.ys[qq][c] = xv[t][rr];
+          writeRow<kShared>(Ds + static_cast<int64_t>(r0 + qq) * nrhs + rg + c, xv[t][rr]);
+        }
+      }
+  }
+  __syncthreads();
+  // the earlier blocks need x only through the work block: release it, then write X
+  if (kShared && publish) publishBlock(fl + w, gen);
+  for (int e = tid; e < nr * gw; e += NT) {
+    const int c = e % gw, r = e / gw;
+    X[static_cast<int64_t>(S.rixA[r]) * nrhs + rg + c] = S.ys[r][c];
+  }
+  __syncthreads();
+}
+
 }  // namespace
 
-// One increment per solve: the generation that the sweep flags are compared
+// One increment per solve: the generation that the chain flags are compared
 // with, kept on the device so that a recorded solve stays valid on replay.
 __global__ void k_solve_begin(unsigned* gen) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *gen = *gen + 1u;
 }
 
-// Forward step 1 (solve.hpp:327-347) through a gather map: for each
-// pre-permutation position t of front s, the children's delta entries that
-// land on it, in the children's order (mapPtr over idxOff positions, mapSrc
-// = (child, child position)).  Thread p of a front takes t = perm[p]: G[t]
-// is the sum of those entries in that order (the same sums as adding the
-// children one after another into a zeroed G) and, for an own row p < nF,
-// y0 = Bp - G.  G and Dl are m x nrhs per-front work blocks at gOff[s] *
-// nrhs (gOff: running sum of the front orders, independent of nrhs); Dl
-// rows [0, nF) hold y, rows [nF, m) the delta handed to the parent.  The
-// right-hand sides are split over the grid's y dimension, kGatherCols each.
-constexpr int kGatherCols = 16;
-__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                     int n, int nrhs, const double* __restrict__ Bp,
-                                                     double* __restrict__ G, double* __restrict__ Dl,
-                                                     const int64_t* __restrict__ gOff,
+// Forward step 1 (solve.hpp:327-347) through a gather map, for every front
+// of a level: front position p (pre-permutation position t = perm[p]) gets
+// g = the sum of the children's delta entries that land on t, in the
+// children's order (mapPtr over idxOff positions, mapSrc = (child, child
+// position)) -- the sums of adding the children one after another into a
+// zeroed block.  An own row p < nF gets y0 = Bp - g, the others g, the start
+// of the delta handed to the parent.  The level's positions are numbered
+// front by front (posStart); lp lanes (a power of two) take one position,
+// each a strided subset of the right-hand sides.
+__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int64_t* posStart, int lpShift,
+                                                     DevFronts fd, int nrhs, const double* __restrict__ Bp,
+                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff,
                                                      const int64_t* __restrict__ mapPtr,
                                                      const int2* __restrict__ mapSrc) {
-  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t item = g >> lpShift;
+  const int l = static_cast<int>(g & ((1 << lpShift) - 1));
+  if (item >= posStart[count]) return;
+  const int f = segmentOf<int64_t>(posStart, count, item);
   const int s = list[f];
-  const int p = (blockIdx.x - blockStart[f]) * blockDim.x + threadIdx.x;
+  const int p = static_cast<int>(item - posStart[f]);
   const int m = fd.m[s];
   if (p >= m) return;
   const int nf = fd.nF[s];
@@ -173,35 +489,20 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
   const int t = fd.perm[io + p];
   const int64_t e0 = mapPtr[io + t], e1 = mapPtr[io + t + 1];
   const bool own = p < nf;
-  const int rix = own ? fd.rowIdx[io + p] : 0;
-  double* Gs = G + gOff[s] * nrhs;
-  double* Ds = Dl + gOff[s] * nrhs;
-  {
-    const int c0 = blockIdx.y * kGatherCols;
-    const int w = min(kGatherCols, nrhs - c0);
-    double g[kGatherCols];
-#pragma unroll
-    for (int j = 0; j < kGatherCols; ++j) g[j] = 0.0;
+  const int64_t rix = own ? fd.rowIdx[io + p] : 0;
+  double* dst = Dl + (gOff[s] + p) * nrhs;
+  for (int c = l; c < nrhs; c += 1 << lpShift) {
+    double acc = 0.0;
     for (int64_t e = e0; e < e1; ++e) {
       const int2 src = mapSrc[e];
-      const int mc = fd.m[src.x];
-      const double* Dc = Dl + gOff[src.x] * nrhs + src.y + static_cast<int64_t>(c0) * mc;
-#pragma unroll
-      for (int j = 0; j < kGatherCols; ++j)
-        if (j < w) g[j] += Dc[static_cast<int64_t>(j) * mc];
+      acc += Dl[(gOff[src.x] + src.y) * nrhs + c];
     }
-#pragma unroll
-    for (int j = 0; j < kGatherCols; ++j) {
-      if (j >= w) break;
-      const int64_t col = c0 + j;
-      Gs[t + col * m] = g[j];
-      if (own) Ds[p + col * m] = Bp[rix + col * n] - g[j];
-    }
+    dst[c] = own ? Bp[rix * nrhs + c] - acc : acc;
   }
 }
 
-// Inverses of the unit lower diagonal blocks of L that the sweeps solve
-// with: block b of front s (own rows [32b, 32b + 32) of [0, nF)) gets
+// Inverses of the unit lower diagonal blocks of L that the solve applies:
+// block b of front s (own rows [32b, 32b + 32) of [0, nF)) gets
 // M = L_bb^{-1}, row-major 32 x 32 at inv + invOff[s] + 1024 b (FP64; rows
 // and columns past nF hold the identity).  One workgroup per block; thread c
 // forms column c by forward substitution.  Applying M is the substitution
@@ -212,7 +513,7 @@ __global__ __launch_bounds__(64) void k_diag_inverse(const int* list, int count,
   __shared__ double Ls[SB][SB + 1];
   __shared__ double Ms[SB][SB + 1];
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
   const int s = list[f];
   const int b = blockIdx.x - blockStart[f];
   const int m = fd.m[s];
@@ -248,342 +549,226 @@ __global__ __launch_bounds__(64) void k_diag_inverse(const int* list, int count,
   for (int e = tid; e < SB * SB; e += 64) out[e] = Ms[e / SB][e % SB];
 }
 
-// Forward sweep of one tree level (solve.hpp:349-361 and :434-455).  The
-// workgroups of a front take, in order, the own-row blocks b = 0 .. nOwn-1
-// (SB rows each) and then the blocks of the rows [nF, m).  Own block b
-// subtracts L(b, j) y_j for j < b as the blocks j are published, solves its
-// unit lower diagonal block, applies D^{-1} to its rows and publishes y_b.
-// A block of the rows [nF, m) accumulates L(i, :) y over all own blocks and
-// stores delta(i) = G(perm(i)) + L(i, :) y.  A workgroup only waits for
-// blocks of the same front with a lower index, which were dispatched
-// before it.
+// Forward own rows of fronts with a short own-row chain: one workgroup per
+// front runs the blocks b = 0, 1, ... in order, reading the y_j it has
+// written back from the work block (its own barriers order those stores and
+// loads), so no block waits on another workgroup.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                    int n, int nrhs, const double* __restrict__ G,
-                                                    double* __restrict__ Z, double* __restrict__ Dl,
-                                                    const int64_t* __restrict__ gOff, unsigned* flags,
-                                                    const int64_t* __restrict__ flagOff, const unsigned* genp,
-                                                    int* err, const double* __restrict__ inv,
-                                                    const int64_t* __restrict__ invOff) {
-  using M = SweepMfma<T>;
-  using E = typename M::E;
-  using MD = SweepMfma<double>;
-  constexpr int NT = 256;
-  __shared__ E Lt[SB][SB + 1];          // [row][col] operand tile of L
-  __shared__ double ybuf[SB * YW];      // y_j operand tile (E, [q][rhs]); then this block's y (double)
-  __shared__ double Ld[SB][SB + 1];     // own block: the inverse of its unit lower diagonal block
-  __shared__ double dA[SB + 1], dB[SB + 1];  // own block: d_diag, d_off of positions r0 - 1 .. r0 + SB - 1
-  __shared__ int pkA[SB + 1], rixA[SB + 1];  // and their pivot kinds and final-order rows
-  E* yt = reinterpret_cast<E*>(ybuf);
-  double (*ys)[YW] = reinterpret_cast<double (*)[YW]>(ybuf);
-  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf(blockStart, count, blockIdx.x);
-  const int s = list[f];
-  const int b = blockIdx.x - blockStart[f];
-  const int m = fd.m[s];
-  const int nf = fd.nF[s];
-  const int nOwn = (nf + SB - 1) / SB;
-  const int nDel = (m - nf + SB - 1) / SB;
-  if (b >= nOwn + nDel) return;
-  const unsigned gen = *genp;
-  const bool own = b < nOwn;
-  const int r0 = own ? b * SB : nf + (b - nOwn) * SB;
-  const int r1 = own ? min(r0 + SB, nf) : min(r0 + SB, m);
-  const int nr = r1 - r0;
-  const int jEnd = own ? b : nOwn;
-  const int64_t io = fd.idxOff[s];
-  const int* pk = fd.pkind + io;
-  const T* F = static_cast<const T*>(fd.F[s]);
-  const double* Gs = G + gOff[s] * nrhs;
+__global__ __launch_bounds__(256) void k_fwd_own(const int* list, int count, DevFronts fd, int nrhs,
+                                                 double* __restrict__ Z, double* __restrict__ Dl,
+                                                 const int64_t* __restrict__ gOff, const double* __restrict__ inv,
+                                                 const int64_t* __restrict__ invOff) {
+  __shared__ BlockLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int nOwn = (fd.nF[s] + SB - 1) / SB;
   double* Ds = Dl + gOff[s] * nrhs;
-  unsigned* fl = flags + flagOff[s];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  // wave wv owns output rows tr .. tr + 15 and right-hand sides tc .. tc + 31
-  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
-  const int lr = lane & 15, lk = lane >> 4;
-  // What the own block's tail needs does not depend on other blocks: stage
-  // it before the first wait (visible after the next barrier).
-  if (own) {
-    const double* Mi = inv + invOff[s] + static_cast<int64_t>(b) * SB * SB;
-    for (int e = tid; e < SB * SB; e += NT) Ld[e / SB][e % SB] = Mi[e];
-    for (int r = tid; r <= SB; r += NT) {
-      const int p = r0 - 1 + r;
-      const bool ok = p >= 0 && p < r1;
-      dA[r] = ok ? fd.dDiag[io + p] : 0.0;
-      dB[r] = ok ? fd.dOff[io + p] : 0.0;
-      pkA[r] = ok ? pk[p] : 0;
-      rixA[r] = ok ? fd.rowIdx[io + p] : 0;
-    }
-  }
-  // L(b, j) tile of the workgroup's rows: four elements per thread, loaded
-  // one tile ahead so the load overlaps the wait and the products.
-  E lpre[4];
-  auto loadL = [&](int j) {
-    const int c0 = j * SB, cn = min(SB, nf - c0);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + u * NT;
-      const int r = e % SB, qq = e / SB;
-      lpre[u] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
-    }
-  };
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
-    // an own block's y0 (from the gather) does not depend on other blocks:
-    // load it before the waits, off the block chain's critical path
-    double y0p[2][4];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        y0p[t][rr] = (own && r < nr && c < gw) ? Ds[(r0 + r) + static_cast<int64_t>(rg + c) * m] : 0.0;
-      }
-    if (jEnd > 0) loadL(0);
-    for (int j = 0; j < jEnd; ++j) {
-      if (rg == 0) waitBlock(fl + j, gen, err);
-      const int c0 = j * SB, cn = min(SB, nf - c0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int e = tid + u * NT;
-        Lt[e % SB][e / SB] = lpre[u];
-      }
-      for (int e = tid; e < SB * 64; e += NT) {
-        const int qq = e % SB, c = e / SB;
-        yt[qq * YW + c] =
-            (qq < cn && c < gw) ? static_cast<E>(loadShared(Ds + (c0 + qq) + static_cast<int64_t>(rg + c) * m)) : E(0);
-      }
-      if (j + 1 < jEnd) loadL(j + 1);
-      __syncthreads();
-      typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-#pragma unroll
-      for (int kk = 0; kk < SB; kk += 4) {
-        const E av = Lt[tr + lr][kk + lk];
-        part[0] = M::mma(av, yt[(kk + lk) * YW + tc + lr], part[0]);
-        part[1] = M::mma(av, yt[(kk + lk) * YW + tc + 16 + lr], part[1]);
-      }
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
-      __syncthreads();
-    }
-    if (!own) {
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-          if (r < nr && c < gw) {
-            const int i = r0 + r;
-            const int64_t cc = rg + c;
-            Ds[i + cc * m] = Gs[fd.perm[io + i] + cc * m] + acc[t][rr];
-          }
-        }
-      continue;
-    }
-    // y_b = Ltop(b, b)^{-1} (y0_b - sum_j L(b, j) y_j), the inverse applied
-    // as an FP64 tile product (rows past nr are zero so they add nothing)
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (c < gw) ys[r][c] = (r < nr) ? y0p[t][rr] - acc[t][rr] : 0.0;
-      }
-    __syncthreads();
-    {
-      typename MD::v4 yv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-#pragma unroll
-      for (int kk = 0; kk < SB; kk += 4) {
-        const double av = Ld[tr + lr][kk + lk];
-        yv[0] = MD::mma(av, (tc + lr < gw) ? ys[kk + lk][tc + lr] : 0.0, yv[0]);
-        yv[1] = MD::mma(av, (tc + 16 + lr < gw) ? ys[kk + lk][tc + 16 + lr] : 0.0, yv[1]);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int r = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
-          if (r < nr && c < gw) {
-            ys[r][c] = yv[t][rr];
-            storeShared(Ds + (r0 + r) + static_cast<int64_t>(rg + c) * m, yv[t][rr]);
-          }
-        }
-    }
-    __syncthreads();
-    // the later blocks need y only: release it before forming z
-    if (rg + 64 >= nrhs) publishBlock(fl + b, gen);
-    // z = D^{-1} y for the pivot blocks that end in this block (slot r + 1
-    // of the staged pivot data is position r0 + r)
-    for (int e = tid; e < nr * gw; e += NT) {
-      const int r = e % nr, c = e / nr;
-      const int p = r0 + r;
-      const int64_t cc = rg + c;
-      const int kind = pkA[r + 1];
-      if (kind == 1) {
-        Z[rixA[r + 1] + cc * n] = ys[r][c] / dA[r + 1];
-      } else if (kind == 2 && r + 1 < nr) {
-        const double d11 = dA[r + 1], d21 = dB[r + 1], d22 = dA[r + 2];
-        const double det = d11 * d22 - d21 * d21;
-        const double y0 = ys[r][c], y1 = ys[r + 1][c];
-        Z[rixA[r + 1] + cc * n] = (d22 * y0 - d21 * y1) / det;
-        Z[rixA[r + 2] + cc * n] = (-d21 * y0 + d11 * y1) / det;
-      } else if (kind == 0 && r == 0 && p > 0) {
-        // second row of a 2x2 block that starts in the previous block
-        const double d11 = dA[0], d21 = dB[0], d22 = dA[1];
-        const double det = d11 * d22 - d21 * d21;
-        const double y0 = loadShared(Ds + (p - 1) + cc * m), y1 = ys[0][c];
-        Z[rixA[0] + cc * n] = (d22 * y0 - d21 * y1) / det;
-        Z[rixA[1] + cc * n] = (-d21 * y0 + d11 * y1) / det;
-      }
-    }
-    __syncthreads();
+    for (int b = 0; b < nOwn; ++b)
+      fwdOwnBlock<T, false>(S, fd, s, b, rg, gw, nrhs, Z, Ds, inv + invOff[s] + static_cast<int64_t>(b) * SB * SB,
+                            nullptr, 0u, nullptr, false);
   }
 }
 
-// Backward sweep of one tree level (solve.hpp:364-383).  The workgroups of a
-// front take the own-row blocks from the last to the first.  Block w forms
-// x_w = z_w - L(anc, w)^T x_anc - sum_{j > w} L(j, w)^T x_j (the ancestors'
-// x is final from the levels above; the blocks j of the same front are
-// waited for), solves its unit upper diagonal block and publishes x_w.
+// Forward own rows of fronts with a long own-row chain: one workgroup per
+// own block; block b takes the products with the blocks j < b as they are
+// published (flag per block, bounded wait), so the chain's products run on
+// many workgroups at once.  A workgroup only waits for blocks of the same
+// front with a lower index, which were dispatched before it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                    int n, int nrhs, const double* __restrict__ Z,
-                                                    double* __restrict__ X, double* __restrict__ Dl,
-                                                    const int64_t* __restrict__ gOff, unsigned* flags,
-                                                    const int64_t* __restrict__ flagOff, const unsigned* genp,
-                                                    int* err, const double* __restrict__ inv,
-                                                    const int64_t* __restrict__ invOff, int ancGathered) {
+__global__ __launch_bounds__(256) void k_fwd_chain(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                   int nrhs, double* __restrict__ Z, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff, unsigned* flags,
+                                                   const int64_t* __restrict__ flagOff, const unsigned* genp,
+                                                   int* err, const double* __restrict__ inv,
+                                                   const int64_t* __restrict__ invOff) {
+  __shared__ BlockLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
+  const int nOwn = (fd.nF[s] + SB - 1) / SB;
+  if (b == 0 && threadIdx.x == 0 && nOwn > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (b >= nOwn) return;
+  const unsigned gen = *genp;
+  double* Ds = Dl + gOff[s] * nrhs;
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    fwdOwnBlock<T, true>(S, fd, s, b, rg, gw, nrhs, Z, Ds, inv + invOff[s] + static_cast<int64_t>(b) * SB * SB,
+                         flags + flagOff[s], gen, err, rg + 64 >= nrhs);
+  }
+}
+
+// Rows [nF, m) of a front once its own rows are solved (solve.hpp:349-361):
+// delta(i) = g(i) + L(i, 0:nF) y, g already in the work block.  One
+// workgroup per 64-row chunk (waves of 32 rows x 32 right-hand sides); the
+// product runs over the own columns in 32-column tiles, the next tiles'
+// operands loaded while the current ones are multiplied.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                   int nrhs, double* __restrict__ Dl, const int64_t* __restrict__ gOff,
+                                                   int* err) {
   using M = SweepMfma<T>;
   using E = typename M::E;
-  using MD = SweepMfma<double>;
   constexpr int NT = 256;
-  __shared__ E Lt[DT][SB + 1];          // [row i][own column q] operand tile of L
-  __shared__ double xbuf[DT * YW];      // x operand tile (E, [row i][rhs]); then this block's x (double)
-  __shared__ double Ld[SB][SB + 1];     // the inverse of the block's unit lower diagonal block
-  __shared__ int rixA[SB];              // final-order rows of the block
-  E* xt = reinterpret_cast<E*>(xbuf);
-  double (*xs)[YW] = reinterpret_cast<double (*)[YW]>(xbuf);
+  __shared__ E Lt[DT][SB + 1];  // [row i][own column q]
+  __shared__ E yt[SB][YW];      // [own column q][right-hand side]
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf(blockStart, count, blockIdx.x);
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
   const int s = list[f];
   const int b = blockIdx.x - blockStart[f];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
-  const int nOwn = (nf + SB - 1) / SB;
-  if (b >= nOwn) return;
-  const unsigned gen = *genp;
-  const int w = nOwn - 1 - b;
-  const int r0 = w * SB;
-  const int nr = min(r0 + SB, nf) - r0;
-  const int64_t io = fd.idxOff[s];
-  const int* pk = fd.pkind + io;
+  const int nD = (m - nf + DT - 1) / DT;
+  if (b == 0 && threadIdx.x == 0 && nD > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (b >= nD) return;
+  const int i0 = nf + b * DT, ni = min(DT, m - i0);
   const T* F = static_cast<const T*>(fd.F[s]);
   double* Ds = Dl + gOff[s] * nrhs;
-  unsigned* fl = flags + flagOff[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  // wave wv owns own columns tr .. tr + 15 and right-hand sides tc .. tc + 31
-  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  // staged before any wait (visible after the next barrier)
-  {
-    const double* Mi = inv + invOff[s] + static_cast<int64_t>(w) * SB * SB;
-    for (int e = tid; e < SB * SB; e += NT) Ld[e / SB][e % SB] = Mi[e];
-  }
-  for (int q = tid; q < SB; q += NT) rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
-  // the block's tail reads rixA and Ld from every wave, and a block with no
-  // ancestors' rows and no later own blocks reaches it with no other barrier
-  __syncthreads();
-  // L tiles loaded one tile ahead: the ancestors' rows (DT x SB, eight
-  // elements per thread), then the own blocks below this one (SB x SB, four)
-  E apre[8];
-  auto loadA = [&](int i0) {
-    const int in = min(DT, m - i0);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = tid + u * NT;
-      const int i = e % DT, qq = e / DT;
-      apre[u] = (i < in && qq < nr) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(r0 + qq) * m]) : E(0);
-    }
-  };
-  E opre[4];
-  auto loadO = [&](int j) {
-    const int i0 = j * SB, in = min(SB, nf - i0);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + u * NT;
-      const int i = e % SB, qq = e / SB;
-      opre[u] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
-    }
-  };
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
-    // the block's z does not depend on other blocks: load it before the waits
-    double z0p[2][4];
+    E lpre[8], ypre[8];
+    auto loadT = [&](int k0) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        z0p[t][rr] = (qq < nr && c < gw) ? Z[rixA[qq] + static_cast<int64_t>(rg + c) * n] : 0.0;
-      }
-    // acc(q, c) += sum over rows i of the tile of L(i, r0 + q) x(i, c)
-    auto tileProduct = [&](int kn) {
-      typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-      for (int kk = 0; kk < kn; kk += 4) {
-        const E av = Lt[kk + lk][tr + lr];
-        part[0] = M::mma(av, xt[(kk + lk) * YW + tc + lr], part[0]);
-        part[1] = M::mma(av, xt[(kk + lk) * YW + tc + 16 + lr], part[1]);
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * NT;
+        const int i = e & (DT - 1), q = e / DT;
+        lpre[u] = (i < ni && k0 + q < nf) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(k0 + q) * m]) : E(0);
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * NT;
+        const int c = e & 63, q = e >> 6;
+        ypre[u] = (k0 + q < nf && c < gw) ? static_cast<E>(Ds[static_cast<int64_t>(k0 + q) * nrhs + rg + c]) : E(0);
+      }
     };
-    // rows of the ancestors (their x is final from the levels above)
-    if (nf < m) loadA(nf);
-    for (int i0 = nf; i0 < m; i0 += DT) {
+    double acc[2][2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) acc[bi][bj][rr] = 0.0;
+    if (nf > 0) loadT(0);
+    for (int k0 = 0; k0 < nf; k0 += SB) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * NT;
+        Lt[e & (DT - 1)][e / DT] = lpre[u];
+        yt[e >> 6][e & 63] = ypre[u];
+      }
+      if (k0 + SB < nf) loadT(k0 + SB);
+      __syncthreads();
+      typename M::v4 part[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+#pragma unroll
+      for (int kk = 0; kk < SB; kk += 4) {
+        const E a0 = Lt[wr + lr][kk + lk], a1 = Lt[wr + 16 + lr][kk + lk];
+        const E b0 = yt[kk + lk][wc + lr], b1 = yt[kk + lk][wc + 16 + lr];
+        part[0][0] = M::mma(a0, b0, part[0][0]);
+        part[0][1] = M::mma(a0, b1, part[0][1]);
+        part[1][0] = M::mma(a1, b0, part[1][0]);
+        part[1][1] = M::mma(a1, b1, part[1][1]);
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) acc[bi][bj][rr] += static_cast<double>(part[bi][bj][rr]);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int r = wr + 16 * bi + M::row(lane, rr), c = wc + 16 * bj + lr;
+          if (r < ni && c < gw) Ds[static_cast<int64_t>(i0 + r) * nrhs + rg + c] += acc[bi][bj][rr];
+        }
+  }
+}
+
+// Backward step, the ancestors' part (solve.hpp:364-383): for own block w of
+// a front, P_w = z_w - L(nF:m, w)^T x(nF:m), the ancestors' x final from the
+// levels above and read from X through the front's row indices (one
+// contiguous row each).  One workgroup per own block, the rows taken in
+// 64-row tiles; P_w goes to the block's work rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                 int nrhs, const double* __restrict__ Z, const double* __restrict__ X,
+                                                 double* __restrict__ Dl, const int64_t* __restrict__ gOff, int* err) {
+  using M = SweepMfma<T>;
+  using E = typename M::E;
+  constexpr int NT = 256;
+  __shared__ E Lt[DT][SB + 1];  // [ancestor row i][own column q]
+  __shared__ E xt[DT][YW];      // [ancestor row i][right-hand side]
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int w = blockIdx.x - blockStart[f];
+  const int m = fd.m[s];
+  const int nf = fd.nF[s];
+  const int nOwn = (nf + SB - 1) / SB;
+  if (w == 0 && threadIdx.x == 0 && nOwn > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (w >= nOwn) return;
+  const int r0 = w * SB, nr = min(SB, nf - r0);
+  const int64_t io = fd.idxOff[s];
+  const T* F = static_cast<const T*>(fd.F[s]);
+  double* Ds = Dl + gOff[s] * nrhs;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    E lpre[8], xpre[16];
+    auto loadT = [&](int i0) {
       const int in = min(DT, m - i0);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = tid + u * NT;
-        Lt[e % DT][e / DT] = apre[u];
+        const int i = e & (DT - 1), q = e / DT;
+        lpre[u] = (i < in && q < nr) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(r0 + q) * m]) : E(0);
       }
-      for (int e = tid; e < DT * 64; e += NT) {
-        const int i = e % DT, c = e / DT;
-        double v = 0.0;
-        if (i < in && c < gw)
-          v = ancGathered ? Ds[(i0 + i) + static_cast<int64_t>(rg + c) * m]
-                          : X[fd.rowIdx[io + i0 + i] + static_cast<int64_t>(rg + c) * n];
-        xt[i * YW + c] = static_cast<E>(v);
-      }
-      if (i0 + DT < m) loadA(i0 + DT);
-      __syncthreads();
-      tileProduct((in + 3) & ~3);
-      __syncthreads();
-    }
-    // own blocks below this one, as they are published
-    if (nOwn - 1 > w) loadO(nOwn - 1);
-    for (int j = nOwn - 1; j > w; --j) {
-      if (rg == 0) waitBlock(fl + j, gen, err);
-      const int i0 = j * SB, in = min(SB, nf - i0);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int e = tid + u * NT;
-        Lt[e % SB][e / SB] = opre[u];
+        const int c = e & 63, i = e >> 6;
+        xpre[u] = (i < in && c < gw)
+                      ? static_cast<E>(X[static_cast<int64_t>(fd.rowIdx[io + i0 + i]) * nrhs + rg + c])
+                      : E(0);
       }
-      for (int e = tid; e < SB * 64; e += NT) {
-        const int i = e % SB, c = e / SB;
-        xt[i * YW + c] =
-            (i < in && c < gw) ? static_cast<E>(loadShared(Ds + (i0 + i) + static_cast<int64_t>(rg + c) * m)) : E(0);
+    };
+    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    if (nf < m) loadT(nf);
+    for (int i0 = nf; i0 < m; i0 += DT) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * NT;
+        Lt[e & (DT - 1)][e / DT] = lpre[u];
       }
-      if (j - 1 > w) loadO(j - 1);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = tid + u * NT;
+        xt[e >> 6][e & 63] = xpre[u];
+      }
+      if (i0 + DT < m) loadT(i0 + DT);
       __syncthreads();
-      tileProduct((in + 3) & ~3);
+      typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+      for (int kk = 0; kk < DT; kk += 4) {
+        const E av = Lt[kk + lk][tr + lr];
+        part[0] = M::mma(av, xt[kk + lk][tc + lr], part[0]);
+        part[1] = M::mma(av, xt[kk + lk][tc + 16 + lr], part[1]);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) acc[t][rr] += static_cast<double>(part[t][rr]);
       __syncthreads();
     }
 #pragma unroll
@@ -591,59 +776,61 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(const int* list, int count, c
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (c < gw) xs[qq][c] = (qq < nr) ? z0p[t][rr] - acc[t][rr] : 0.0;
-      }
-    __syncthreads();
-    {
-      // x_w = L_ww^{-T} (z_w - ...): the transposed inverse as an FP64 tile product
-      typename MD::v4 xv[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-#pragma unroll
-      for (int kk = 0; kk < SB; kk += 4) {
-        const double av = Ld[kk + lk][tr + lr];
-        xv[0] = MD::mma(av, (tc + lr < gw) ? xs[kk + lk][tc + lr] : 0.0, xv[0]);
-        xv[1] = MD::mma(av, (tc + 16 + lr < gw) ? xs[kk + lk][tc + 16 + lr] : 0.0, xv[1]);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int qq = tr + MD::row(lane, rr), c = tc + 16 * t + lr;
-          if (qq < nr && c < gw) {
-            xs[qq][c] = xv[t][rr];
-            storeShared(Ds + (r0 + qq) + static_cast<int64_t>(rg + c) * m, xv[t][rr]);
-          }
+        if (qq < nr && c < gw) {
+          const double z = Z[static_cast<int64_t>(fd.rowIdx[io + r0 + qq]) * nrhs + rg + c];
+          Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] = z - acc[t][rr];
         }
-    }
-    __syncthreads();
-    // the earlier blocks need x only through Ds: release it, then write X
-    if (rg + 64 >= nrhs) publishBlock(fl + w, gen);
-    for (int e = tid; e < nr * gw; e += NT) {
-      const int r = e % nr, c = e / nr;
-      X[rixA[r] + static_cast<int64_t>(rg + c) * n] = xs[r][c];
-    }
-    __syncthreads();
+      }
   }
 }
 
-// Backward step 0: the ancestors' x of every front of a level, gathered once
-// into the front's work block rows [nF, m) (free after the forward sweep), so
-// that the backward sweep's workgroups read it contiguously.  Workgroups per
-// front as the forward gather (blockStart: ceil(m / 256)); fronts of at most
-// kTinyFront rows are skipped (their kernel reads X itself).
-__global__ __launch_bounds__(256) void k_bwd_gather(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                     int n, int nrhs, const double* __restrict__ X,
-                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
-  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf(blockStart, count, blockIdx.x);
-  const int s = list[f];
-  const int m = fd.m[s];
-  if (m <= kTinyFront) return;
-  const int i = fd.nF[s] + (blockIdx.x - blockStart[f]) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const int r = fd.rowIdx[fd.idxOff[s] + i];
+// Backward own rows of fronts with a short own-row chain: one workgroup per
+// front, blocks w = nOwn - 1 .. 0 in order, the x_j read back from the work
+// block this workgroup wrote.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_own(const int* list, int count, DevFronts fd, int nrhs,
+                                                 double* __restrict__ X, double* __restrict__ Dl,
+                                                 const int64_t* __restrict__ gOff, const double* __restrict__ inv,
+                                                 const int64_t* __restrict__ invOff) {
+  __shared__ BlockLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
+  const int nOwn = (fd.nF[s] + SB - 1) / SB;
   double* Ds = Dl + gOff[s] * nrhs;
-  for (int64_t c = 0; c < nrhs; ++c) Ds[i + c * m] = X[r + c * n];
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    for (int w = nOwn - 1; w >= 0; --w)
+      bwdOwnBlock<T, false>(S, fd, s, w, rg, gw, nrhs, X, Ds, inv + invOff[s] + static_cast<int64_t>(w) * SB * SB,
+                            nullptr, 0u, nullptr, false);
+  }
+}
+
+// Backward own rows of fronts with a long own-row chain: one workgroup per
+// own block, taken from the last block to the first; block w waits for the
+// blocks j > w of its front (dispatched before it).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_chain(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                   int nrhs, double* __restrict__ X, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff, unsigned* flags,
+                                                   const int64_t* __restrict__ flagOff, const unsigned* genp,
+                                                   int* err, const double* __restrict__ inv,
+                                                   const int64_t* __restrict__ invOff) {
+  __shared__ BlockLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
+  const int nOwn = (fd.nF[s] + SB - 1) / SB;
+  if (b == 0 && threadIdx.x == 0 && nOwn > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (b >= nOwn) return;
+  const int w = nOwn - 1 - b;
+  const unsigned gen = *genp;
+  double* Ds = Dl + gOff[s] * nrhs;
+  for (int rg = 0; rg < nrhs; rg += 64) {
+    const int gw = min(64, nrhs - rg);
+    bwdOwnBlock<T, true>(S, fd, s, w, rg, gw, nrhs, X, Ds, inv + invOff[s] + static_cast<int64_t>(w) * SB * SB,
+                         flags + flagOff[s], gen, err, rg + 64 >= nrhs);
+  }
 }
 
 // Fronts of at most 64 rows: one wave per front, lane i owning row i, and
@@ -651,9 +838,9 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const int* list, int count, 
 // unit-lower solve of the own rows, D^{-1} into Z, and the delta of the
 // remaining rows (solve.hpp:303-362, :434-455) in one pass; no flags needed.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, DevFronts fd, int n, int nrhs,
-                                                   const double* __restrict__ G, double* __restrict__ Z,
-                                                   double* __restrict__ Dl, const int64_t* __restrict__ gOff) {
+__global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, DevFronts fd, int nrhs,
+                                                   double* __restrict__ Z, double* __restrict__ Dl,
+                                                   const int64_t* __restrict__ gOff) {
   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (f >= count) return;
   const int s = list[f];
@@ -663,17 +850,15 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
   const int64_t io = fd.idxOff[s];
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
-  const double* Gs = G + gOff[s] * nrhs;
-  double* Ds = Dl + gOff[s] * nrhs;
+  double* Dr = Dl + (gOff[s] + lane) * nrhs;  // this lane's work row
   const bool row = lane < m;
   const bool own = lane < nf;
   const int kind = own ? pk[lane] : 0;
-  const int prow = row ? fd.rowIdx[io + lane] : 0;
-  const int gpos = row ? fd.perm[io + lane] : 0;
+  const int64_t prow = own ? fd.rowIdx[io + lane] : 0;
   for (int cg = 0; cg < nrhs; cg += 8) {
     double v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = (own && cg + c < nrhs) ? Ds[lane + static_cast<int64_t>(cg + c) * m] : 0.0;
+    for (int c = 0; c < 8; ++c) v[c] = (own && cg + c < nrhs) ? Dr[cg + c] : 0.0;
     for (int j = 0; j < nf; ++j) {
       const double l = (row && lane > j) ? lval(F, m, pk, lane, j) : 0.0;
 #pragma unroll
@@ -685,25 +870,25 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       if (cg + c >= nrhs) break;
-      const int64_t cc = cg + c;
+      const int cc = cg + c;
       // lanes below nF hold y; the others hold -L(i, :) y
       const double up = __shfl(v[c], min(lane + 1, 63), 64);
       const double dn = __shfl(v[c], max(lane - 1, 0), 64);
       if (own) {
-        Ds[lane + cc * m] = v[c];
+        Dr[cc] = v[c];
         if (kind == 1) {
-          Z[prow + cc * n] = v[c] / fd.dDiag[io + lane];
+          Z[prow * nrhs + cc] = v[c] / fd.dDiag[io + lane];
         } else if (kind == 2) {
           const double d11 = fd.dDiag[io + lane], d21 = fd.dOff[io + lane], d22 = fd.dDiag[io + lane + 1];
           const double det = d11 * d22 - d21 * d21;
-          Z[prow + cc * n] = (d22 * v[c] - d21 * up) / det;
+          Z[prow * nrhs + cc] = (d22 * v[c] - d21 * up) / det;
         } else {
           const double d11 = fd.dDiag[io + lane - 1], d21 = fd.dOff[io + lane - 1], d22 = fd.dDiag[io + lane];
           const double det = d11 * d22 - d21 * d21;
-          Z[prow + cc * n] = (-d21 * dn + d11 * v[c]) / det;
+          Z[prow * nrhs + cc] = (-d21 * dn + d11 * v[c]) / det;
         }
       } else if (row) {
-        Ds[lane + cc * m] = Gs[gpos + cc * m] - v[c];
+        Dr[cc] = Dr[cc] - v[c];
       }
     }
   }
@@ -713,7 +898,7 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
 // x_own = z - L(anc, :)^T x_anc, then the unit upper solve with L_top^T
 // (solve.hpp:364-383).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, DevFronts fd, int n, int nrhs,
+__global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, DevFronts fd, int nrhs,
                                                    const double* __restrict__ Z, double* __restrict__ X) {
   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (f >= count) return;
@@ -725,16 +910,16 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
   const bool own = lane < nf;
-  const int prow = own ? fd.rowIdx[io + lane] : 0;
+  const int64_t prow = own ? fd.rowIdx[io + lane] : 0;
   // lane t holds the ancestors' x of the front's row nF + t (m <= 64)
   const int ncb = m - nf;
-  const int crow = (lane < ncb) ? fd.rowIdx[io + nf + lane] : 0;
+  const int64_t crow = (lane < ncb) ? fd.rowIdx[io + nf + lane] : 0;
   for (int cg = 0; cg < nrhs; cg += 8) {
     double x[8], xa[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      x[c] = (own && cg + c < nrhs) ? Z[prow + static_cast<int64_t>(cg + c) * n] : 0.0;
-      xa[c] = (lane < ncb && cg + c < nrhs) ? X[crow + static_cast<int64_t>(cg + c) * n] : 0.0;
+      x[c] = (own && cg + c < nrhs) ? Z[prow * nrhs + cg + c] : 0.0;
+      xa[c] = (lane < ncb && cg + c < nrhs) ? X[crow * nrhs + cg + c] : 0.0;
     }
     for (int i0 = 0; i0 < ncb; i0 += 8) {
       double lv[8];
@@ -758,7 +943,7 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-      if (own && cg + c < nrhs) X[prow + static_cast<int64_t>(cg + c) * n] = x[c];
+      if (own && cg + c < nrhs) X[prow * nrhs + cg + c] = x[c];
   }
 }
 
@@ -1073,23 +1258,30 @@ static int gridFor(int64_t total, int threads) {
   return static_cast<int>(b);
 }
 void launchPermuteIn(int n, int nrhs, const int* perm, const double* B, int64_t ldb, double* Bp, hipStream_t st) {
-  k_permute_in<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, B, ldb, Bp);
+  if (n <= 0 || nrhs <= 0) return;
+  const dim3 grid((n + 63) / 64, (nrhs + 63) / 64);
+  k_permute_in<<<grid, 256, 0, st>>>(n, nrhs, perm, B, ldb, Bp);
   hipCheck(hipGetLastError(), "launchPermuteIn");
 }
 void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double* Xo, int64_t ldx, hipStream_t st) {
-  k_permute_out<<<gridFor(static_cast<int64_t>(n) * nrhs, 256), 256, 0, st>>>(n, nrhs, perm, X, Xo, ldx);
+  if (n <= 0 || nrhs <= 0) return;
+  const dim3 grid((n + 63) / 64, (nrhs + 63) / 64);
+  k_permute_out<<<grid, 256, 0, st>>>(n, nrhs, perm, X, Xo, ldx);
   hipCheck(hipGetLastError(), "launchPermuteOut");
 }
 void launchSolveBegin(unsigned* gen, hipStream_t st) {
   k_solve_begin<<<1, 64, 0, st>>>(gen);
   hipCheck(hipGetLastError(), "launchSolveBegin");
 }
-void launchFwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* Bp, double* G, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
+void launchFwdGather(const int* list, int count, const int64_t* posStart, int64_t positions, const DevFronts& fd,
+                     int nrhs, const double* Bp, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
                      const int2* mapSrc, hipStream_t st) {
-  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
-  const dim3 grid(blocks, (nrhs + kGatherCols - 1) / kGatherCols);
-  k_fwd_gather<<<grid, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Bp, G, Dl, gOff, mapPtr, mapSrc);
+  if (count <= 0 || positions <= 0 || nrhs <= 0) return;
+  int lpShift = 0;  // lanes per position: the power of two at or above nrhs, at most 64
+  while ((1 << lpShift) < nrhs && lpShift < 6) ++lpShift;
+  const int64_t blocks = ((positions << lpShift) + 255) / 256;
+  k_fwd_gather<<<static_cast<unsigned>(blocks), 256, 0, st>>>(list, count, posStart, lpShift, fd, nrhs, Bp, Dl, gOff,
+                                                              mapPtr, mapSrc);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
@@ -1101,57 +1293,84 @@ void launchDiagInverse(const int* list, int count, const int* blockStart, int bl
     k_diag_inverse<double><<<blocks, 64, 0, st>>>(list, count, blockStart, fd, inv, invOff);
   hipCheck(hipGetLastError(), "launchDiagInverse");
 }
-void launchFwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                    int nrhs, const double* G, double* Z, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, hipStream_t st) {
-  if (count <= 0 || blocks <= 0) return;
+void launchFwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,
+                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st) {
+  if (count <= 0 || nrhs <= 0) return;
   if (fd.fp32)
-    k_fwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
-                                               gen, err, inv, invOff);
+    k_fwd_own<float><<<count, 256, 0, st>>>(list, count, fd, nrhs, Z, Dl, gOff, inv, invOff);
   else
-    k_fwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, G, Z, Dl, gOff, flags, flagOff,
-                                                gen, err, inv, invOff);
-  hipCheck(hipGetLastError(), "launchFwdSweep");
+    k_fwd_own<double><<<count, 256, 0, st>>>(list, count, fd, nrhs, Z, Dl, gOff, inv, invOff);
+  hipCheck(hipGetLastError(), "launchFwdOwn");
 }
-void launchBwdGather(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                     int nrhs, const double* X, double* Dl, const int64_t* gOff, hipStream_t st) {
-  if (count <= 0 || blocks <= 0) return;
-  k_bwd_gather<<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, X, Dl, gOff);
-  hipCheck(hipGetLastError(), "launchBwdGather");
-}
-void launchBwdSweep(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int n,
-                    int nrhs, const double* Z, double* X, double* Dl, const int64_t* gOff, unsigned* flags,
-                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv,
-                    const int64_t* invOff, bool ancGathered, hipStream_t st) {
-  if (count <= 0 || blocks <= 0) return;
-  const int ag = ancGathered ? 1 : 0;
+void launchFwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* Z, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
+                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
   if (fd.fp32)
-    k_bwd_sweep<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                               gen, err, inv, invOff, ag);
+    k_fwd_chain<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, Dl, gOff, flags, flagOff, gen,
+                                               err, inv, invOff);
   else
-    k_bwd_sweep<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, n, nrhs, Z, X, Dl, gOff, flags, flagOff,
-                                                gen, err, inv, invOff, ag);
-  hipCheck(hipGetLastError(), "launchBwdSweep");
+    k_fwd_chain<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, Dl, gOff, flags, flagOff, gen,
+                                                err, inv, invOff);
+  hipCheck(hipGetLastError(), "launchFwdChain");
 }
-void launchFwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* G, double* Z,
-                   double* Dl, const int64_t* gOff, hipStream_t st) {
-  if (count <= 0) return;
+void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* Dl, const int64_t* gOff, int* err, hipStream_t st) {
+  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
+  if (fd.fp32)
+    k_fwd_delta<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Dl, gOff, err);
+  else
+    k_fwd_delta<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Dl, gOff, err);
+  hipCheck(hipGetLastError(), "launchFwdDelta");
+}
+void launchBwdAnc(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                  const double* Z, const double* X, double* Dl, const int64_t* gOff, int* err, hipStream_t st) {
+  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
+  if (fd.fp32)
+    k_bwd_anc<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, err);
+  else
+    k_bwd_anc<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, err);
+  hipCheck(hipGetLastError(), "launchBwdAnc");
+}
+void launchBwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* X, double* Dl,
+                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st) {
+  if (count <= 0 || nrhs <= 0) return;
+  if (fd.fp32)
+    k_bwd_own<float><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, inv, invOff);
+  else
+    k_bwd_own<double><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, inv, invOff);
+  hipCheck(hipGetLastError(), "launchBwdOwn");
+}
+void launchBwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
+                    double* X, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
+                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st) {
+  if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
+  if (fd.fp32)
+    k_bwd_chain<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, flags, flagOff, gen,
+                                               err, inv, invOff);
+  else
+    k_bwd_chain<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, flags, flagOff, gen,
+                                                err, inv, invOff);
+  hipCheck(hipGetLastError(), "launchBwdChain");
+}
+void launchFwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,
+                   const int64_t* gOff, hipStream_t st) {
+  if (count <= 0 || nrhs <= 0) return;
   const int blocks = (count + 3) / 4;
   if (fd.fp32)
-    k_fwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
+    k_fwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, nrhs, Z, Dl, gOff);
   else
-    k_fwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, G, Z, Dl, gOff);
+    k_fwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, nrhs, Z, Dl, gOff);
   hipCheck(hipGetLastError(), "launchFwdTiny");
 }
-void launchBwdTiny(const int* list, int count, const DevFronts& fd, int n, int nrhs, const double* Z, double* X,
+void launchBwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, const double* Z, double* X,
                    hipStream_t st) {
-  if (count <= 0) return;
+  if (count <= 0 || nrhs <= 0) return;
   const int blocks = (count + 3) / 4;
   if (fd.fp32)
-    k_bwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
+    k_bwd_tiny<float><<<blocks, 256, 0, st>>>(list, count, fd, nrhs, Z, X);
   else
-    k_bwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, n, nrhs, Z, X);
+    k_bwd_tiny<double><<<blocks, 256, 0, st>>>(list, count, fd, nrhs, Z, X);
   hipCheck(hipGetLastError(), "launchBwdTiny");
 }
 void launchResidual(int n, int nrhs, const int64_t* rowPtr, const int* colIdx, const double* aSym, const double* B,
