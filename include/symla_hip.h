@@ -183,8 +183,9 @@ int symla_solve_device(symla_plan* p, const double* dB, int64_t ldb, int nrhs, d
 /* No counterpart in the reference: symla_solve_device for callers that
  * cannot allocate device memory (code under stream capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The same kernels and results as
- * symla_solve_device; the per-call workspace, (3 n + sum of front orders)
- * x nrhs doubles, comes from a static 128 MiB device array of the library
+ * symla_solve_device; the per-call workspace, (3 n + w) x nrhs doubles
+ * (w: the solve's work rows, the sum of the front orders plus the partial
+ * rows of the split backward products), comes from a static 128 MiB device array of the library
  * instead of plan buffers, and the call makes no hipMalloc / hipFree.  A
  * solve needing more returns SYMLA_ERR_INVALID (nothing runs).  Call
  * symla_factorize first, on the current device (the plan's).  The array of

@@ -19,6 +19,16 @@ constexpr int kPanelThreads = 512;
 // Per-front metadata and outputs, all device arrays indexed by supernode id.
 // Per-position arrays (frontIdx, perm, rowIdx, dDiag, dOff, pkind, relmap)
 // are indexed by idxOff[s] + local position, local position in [0, m[s]).
+// The backward pass's product with a front's ancestors' rows, L(nF:m, own)^T
+// x_anc, split over those rows in `split` chunks: chunk 0 leaves z less its
+// part in the own work rows, chunk k >= 1 its part in the work rows
+// off + (k - 1) * stride + (own row), Dl rows past the fronts' blocks.
+struct AncPart {
+  int64_t off;
+  int stride;
+  int split;
+};
+
 struct DevFronts {
   const int* m;          // front order
   const int* nE;         // eligible (fully summed) columns

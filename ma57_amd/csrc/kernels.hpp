@@ -51,12 +51,11 @@ void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double*
 void launchSolveBegin(unsigned* gen, hipStream_t st);
 // Inside a solve, Bp, Z and X are n x nrhs row-major (ld nrhs) and a front's
 // work block Dl is m x nrhs row-major at gOff[s] * nrhs.
-// forward gather over every front of a level: posStart[f] is the running sum
-// of the fronts' orders (positions = posStart[count]); mapPtr / mapSrc: per
-// pre-permutation position, the (child, child position) entries summed into it
-void launchFwdGather(const int* list, int count, const int64_t* posStart, int64_t positions, const DevFronts& fd,
-                     int nrhs, const double* Bp, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
-                     const int2* mapSrc, hipStream_t st);
+// forward gather over every front of a level: pos[k] = (front, position)
+// for k < positions; mapPtr / mapSrc: per pre-permutation position, the
+// (child, child position) entries summed into it
+void launchFwdGather(const int2* pos, int64_t positions, const DevFronts& fd, int nrhs, const double* Bp, double* Dl,
+                     const int64_t* gOff, const int64_t* mapPtr, const int2* mapSrc, hipStream_t st);
 // own rows per solve block: the solve applies the inverse of each block's
 // unit lower diagonal block (kSolveBlock^2 doubles, row-major, at
 // invOff[s] + kSolveBlock^2 * block), formed once per factorization
@@ -73,14 +72,17 @@ void launchFwdChain(const int* list, int count, const int* blockStart, int block
 // rows [nF, m) after the own rows (blockStart: ceil((m - nF) / 64) per front)
 void launchFwdDelta(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
                     double* Dl, const int64_t* gOff, int* err, hipStream_t st);
-// P = z - L(anc, own)^T x_anc into the own work rows (blockStart: ceil(nF / kSolveBlock) per front)
+// blockStart: split * ceil(nF / kSolveBlock) workgroups per front
 void launchBwdAnc(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                  const double* Z, const double* X, double* Dl, const int64_t* gOff, int* err, hipStream_t st);
+                  const double* Z, const double* X, double* Dl, const int64_t* gOff, const AncPart* parts, int* err,
+                  hipStream_t st);
 void launchBwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* X, double* Dl,
-                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st);
+                  const int64_t* gOff, const AncPart* parts, const double* inv, const int64_t* invOff,
+                  hipStream_t st);
 void launchBwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                    double* X, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
-                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st);
+                    double* X, double* Dl, const int64_t* gOff, const AncPart* parts, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv, const int64_t* invOff,
+                    hipStream_t st);
 // fronts of at most kTinyFront rows: one wave per front, no dependency flags
 constexpr int kTinyFront = 64;
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,

@@ -1033,7 +1033,7 @@ void Plan::ensureSolveWs(int nrhs) {
   dBp_.ensure(nn);
   dZ_.ensure(nn);
   dX_.ensure(nn);
-  dDl_.ensure(static_cast<size_t>(sumM_) * nrhs);
+  dDl_.ensure(static_cast<size_t>(workRows_) * nrhs);
 }
 
 void Plan::ensureSolveLayout() {
@@ -1071,7 +1071,9 @@ void Plan::ensureSolveLayout() {
     const int nl = static_cast<int>(levelFronts_.size());
     solveLv_.assign(nl, {});
     std::vector<int> lists, starts;
-    std::vector<int64_t> pos;
+    std::vector<int2> pos;
+    std::vector<AncPart> parts(numSN_, AncPart{0, 0, 1});
+    int64_t partRows = 0;
     auto addLaunch = [&](SolveLaunch& ln, const std::vector<int>& fronts, auto&& blocksOf) {
       ln.list = static_cast<int64_t>(lists.size());
       ln.start = static_cast<int64_t>(starts.size());
@@ -1088,17 +1090,10 @@ void Plan::ensureSolveLayout() {
     for (int L = 0; L < nl; ++L) {
       SolveLevel& lv = solveLv_[L];
       const auto& fr = levelFronts_[L];
-      lv.gatherList = static_cast<int64_t>(lists.size());
-      lv.gatherCount = static_cast<int>(fr.size());
       lv.posOff = static_cast<int64_t>(pos.size());
-      int64_t p = 0;
-      for (int s : fr) {
-        lists.push_back(s);
-        pos.push_back(p);
-        p += hm_[s];
-      }
-      pos.push_back(p);
-      lv.positions = p;
+      for (int s : fr)
+        for (int p = 0; p < hm_[s]; ++p) pos.push_back(make_int2(s, p));
+      lv.positions = static_cast<int64_t>(pos.size()) - lv.posOff;
       std::vector<int> own, chain, delta, anc, tiny;
       for (int s : fr) {
         const int nfs = layoutDirty_ ? hnF_[s] : hnE_[s];
@@ -1115,15 +1110,35 @@ void Plan::ensureSolveLayout() {
       }
       auto nfOf = [&](int s) { return layoutDirty_ ? hnF_[s] : hnE_[s]; };
       auto ownBlocks = [&](int s) { return (nfOf(s) + kSolveBlock - 1) / kSolveBlock; };
+      // The ancestors' product of a level: split each front's ancestor rows
+      // in chunks of at least kAncChunkRows until the level has about
+      // kAncLevelBlocks workgroups (the top levels have few fronts with
+      // many rows); chunk k >= 1 leaves its part in rows past the fronts.
+      int levelOwn = 0;
+      for (int s : anc) levelOwn += ownBlocks(s);
+      for (int s : anc) {
+        const int rowsAnc = hm_[s] - nfOf(s);
+        int split = 1;
+        if (rowsAnc > 0 && levelOwn > 0)
+          split = std::max(1, std::min({(rowsAnc + kAncChunkRows - 1) / kAncChunkRows,
+                                        (kAncLevelBlocks + levelOwn - 1) / levelOwn, kAncMaxSplit}));
+        AncPart& pa = parts[s];
+        pa.split = split;
+        pa.stride = ownBlocks(s) * kSolveBlock;
+        pa.off = sumM_ + partRows;
+        partRows += static_cast<int64_t>(split - 1) * pa.stride;
+      }
       addLaunch(lv.own, own, [](int) { return 1; });
       addLaunch(lv.chain, chain, ownBlocks);
       addLaunch(lv.delta, delta, [&](int s) { return (hm_[s] - nfOf(s) + 63) / 64; });
-      addLaunch(lv.anc, anc, ownBlocks);
+      addLaunch(lv.anc, anc, [&](int s) { return ownBlocks(s) * parts[s].split; });
       addLaunch(lv.tiny, tiny, [](int) { return 1; });
     }
+    workRows_ = sumM_ + partRows;
+    dancPart_.upload(parts, stream_);
     dsolveLists_.upload(lists, stream_);
     dsolveStarts_.upload(starts, stream_);
-    dposStart_.upload(pos, stream_);
+    dpos_.upload(pos, stream_);
     std::vector<int64_t> fo(numSN_ + 1, 0);
     for (int s = 0; s < numSN_; ++s) fo[s + 1] = fo[s] + (hm_[s] + 31) / 32;
     dflagOff_.upload(fo, stream_);
@@ -1174,8 +1189,8 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   for (int L = 0; L < nl; ++L) {
     const SolveLevel& lv = solveLv_[L];
     profBegin(kProfFwd);
-    launchFwdGather(lists + lv.gatherList, lv.gatherCount, dposStart_.p + lv.posOff, lv.positions, fd, nrhs, w.Bp,
-                    w.Dl, dgOff_.p, dmapPtr_.p, dmapSrc_.p, stream_);
+    launchFwdGather(dpos_.p + lv.posOff, lv.positions, fd, nrhs, w.Bp, w.Dl, dgOff_.p, dmapPtr_.p, dmapSrc_.p,
+                    stream_);
     launchFwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.Z, w.Dl, dgOff_.p, dinv_.p, dinvOff_.p, stream_);
     launchFwdChain(lists + lv.chain.list, lv.chain.count, starts + lv.chain.start, lv.chain.blocks, fd, nrhs, w.Z,
                    w.Dl, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p, stream_);
@@ -1188,10 +1203,12 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
     const SolveLevel& lv = solveLv_[L];
     profBegin(kProfBwd);
     launchBwdAnc(lists + lv.anc.list, lv.anc.count, starts + lv.anc.start, lv.anc.blocks, fd, nrhs, w.Z, w.X, w.Dl,
-                 dgOff_.p, err, stream_);
-    launchBwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.X, w.Dl, dgOff_.p, dinv_.p, dinvOff_.p, stream_);
+                 dgOff_.p, dancPart_.p, err, stream_);
+    launchBwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.X, w.Dl, dgOff_.p, dancPart_.p, dinv_.p, dinvOff_.p,
+                 stream_);
     launchBwdChain(lists + lv.chain.list, lv.chain.count, starts + lv.chain.start, lv.chain.blocks, fd, nrhs, w.X,
-                   w.Dl, dgOff_.p, dflagsB_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p, stream_);
+                   w.Dl, dgOff_.p, dancPart_.p, dflagsB_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p,
+                   stream_);
     launchBwdTiny(lists + lv.tiny.list, lv.tiny.count, fd, nrhs, w.Z, w.X, stream_);
     profEnd(kProfBwd);
   }
@@ -1240,11 +1257,11 @@ void Plan::solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int
   if (nrhs == 0 || n_ == 0) return;
   if (!solveLayoutReady_) throw std::logic_error("symla: the solve layout is not built (factorize first)");
   const size_t nn = static_cast<size_t>(n_) * nrhs;
-  const size_t gm = static_cast<size_t>(sumM_) * nrhs;
+  const size_t gm = static_cast<size_t>(workRows_) * nrhs;
   const size_t need = 3 * nn + gm;
   if (need > capacity)
     throw std::invalid_argument("symla_solve_device_noalloc: this solve needs " + std::to_string(need * 8 >> 20) +
-                                " MiB of workspace ((3 n + sum of front orders) x nrhs doubles), above the " +
+                                " MiB of workspace ((3 n + solve work rows) x nrhs doubles), above the " +
                                 std::to_string(capacity * 8 >> 20) +
                                 " MiB static scratch; use fewer right-hand sides per call or symla_solve_device");
   const SolveWs w{scratch, scratch + nn, scratch + 2 * nn, scratch + 3 * nn};

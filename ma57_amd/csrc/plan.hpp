@@ -135,6 +135,11 @@ constexpr int kCbChunk = 2048;
 // fronts with at most this many 32-row own blocks solve their own rows on
 // one workgroup; longer chains take one workgroup per block
 constexpr int kOwnChainBlocks = 10;
+// the backward ancestors' product: chunks of at least this many rows, split
+// until a level has about kAncLevelBlocks workgroups, at most kAncMaxSplit
+constexpr int kAncChunkRows = 512;
+constexpr int kAncLevelBlocks = 2048;
+constexpr int kAncMaxSplit = 16;
 struct CbTask {
   int s, m, nE;
 };
@@ -210,7 +215,7 @@ class Plan {
                                                double tol, int ngroups, const int* groupStart);
   // The device solve with its per-call workspace in `scratch` (capacity in
   // doubles) instead of plan buffers: allocates nothing.  Throws
-  // std::invalid_argument when (3 n + sum of front orders) x nrhs doubles
+  // std::invalid_argument when (3 n + solve work rows) x nrhs doubles
   // exceed the capacity.
   void solveNoAlloc(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx, double* scratch,
                     size_t capacity);
@@ -362,6 +367,10 @@ class Plan {
   DBuf<double> dBp_, dZ_, dX_, dDl_, dBin_, dXout_;
   DBuf<int64_t> dgOff_;  // running sum of the front orders: front s's work blocks start at dgOff_[s] * nrhs
   int64_t sumM_ = 0;
+  // work rows of a solve: the fronts' blocks (sumM_) then the split
+  // ancestors' products' partial rows (dancPart_, one entry per front)
+  int64_t workRows_ = 0;
+  DBuf<AncPart> dancPart_;
   bool solveLayoutReady_ = false;
   // forward gather map: per front position (idxOff-based), the children's
   // delta entries (child, child position) that sum into it, in child order
@@ -375,8 +384,8 @@ class Plan {
   DBuf<double> dinv_;
   int invCount_ = 0, invBlocks_ = 0;
   void launchInverse();
-  // Per level, the solve's launches: the gather over every front (fronts in
-  // solveLists_ at posOff's list offset, positions numbered front by front),
+  // Per level, the solve's launches: the gather over every front (positions
+  // (front, row) at dpos_ + posOff),
   // fronts whose own rows take one workgroup (own), fronts whose own rows
   // take one workgroup per block (chain), fronts with rows past their own
   // (delta), every front above kTinyFront rows (anc: the backward ancestors'
@@ -387,13 +396,12 @@ class Plan {
     int count = 0, blocks = 0;
   };
   struct SolveLevel {
-    int64_t gatherList = 0, posOff = 0, positions = 0;
-    int gatherCount = 0;
+    int64_t posOff = 0, positions = 0;
     SolveLaunch own, chain, delta, anc, tiny;
   };
   std::vector<SolveLevel> solveLv_;
   DBuf<int> dsolveLists_, dsolveStarts_;
-  DBuf<int64_t> dposStart_;
+  DBuf<int2> dpos_;
   // dependency flags of the sweeps (one per own-row block), the solve
   // generation they are compared with, and the bounded-wait error word
   DBuf<int64_t> dflagOff_;
@@ -486,7 +494,7 @@ class Plan {
   void ensureSolveLayout();
   void ensureSolveWs(int nrhs);
   // per-call solve workspace: permuted B, Z, X (n x nrhs each, row-major) and
-  // the per-front work blocks Dl (sum of front orders x nrhs, row-major)
+  // the work rows Dl (workRows_ x nrhs, row-major)
   struct SolveWs {
     double *Bp = nullptr, *Z = nullptr, *X = nullptr, *Dl = nullptr;
   };

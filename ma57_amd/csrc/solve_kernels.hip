@@ -346,8 +346,9 @@ __device__ void fwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
 // ancestors' part) in the block's work rows, x_w written over it and into X.
 template <typename T, bool kShared>
 __device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFronts& fd, int s, int w, int rg,
-                            int gw, int nrhs, double* __restrict__ X, double* Ds, const double* __restrict__ Mi,
-                            unsigned* fl, unsigned gen, int* err, bool publish) {
+                            int gw, int nrhs, double* __restrict__ X, double* Ds, const double* __restrict__ Dl,
+                            const AncPart pa, const double* __restrict__ Mi, unsigned* fl, unsigned gen, int* err,
+                            bool publish) {
   using M = SweepMfma<T>;
   using E = typename M::E;
   using MD = SweepMfma<double>;
@@ -365,13 +366,20 @@ __device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
   const int lr = lane & 15, lk = lane >> 4;
   for (int e = tid; e < SB * SB; e += NT) S.Ld[e / SB][e % SB] = Mi[e];
   for (int q = tid; q < SB; q += NT) S.rixA[q] = (q < nr) ? fd.rowIdx[io + r0 + q] : 0;
+  // P_w: chunk 0's value in the own rows less the other chunks' parts, in chunk order
   double p0[2][4];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-      p0[t][rr] = (qq < nr && c < gw) ? Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] : 0.0;
+      double v = 0.0;
+      if (qq < nr && c < gw) {
+        v = Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c];
+        for (int k = 1; k < pa.split; ++k)
+          v -= Dl[(pa.off + static_cast<int64_t>(k - 1) * pa.stride + r0 + qq) * nrhs + rg + c];
+      }
+      p0[t][rr] = v;
     }
   E opre[4];
   auto loadO = [&](int j) {
@@ -467,10 +475,10 @@ __global__ void k_solve_begin(unsigned* gen) {
 // children's order (mapPtr over idxOff positions, mapSrc = (child, child
 // position)) -- the sums of adding the children one after another into a
 // zeroed block.  An own row p < nF gets y0 = Bp - g, the others g, the start
-// of the delta handed to the parent.  The level's positions are numbered
-// front by front (posStart); lp lanes (a power of two) take one position,
-// each a strided subset of the right-hand sides.
-__global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, const int64_t* posStart, int lpShift,
+// of the delta handed to the parent.  pos[k] = (front, position) numbers the
+// level's positions; 2^lpShift lanes take one, each a strided subset of the
+// right-hand sides.
+__global__ __launch_bounds__(256) void k_fwd_gather(const int2* __restrict__ pos, int64_t positions, int lpShift,
                                                      DevFronts fd, int nrhs, const double* __restrict__ Bp,
                                                      double* __restrict__ Dl, const int64_t* __restrict__ gOff,
                                                      const int64_t* __restrict__ mapPtr,
@@ -478,10 +486,9 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const int* list, int count, 
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t item = g >> lpShift;
   const int l = static_cast<int>(g & ((1 << lpShift) - 1));
-  if (item >= posStart[count]) return;
-  const int f = segmentOf<int64_t>(posStart, count, item);
-  const int s = list[f];
-  const int p = static_cast<int>(item - posStart[f]);
+  if (item >= positions) return;
+  const int2 sp = pos[item];
+  const int s = sp.x, p = sp.y;
   const int m = fd.m[s];
   if (p >= m) return;
   const int nf = fd.nF[s];
@@ -696,12 +703,16 @@ __global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, c
 // Backward step, the ancestors' part (solve.hpp:364-383): for own block w of
 // a front, P_w = z_w - L(nF:m, w)^T x(nF:m), the ancestors' x final from the
 // levels above and read from X through the front's row indices (one
-// contiguous row each).  One workgroup per own block, the rows taken in
-// 64-row tiles; P_w goes to the block's work rows.
+// contiguous row each; the indices of a tile are read two tiles ahead, as
+// wave-uniform values).  The ancestors' rows are split in `split` chunks of
+// whole 64-row tiles: workgroup (w, k) takes chunk k of own block w, chunk 0
+// writing z_w less its part into the block's work rows, chunk k >= 1 its part
+// into the front's partial rows (AncPart).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, const int* blockStart, DevFronts fd,
                                                  int nrhs, const double* __restrict__ Z, const double* __restrict__ X,
-                                                 double* __restrict__ Dl, const int64_t* __restrict__ gOff, int* err) {
+                                                 double* __restrict__ Dl, const int64_t* __restrict__ gOff,
+                                                 const AncPart* __restrict__ parts, int* err) {
   using M = SweepMfma<T>;
   using E = typename M::E;
   constexpr int NT = 256;
@@ -710,53 +721,64 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
   if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
   const int f = segmentOf<int>(blockStart, count, blockIdx.x);
   const int s = list[f];
-  const int w = blockIdx.x - blockStart[f];
+  const int bb = blockIdx.x - blockStart[f];
+  const AncPart pa = parts[s];
+  const int alloc = (blockStart[f + 1] - blockStart[f]) / pa.split;  // own blocks planned
+  const int w = bb / pa.split, k = bb - w * pa.split;
   const int m = fd.m[s];
   const int nf = fd.nF[s];
   const int nOwn = (nf + SB - 1) / SB;
-  if (w == 0 && threadIdx.x == 0 && nOwn > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (bb == 0 && threadIdx.x == 0 && nOwn > alloc) raiseError(err, 2);
   if (w >= nOwn) return;
   const int r0 = w * SB, nr = min(SB, nf - r0);
+  const int chunk = ((m - nf + pa.split - 1) / pa.split + DT - 1) / DT * DT;
+  const int a0 = nf + k * chunk, a1 = min(m, a0 + chunk);
   const int64_t io = fd.idxOff[s];
   const T* F = static_cast<const T*>(fd.F[s]);
   double* Ds = Dl + gOff[s] * nrhs;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wu = __builtin_amdgcn_readfirstlane(wv);
   const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
+    // staging: thread (wave wu, lane) loads L rows i0 + lane of own columns
+    // wu + 4u, and x rows i0 + wu + 4u (right-hand side rg + lane)
     E lpre[8], xpre[16];
-    auto loadT = [&](int i0) {
-      const int in = min(DT, m - i0);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = tid + u * NT;
-        const int i = e & (DT - 1), q = e / DT;
-        lpre[u] = (i < in && q < nr) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(r0 + q) * m]) : E(0);
-      }
+    int64_t rcur[16], rnext[16];
+    auto rows = [&](int i0, int64_t* out) {
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const int e = tid + u * NT;
-        const int c = e & 63, i = e >> 6;
-        xpre[u] = (i < in && c < gw)
-                      ? static_cast<E>(X[static_cast<int64_t>(fd.rowIdx[io + i0 + i]) * nrhs + rg + c])
-                      : E(0);
+        const int i = i0 + wu + 4 * u;
+        out[u] = (i < a1) ? static_cast<int64_t>(fd.rowIdx[io + i]) * nrhs : -1;
       }
     };
-    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    if (nf < m) loadT(nf);
-    for (int i0 = nf; i0 < m; i0 += DT) {
+    auto loadT = [&](int i0, const int64_t* xr) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int e = tid + u * NT;
-        Lt[e & (DT - 1)][e / DT] = lpre[u];
+        const int q = wv + 4 * u;
+        lpre[u] = (i0 + lane < a1 && q < nr) ? static_cast<E>(F[(i0 + lane) + static_cast<size_t>(r0 + q) * m]) : E(0);
       }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int e = tid + u * NT;
-        xt[e >> 6][e & 63] = xpre[u];
+      for (int u = 0; u < 16; ++u) xpre[u] = (xr[u] >= 0 && lane < gw) ? static_cast<E>(X[xr[u] + rg + lane]) : E(0);
+    };
+    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    if (a0 < a1) {
+      rows(a0, rcur);
+      loadT(a0, rcur);
+      if (a0 + DT < a1) rows(a0 + DT, rnext);
+    }
+    for (int i0 = a0; i0 < a1; i0 += DT) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Lt[lane][wv + 4 * u] = lpre[u];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) xt[wv + 4 * u][lane] = xpre[u];
+      if (i0 + DT < a1) {
+        loadT(i0 + DT, rnext);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) rcur[u] = rnext[u];
+        if (i0 + 2 * DT < a1) rows(i0 + 2 * DT, rnext);
       }
-      if (i0 + DT < m) loadT(i0 + DT);
       __syncthreads();
       typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
@@ -777,8 +799,12 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
         if (qq < nr && c < gw) {
-          const double z = Z[static_cast<int64_t>(fd.rowIdx[io + r0 + qq]) * nrhs + rg + c];
-          Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] = z - acc[t][rr];
+          if (k == 0) {
+            const double z = Z[static_cast<int64_t>(fd.rowIdx[io + r0 + qq]) * nrhs + rg + c];
+            Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] = z - acc[t][rr];
+          } else {
+            Dl[(pa.off + static_cast<int64_t>(k - 1) * pa.stride + r0 + qq) * nrhs + rg + c] = acc[t][rr];
+          }
         }
       }
   }
@@ -790,8 +816,8 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_own(const int* list, int count, DevFronts fd, int nrhs,
                                                  double* __restrict__ X, double* __restrict__ Dl,
-                                                 const int64_t* __restrict__ gOff, const double* __restrict__ inv,
-                                                 const int64_t* __restrict__ invOff) {
+                                                 const int64_t* __restrict__ gOff, const AncPart* __restrict__ parts,
+                                                 const double* __restrict__ inv, const int64_t* __restrict__ invOff) {
   __shared__ BlockLds<typename SweepMfma<T>::E> S;
   if (static_cast<int>(blockIdx.x) >= count) return;
   const int s = list[blockIdx.x];
@@ -800,8 +826,8 @@ __global__ __launch_bounds__(256) void k_bwd_own(const int* list, int count, Dev
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
     for (int w = nOwn - 1; w >= 0; --w)
-      bwdOwnBlock<T, false>(S, fd, s, w, rg, gw, nrhs, X, Ds, inv + invOff[s] + static_cast<int64_t>(w) * SB * SB,
-                            nullptr, 0u, nullptr, false);
+      bwdOwnBlock<T, false>(S, fd, s, w, rg, gw, nrhs, X, Ds, Dl, parts[s],
+                            inv + invOff[s] + static_cast<int64_t>(w) * SB * SB, nullptr, 0u, nullptr, false);
   }
 }
 
@@ -811,7 +837,8 @@ __global__ __launch_bounds__(256) void k_bwd_own(const int* list, int count, Dev
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_chain(const int* list, int count, const int* blockStart, DevFronts fd,
                                                    int nrhs, double* __restrict__ X, double* __restrict__ Dl,
-                                                   const int64_t* __restrict__ gOff, unsigned* flags,
+                                                   const int64_t* __restrict__ gOff,
+                                                   const AncPart* __restrict__ parts, unsigned* flags,
                                                    const int64_t* __restrict__ flagOff, const unsigned* genp,
                                                    int* err, const double* __restrict__ inv,
                                                    const int64_t* __restrict__ invOff) {
@@ -828,20 +855,32 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const int* list, int count, c
   double* Ds = Dl + gOff[s] * nrhs;
   for (int rg = 0; rg < nrhs; rg += 64) {
     const int gw = min(64, nrhs - rg);
-    bwdOwnBlock<T, true>(S, fd, s, w, rg, gw, nrhs, X, Ds, inv + invOff[s] + static_cast<int64_t>(w) * SB * SB,
-                         flags + flagOff[s], gen, err, rg + 64 >= nrhs);
+    bwdOwnBlock<T, true>(S, fd, s, w, rg, gw, nrhs, X, Ds, Dl, parts[s],
+                         inv + invOff[s] + static_cast<int64_t>(w) * SB * SB, flags + flagOff[s], gen, err,
+                         rg + 64 >= nrhs);
   }
 }
 
-// Fronts of at most 64 rows: one wave per front, lane i owning row i, and
-// the right-hand sides in groups of 8 held in registers.  Forward: the
-// unit-lower solve of the own rows, D^{-1} into Z, and the delta of the
-// remaining rows (solve.hpp:303-362, :434-455) in one pass; no flags needed.
+// A double of lane l (wave-uniform l) to every lane.
+__device__ inline double readLane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Fronts of at most 64 rows: one wave per front, lane i owning row i, the
+// right-hand sides in groups of 8 held in registers.  The lane's row of L is
+// read once into registers (the loops over the front's columns are unrolled,
+// their bounds wave-uniform branches), and the solved value of row j reaches
+// the other lanes by a lane read.  Forward: the unit-lower solve of the own
+// rows, D^{-1} into Z, and the delta of the remaining rows
+// (solve.hpp:303-362, :434-455) in one pass.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, DevFronts fd, int nrhs,
                                                    double* __restrict__ Z, double* __restrict__ Dl,
                                                    const int64_t* __restrict__ gOff) {
-  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int f = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (f >= count) return;
   const int s = list[f];
   const int lane = threadIdx.x & 63;
@@ -853,54 +892,62 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
   double* Dr = Dl + (gOff[s] + lane) * nrhs;  // this lane's work row
   const bool row = lane < m;
   const bool own = lane < nf;
+  // own rows subtract L(i, :) y, the others add it
+  double lr[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    const double l = (row && j < nf && j < lane) ? lval(F, m, pk, lane, j) : 0.0;
+    lr[j] = own ? -l : l;
+  }
+  // keep the row in registers: loaded once, not again inside the loops below
+#pragma unroll
+  for (int j = 0; j < 64; ++j) asm volatile("" : "+v"(lr[j]));
   const int kind = own ? pk[lane] : 0;
   const int64_t prow = own ? fd.rowIdx[io + lane] : 0;
   for (int cg = 0; cg < nrhs; cg += 8) {
     double v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = (own && cg + c < nrhs) ? Dr[cg + c] : 0.0;
-    for (int j = 0; j < nf; ++j) {
-      const double l = (row && lane > j) ? lval(F, m, pk, lane, j) : 0.0;
+    for (int c = 0; c < 8; ++c) v[c] = (row && cg + c < nrhs) ? Dr[cg + c] : 0.0;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double yj = __shfl(v[c], j, 64);
-        v[c] = fma(-l, yj, v[c]);
-      }
+    for (int j = 0; j < 64; ++j) {
+      if (j >= nf) break;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) v[c] = fma(lr[j], readLane(v[c], j), v[c]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       if (cg + c >= nrhs) break;
       const int cc = cg + c;
-      // lanes below nF hold y; the others hold -L(i, :) y
       const double up = __shfl(v[c], min(lane + 1, 63), 64);
       const double dn = __shfl(v[c], max(lane - 1, 0), 64);
+      if (row) Dr[cc] = v[c];
       if (own) {
-        Dr[cc] = v[c];
+        double z;
         if (kind == 1) {
-          Z[prow * nrhs + cc] = v[c] / fd.dDiag[io + lane];
+          z = v[c] / fd.dDiag[io + lane];
         } else if (kind == 2) {
           const double d11 = fd.dDiag[io + lane], d21 = fd.dOff[io + lane], d22 = fd.dDiag[io + lane + 1];
           const double det = d11 * d22 - d21 * d21;
-          Z[prow * nrhs + cc] = (d22 * v[c] - d21 * up) / det;
+          z = (d22 * v[c] - d21 * up) / det;
         } else {
           const double d11 = fd.dDiag[io + lane - 1], d21 = fd.dOff[io + lane - 1], d22 = fd.dDiag[io + lane];
           const double det = d11 * d22 - d21 * d21;
-          Z[prow * nrhs + cc] = (-d21 * dn + d11 * v[c]) / det;
+          z = (-d21 * dn + d11 * v[c]) / det;
         }
-      } else if (row) {
-        Dr[cc] = Dr[cc] - v[c];
+        Z[prow * nrhs + cc] = z;
       }
     }
   }
 }
 
-// Backward pass of fronts of at most 64 rows: lane p owns own row p;
-// x_own = z - L(anc, :)^T x_anc, then the unit upper solve with L_top^T
-// (solve.hpp:364-383).
+// Backward pass of fronts of at most 64 rows, the same layout with the
+// lane's column of L in registers: x_own = z - L(anc, :)^T x_anc, then the
+// unit upper solve with L_top^T (solve.hpp:364-383).  Lane t also holds the
+// ancestors' x of the front's row nF + t.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, DevFronts fd, int nrhs,
                                                    const double* __restrict__ Z, double* __restrict__ X) {
-  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int f = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (f >= count) return;
   const int s = list[f];
   const int lane = threadIdx.x & 63;
@@ -910,8 +957,14 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
   const int* pk = fd.pkind + io;
   const T* F = static_cast<const T*>(fd.F[s]);
   const bool own = lane < nf;
+  // column `lane` of L below the diagonal, rows [lane + 1, m)
+  double lc[64];
+#pragma unroll
+  for (int a = 0; a < 64; ++a) lc[a] = (own && a > lane && a < m) ? lval(F, m, pk, a, lane) : 0.0;
+  // keep the column in registers: loaded once, not again inside the loops below
+#pragma unroll
+  for (int a = 0; a < 64; ++a) asm volatile("" : "+v"(lc[a]));
   const int64_t prow = own ? fd.rowIdx[io + lane] : 0;
-  // lane t holds the ancestors' x of the front's row nF + t (m <= 64)
   const int ncb = m - nf;
   const int64_t crow = (lane < ncb) ? fd.rowIdx[io + nf + lane] : 0;
   for (int cg = 0; cg < nrhs; cg += 8) {
@@ -921,25 +974,18 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
       x[c] = (own && cg + c < nrhs) ? Z[prow * nrhs + cg + c] : 0.0;
       xa[c] = (lane < ncb && cg + c < nrhs) ? X[crow * nrhs + cg + c] : 0.0;
     }
-    for (int i0 = 0; i0 < ncb; i0 += 8) {
-      double lv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        lv[u] = (own && i0 + u < ncb) ? static_cast<double>(F[(nf + i0 + u) + static_cast<size_t>(lane) * m]) : 0.0;
+    for (int a = 0; a < 64; ++a) {
+      if (a < nf) continue;
+      if (a >= m) break;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (i0 + u >= ncb) break;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) x[c] = fma(-lv[u], __shfl(xa[c], i0 + u, 64), x[c]);
-      }
+      for (int c = 0; c < 8; ++c) x[c] = fma(-lc[a], readLane(xa[c], a - nf), x[c]);
     }
-    for (int j = nf - 1; j > 0; --j) {
-      const double l = (lane < j) ? lval(F, m, pk, j, lane) : 0.0;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double xj = __shfl(x[c], j, 64);
-        x[c] = fma(-l, xj, x[c]);
-      }
+    for (int j = 63; j > 0; --j) {
+      if (j >= nf) continue;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) x[c] = fma(-lc[j], readLane(x[c], j), x[c]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
@@ -1273,15 +1319,14 @@ void launchSolveBegin(unsigned* gen, hipStream_t st) {
   k_solve_begin<<<1, 64, 0, st>>>(gen);
   hipCheck(hipGetLastError(), "launchSolveBegin");
 }
-void launchFwdGather(const int* list, int count, const int64_t* posStart, int64_t positions, const DevFronts& fd,
-                     int nrhs, const double* Bp, double* Dl, const int64_t* gOff, const int64_t* mapPtr,
-                     const int2* mapSrc, hipStream_t st) {
-  if (count <= 0 || positions <= 0 || nrhs <= 0) return;
+void launchFwdGather(const int2* pos, int64_t positions, const DevFronts& fd, int nrhs, const double* Bp, double* Dl,
+                     const int64_t* gOff, const int64_t* mapPtr, const int2* mapSrc, hipStream_t st) {
+  if (positions <= 0 || nrhs <= 0) return;
   int lpShift = 0;  // lanes per position: the power of two at or above nrhs, at most 64
   while ((1 << lpShift) < nrhs && lpShift < 6) ++lpShift;
   const int64_t blocks = ((positions << lpShift) + 255) / 256;
-  k_fwd_gather<<<static_cast<unsigned>(blocks), 256, 0, st>>>(list, count, posStart, lpShift, fd, nrhs, Bp, Dl, gOff,
-                                                              mapPtr, mapSrc);
+  k_fwd_gather<<<static_cast<unsigned>(blocks), 256, 0, st>>>(pos, positions, lpShift, fd, nrhs, Bp, Dl, gOff, mapPtr,
+                                                              mapSrc);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
@@ -1324,33 +1369,36 @@ void launchFwdDelta(const int* list, int count, const int* blockStart, int block
   hipCheck(hipGetLastError(), "launchFwdDelta");
 }
 void launchBwdAnc(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                  const double* Z, const double* X, double* Dl, const int64_t* gOff, int* err, hipStream_t st) {
+                  const double* Z, const double* X, double* Dl, const int64_t* gOff, const AncPart* parts, int* err,
+                  hipStream_t st) {
   if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
   if (fd.fp32)
-    k_bwd_anc<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, err);
+    k_bwd_anc<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, parts, err);
   else
-    k_bwd_anc<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, err);
+    k_bwd_anc<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, Z, X, Dl, gOff, parts, err);
   hipCheck(hipGetLastError(), "launchBwdAnc");
 }
 void launchBwdOwn(const int* list, int count, const DevFronts& fd, int nrhs, double* X, double* Dl,
-                  const int64_t* gOff, const double* inv, const int64_t* invOff, hipStream_t st) {
+                  const int64_t* gOff, const AncPart* parts, const double* inv, const int64_t* invOff,
+                  hipStream_t st) {
   if (count <= 0 || nrhs <= 0) return;
   if (fd.fp32)
-    k_bwd_own<float><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, inv, invOff);
+    k_bwd_own<float><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, parts, inv, invOff);
   else
-    k_bwd_own<double><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, inv, invOff);
+    k_bwd_own<double><<<count, 256, 0, st>>>(list, count, fd, nrhs, X, Dl, gOff, parts, inv, invOff);
   hipCheck(hipGetLastError(), "launchBwdOwn");
 }
 void launchBwdChain(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd, int nrhs,
-                    double* X, double* Dl, const int64_t* gOff, unsigned* flags, const int64_t* flagOff,
-                    const unsigned* gen, int* err, const double* inv, const int64_t* invOff, hipStream_t st) {
+                    double* X, double* Dl, const int64_t* gOff, const AncPart* parts, unsigned* flags,
+                    const int64_t* flagOff, const unsigned* gen, int* err, const double* inv, const int64_t* invOff,
+                    hipStream_t st) {
   if (count <= 0 || blocks <= 0 || nrhs <= 0) return;
   if (fd.fp32)
-    k_bwd_chain<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, flags, flagOff, gen,
-                                               err, inv, invOff);
+    k_bwd_chain<float><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, parts, flags, flagOff,
+                                               gen, err, inv, invOff);
   else
-    k_bwd_chain<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, flags, flagOff, gen,
-                                                err, inv, invOff);
+    k_bwd_chain<double><<<blocks, 256, 0, st>>>(list, count, blockStart, fd, nrhs, X, Dl, gOff, parts, flags,
+                                                flagOff, gen, err, inv, invOff);
   hipCheck(hipGetLastError(), "launchBwdChain");
 }
 void launchFwdTiny(const int* list, int count, const DevFronts& fd, int nrhs, double* Z, double* Dl,
