@@ -1625,6 +1625,18 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
 // rest (still holding current values) to the exact panel kernel, which
 // continues from there.
 // ---------------------------------------------------------------------------
+// 1 / d for a pivot on a dependent chain: the hardware reciprocal refined by
+// two Newton steps (within an ulp of the quotient), an IEEE division for
+// zero, denormal and non-finite d.
+__device__ inline double pivotRecip(double d) {
+  const double a = fabs(d);
+  if (!(a >= DBL_MIN && a <= DBL_MAX)) return 1.0 / d;
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return r;
+}
+
 struct SpecRec {
   unsigned long long* colmax;  // [2][kPanel]
   double* cdiag;
@@ -1645,10 +1657,11 @@ __device__ inline SpecRec specRec(double* ws, int nE) {
 
 // One row below the diagonal block: x holds its raw values in the panel
 // columns; for q < lim in order, c_q = x_q - sum_{j<q} l_j W[q][j] with
-// W[q][j] = d_j l_qj, then x[q] = l_q = T(c_q / d_q).  onC(q, c_q) sees every
+// W[q][j] = d_j l_qj, then x[q] = l_q = T(c_q r_q), r_q = 1 / d_q (rdv: the
+// pivots' reciprocals, so the chain over q carries no division).  onC(q, c_q) sees every
 // current value (the same arithmetic in both passes, so both see the same c).
 template <typename T, typename OnC>
-__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1], const double* dv, int lim,
+__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1], const double* rdv, int lim,
                                OnC onC) {
 #pragma unroll
   for (int q = 0; q < kPanel; ++q) {
@@ -1663,7 +1676,7 @@ __device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1
       }
       const double c = c0 + c1;
       onC(q, c);
-      x[q] = static_cast<double>(static_cast<T>(c / dv[q]));
+      x[q] = static_cast<double>(static_cast<T>(c * rdv[q]));
     }
   }
 }
@@ -1680,7 +1693,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   constexpr int P = kPanel;
   __shared__ double W[P][P + 1];
   __shared__ double cvec[2][P];
-  __shared__ double dv[P];
+  __shared__ double dv[P], rdv[P];
   __shared__ int sgn[P];
   __shared__ double red[4][P];
   __shared__ float cb[16][kSpecRows + 1];  // |c| of 16 columns x the workgroup's rows, rounded up
@@ -1719,7 +1732,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   }
   __syncthreads();
   // Right-looking, one column per step: a_ij -= c_iq l_jq (c_iq = d_q l_iq
-  // up to the rounding of the stored multiplier), one division per thread.
+  // up to the rounding of the stored multiplier), l_jq = T(c_jq (1 / d_q)).
   for (int q = 0; q < pw; ++q) {
     double* cv = cvec[q & 1];
     if (j == q) {
@@ -1735,9 +1748,13 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
       double applied;
       d = staticPivot(d, sgn[q], delta, sfloor, applied);
     }
-    if (tid == 0) dv[q] = d;
+    const double rd = pivotRecip(d);
+    if (tid == 0) {
+      dv[q] = d;
+      rdv[q] = rd;
+    }
     if (j > q && j < pw) {
-      const double lj = static_cast<double>(static_cast<T>(cv[j] / d));
+      const double lj = static_cast<double>(static_cast<T>(cv[j] * rd));
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int i = ib + 4 * r;
@@ -1747,7 +1764,10 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   }
   __syncthreads();
   // columns past the block: a neutral pivot for the branch-free row solve
-  for (int q = pw + tid; q < P; q += 256) dv[q] = 1.0;
+  for (int q = pw + tid; q < P; q += 256) {
+    dv[q] = 1.0;
+    rdv[q] = 1.0;
+  }
   // a now holds c_ij (current value at pivot j) for i > j and a_jj^(j) on the diagonal
   double bm = 0.0;
 #pragma unroll
@@ -1756,7 +1776,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
     if (i > j && (i >= pw || j >= pw)) W[i][j] = 0.0;
     if (j < pw && i < pw) {
       if (i > j) {
-        const double l = static_cast<double>(static_cast<T>(a[r] / dv[j]));
+        const double l = static_cast<double>(static_cast<T>(a[r] * rdv[j]));
         W[i][j] = dv[j] * l;
         bm = fmax(bm, fabs(a[r]));
         if (w == 0) rec.lblk[i * P + j] = l;
@@ -1780,7 +1800,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   // Column maxima through LDS, 16 columns at a time: |c| rounded up to
   // float (a bound that is never below the value, so a borderline column can
   // only fail the test here and be decided by the exact kernel).
-  specRow<T>(x, W, dv, pw, [&](int q, double c) {
+  specRow<T>(x, W, rdv, pw, [&](int q, double c) {
     const double ac = fabs(c);
     float f = static_cast<float>(ac);
     if (static_cast<double>(f) < ac) f = nextafterf(f, INFINITY);
@@ -1815,7 +1835,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
                                                     const int64_t* wsOff) {
   constexpr int P = kPanel;
   __shared__ double W[P][P + 1];
-  __shared__ double dv[P];
+  __shared__ double dv[P], rdv[P];
   __shared__ int qsh;
   __shared__ double redV[4];
   if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
@@ -1844,7 +1864,10 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   double origMax = bitsToDouble(fd.origMaxBits[s]);
   if (!stat && origMax < prm.zeroTol) origMax = 1.0;
   if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
-  for (int q = tid; q < P; q += 256) dv[q] = (q < pw) ? rec.dval[q] : 1.0;
+  for (int q = tid; q < P; q += 256) {
+    dv[q] = (q < pw) ? rec.dval[q] : 1.0;
+    rdv[q] = pivotRecip(dv[q]);
+  }
   for (int e = tid; e < P * P; e += 256) {
     const int q = e / P, jj = e % P;
     W[q][jj] = (jj < q && q < pw) ? rec.dval[jj] * rec.lblk[q * P + jj] : 0.0;
@@ -1871,7 +1894,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
     double x[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) x[q] = (q < lim) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
-    specRow<T>(x, W, dv, lim, [&](int q, double c) {
+    specRow<T>(x, W, rdv, lim, [&](int q, double c) {
       if (q < lim) tmax = fmax(tmax, fabs(c));
     });
 #pragma unroll
