@@ -19,6 +19,15 @@ constexpr int kPanelThreads = 512;
 // Per-front metadata and outputs, all device arrays indexed by supernode id.
 // Per-position arrays (frontIdx, perm, rowIdx, dDiag, dOff, pkind, relmap)
 // are indexed by idxOff[s] + local position, local position in [0, m[s]).
+// One front position of the solve's forward gather: its work row, the
+// children's delta rows summed into it (mapRow[e0 .. e0 + cnt)), and the
+// final-order row of B it starts from (own rows) or -1.
+struct PosInfo {
+  int64_t dst;
+  int64_t e0;
+  int cnt;
+  int rix;
+};
 // The backward pass's product with a front's ancestors' rows, L(nF:m, own)^T
 // x_anc, split over those rows in `split` chunks: chunk 0 leaves z less its
 // part in the own work rows, chunk k >= 1 its part in the work rows

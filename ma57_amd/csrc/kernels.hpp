@@ -51,11 +51,14 @@ void launchPermuteOut(int n, int nrhs, const int* perm, const double* X, double*
 void launchSolveBegin(unsigned* gen, hipStream_t st);
 // Inside a solve, Bp, Z and X are n x nrhs row-major (ld nrhs) and a front's
 // work block Dl is m x nrhs row-major at gOff[s] * nrhs.
-// forward gather over every front of a level: pos[k] = (front, position)
-// for k < positions; mapPtr / mapSrc: per pre-permutation position, the
-// (child, child position) entries summed into it
-void launchFwdGather(const int2* pos, int64_t positions, const DevFronts& fd, int nrhs, const double* Bp, double* Dl,
-                     const int64_t* gOff, const int64_t* mapPtr, const int2* mapSrc, hipStream_t st);
+// per-position records of the forward gather (pos[k] = (front, position)),
+// formed at the start of each solve
+void launchPosInfo(const int2* pos, int64_t positions, const DevFronts& fd, const int64_t* gOff,
+                   const int64_t* mapPtr, PosInfo* info, hipStream_t st);
+// forward gather over the positions info[0 .. positions); mapRow: the
+// children's delta rows (work-row numbers) summed into each position
+void launchFwdGather(const PosInfo* info, int64_t positions, int nrhs, const double* Bp, double* Dl,
+                     const int64_t* mapRow, hipStream_t st);
 // own rows per solve block: the solve applies the inverse of each block's
 // unit lower diagonal block (kSolveBlock^2 doubles, row-major, at
 // invOff[s] + kSolveBlock^2 * block), formed once per factorization

@@ -1061,10 +1061,10 @@ void Plan::ensureSolveLayout() {
       forEach([&](int64_t pos, int, int) { ++cnt[pos + 1]; });
       for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
       std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-      std::vector<int2> src(static_cast<size_t>(std::max<int64_t>(cnt.back(), 1)));
-      forEach([&](int64_t pos, int c, int a) { src[fill[pos]++] = make_int2(c, a); });
+      std::vector<int64_t> src(static_cast<size_t>(std::max<int64_t>(cnt.back(), 1)));
+      forEach([&](int64_t pos, int c, int a) { src[fill[pos]++] = off[c] + a; });
       dmapPtr_.upload(cnt, stream_);
-      dmapSrc_.upload(src, stream_);
+      dmapRow_.upload(src, stream_);
     }
     // Per level launch lists (see SolveLevel); the pivots each front took
     // are known here (the static layout takes every eligible pivot).
@@ -1139,6 +1139,8 @@ void Plan::ensureSolveLayout() {
     dsolveLists_.upload(lists, stream_);
     dsolveStarts_.upload(starts, stream_);
     dpos_.upload(pos, stream_);
+    dposInfo_.ensure(std::max<size_t>(pos.size(), 1));
+    posTotal_ = static_cast<int64_t>(pos.size());
     std::vector<int64_t> fo(numSN_ + 1, 0);
     for (int s = 0; s < numSN_; ++s) fo[s + 1] = fo[s] + (hm_[s] + 31) / 32;
     dflagOff_.upload(fo, stream_);
@@ -1182,6 +1184,7 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   DevFronts fd = devFronts();
   launchSolveBegin(dgen_.p, stream_);
   launchPermuteIn(n_, nrhs, dpermVec_.p, dB, ldb, w.Bp, stream_);
+  launchPosInfo(dpos_.p, posTotal_, fd, dgOff_.p, dmapPtr_.p, dposInfo_.p, stream_);
   const int* lists = dsolveLists_.p;
   const int* starts = dsolveStarts_.p;
   int* err = dsolveErr_.p;
@@ -1189,8 +1192,7 @@ void Plan::solveCore(const double* dB, int64_t ldb, int nrhs, double* dX, int64_
   for (int L = 0; L < nl; ++L) {
     const SolveLevel& lv = solveLv_[L];
     profBegin(kProfFwd);
-    launchFwdGather(dpos_.p + lv.posOff, lv.positions, fd, nrhs, w.Bp, w.Dl, dgOff_.p, dmapPtr_.p, dmapSrc_.p,
-                    stream_);
+    launchFwdGather(dposInfo_.p + lv.posOff, lv.positions, nrhs, w.Bp, w.Dl, dmapRow_.p, stream_);
     launchFwdOwn(lists + lv.own.list, lv.own.count, fd, nrhs, w.Z, w.Dl, dgOff_.p, dinv_.p, dinvOff_.p, stream_);
     launchFwdChain(lists + lv.chain.list, lv.chain.count, starts + lv.chain.start, lv.chain.blocks, fd, nrhs, w.Z,
                    w.Dl, dgOff_.p, dflagsF_.p, dflagOff_.p, dgen_.p, err, dinv_.p, dinvOff_.p, stream_);

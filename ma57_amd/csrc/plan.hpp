@@ -375,7 +375,7 @@ class Plan {
   // forward gather map: per front position (idxOff-based), the children's
   // delta entries (child, child position) that sum into it, in child order
   DBuf<int64_t> dmapPtr_;
-  DBuf<int2> dmapSrc_;
+  DBuf<int64_t> dmapRow_;  // the entries' child work rows (gOff[child] + child position)
   // inverses of the sweeps' diagonal blocks: fronts above kTinyFront rows,
   // their flat workgroup starts (one per kSolveBlock eligible rows) and
   // per-front offsets into dinv_
@@ -402,6 +402,8 @@ class Plan {
   std::vector<SolveLevel> solveLv_;
   DBuf<int> dsolveLists_, dsolveStarts_;
   DBuf<int2> dpos_;
+  DBuf<PosInfo> dposInfo_;  // the gather's records of the positions in dpos_, formed per solve
+  int64_t posTotal_ = 0;
   // dependency flags of the sweeps (one per own-row block), the solve
   // generation they are compared with, and the bounded-wait error word
   DBuf<int64_t> dflagOff_;

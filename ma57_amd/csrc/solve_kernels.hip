@@ -165,6 +165,47 @@ struct SweepMfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// FP64 sums of a run of tile products: FP64 operands accumulate in the MFMA
+// result registers directly; the FP32 products of one tile are summed in
+// FP32 and then added in FP64.  begin() gives the tile's accumulators,
+// end() closes the tile.
+template <typename T, int N>
+struct TileAcc;
+template <int N>
+struct TileAcc<double, N> {
+  typename SweepMfma<double>::v4 a[N];
+  __device__ void zero() {
+#pragma unroll
+    for (int n = 0; n < N; ++n) a[n] = typename SweepMfma<double>::v4{0, 0, 0, 0};
+  }
+  __device__ typename SweepMfma<double>::v4* begin() { return a; }
+  __device__ void end() {}
+  __device__ double get(int n, int r) const { return a[n][r]; }
+};
+template <int N>
+struct TileAcc<float, N> {
+  double a[N][4];
+  typename SweepMfma<float>::v4 p[N];
+  __device__ void zero() {
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[n][r] = 0.0;
+  }
+  __device__ typename SweepMfma<float>::v4* begin() {
+#pragma unroll
+    for (int n = 0; n < N; ++n) p[n] = typename SweepMfma<float>::v4{0, 0, 0, 0};
+    return p;
+  }
+  __device__ void end() {
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[n][r] += static_cast<double>(p[n][r]);
+  }
+  __device__ double get(int n, int r) const { return a[n][r]; }
+};
+
 // LDS of one own-row block step: the L operand tile, the right-hand-side
 // operand tile (reused for the block's rows in FP64 once the products are
 // done), the inverse of its unit diagonal block and its pivot data.
@@ -475,37 +516,61 @@ __global__ void k_solve_begin(unsigned* gen) {
 // children's order (mapPtr over idxOff positions, mapSrc = (child, child
 // position)) -- the sums of adding the children one after another into a
 // zeroed block.  An own row p < nF gets y0 = Bp - g, the others g, the start
-// of the delta handed to the parent.  pos[k] = (front, position) numbers the
-// level's positions; 2^lpShift lanes take one, each a strided subset of the
-// right-hand sides.
-__global__ __launch_bounds__(256) void k_fwd_gather(const int2* __restrict__ pos, int64_t positions, int lpShift,
-                                                     DevFronts fd, int nrhs, const double* __restrict__ Bp,
-                                                     double* __restrict__ Dl, const int64_t* __restrict__ gOff,
-                                                     const int64_t* __restrict__ mapPtr,
-                                                     const int2* __restrict__ mapSrc) {
+// of the delta handed to the parent.  info[k] describes the level's position
+// k; 2^lpShift lanes take one, each a strided subset of the right-hand sides.
+__global__ __launch_bounds__(256) void k_fwd_gather(const PosInfo* __restrict__ info, int64_t positions, int lpShift,
+                                                     int nrhs, const double* __restrict__ Bp, double* __restrict__ Dl,
+                                                     const int64_t* __restrict__ mapRow) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t item = g >> lpShift;
-  const int l = static_cast<int>(g & ((1 << lpShift) - 1));
+  const int lp = 1 << lpShift;
+  const int l = static_cast<int>(g & (lp - 1));
   if (item >= positions) return;
-  const int2 sp = pos[item];
-  const int s = sp.x, p = sp.y;
-  const int m = fd.m[s];
-  if (p >= m) return;
-  const int nf = fd.nF[s];
-  const int64_t io = fd.idxOff[s];
-  const int t = fd.perm[io + p];
-  const int64_t e0 = mapPtr[io + t], e1 = mapPtr[io + t + 1];
-  const bool own = p < nf;
-  const int64_t rix = own ? fd.rowIdx[io + p] : 0;
-  double* dst = Dl + (gOff[s] + p) * nrhs;
-  for (int c = l; c < nrhs; c += 1 << lpShift) {
-    double acc = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-      const int2 src = mapSrc[e];
-      acc += Dl[(gOff[src.x] + src.y) * nrhs + c];
+  const PosInfo pi = info[item];
+  double* dst = Dl + pi.dst * nrhs;
+  const double* bp = pi.rix >= 0 ? Bp + static_cast<int64_t>(pi.rix) * nrhs : nullptr;
+  // four right-hand sides per lane at a time: the children's rows are read
+  // with four independent loads per entry
+  for (int c0 = l; c0 < nrhs; c0 += 4 * lp) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = 0; e < pi.cnt; ++e) {
+      const double* src = Dl + mapRow[pi.e0 + e] * nrhs;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (c0 + u * lp < nrhs) acc[u] += src[c0 + u * lp];
     }
-    dst[c] = own ? Bp[rix * nrhs + c] - acc : acc;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + u * lp;
+      if (c < nrhs) dst[c] = bp ? bp[c] - acc[u] : acc[u];
+    }
   }
+}
+
+// The gather's per-position records for the current factorization (the
+// pivoting permutation and the row identities change with the values):
+// position k = (front s, position p) of the solve's position list.
+__global__ __launch_bounds__(256) void k_pos_info(const int2* __restrict__ pos, int64_t positions, DevFronts fd,
+                                                   const int64_t* __restrict__ gOff,
+                                                   const int64_t* __restrict__ mapPtr, PosInfo* __restrict__ info) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= positions) return;
+  const int2 sp = pos[k];
+  const int s = sp.x, p = sp.y;
+  const int64_t io = fd.idxOff[s];
+  PosInfo pi;
+  pi.dst = gOff[s] + p;
+  if (p < fd.m[s]) {
+    const int t = fd.perm[io + p];
+    pi.e0 = mapPtr[io + t];
+    pi.cnt = static_cast<int>(mapPtr[io + t + 1] - pi.e0);
+    pi.rix = p < fd.nF[s] ? fd.rowIdx[io + p] : -1;
+  } else {
+    pi.e0 = 0;
+    pi.cnt = 0;
+    pi.rix = -1;
+  }
+  info[k] = pi;
 }
 
 // Inverses of the unit lower diagonal blocks of L that the solve applies:
@@ -608,31 +673,25 @@ __global__ __launch_bounds__(256) void k_fwd_chain(const int* list, int count, c
 }
 
 // Rows [nF, m) of a front once its own rows are solved (solve.hpp:349-361):
-// delta(i) = g(i) + L(i, 0:nF) y, g already in the work block.  One
-// workgroup per 64-row chunk (waves of 32 rows x 32 right-hand sides); the
+// delta(i) = g(i) + L(i, 0:nF) y, g already in the work block.  A workgroup
+// takes one 64-row chunk b (waves of 32 rows x 32 right-hand sides); the
 // product runs over the own columns in 32-column tiles, the next tiles'
 // operands loaded while the current ones are multiplied.
+template <typename E>
+struct DeltaLds {
+  E Lt[DT][SB + 1];  // [row i][own column q]
+  E yt[SB][YW];      // [own column q][right-hand side]
+};
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                   int nrhs, double* __restrict__ Dl, const int64_t* __restrict__ gOff,
-                                                   int* err) {
+__device__ void fwdDeltaChunk(DeltaLds<typename SweepMfma<T>::E>& S, const DevFronts& fd, int s, int b, int nrhs,
+                              double* __restrict__ Ds) {
   using M = SweepMfma<T>;
   using E = typename M::E;
   constexpr int NT = 256;
-  __shared__ E Lt[DT][SB + 1];  // [row i][own column q]
-  __shared__ E yt[SB][YW];      // [own column q][right-hand side]
-  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
-  const int s = list[f];
-  const int b = blockIdx.x - blockStart[f];
   const int m = fd.m[s];
   const int nf = fd.nF[s];
-  const int nD = (m - nf + DT - 1) / DT;
-  if (b == 0 && threadIdx.x == 0 && nD > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
-  if (b >= nD) return;
   const int i0 = nf + b * DT, ni = min(DT, m - i0);
   const T* F = static_cast<const T*>(fd.F[s]);
-  double* Ds = Dl + gOff[s] * nrhs;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
@@ -653,39 +712,29 @@ __global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, c
         ypre[u] = (k0 + q < nf && c < gw) ? static_cast<E>(Ds[static_cast<int64_t>(k0 + q) * nrhs + rg + c]) : E(0);
       }
     };
-    double acc[2][2][4];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) acc[bi][bj][rr] = 0.0;
+    TileAcc<T, 4> acc;  // block (bi, bj) at 2 bi + bj
+    acc.zero();
     if (nf > 0) loadT(0);
     for (int k0 = 0; k0 < nf; k0 += SB) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = tid + u * NT;
-        Lt[e & (DT - 1)][e / DT] = lpre[u];
-        yt[e >> 6][e & 63] = ypre[u];
+        S.Lt[e & (DT - 1)][e / DT] = lpre[u];
+        S.yt[e >> 6][e & 63] = ypre[u];
       }
       if (k0 + SB < nf) loadT(k0 + SB);
       __syncthreads();
-      typename M::v4 part[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
-#pragma unroll
+      typename M::v4* part = acc.begin();
+#pragma unroll 2
       for (int kk = 0; kk < SB; kk += 4) {
-        const E a0 = Lt[wr + lr][kk + lk], a1 = Lt[wr + 16 + lr][kk + lk];
-        const E b0 = yt[kk + lk][wc + lr], b1 = yt[kk + lk][wc + 16 + lr];
-        part[0][0] = M::mma(a0, b0, part[0][0]);
-        part[0][1] = M::mma(a0, b1, part[0][1]);
-        part[1][0] = M::mma(a1, b0, part[1][0]);
-        part[1][1] = M::mma(a1, b1, part[1][1]);
+        const E a0 = S.Lt[wr + lr][kk + lk], a1 = S.Lt[wr + 16 + lr][kk + lk];
+        const E b0 = S.yt[kk + lk][wc + lr], b1 = S.yt[kk + lk][wc + 16 + lr];
+        part[0] = M::mma(a0, b0, part[0]);
+        part[1] = M::mma(a0, b1, part[1]);
+        part[2] = M::mma(a1, b0, part[2]);
+        part[3] = M::mma(a1, b1, part[3]);
       }
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) acc[bi][bj][rr] += static_cast<double>(part[bi][bj][rr]);
+      acc.end();
       __syncthreads();
     }
 #pragma unroll
@@ -695,9 +744,25 @@ __global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, c
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
           const int r = wr + 16 * bi + M::row(lane, rr), c = wc + 16 * bj + lr;
-          if (r < ni && c < gw) Ds[static_cast<int64_t>(i0 + r) * nrhs + rg + c] += acc[bi][bj][rr];
+          if (r < ni && c < gw) Ds[static_cast<int64_t>(i0 + r) * nrhs + rg + c] += acc.get(2 * bi + bj, rr);
         }
   }
+}
+
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                   int nrhs, double* __restrict__ Dl, const int64_t* __restrict__ gOff,
+                                                   int* err) {
+  __shared__ DeltaLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int b = blockIdx.x - blockStart[f];
+  const int nD = (fd.m[s] - fd.nF[s] + DT - 1) / DT;
+  if (b == 0 && threadIdx.x == 0 && nD > blockStart[f + 1] - blockStart[f]) raiseError(err, 2);
+  if (b >= nD) return;
+  fwdDeltaChunk<T>(S, fd, s, b, nrhs, Dl + gOff[s] * nrhs);
 }
 
 // Backward step, the ancestors' part (solve.hpp:364-383): for own block w of
@@ -708,34 +773,25 @@ __global__ __launch_bounds__(256) void k_fwd_delta(const int* list, int count, c
 // whole 64-row tiles: workgroup (w, k) takes chunk k of own block w, chunk 0
 // writing z_w less its part into the block's work rows, chunk k >= 1 its part
 // into the front's partial rows (AncPart).
+template <typename E>
+struct AncLds {
+  E Lt[DT][SB + 1];  // [ancestor row i][own column q]
+  E xt[DT][YW];      // [ancestor row i][right-hand side]
+};
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, const int* blockStart, DevFronts fd,
-                                                 int nrhs, const double* __restrict__ Z, const double* __restrict__ X,
-                                                 double* __restrict__ Dl, const int64_t* __restrict__ gOff,
-                                                 const AncPart* __restrict__ parts, int* err) {
+__device__ void bwdAncChunk(AncLds<typename SweepMfma<T>::E>& S, const DevFronts& fd, int s, int w, int k,
+                            const AncPart pa, int nrhs, const double* __restrict__ Z, const double* __restrict__ X,
+                            double* __restrict__ Dl, double* __restrict__ Ds) {
   using M = SweepMfma<T>;
   using E = typename M::E;
   constexpr int NT = 256;
-  __shared__ E Lt[DT][SB + 1];  // [ancestor row i][own column q]
-  __shared__ E xt[DT][YW];      // [ancestor row i][right-hand side]
-  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
-  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
-  const int s = list[f];
-  const int bb = blockIdx.x - blockStart[f];
-  const AncPart pa = parts[s];
-  const int alloc = (blockStart[f + 1] - blockStart[f]) / pa.split;  // own blocks planned
-  const int w = bb / pa.split, k = bb - w * pa.split;
   const int m = fd.m[s];
   const int nf = fd.nF[s];
-  const int nOwn = (nf + SB - 1) / SB;
-  if (bb == 0 && threadIdx.x == 0 && nOwn > alloc) raiseError(err, 2);
-  if (w >= nOwn) return;
   const int r0 = w * SB, nr = min(SB, nf - r0);
   const int chunk = ((m - nf + pa.split - 1) / pa.split + DT - 1) / DT * DT;
   const int a0 = nf + k * chunk, a1 = min(m, a0 + chunk);
   const int64_t io = fd.idxOff[s];
   const T* F = static_cast<const T*>(fd.F[s]);
-  double* Ds = Dl + gOff[s] * nrhs;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wu = __builtin_amdgcn_readfirstlane(wv);
   const int tr = (wv & 1) * 16, tc = (wv >> 1) * 32;
@@ -770,9 +826,9 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
     }
     for (int i0 = a0; i0 < a1; i0 += DT) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) Lt[lane][wv + 4 * u] = lpre[u];
+      for (int u = 0; u < 8; ++u) S.Lt[lane][wv + 4 * u] = lpre[u];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) xt[wv + 4 * u][lane] = xpre[u];
+      for (int u = 0; u < 16; ++u) S.xt[wv + 4 * u][lane] = xpre[u];
       if (i0 + DT < a1) {
         loadT(i0 + DT, rnext);
 #pragma unroll
@@ -781,11 +837,11 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
       }
       __syncthreads();
       typename M::v4 part[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-#pragma unroll
+#pragma unroll 4
       for (int kk = 0; kk < DT; kk += 4) {
-        const E av = Lt[kk + lk][tr + lr];
-        part[0] = M::mma(av, xt[kk + lk][tc + lr], part[0]);
-        part[1] = M::mma(av, xt[kk + lk][tc + 16 + lr], part[1]);
+        const E av = S.Lt[kk + lk][tr + lr];
+        part[0] = M::mma(av, S.xt[kk + lk][tc + lr], part[0]);
+        part[1] = M::mma(av, S.xt[kk + lk][tc + 16 + lr], part[1]);
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -808,6 +864,26 @@ __global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, con
         }
       }
   }
+}
+
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_anc(const int* list, int count, const int* blockStart, DevFronts fd,
+                                                 int nrhs, const double* __restrict__ Z, const double* __restrict__ X,
+                                                 double* __restrict__ Dl, const int64_t* __restrict__ gOff,
+                                                 const AncPart* __restrict__ parts, int* err) {
+  __shared__ AncLds<typename SweepMfma<T>::E> S;
+  if (static_cast<int>(blockIdx.x) >= blockStart[count]) return;
+  const int f = segmentOf<int>(blockStart, count, blockIdx.x);
+  const int s = list[f];
+  const int bb = blockIdx.x - blockStart[f];
+  const AncPart pa = parts[s];
+  const int alloc = (blockStart[f + 1] - blockStart[f]) / pa.split;  // own blocks planned
+  const int w = bb / pa.split, k = bb - w * pa.split;
+  const int nOwn = (fd.nF[s] + SB - 1) / SB;
+  if (bb == 0 && threadIdx.x == 0 && nOwn > alloc) raiseError(err, 2);
+  if (w >= nOwn) return;
+  bwdAncChunk<T>(S, fd, s, w, k, pa, nrhs, Z, X, Dl, Dl + gOff[s] * nrhs);
 }
 
 // Backward own rows of fronts with a short own-row chain: one workgroup per
@@ -1319,14 +1395,19 @@ void launchSolveBegin(unsigned* gen, hipStream_t st) {
   k_solve_begin<<<1, 64, 0, st>>>(gen);
   hipCheck(hipGetLastError(), "launchSolveBegin");
 }
-void launchFwdGather(const int2* pos, int64_t positions, const DevFronts& fd, int nrhs, const double* Bp, double* Dl,
-                     const int64_t* gOff, const int64_t* mapPtr, const int2* mapSrc, hipStream_t st) {
+void launchPosInfo(const int2* pos, int64_t positions, const DevFronts& fd, const int64_t* gOff,
+                   const int64_t* mapPtr, PosInfo* info, hipStream_t st) {
+  if (positions <= 0) return;
+  k_pos_info<<<static_cast<unsigned>((positions + 255) / 256), 256, 0, st>>>(pos, positions, fd, gOff, mapPtr, info);
+  hipCheck(hipGetLastError(), "launchPosInfo");
+}
+void launchFwdGather(const PosInfo* info, int64_t positions, int nrhs, const double* Bp, double* Dl,
+                     const int64_t* mapRow, hipStream_t st) {
   if (positions <= 0 || nrhs <= 0) return;
-  int lpShift = 0;  // lanes per position: the power of two at or above nrhs, at most 64
-  while ((1 << lpShift) < nrhs && lpShift < 6) ++lpShift;
+  int lpShift = 0;  // lanes per position: the power of two at or above nrhs / 4, at most 16
+  while ((4 << lpShift) < nrhs && lpShift < 4) ++lpShift;
   const int64_t blocks = ((positions << lpShift) + 255) / 256;
-  k_fwd_gather<<<static_cast<unsigned>(blocks), 256, 0, st>>>(pos, positions, lpShift, fd, nrhs, Bp, Dl, gOff, mapPtr,
-                                                              mapSrc);
+  k_fwd_gather<<<static_cast<unsigned>(blocks), 256, 0, st>>>(info, positions, lpShift, nrhs, Bp, Dl, mapRow);
   hipCheck(hipGetLastError(), "launchFwdGather");
 }
 void launchDiagInverse(const int* list, int count, const int* blockStart, int blocks, const DevFronts& fd,
