@@ -744,8 +744,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       int q = 0;
       for (; q + U <= p; q += U) {
         double lv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) lv[u] = static_cast<double>(Fat(i, k0 + q + u));
+        loadWiden<U>(lv, static_cast<const T*>(F), [&](int u) -> int64_t { return i + static_cast<int64_t>(k0 + q + u) * m; });
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -1625,6 +1624,24 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
 // rest (still holding current values) to the exact panel kernel, which
 // continues from there.
 // ---------------------------------------------------------------------------
+// Row values loaded in their storage type, every load issued before any is
+// used: each load is unconditional (a skipped element reads the front's first
+// entry, a valid address, and is zeroed afterwards), since a load under a
+// branch completes before the branch joins.
+template <int N, typename T, typename Off>
+__device__ inline void loadWiden(double (&x)[N], const T* base, Off off) {
+  T raw[N];
+  bool ok[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    const int64_t o = off(q);
+    ok[q] = o >= 0;
+    raw[q] = base[ok[q] ? o : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) x[q] = ok[q] ? static_cast<double>(raw[q]) : 0.0;
+}
+
 // 1 / d for a pivot on a dependent chain: the hardware reciprocal refined by
 // two Newton steps (within an ulp of the quotient), an IEEE division for
 // zero, denormal and non-finite d.
@@ -1725,11 +1742,10 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   // block entry (i, j), i >= j: column j = tid % 64, rows i = tid / 64 + 4 r
   const int j = tid & 63, ib = tid >> 6;
   double a[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
+  loadWiden<16>(a, F, [&](int r) -> int64_t {
     const int i = ib + 4 * r;
-    a[r] = (i < pw && j < pw && i >= j) ? Fat(k0 + i, k0 + j) : 0.0;
-  }
+    return (i < pw && j < pw && i >= j) ? (k0 + i) + static_cast<int64_t>(k0 + j) * m : -1;
+  });
   __syncthreads();
   // Right-looking, one column per step: a_ij -= c_iq l_jq (c_iq = d_q l_iq
   // up to the rounding of the stored multiplier), l_jq = T(c_jq (1 / d_q)).
@@ -1795,8 +1811,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   const int i = rBeg + tid;
   const bool valid = i < nE;
   double x[P];
-#pragma unroll
-  for (int q = 0; q < P; ++q) x[q] = (valid && q < pw) ? Fat(i, k0 + q) : 0.0;
+  loadWiden<P>(x, F, [&](int q) -> int64_t { return (valid && q < pw) ? i + static_cast<int64_t>(k0 + q) * m : -1; });
   // Column maxima through LDS, 16 columns at a time: |c| rounded up to
   // float (a bound that is never below the value, so a borderline column can
   // only fail the test here and be decided by the exact kernel).
@@ -1892,8 +1907,9 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   const int lim = (i < nE) ? qs : ((i < m && qs == pw) ? pw : 0);
   if (lim > 0) {
     double x[P];
-#pragma unroll
-    for (int q = 0; q < P; ++q) x[q] = (q < lim) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
+    loadWiden<P>(x, static_cast<const T*>(F), [&](int q) -> int64_t {
+      return (q < lim) ? i + static_cast<int64_t>(k0 + q) * m : -1;
+    });
     specRow<T>(x, W, rdv, lim, [&](int q, double c) {
       if (q < lim) tmax = fmax(tmax, fabs(c));
     });
@@ -2060,8 +2076,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int* list, int count, 
   const int i = nE + rowBlock * blockDim.x + threadIdx.x;
   if (i >= m) return;
   double x[P];
-#pragma unroll
-  for (int q = 0; q < P; ++q) x[q] = (q < p) ? static_cast<double>(Fat(i, k0 + q)) : 0.0;
+  loadWiden<P>(x, static_cast<const T*>(F), [&](int q) -> int64_t {
+    return (q < p) ? i + static_cast<int64_t>(k0 + q) * m : -1;
+  });
   // y = x M^T in place, last column first (y_q needs x_0 .. x_q only)
 #pragma unroll
   for (int q = P - 1; q > 0; --q) {
