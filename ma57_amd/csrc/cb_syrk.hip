@@ -181,22 +181,30 @@ __global__ __launch_bounds__(64 * SyrkMfma<T>::kWaves) void k_cb_syrk(int N, int
   }
 
   // C(row, col) -= acc: register r of block (bi, bj) holds row lr of the
-  // block and column M::col(lane, r)
+  // block and column M::col(lane, r).  Every C value is loaded before any is
+  // used (entries outside the triangle read C's first entry and are not
+  // stored), so the loads overlap instead of waiting one by one.
+  T cv[BI][4][4];
 #pragma unroll
-  for (int bi = 0; bi < BI; ++bi) {
-    const int row = I0 + wr + bi * 16 + lr;
-    if (row >= N) continue;
+  for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int col = J0 + wc + bj * 16 + M::col(lane, r);
-        if (col < N && (ti != tj || row >= col)) {
-          T* c = C + row + static_cast<size_t>(col) * ldc;
-          *c = *c - acc[bi][bj][r];
-        }
+        const int row = I0 + wr + bi * 16 + lr, col = J0 + wc + bj * 16 + M::col(lane, r);
+        const bool ok = row < N && col < N && (ti != tj || row >= col);
+        cv[bi][bj][r] = C[ok ? row + static_cast<size_t>(col) * ldc : 0];
       }
-  }
+#pragma unroll
+  for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = I0 + wr + bi * 16 + lr, col = J0 + wc + bj * 16 + M::col(lane, r);
+        if (row < N && col < N && (ti != tj || row >= col))
+          C[row + static_cast<size_t>(col) * ldc] = cv[bi][bj][r] - acc[bi][bj][r];
+      }
 }
 
 // BF16x3 split products (Henry, Tang and Heinecke, "Leveraging the bfloat16
@@ -299,21 +307,28 @@ __global__ __launch_bounds__(256) void k_cb_syrk_bf16x3(int N, int K, const floa
     if (kt + 1 < nk) lstore(buf ^ 1);
     __syncthreads();
   }
+  // C -= acc, every C value loaded before any is used (as in k_cb_syrk)
+  float cv[4][4][4];
 #pragma unroll
-  for (int bi = 0; bi < 4; ++bi) {
-    const int row = I0 + wr + bi * 16 + lr;
-    if (row >= N) continue;
+  for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int col = J0 + wc + bj * 16 + 4 * lk + r;
-        if (col < N && (ti != tj || row >= col)) {
-          float* c = C + row + static_cast<size_t>(col) * ldc;
-          *c = *c - acc[bi][bj][r];
-        }
+        const int row = I0 + wr + bi * 16 + lr, col = J0 + wc + bj * 16 + 4 * lk + r;
+        const bool ok = row < N && col < N && (ti != tj || row >= col);
+        cv[bi][bj][r] = C[ok ? row + static_cast<size_t>(col) * ldc : 0];
       }
-  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = I0 + wr + bi * 16 + lr, col = J0 + wc + bj * 16 + 4 * lk + r;
+        if (row < N && col < N && (ti != tj || row >= col))
+          C[row + static_cast<size_t>(col) * ldc] = cv[bi][bj][r] - acc[bi][bj][r];
+      }
 }
 
 }  // namespace

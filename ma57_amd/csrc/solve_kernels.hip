@@ -141,6 +141,23 @@ __device__ inline double lval(const T* F, int m, const int* pk, int row, int col
   return (row == col + 1 && pk[col] == 2) ? 0.0 : v;
 }
 
+// Elements that may lie outside their block, loaded unconditionally: outside,
+// the block's first element (a valid address) is read and zero returned.  A
+// load under a branch completes before the branch joins, so conditional
+// loads would wait for one another; these are issued back to back.
+template <typename T>
+__device__ inline T ldOr0(const T* base, int64_t off, bool ok) {
+  const T v = base[ok ? off : 0];
+  return ok ? v : T(0);
+}
+template <typename T>
+__device__ inline double lvalOr0(const T* F, int m, const int* pk, int row, int col, bool ok) {
+  const int r = ok ? row : 0, c = ok ? col : 0;
+  const double v = F[r + static_cast<size_t>(c) * m];
+  const int k = pk[c];
+  return (ok && !(r == c + 1 && k == 2)) ? v : 0.0;
+}
+
 // The solve's tile products on the matrix cores, in the storage type of the
 // factor: v_mfma_f64_16x16x4f64 for FP64 fronts, v_mfma_f32_16x16x4f32 for
 // FP32 fronts (operands rounded to FP32).  A[i][k] comes from lane i + 16k,
@@ -278,7 +295,7 @@ __device__ void fwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int r = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-      y0p[t][rr] = (r < nr && c < gw) ? Ds[static_cast<int64_t>(r0 + r) * nrhs + rg + c] : 0.0;
+      y0p[t][rr] = ldOr0(static_cast<const double*>(Ds), static_cast<int64_t>(r0 + r) * nrhs + rg + c, r < nr && c < gw);
     }
   // L(b, j) tiles four elements per thread, loaded one tile ahead
   E lpre[4];
@@ -288,7 +305,7 @@ __device__ void fwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
     for (int u = 0; u < 4; ++u) {
       const int e = tid + u * NT;
       const int r = e % SB, qq = e / SB;
-      lpre[u] = (r < nr && qq < cn) ? static_cast<E>(lval(F, m, pk, r0 + r, c0 + qq)) : E(0);
+      lpre[u] = static_cast<E>(lvalOr0(F, m, pk, r0 + r, c0 + qq, r < nr && qq < cn));
     }
   };
   double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};  // FP64 sums of the per-tile products
@@ -303,9 +320,9 @@ __device__ void fwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
     }
     for (int e = tid; e < SB * 64; e += NT) {
       const int c = e & 63, qq = e >> 6;
-      S.yt[qq][c] = (qq < cn && c < gw)
-                        ? static_cast<E>(readRow<kShared>(Ds + static_cast<int64_t>(c0 + qq) * nrhs + rg + c))
-                        : E(0);
+      const bool ok = qq < cn && c < gw;
+      const double v = readRow<kShared>(Ds + (ok ? static_cast<int64_t>(c0 + qq) * nrhs + rg + c : 0));
+      S.yt[qq][c] = ok ? static_cast<E>(v) : E(0);
     }
     if (j + 1 < b) loadL(j + 1);
     __syncthreads();
@@ -414,12 +431,11 @@ __device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-      double v = 0.0;
-      if (qq < nr && c < gw) {
-        v = Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c];
-        for (int k = 1; k < pa.split; ++k)
-          v -= Dl[(pa.off + static_cast<int64_t>(k - 1) * pa.stride + r0 + qq) * nrhs + rg + c];
-      }
+      const bool ok = qq < nr && c < gw;
+      double v = ldOr0(static_cast<const double*>(Ds), static_cast<int64_t>(r0 + qq) * nrhs + rg + c, ok);
+      for (int k = 1; k < pa.split; ++k)
+        v -= ldOr0(static_cast<const double*>(Dl),
+                   (pa.off + static_cast<int64_t>(k - 1) * pa.stride + r0 + qq) * nrhs + rg + c, ok);
       p0[t][rr] = v;
     }
   E opre[4];
@@ -429,7 +445,7 @@ __device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
     for (int u = 0; u < 4; ++u) {
       const int e = tid + u * NT;
       const int i = e % SB, qq = e / SB;
-      opre[u] = (i < in && qq < nr) ? static_cast<E>(lval(F, m, pk, i0 + i, r0 + qq)) : E(0);
+      opre[u] = static_cast<E>(lvalOr0(F, m, pk, i0 + i, r0 + qq, i < in && qq < nr));
     }
   };
   double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
@@ -444,9 +460,9 @@ __device__ void bwdOwnBlock(BlockLds<typename SweepMfma<T>::E>& S, const DevFron
     }
     for (int e = tid; e < SB * 64; e += NT) {
       const int c = e & 63, i = e >> 6;
-      S.yt[i][c] = (i < in && c < gw)
-                       ? static_cast<E>(readRow<kShared>(Ds + static_cast<int64_t>(i0 + i) * nrhs + rg + c))
-                       : E(0);
+      const bool ok = i < in && c < gw;
+      const double v = readRow<kShared>(Ds + (ok ? static_cast<int64_t>(i0 + i) * nrhs + rg + c : 0));
+      S.yt[i][c] = ok ? static_cast<E>(v) : E(0);
     }
     if (j - 1 > w) loadO(j - 1);
     __syncthreads();
@@ -703,13 +719,14 @@ __device__ void fwdDeltaChunk(DeltaLds<typename SweepMfma<T>::E>& S, const DevFr
       for (int u = 0; u < 8; ++u) {
         const int e = tid + u * NT;
         const int i = e & (DT - 1), q = e / DT;
-        lpre[u] = (i < ni && k0 + q < nf) ? static_cast<E>(F[(i0 + i) + static_cast<size_t>(k0 + q) * m]) : E(0);
+        lpre[u] = static_cast<E>(ldOr0(F, (i0 + i) + static_cast<int64_t>(k0 + q) * m, i < ni && k0 + q < nf));
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = tid + u * NT;
         const int c = e & 63, q = e >> 6;
-        ypre[u] = (k0 + q < nf && c < gw) ? static_cast<E>(Ds[static_cast<int64_t>(k0 + q) * nrhs + rg + c]) : E(0);
+        ypre[u] = static_cast<E>(ldOr0(static_cast<const double*>(Ds), static_cast<int64_t>(k0 + q) * nrhs + rg + c,
+                                        k0 + q < nf && c < gw));
       }
     };
     TileAcc<T, 4> acc;  // block (bi, bj) at 2 bi + bj
@@ -737,6 +754,8 @@ __device__ void fwdDeltaChunk(DeltaLds<typename SweepMfma<T>::E>& S, const DevFr
       acc.end();
       __syncthreads();
     }
+    // g of the chunk's rows, all loaded before any is used
+    double g[2][2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -744,7 +763,16 @@ __device__ void fwdDeltaChunk(DeltaLds<typename SweepMfma<T>::E>& S, const DevFr
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
           const int r = wr + 16 * bi + M::row(lane, rr), c = wc + 16 * bj + lr;
-          if (r < ni && c < gw) Ds[static_cast<int64_t>(i0 + r) * nrhs + rg + c] += acc.get(2 * bi + bj, rr);
+          g[bi][bj][rr] = ldOr0(static_cast<const double*>(Ds), static_cast<int64_t>(i0 + r) * nrhs + rg + c, r < ni && c < gw);
+        }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int r = wr + 16 * bi + M::row(lane, rr), c = wc + 16 * bj + lr;
+          if (r < ni && c < gw) Ds[static_cast<int64_t>(i0 + r) * nrhs + rg + c] = g[bi][bj][rr] + acc.get(2 * bi + bj, rr);
         }
   }
 }
@@ -803,20 +831,24 @@ __device__ void bwdAncChunk(AncLds<typename SweepMfma<T>::E>& S, const DevFronts
     E lpre[8], xpre[16];
     int64_t rcur[16], rnext[16];
     auto rows = [&](int i0, int64_t* out) {
+      // loaded unconditionally (a1 > i0 here) and held as loaded, so the
+      // sixteen loads are in flight together
+      int rr[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int i = i0 + wu + 4 * u;
-        out[u] = (i < a1) ? static_cast<int64_t>(fd.rowIdx[io + i]) * nrhs : -1;
-      }
+      for (int u = 0; u < 16; ++u) rr[u] = fd.rowIdx[io + min(i0 + wu + 4 * u, a1 - 1)];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(rr[u]));
+#pragma unroll
+      for (int u = 0; u < 16; ++u) out[u] = (i0 + wu + 4 * u < a1) ? static_cast<int64_t>(rr[u]) * nrhs : -1;
     };
     auto loadT = [&](int i0, const int64_t* xr) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int q = wv + 4 * u;
-        lpre[u] = (i0 + lane < a1 && q < nr) ? static_cast<E>(F[(i0 + lane) + static_cast<size_t>(r0 + q) * m]) : E(0);
+        lpre[u] = static_cast<E>(ldOr0(F, (i0 + lane) + static_cast<int64_t>(r0 + q) * m, i0 + lane < a1 && q < nr));
       }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) xpre[u] = (xr[u] >= 0 && lane < gw) ? static_cast<E>(X[xr[u] + rg + lane]) : E(0);
+      for (int u = 0; u < 16; ++u) xpre[u] = static_cast<E>(ldOr0(X, xr[u] + rg + lane, xr[u] >= 0 && lane < gw));
     };
     double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     if (a0 < a1) {
@@ -854,9 +886,11 @@ __device__ void bwdAncChunk(AncLds<typename SweepMfma<T>::E>& S, const DevFronts
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int qq = tr + M::row(lane, rr), c = tc + 16 * t + lr;
-        if (qq < nr && c < gw) {
+        const bool ok = qq < nr && c < gw;
+        const int zr = fd.rowIdx[io + r0 + (ok ? qq : 0)];
+        const double z = (k == 0) ? ldOr0(Z, static_cast<int64_t>(zr) * nrhs + rg + c, ok) : 0.0;
+        if (ok) {
           if (k == 0) {
-            const double z = Z[static_cast<int64_t>(fd.rowIdx[io + r0 + qq]) * nrhs + rg + c];
             Ds[static_cast<int64_t>(r0 + qq) * nrhs + rg + c] = z - acc[t][rr];
           } else {
             Dl[(pa.off + static_cast<int64_t>(k - 1) * pa.stride + r0 + qq) * nrhs + rg + c] = acc[t][rr];
@@ -972,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
   double lr[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) {
-    const double l = (row && j < nf && j < lane) ? lval(F, m, pk, lane, j) : 0.0;
+    const double l = lvalOr0(F, m, pk, lane, j, row && j < nf && j < lane);
     lr[j] = own ? -l : l;
   }
   // keep the row in registers: loaded once, not again inside the loops below
@@ -983,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
   for (int cg = 0; cg < nrhs; cg += 8) {
     double v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = (row && cg + c < nrhs) ? Dr[cg + c] : 0.0;
+    for (int c = 0; c < 8; ++c) v[c] = ldOr0(static_cast<const double*>(Dl), (gOff[s] + lane) * nrhs + cg + c, row && cg + c < nrhs);
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
       if (j >= nf) break;
@@ -1036,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
   // column `lane` of L below the diagonal, rows [lane + 1, m)
   double lc[64];
 #pragma unroll
-  for (int a = 0; a < 64; ++a) lc[a] = (own && a > lane && a < m) ? lval(F, m, pk, a, lane) : 0.0;
+  for (int a = 0; a < 64; ++a) lc[a] = lvalOr0(F, m, pk, a, lane, own && a > lane && a < m);
   // keep the column in registers: loaded once, not again inside the loops below
 #pragma unroll
   for (int a = 0; a < 64; ++a) asm volatile("" : "+v"(lc[a]));
@@ -1047,8 +1081,8 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int* list, int count, De
     double x[8], xa[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      x[c] = (own && cg + c < nrhs) ? Z[prow * nrhs + cg + c] : 0.0;
-      xa[c] = (lane < ncb && cg + c < nrhs) ? X[crow * nrhs + cg + c] : 0.0;
+      x[c] = ldOr0(Z, prow * nrhs + cg + c, own && cg + c < nrhs);
+      xa[c] = ldOr0(static_cast<const double*>(X), crow * nrhs + cg + c, lane < ncb && cg + c < nrhs);
     }
 #pragma unroll
     for (int a = 0; a < 64; ++a) {
