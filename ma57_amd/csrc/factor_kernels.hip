@@ -214,11 +214,26 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* tasks, int ntasks,
 
   const int nc = fd.ncols[s];
   const int fc = fd.firstCol[s];
+  // A's entries of the own columns, four at a time: the rows and values of a
+  // batch are loaded together, then added in entry order (a caller's
+  // repeated entry lands on the same row and adds in turn)
   for (int j = c0 + threadIdx.x; j < min(c1, nc); j += blockDim.x) {
     const int gcol = fc + j;
     T* col = F + static_cast<size_t>(j) * m;
-    for (int64_t t = fd.acolPtr[gcol]; t < fd.acolPtr[gcol + 1]; ++t)
-      col[fd.aRow[t]] += static_cast<T>(aVals[fd.aSrc[t]]);
+    const int64_t tb = fd.acolPtr[gcol], te = fd.acolPtr[gcol + 1];
+    for (int64_t t0 = tb; t0 < te; t0 += 4) {
+      int row[4];
+      double av[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t t = min(t0 + u, te - 1);
+        row[u] = fd.aRow[t];
+        av[u] = aVals[fd.aSrc[t]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t0 + u < te) col[row[u]] += static_cast<T>(av[u]);
+    }
   }
   __syncthreads();
 
@@ -254,13 +269,32 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* tasks, int ntasks,
       const int lb = rel[b];
       if (lb < 0) continue;
       const T* ccol = Fc + static_cast<size_t>(b) * mc;
-      for (int a = b + lane; a < mc; a += 64) {
-        const int la = rel[a];
-        if (la < 0) continue;
-        const int tr = la > lb ? la : lb;
-        const int tc = la > lb ? lb : la;
-        if (tc < c0 || tc >= c1) continue;
-        F[tr + static_cast<size_t>(tc) * m] += ccol[a];
+      // four rows per lane at a time, every load of a batch in flight before
+      // any add (a skipped row reads the child's own entry (b, b) instead);
+      // the relabelling is one to one, so no two rows meet in the parent
+      for (int a0 = b + lane; a0 < mc; a0 += 256) {
+        int la[4];
+        T cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int a = min(a0 + 64 * u, mc - 1);
+          la[u] = rel[a];
+          cv[u] = ccol[a];
+        }
+        size_t idx[4];
+        bool ok[4];
+        T old[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int tr = la[u] > lb ? la[u] : lb;
+          const int tc = la[u] > lb ? lb : la[u];
+          ok[u] = a0 + 64 * u < mc && la[u] >= 0 && tc >= c0 && tc < c1;
+          idx[u] = tr + static_cast<size_t>(tc) * m;
+          old[u] = ok[u] ? F[idx[u]] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (ok[u]) F[idx[u]] = old[u] + cv[u];
       }
     }
     __syncthreads();
@@ -735,11 +769,11 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(const int* list, int co
       const int i = k0 + tid + jj * NT;
       if (i < k || i >= nE) continue;
       double acc[SW];
-#pragma unroll
-      for (int j = 0; j < SW; ++j) {
+      loadWiden<SW>(acc, static_cast<const T*>(F), [&](int j) -> int64_t {
         const int col = c0 + j;
-        acc[j] = (j < sw) ? static_cast<double>(i >= col ? Fat(i, col) : Fat(col, i)) : 0.0;
-      }
+        if (j >= sw) return -1;
+        return i >= col ? i + static_cast<int64_t>(col) * m : col + static_cast<int64_t>(i) * m;
+      });
       constexpr int U = 8;
       int q = 0;
       for (; q + U <= p; q += U) {
@@ -1264,20 +1298,26 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_mw(const int* list, int
       const int i = r0 + tid + jj * NT;
       if (i < k || i >= nE) continue;
       double acc[SW];
-#pragma unroll
-      for (int j = 0; j < SW; ++j) {
+      loadWiden<SW>(acc, static_cast<const T*>(F), [&](int j) -> int64_t {
         const int col = c0 + j;
-        acc[j] = (j < sw) ? static_cast<double>(i >= col ? Fat(i, col) : Fat(col, i)) : 0.0;
-      }
+        if (j >= sw) return -1;
+        return i >= col ? i + static_cast<int64_t>(col) * m : col + static_cast<int64_t>(i) * m;
+      });
       // the panel's own rows were stored device-coherent: read them the same way
       const bool panelRow = i < k0 + kPanel + 2;
       constexpr int U = 8;
       int q = 0;
       for (; q + U <= p; q += U) {
         double lv[U];
+        if (panelRow) {
+          T raw[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-          lv[u] = static_cast<double>(panelRow ? loadCoherent(&Fat(i, k0 + q + u)) : Fat(i, k0 + q + u));
+          for (int u = 0; u < U; ++u) raw[u] = loadCoherent(&Fat(i, k0 + q + u));
+#pragma unroll
+          for (int u = 0; u < U; ++u) lv[u] = static_cast<double>(raw[u]);
+        } else {
+          loadWiden<U>(lv, static_cast<const T*>(F), [&](int u) -> int64_t { return i + static_cast<int64_t>(k0 + q + u) * m; });
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
