@@ -639,7 +639,7 @@ int symla_dense_factor(int n, double* A, int lda, int n_eligible, const symla_de
     dpert.alloc(1);
     dom.alloc(1);
     drm.alloc(1);
-    dws.alloc(static_cast<size_t>(panelWorkspace(nE)));
+    dws.alloc(static_cast<size_t>(panelWorkspace(n, nE, !(opts && opts->exact_panel))));
     dF.upload(std::vector<void*>{dA.p}, st);
     upI(dlist, {0});
     const int* signPtr = nullptr;

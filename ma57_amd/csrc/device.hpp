@@ -109,15 +109,25 @@ __host__ __device__ inline bool specFront(const FactorParams& p, int nE) { retur
 // column at once; the prefix that passes is kept and the exact panel kernel
 // continues from the first column that fails.  Its per-front record lives in
 // the front's workspace after the two column buffers (2 * nE doubles):
-//   colmax[2][kPanel]  max |c_iq| over the eligible rows below the block
-//                      (bits of non-negative doubles; double-buffered by
-//                      panel iteration parity)
+//   colmax[2][kPanel]  max |c_iq| over the eligible rows below the block,
+//                      rounded up to float (bits of non-negative doubles;
+//                      double-buffered by panel iteration parity)
+//   emax[kPanel]       the same maxima, exact
+//   nmax[kPanel]       max |c_iq| over the rows [nE, m), exact
 //   cdiag[kPanel]      a_qq at pivot q (before any static perturbation)
-//   dval[kPanel]       the pivot d_q
+//   dval[kPanel]       the pivot d_q (1 past the block)
 //   bmax[kPanel]       max |c_iq| over the block rows below q
+//   rdiv[kPanel]       1 / d_q (1 past the block)
 //   lblk[kPanel * kPanel]  multipliers of the block, l_iq at [i * kPanel + q]
-constexpr int kSpecRec = 5 * kPanel + kPanel * kPanel;
+//   wmat[kPanel * kPanel]  d_j l_qj at [q * kPanel + j] (j < q): the row
+//                      substitution's coefficients
+// followed, for fronts that take the step, by the backup of the raw panel
+// values of the rows below the block (m x kPanel front elements, row i of
+// panel column q at i + q m), from which a failed test restores them.
+constexpr int kSpecRec = 8 * kPanel + 2 * kPanel * kPanel;
 constexpr int kSpecRows = 256;  // rows below the block per workgroup
-__host__ __device__ inline int64_t panelWorkspace(int nE) { return 2 * static_cast<int64_t>(nE) + kSpecRec; }
+__host__ __device__ inline int64_t panelWorkspace(int m, int nE, bool spec) {
+  return 2 * static_cast<int64_t>(nE) + kSpecRec + (spec ? static_cast<int64_t>(m) * kPanel : 0);
+}
 
 }  // namespace symla_hip

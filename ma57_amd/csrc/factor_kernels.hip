@@ -1696,29 +1696,39 @@ __device__ inline double pivotRecip(double d) {
 
 struct SpecRec {
   unsigned long long* colmax;  // [2][kPanel]
+  unsigned long long* emax;
+  unsigned long long* nmax;
   double* cdiag;
   double* dval;
   double* bmax;
+  double* rdiv;
   double* lblk;
+  double* wmat;
+  double* backup;
 };
 __device__ inline SpecRec specRec(double* ws, int nE) {
   double* r = ws + 2 * static_cast<int64_t>(nE);
   SpecRec x;
   x.colmax = reinterpret_cast<unsigned long long*>(r);
-  x.cdiag = r + 2 * kPanel;
-  x.dval = r + 3 * kPanel;
-  x.bmax = r + 4 * kPanel;
-  x.lblk = r + 5 * kPanel;
+  x.emax = reinterpret_cast<unsigned long long*>(r + 2 * kPanel);
+  x.nmax = reinterpret_cast<unsigned long long*>(r + 3 * kPanel);
+  x.cdiag = r + 4 * kPanel;
+  x.dval = r + 5 * kPanel;
+  x.bmax = r + 6 * kPanel;
+  x.rdiv = r + 7 * kPanel;
+  x.lblk = r + 8 * kPanel;
+  x.wmat = r + 8 * kPanel + kPanel * kPanel;
+  x.backup = r + kSpecRec;
   return x;
 }
 
 // One row below the diagonal block: x holds its raw values in the panel
 // columns; for q < lim in order, c_q = x_q - sum_{j<q} l_j W[q][j] with
 // W[q][j] = d_j l_qj, then x[q] = l_q = T(c_q r_q), r_q = 1 / d_q (rdv: the
-// pivots' reciprocals, so the chain over q carries no division).  onC(q, c_q) sees every
-// current value (the same arithmetic in both passes, so both see the same c).
+// pivots' reciprocals, so the chain over q carries no division).  onC(q, c_q)
+// sees every current value.
 template <typename T, typename OnC>
-__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1], const double* rdv, int lim,
+__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel], const double* rdv, int lim,
                                OnC onC) {
 #pragma unroll
   for (int q = 0; q < kPanel; ++q) {
@@ -1738,37 +1748,34 @@ __device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel + 1
   }
 }
 
-// Pass 1: every workgroup of a front factors the diagonal block (pw x pw,
-// pw = min(kPanel, nE - k0)) in registers, 1x1 pivots in order, and takes
-// kSpecRows eligible rows below it; the column maxima of the rows go to
-// colmax[parity] with atomics.  Workgroup 0 also stores the block's
-// multipliers, pivots, a_qq and in-block column maxima in the record.
+// Smallest float at or above v (v >= 0): a bound that is never below the value.
+__device__ inline double roundUpToFloat(double v) {
+  float f = static_cast<float>(v);
+  if (static_cast<double>(f) < v) f = nextafterf(f, INFINITY);
+  return static_cast<double>(f);
+}
+
+// Pass 0: one workgroup per front factors the diagonal block (pw x pw,
+// pw = min(kPanel, nE - k0)) in registers, 1x1 pivots in order, and writes
+// the record: multipliers, pivots and their reciprocals, a_qq, the in-block
+// column maxima and W = (d_j l_qj) for the row passes.
 template <typename T>
-__global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count, const int* wgStart, int parity,
-                                                     DevFronts fd, FactorParams prm, double* wsBase,
-                                                     const int64_t* wsOff) {
+__global__ __launch_bounds__(256) void k_spec_block(const int* list, int count, DevFronts fd, FactorParams prm,
+                                                    double* wsBase, const int64_t* wsOff) {
   constexpr int P = kPanel;
-  __shared__ double W[P][P + 1];
   __shared__ double cvec[2][P];
   __shared__ double dv[P], rdv[P];
   __shared__ int sgn[P];
   __shared__ double red[4][P];
-  __shared__ float cb[16][kSpecRows + 1];  // |c| of 16 columns x the workgroup's rows, rounded up
-  __shared__ float part[4][P];
-  if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
-  const int f = findSegment(wgStart, count, blockIdx.x);
-  const int s = list[f];
-  const int w = blockIdx.x - wgStart[f];
+  if (static_cast<int>(blockIdx.x) >= count) return;
+  const int s = list[blockIdx.x];
   const int nE = fd.nE[s];
   if (!specFront(prm, nE) || fd.stopped[s]) return;
   const int m = fd.m[s];
   const int k0 = fd.kcur[s];
   if (k0 >= nE) return;
   const int pw = min(P, nE - k0);
-  const int rBeg = k0 + pw + w * kSpecRows;
-  if (w > 0 && rBeg >= nE) return;
   const T* F = static_cast<const T*>(fd.F[s]);
-  auto Fat = [&](int i, int j) -> double { return static_cast<double>(F[i + static_cast<size_t>(j) * m]); };
   const int64_t io = fd.idxOff[s];
   const SpecRec rec = specRec(wsBase + wsOff[s], nE);
   const int tid = threadIdx.x;
@@ -1789,16 +1796,20 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   __syncthreads();
   // Right-looking, one column per step: a_ij -= c_iq l_jq (c_iq = d_q l_iq
   // up to the rounding of the stored multiplier), l_jq = T(c_jq (1 / d_q)).
+  // Column q goes through LDS whole (its rows above q hold zeros); every lane
+  // reads its 16 rows unconditionally and keeps the update only where it
+  // applies, so the reads of a step are in flight together.
   for (int q = 0; q < pw; ++q) {
     double* cv = cvec[q & 1];
     if (j == q) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = ib + 4 * r;
-        if (i >= q && i < pw) cv[i] = a[r];
-      }
+      for (int r = 0; r < 16; ++r) cv[ib + 4 * r] = a[r];
     }
     __syncthreads();
+    double ci[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) ci[r] = cv[ib + 4 * r];
+    const double cj = cv[j];
     double d = cv[q];
     if (stat) {
       double applied;
@@ -1809,90 +1820,156 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
       dv[q] = d;
       rdv[q] = rd;
     }
-    if (j > q && j < pw) {
-      const double lj = static_cast<double>(static_cast<T>(cv[j] * rd));
+    const double lj = static_cast<double>(static_cast<T>(cj * rd));
+    const bool live = j > q && j < pw;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = ib + 4 * r;
-        if (i >= j && i < pw) a[r] = fma(-cv[i], lj, a[r]);
-      }
+    for (int r = 0; r < 16; ++r) {
+      const int i = ib + 4 * r;
+      const double u = fma(-ci[r], lj, a[r]);
+      a[r] = (live && i >= j && i < pw) ? u : a[r];
     }
   }
-  __syncthreads();
   // columns past the block: a neutral pivot for the branch-free row solve
   for (int q = pw + tid; q < P; q += 256) {
     dv[q] = 1.0;
     rdv[q] = 1.0;
   }
+  __syncthreads();
   // a now holds c_ij (current value at pivot j) for i > j and a_jj^(j) on the diagonal
   double bm = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int i = ib + 4 * r;
-    if (i > j && (i >= pw || j >= pw)) W[i][j] = 0.0;
     if (j < pw && i < pw) {
       if (i > j) {
         const double l = static_cast<double>(static_cast<T>(a[r] * rdv[j]));
-        W[i][j] = dv[j] * l;
+        rec.wmat[i * P + j] = dv[j] * l;
+        rec.lblk[i * P + j] = l;
         bm = fmax(bm, fabs(a[r]));
-        if (w == 0) rec.lblk[i * P + j] = l;
-      } else if (i == j && w == 0) {
+      } else if (i == j) {
         rec.cdiag[j] = a[r];
       }
     }
   }
   red[ib][j] = bm;
   __syncthreads();
-  if (w == 0 && tid < pw) {
+  if (tid < P) {
     rec.dval[tid] = dv[tid];
+    rec.rdiv[tid] = rdv[tid];
     rec.bmax[tid] = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
-  }
-  // eligible rows below the block: column maxima of the current values
-  const int i = rBeg + tid;
-  const bool valid = i < nE;
-  double x[P];
-  loadWiden<P>(x, F, [&](int q) -> int64_t { return (valid && q < pw) ? i + static_cast<int64_t>(k0 + q) * m : -1; });
-  // Column maxima through LDS, 16 columns at a time: |c| rounded up to
-  // float (a bound that is never below the value, so a borderline column can
-  // only fail the test here and be decided by the exact kernel).
-  specRow<T>(x, W, rdv, pw, [&](int q, double c) {
-    const double ac = fabs(c);
-    float f = static_cast<float>(ac);
-    if (static_cast<double>(f) < ac) f = nextafterf(f, INFINITY);
-    cb[q & 15][tid] = f;
-    if ((q & 15) == 15 || q == pw - 1) {
-      __syncthreads();
-      const int col = tid & 15, seg = tid >> 4;  // 16 rows per thread
-      float v = 0.0f;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v = fmaxf(v, cb[col][seg * 16 + r]);
-      v = fmaxf(v, __shfl_xor(v, 16, 64));
-      v = fmaxf(v, __shfl_xor(v, 32, 64));
-      if ((tid & 63) < 16) part[tid >> 6][(q & ~15) + col] = v;
-      __syncthreads();
-    }
-  });
-  if (tid < pw) {
-    const double v = fmax(fmax(part[0][tid], part[1][tid]), fmax(part[2][tid], part[3][tid]));
-    if (v > 0.0) atomicMaxNonNeg(rec.colmax + parity * P + tid, v);
   }
 }
 
-// Pass 2: every workgroup finds the first column q* whose test fails (all
-// of them when static), writes the multipliers of its rows for the columns
-// before q* (and of the rows [nE, m) for the whole panel when q* = pw), and
-// workgroup 0 commits the block's columns before q*, the pivots, inertia and
-// perturbation counts, sets kcur = k0 + q*, pcount = q*, and clears the
-// other colmax buffer for the next panel.
+// Pass 1: every workgroup takes kSpecRows rows below the block of a front
+// (eligible rows [k0 + pw, nE) and rows [nE, m) alike), saves their raw panel
+// values in the record's backup, forms their multipliers for the whole block
+// and stores them in the front.  The column maxima of the current values go
+// to the record: over the eligible rows rounded up to float (colmax[parity],
+// the threshold test) and exact (emax), over the rows [nE, m) exact (nmax).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count, const int* wgStart, int parity,
+                                                     DevFronts fd, FactorParams prm, double* wsBase,
+                                                     const int64_t* wsOff) {
+  constexpr int P = kPanel;
+  __shared__ __align__(16) double W[P][P];  // whole 16-byte rows: broadcast reads of two coefficients at once
+  __shared__ double rdv[P];
+  __shared__ double cm[16][kSpecRows + 1];  // |c| (eligible rows) or -|c| (rows [nE, m)) of 16 columns
+  __shared__ double partE[4][P], partN[4][P];
+  if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
+  const int f = findSegment(wgStart, count, blockIdx.x);
+  const int s = list[f];
+  const int w = blockIdx.x - wgStart[f];
+  const int nE = fd.nE[s];
+  if (!specFront(prm, nE) || fd.stopped[s]) return;
+  const int m = fd.m[s];
+  const int k0 = fd.kcur[s];
+  if (k0 >= nE) return;
+  const int pw = min(P, nE - k0);
+  const int rBeg = k0 + pw + w * kSpecRows;
+  if (rBeg >= m) return;
+  T* F = static_cast<T*>(fd.F[s]);
+  const SpecRec rec = specRec(wsBase + wsOff[s], nE);
+  const int tid = threadIdx.x;
+  // the block's substitution coefficients and reciprocal pivots, loaded
+  // together (entries outside the block's lower triangle are read and dropped)
+  {
+    double wv[P * P / 256];
+#pragma unroll
+    for (int u = 0; u < P * P / 256; ++u) wv[u] = rec.wmat[tid + 256 * u];
+#pragma unroll
+    for (int u = 0; u < P * P / 256; ++u) {
+      const int e = tid + 256 * u, q = e / P, j = e % P;
+      W[q][j] = (j < q && q < pw) ? wv[u] : 0.0;
+    }
+    if (tid < P) rdv[tid] = rec.rdiv[tid];
+  }
+  const int i = rBeg + tid;
+  const bool valid = i < m;
+  const bool elig = i < nE;
+  double x[P];
+  loadWiden<P>(x, static_cast<const T*>(F),
+               [&](int q) -> int64_t { return (valid && q < pw) ? i + static_cast<int64_t>(k0 + q) * m : -1; });
+  T* bk = reinterpret_cast<T*>(rec.backup);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < pw) bk[i + static_cast<int64_t>(q) * m] = static_cast<T>(x[q]);
+  }
+  __syncthreads();
+  // column maxima through LDS, 16 columns at a time
+  specRow<T>(x, W, rdv, pw, [&](int q, double c) {
+    const double ac = fabs(c);
+    cm[q & 15][tid] = elig ? ac : -ac;
+    if ((q & 15) == 15 || q == pw - 1) {
+      __syncthreads();
+      const int col = tid & 15, seg = tid >> 4;  // 16 rows per thread
+      double ve = 0.0, vn = 0.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double v = cm[col][seg * 16 + r];
+        ve = fmax(ve, v);
+        vn = fmax(vn, -v);
+      }
+      ve = fmax(ve, __shfl_xor(ve, 16, 64));
+      ve = fmax(ve, __shfl_xor(ve, 32, 64));
+      vn = fmax(vn, __shfl_xor(vn, 16, 64));
+      vn = fmax(vn, __shfl_xor(vn, 32, 64));
+      if ((tid & 63) < 16) {
+        partE[tid >> 6][(q & ~15) + col] = ve;
+        partN[tid >> 6][(q & ~15) + col] = vn;
+      }
+      __syncthreads();
+    }
+  });
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < pw) F[i + static_cast<int64_t>(k0 + q) * m] = static_cast<T>(x[q]);
+  }
+  if (tid < pw) {
+    const double ve = fmax(fmax(partE[0][tid], partE[1][tid]), fmax(partE[2][tid], partE[3][tid]));
+    const double vn = fmax(fmax(partN[0][tid], partN[1][tid]), fmax(partN[2][tid], partN[3][tid]));
+    if (ve > 0.0) {
+      atomicMaxNonNeg(rec.colmax + parity * P + tid, roundUpToFloat(ve));
+      atomicMaxNonNeg(rec.emax + tid, ve);
+    }
+    if (vn > 0.0) atomicMaxNonNeg(rec.nmax + tid, vn);
+  }
+}
+
+// Pass 2: every workgroup finds the first column q* whose test fails (all of
+// them pass when static); when q* < pw it restores, for its rows, the raw
+// values the exact kernel continues from (columns [q*, pw) of the eligible
+// rows, the whole block of the rows [nE, m)).  Workgroup 0 commits the block's
+// columns before q*, the pivots, inertia, perturbation counts and the running
+// maximum, sets kcur = k0 + q*, pcount = q*, and clears the other colmax
+// buffer and the exact maxima for the next panel.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, const int* wgStart, int parity,
                                                     DevFronts fd, FactorParams prm, double* wsBase,
                                                     const int64_t* wsOff) {
   constexpr int P = kPanel;
-  __shared__ double W[P][P + 1];
-  __shared__ double dv[P], rdv[P];
   __shared__ int qsh;
-  __shared__ double redV[4];
   if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
   const int f = findSegment(wgStart, count, blockIdx.x);
   const int s = list[f];
@@ -1911,6 +1988,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   }
   const int pw = min(P, nE - k0);
   const int rBeg = k0 + pw + w * kSpecRows;
+  if (w > 0 && rBeg >= m) return;
   T* F = static_cast<T*>(fd.F[s]);
   auto Fat = [&](int i, int j) -> T& { return F[i + static_cast<size_t>(j) * m]; };
   const int64_t io = fd.idxOff[s];
@@ -1919,14 +1997,6 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   double origMax = bitsToDouble(fd.origMaxBits[s]);
   if (!stat && origMax < prm.zeroTol) origMax = 1.0;
   if (stat && origMax < prm.staticAbsFloor) origMax = 1.0;
-  for (int q = tid; q < P; q += 256) {
-    dv[q] = (q < pw) ? rec.dval[q] : 1.0;
-    rdv[q] = pivotRecip(dv[q]);
-  }
-  for (int e = tid; e < P * P; e += 256) {
-    const int q = e / P, jj = e % P;
-    W[q][jj] = (jj < q && q < pw) ? rec.dval[jj] * rec.lblk[q * P + jj] : 0.0;
-  }
   // first failing column: the first wave tests one column per lane
   if (tid < 64) {
     const int q = tid;
@@ -1942,87 +2012,78 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   }
   __syncthreads();
   const int qs = qsh;
-  double tmax = 0.0;
-  const int i = rBeg + tid;
-  const int lim = (i < nE) ? qs : ((i < m && qs == pw) ? pw : 0);
-  if (lim > 0) {
-    double x[P];
-    loadWiden<P>(x, static_cast<const T*>(F), [&](int q) -> int64_t {
-      return (q < lim) ? i + static_cast<int64_t>(k0 + q) * m : -1;
-    });
-    specRow<T>(x, W, rdv, lim, [&](int q, double c) {
-      if (q < lim) tmax = fmax(tmax, fabs(c));
-    });
-#pragma unroll
-    for (int q = 0; q < P; ++q)
-      if (q < lim) Fat(i, k0 + q) = static_cast<T>(x[q]);
-  }
-  if (w == 0) {
-    for (int e = tid; e < pw * pw; e += 256) {
-      const int q = e / pw, r = e % pw;  // column q, row r of the block
-      if (q < qs && r >= q) Fat(k0 + r, k0 + q) = static_cast<T>(r == q ? dv[q] : rec.lblk[r * P + q]);
+  if (qs < pw) {
+    const int i = rBeg + tid;
+    if (i < m) {
+      const T* bk = reinterpret_cast<const T*>(rec.backup);
+      for (int q = (i < nE) ? qs : 0; q < pw; ++q) Fat(i, k0 + q) = bk[i + static_cast<int64_t>(q) * m];
     }
-    for (int q = tid; q < qs; q += 256) {
-      fd.dDiag[io + k0 + q] = dv[q];
+  }
+  if (w != 0) return;
+  double tmax = 0.0;
+  for (int e = tid; e < pw * pw; e += 256) {
+    const int q = e / pw, r = e % pw;  // column q, row r of the block
+    if (q < qs && r >= q) Fat(k0 + r, k0 + q) = static_cast<T>(r == q ? rec.dval[q] : rec.lblk[r * P + q]);
+  }
+  if (tid < P) rec.colmax[(parity ^ 1) * P + tid] = 0ull;
+  if (tid < 64) {
+    // pivots, inertia, perturbation and the maxima of the kept columns: one column per lane
+    const int q = tid;
+    const double ev = bitsToDouble(rec.emax[q]), nv = bitsToDouble(rec.nmax[q]);
+    rec.emax[q] = 0ull;
+    rec.nmax[q] = 0ull;
+    int npos = 0, nneg = 0, nzero = 0, npert = 0;
+    double pert = 0.0;
+    if (q < qs) {
+      const double d = rec.dval[q];
+      fd.dDiag[io + k0 + q] = d;
       fd.dOff[io + k0 + q] = 0.0;
       fd.pkind[io + k0 + q] = 1;
-      tmax = fmax(tmax, fmax(fabs(dv[q]), rec.bmax[q]));
-    }
-    if (tid < P) rec.colmax[(parity ^ 1) * P + tid] = 0ull;
-    if (tid < 64) {
-      // inertia and perturbation of the kept pivots: one column per lane
-      const int q = tid;
-      int npos = 0, nneg = 0, nzero = 0, npert = 0;
-      double pert = 0.0;
-      if (q < qs) {
-        countInertia(dv[q], npos, nneg, nzero);
-        if (stat) {
-          const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
-          const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
-          const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k0 + q]] : 0;
-          double applied;
-          (void)staticPivot(rec.cdiag[q], sign, delta, sfloor, applied);
-          if (applied != 0.0) {
-            pert = fabs(applied);
-            npert = 1;
-          }
+      tmax = fmax(fmax(fabs(d), rec.bmax[q]), ev);
+      countInertia(d, npos, nneg, nzero);
+      if (stat) {
+        const double delta = prm.staticDelta > 0.0 ? prm.staticDelta : sqrt(DBL_EPSILON) * origMax;
+        const double sfloor = fmax(prm.staticAbsFloor, prm.staticRelFloor * origMax);
+        const int sign = prm.expectedSign ? prm.expectedSign[fd.frontIdx[io + k0 + q]] : 0;
+        double applied;
+        (void)staticPivot(rec.cdiag[q], sign, delta, sfloor, applied);
+        if (applied != 0.0) {
+          pert = fabs(applied);
+          npert = 1;
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        npos += __shfl_xor(npos, off, 64);
-        nneg += __shfl_xor(nneg, off, 64);
-        nzero += __shfl_xor(nzero, off, 64);
-        npert += __shfl_xor(npert, off, 64);
-        pert += __shfl_xor(pert, off, 64);
-      }
-      if (tid == 0) {
-        fd.inertia[3 * s] += npos;
-        fd.inertia[3 * s + 1] += nneg;
-        fd.inertia[3 * s + 2] += nzero;
-        fd.pert[s] += pert;
-        fd.npert[s] += npert;
-        fd.kcur[s] = k0 + qs;
-        fd.pcount[s] = qs;
-        fd.cbDone[s] = (qs == pw) ? 1 : 0;
-      }
     }
-  }
-  for (int off = 32; off > 0; off >>= 1) tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
-  if ((tid & 63) == 0) redV[tid >> 6] = tmax;
-  __syncthreads();
-  if (tid == 0) {
-    const double v = fmax(fmax(redV[0], redV[1]), fmax(redV[2], redV[3]));
-    atomicMaxNonNeg(fd.runMaxBits + s, fmax(v, origMax));
+    if (qs == pw && q < pw) tmax = fmax(tmax, nv);
+    for (int off = 32; off > 0; off >>= 1) {
+      npos += __shfl_xor(npos, off, 64);
+      nneg += __shfl_xor(nneg, off, 64);
+      nzero += __shfl_xor(nzero, off, 64);
+      npert += __shfl_xor(npert, off, 64);
+      pert += __shfl_xor(pert, off, 64);
+      tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+    }
+    if (tid == 0) {
+      fd.inertia[3 * s] += npos;
+      fd.inertia[3 * s + 1] += nneg;
+      fd.inertia[3 * s + 2] += nzero;
+      fd.pert[s] += pert;
+      fd.npert[s] += npert;
+      fd.kcur[s] = k0 + qs;
+      fd.pcount[s] = qs;
+      fd.cbDone[s] = (qs == pw) ? 1 : 0;
+      atomicMaxNonNeg(fd.runMaxBits + s, fmax(tmax, origMax));
+    }
   }
 }
 
-// Clears both colmax buffers of the speculative records of a level's fronts.
+// Clears the colmax buffers and exact maxima of the speculative records of a
+// level's fronts.
 __global__ void k_spec_reset(const int* list, int count, DevFronts fd, double* wsBase, const int64_t* wsOff) {
   const int f = blockIdx.x;
   if (f >= count) return;
   const int s = list[f];
   const SpecRec rec = specRec(wsBase + wsOff[s], fd.nE[s]);
-  for (int t = threadIdx.x; t < 2 * kPanel; t += blockDim.x) rec.colmax[t] = 0ull;
+  for (int t = threadIdx.x; t < 4 * kPanel; t += blockDim.x) rec.colmax[t] = 0ull;
 }
 
 // Multipliers of the rows that are not fully summed ([nE, m)) for the last
@@ -2630,10 +2691,14 @@ void launchSpecPanel(const int* list, int count, const int* wgStart, int blocks,
                      const FactorParams& prm, double* ws, const int64_t* wsOff, hipStream_t st) {
   if (count <= 0 || blocks <= 0) return;
   if (fd.fp32) {
+    k_spec_block<float><<<count, 256, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    hipCheck(hipGetLastError(), "k_spec_block");
     k_spec_factor<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_factor");
     k_spec_apply<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
   } else {
+    k_spec_block<double><<<count, 256, 0, st>>>(list, count, fd, prm, ws, wsOff);
+    hipCheck(hipGetLastError(), "k_spec_block");
     k_spec_factor<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_factor");
     k_spec_apply<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);

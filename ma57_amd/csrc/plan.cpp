@@ -415,7 +415,7 @@ void Plan::buildSchedule() {
     int64_t off = 0;
     for (int s : ls.large) {
       hwsOff_[s] = off;
-      off += panelWorkspace(hnE_[s]);
+      off += panelWorkspace(hm_[s], hnE_[s], tune_.specPanel && hnE_[s] > tune_.specMinRows);
     }
     wsNeed = std::max(wsNeed, off);
   }
@@ -856,7 +856,7 @@ void Plan::runDynamic(const double* aValsDev, const FactorParams& prm) {
       int64_t off = 0;
       for (int s : large) {
         wsOffH[s] = off;
-        off += panelWorkspace(hnE_[s]);
+        off += panelWorkspace(hm_[s], hnE_[s], tune_.specPanel && hnE_[s] > tune_.specMinRows);
       }
       dws_.ensure(static_cast<size_t>(std::max<int64_t>(off, 2)));
       DBuf<int> dl, dr, dt, dtp, dmw, dsp;
