@@ -307,7 +307,8 @@ typedef struct {
   int small_front_max;
   int assemble_tile;
   int speculative_panel;    /* 1 (default): speculative blocked step on large fronts; 0: exact steps only */
-  int speculative_min_rows; /* fronts with more eligible rows than this take it (default 2048) */
+  int speculative_min_rows; /* fronts with more eligible rows than this take it (default 2048; 0 when
+                               a front has more than 4096 eligible rows) */
   int deferred_cb_min_rows; /* large fronts with at least 256 eligible columns and a contribution block of at
                                least this many rows update that block once, after their last panel, as
                                library SYRKs (default 2048; -1: every block panel by panel) */

@@ -85,6 +85,14 @@ Plan::Plan(const SymbolicResult& sym, const int64_t* colptr, const int* rowidx) 
     if (aRowidx_[t] < 0 || aRowidx_[t] >= n_) throw std::invalid_argument("symla: row index out of range");
 
   buildStaticLayout();
+  // Default speculative threshold from the largest front: with very large
+  // fronts the mid-size ones carry long contribution blocks too, whose rows
+  // the step maps in the same pass (measured faster on every panel front).
+  {
+    int maxNE = 0;
+    for (int s = 0; s < numSN_; ++s) maxNE = std::max(maxNE, hnE_[s]);
+    if (maxNE > kSpecAllFrontsNE) tune_.specMinRows = 0;
+  }
   // per-front state
   dnF_.alloc(numSN_);
   dinertia_.alloc(3 * static_cast<size_t>(numSN_));

@@ -106,6 +106,9 @@ struct RefineResult {
 constexpr int kMwThreshold = 4096;
 // Fronts with more eligible rows than this take the speculative panel step by default.
 constexpr int kSpecMinRowsDefault = 2048;
+// plans with a front of more eligible rows than this default to the
+// speculative step on every panel front
+constexpr int kSpecAllFrontsNE = 4096;
 constexpr int kMwMaxBlocks = 240;  // cooperative workgroups per launch (all resident at once)
 struct LargeIter {
   int64_t listOff = 0;  // all fronts of the iteration (rows, trailing update)
