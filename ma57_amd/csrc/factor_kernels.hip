@@ -1763,7 +1763,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_spec_block(const int* list, int count, DevFronts fd, FactorParams prm,
                                                     double* wsBase, const int64_t* wsOff) {
   constexpr int P = kPanel;
-  __shared__ double cvec[2][P];
+  __shared__ double cvec[2][2][P];  // [step parity][column q, q + 1]
   __shared__ double dv[P], rdv[P];
   __shared__ int sgn[P];
   __shared__ double red[4][P];
@@ -1794,39 +1794,73 @@ __global__ __launch_bounds__(256) void k_spec_block(const int* list, int count, 
     return (i < pw && j < pw && i >= j) ? (k0 + i) + static_cast<int64_t>(k0 + j) * m : -1;
   });
   __syncthreads();
-  // Right-looking, one column per step: a_ij -= c_iq l_jq (c_iq = d_q l_iq
-  // up to the rounding of the stored multiplier), l_jq = T(c_jq (1 / d_q)).
-  // Column q goes through LDS whole (its rows above q hold zeros); every lane
-  // reads its 16 rows unconditionally and keeps the update only where it
-  // applies, so the reads of a step are in flight together.
-  for (int q = 0; q < pw; ++q) {
-    double* cv = cvec[q & 1];
+  // Right-looking: a_ij -= c_iq l_jq (c_iq = d_q l_iq up to the rounding of
+  // the stored multiplier), l_jq = T(c_jq (1 / d_q)), two pivots per step.
+  // Columns q and q + 1 go through LDS whole after one barrier; every lane
+  // applies pivot q to its copy of column q + 1 (the same operations lane
+  // q + 1 applies to its own) before taking pivot q + 1, so the result is that
+  // of one pivot per step.  A column's lane updates all its rows: entries
+  // above the diagonal or past the block take values that nothing reads, and
+  // the lower-triangle entries combine lower-triangle entries only.
+  for (int q = 0; q < pw; q += 2) {
+    double* cv0 = cvec[(q >> 1) & 1][0];
+    double* cv1 = cvec[(q >> 1) & 1][1];
+    const bool two = q + 1 < pw;
     if (j == q) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) cv[ib + 4 * r] = a[r];
+      for (int r = 0; r < 16; ++r) cv0[ib + 4 * r] = a[r];
+    }
+    if (j == q + 1) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) cv1[ib + 4 * r] = a[r];
     }
     __syncthreads();
-    double ci[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) ci[r] = cv[ib + 4 * r];
-    const double cj = cv[j];
-    double d = cv[q];
-    if (stat) {
-      double applied;
-      d = staticPivot(d, sgn[q], delta, sfloor, applied);
-    }
-    const double rd = pivotRecip(d);
-    if (tid == 0) {
-      dv[q] = d;
-      rdv[q] = rd;
-    }
-    const double lj = static_cast<double>(static_cast<T>(cj * rd));
-    const bool live = j > q && j < pw;
+    double ci0[16], ci1[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int i = ib + 4 * r;
-      const double u = fma(-ci[r], lj, a[r]);
-      a[r] = (live && i >= j && i < pw) ? u : a[r];
+      ci0[r] = cv0[ib + 4 * r];
+      ci1[r] = cv1[ib + 4 * r];
+    }
+    const double cj0 = cv0[j];
+    double cj1 = cv1[j];
+    double d0 = cv0[q];
+    const double b10 = cv0[q + 1];  // c_{q+1,q}
+    double d1 = cv1[q + 1];
+    if (stat) {
+      double applied;
+      d0 = staticPivot(d0, sgn[q], delta, sfloor, applied);
+    }
+    const double rd0 = pivotRecip(d0);
+    const double lj0 = static_cast<double>(static_cast<T>(cj0 * rd0));
+    if (j > q && j < pw) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) a[r] = fma(-ci0[r], lj0, a[r]);
+    }
+    if (tid == 0) {
+      dv[q] = d0;
+      rdv[q] = rd0;
+    }
+    if (two) {
+      // column q + 1 after pivot q
+      const double l10 = static_cast<double>(static_cast<T>(b10 * rd0));
+#pragma unroll
+      for (int r = 0; r < 16; ++r) ci1[r] = fma(-ci0[r], l10, ci1[r]);
+      cj1 = fma(-cj0, l10, cj1);
+      d1 = fma(-b10, l10, d1);
+      if (stat) {
+        double applied;
+        d1 = staticPivot(d1, sgn[q + 1], delta, sfloor, applied);
+      }
+      const double rd1 = pivotRecip(d1);
+      const double lj1 = static_cast<double>(static_cast<T>(cj1 * rd1));
+      if (j > q + 1 && j < pw) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a[r] = fma(-ci1[r], lj1, a[r]);
+      }
+      if (tid == 0) {
+        dv[q + 1] = d1;
+        rdv[q + 1] = rd1;
+      }
     }
   }
   // columns past the block: a neutral pivot for the branch-free row solve
