@@ -1726,25 +1726,48 @@ __device__ inline SpecRec specRec(double* ws, int nE) {
 // columns; for q < lim in order, c_q = x_q - sum_{j<q} l_j W[q][j] with
 // W[q][j] = d_j l_qj, then x[q] = l_q = T(c_q r_q), r_q = 1 / d_q (rdv: the
 // pivots' reciprocals, so the chain over q carries no division).  onC(q, c_q)
-// sees every current value.
-template <typename T, typename OnC>
-__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel], const double* rdv, int lim,
-                               OnC onC) {
+// sees every current value; onGroup(g) runs after columns 16 g .. 16 g + 15
+// (every thread of the workgroup reaches it).
+//
+// The coefficients are broadcast LDS reads, 16 at a time: a chunk's reads are
+// all issued before its products (the empty asm statements keep the compiler
+// from interleaving them), so their latencies overlap.
+template <typename T, typename OnC, typename OnGroup>
+__device__ inline void specRow(double (&x)[kPanel], const double (*W)[kPanel], const double* rdv, int lim, OnC onC,
+                               OnGroup onGroup) {
 #pragma unroll
-  for (int q = 0; q < kPanel; ++q) {
-    if (q < lim) {
-      double c0 = x[q], c1 = 0.0;
+  for (int g = 0; g < kPanel / 16; ++g) {
 #pragma unroll
-      for (int j = 0; j < q; ++j) {
-        if (j & 1)
-          c1 = fma(-x[j], W[q][j], c1);
-        else
-          c0 = fma(-x[j], W[q][j], c0);
+    for (int qq = 0; qq < 16; ++qq) {
+      const int q = 16 * g + qq;
+      if (q < lim) {
+        double c0 = x[q], c1 = 0.0;
+#pragma unroll
+        for (int j0 = 0; j0 < q; j0 += 16) {
+          double w[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (j0 + u < q) w[u] = W[q][j0 + u];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (j0 + u < q) asm volatile("" : "+v"(w[u]));
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int j = j0 + u;
+            if (j < q) {
+              if (j & 1)
+                c1 = fma(-x[j], w[u], c1);
+              else
+                c0 = fma(-x[j], w[u], c0);
+            }
+          }
+        }
+        const double c = c0 + c1;
+        onC(q, c);
+        x[q] = static_cast<double>(static_cast<T>(c * rdv[q]));
       }
-      const double c = c0 + c1;
-      onC(q, c);
-      x[q] = static_cast<double>(static_cast<T>(c * rdv[q]));
     }
+    if (16 * g < lim) onGroup(g);
   }
 }
 
@@ -1951,30 +1974,32 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   }
   __syncthreads();
   // column maxima through LDS, 16 columns at a time
-  specRow<T>(x, W, rdv, pw, [&](int q, double c) {
-    const double ac = fabs(c);
-    cm[q & 15][tid] = elig ? ac : -ac;
-    if ((q & 15) == 15 || q == pw - 1) {
-      __syncthreads();
-      const int col = tid & 15, seg = tid >> 4;  // 16 rows per thread
-      double ve = 0.0, vn = 0.0;
+  specRow<T>(
+      x, W, rdv, pw,
+      [&](int q, double c) {
+        const double ac = fabs(c);
+        cm[q & 15][tid] = elig ? ac : -ac;
+      },
+      [&](int g) {
+        __syncthreads();
+        const int col = tid & 15, seg = tid >> 4;  // 16 rows per thread
+        double ve = 0.0, vn = 0.0;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const double v = cm[col][seg * 16 + r];
-        ve = fmax(ve, v);
-        vn = fmax(vn, -v);
-      }
-      ve = fmax(ve, __shfl_xor(ve, 16, 64));
-      ve = fmax(ve, __shfl_xor(ve, 32, 64));
-      vn = fmax(vn, __shfl_xor(vn, 16, 64));
-      vn = fmax(vn, __shfl_xor(vn, 32, 64));
-      if ((tid & 63) < 16) {
-        partE[tid >> 6][(q & ~15) + col] = ve;
-        partN[tid >> 6][(q & ~15) + col] = vn;
-      }
-      __syncthreads();
-    }
-  });
+        for (int r = 0; r < 16; ++r) {
+          const double v = cm[col][seg * 16 + r];
+          ve = fmax(ve, v);
+          vn = fmax(vn, -v);
+        }
+        ve = fmax(ve, __shfl_xor(ve, 16, 64));
+        ve = fmax(ve, __shfl_xor(ve, 32, 64));
+        vn = fmax(vn, __shfl_xor(vn, 16, 64));
+        vn = fmax(vn, __shfl_xor(vn, 32, 64));
+        if ((tid & 63) < 16) {
+          partE[tid >> 6][16 * g + col] = ve;
+          partN[tid >> 6][16 * g + col] = vn;
+        }
+        __syncthreads();
+      });
   if (valid) {
 #pragma unroll
     for (int q = 0; q < P; ++q)
