@@ -1947,32 +1947,39 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   T* F = static_cast<T*>(fd.F[s]);
   const SpecRec rec = specRec(wsBase + wsOff[s], nE);
   const int tid = threadIdx.x;
-  // the block's substitution coefficients and reciprocal pivots, loaded
-  // together (entries outside the block's lower triangle are read and dropped)
-  {
-    double wv[P * P / 256];
-#pragma unroll
-    for (int u = 0; u < P * P / 256; ++u) wv[u] = rec.wmat[tid + 256 * u];
-#pragma unroll
-    for (int u = 0; u < P * P / 256; ++u) {
-      const int e = tid + 256 * u, q = e / P, j = e % P;
-      W[q][j] = (j < q && q < pw) ? wv[u] : 0.0;
-    }
-    if (tid < P) rdv[tid] = rec.rdiv[tid];
-  }
   const int i = rBeg + tid;
   const bool valid = i < m;
   const bool elig = i < nE;
-  double x[P];
-  loadWiden<P>(x, static_cast<const T*>(F),
-               [&](int q) -> int64_t { return (valid && q < pw) ? i + static_cast<int64_t>(k0 + q) * m : -1; });
-  T* bk = reinterpret_cast<T*>(rec.backup);
-  if (valid) {
+  // the block's substitution coefficients, the reciprocal pivots and the
+  // row's raw values all in flight together (coefficients outside the
+  // block's lower triangle and skipped row entries are read and dropped)
+  double wv[P * P / 256];
 #pragma unroll
-    for (int q = 0; q < P; ++q)
-      if (q < pw) bk[i + static_cast<int64_t>(q) * m] = static_cast<T>(x[q]);
+  for (int u = 0; u < P * P / 256; ++u) wv[u] = rec.wmat[tid + 256 * u];
+  const double rv = rec.rdiv[tid & (P - 1)];
+  T raw[P];
+  bool ok[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    ok[q] = valid && q < pw;
+    raw[q] = F[ok[q] ? i + static_cast<int64_t>(k0 + q) * m : 0];
   }
+#pragma unroll
+  for (int u = 0; u < P * P / 256; ++u) {
+    const int e = tid + 256 * u, q = e / P, j = e % P;
+    W[q][j] = (j < q && q < pw) ? wv[u] : 0.0;
+  }
+  if (tid < P) rdv[tid] = rv;
   __syncthreads();
+  double x[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) x[q] = ok[q] ? static_cast<double>(raw[q]) : 0.0;
+  // the raw values saved for the exact kernel (completed by the first
+  // column group's barrier, long before anything reads them)
+  T* bk = reinterpret_cast<T*>(rec.backup);
+#pragma unroll
+  for (int q = 0; q < P; ++q)
+    if (ok[q]) bk[i + static_cast<int64_t>(q) * m] = raw[q];
   // column maxima through LDS, 16 columns at a time
   specRow<T>(
       x, W, rdv, pw,
