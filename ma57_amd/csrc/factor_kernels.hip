@@ -2317,6 +2317,17 @@ struct SchurMfma<float> {
   static __device__ inline int col(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// A front pointer read from the front table is generic; generic (flat)
+// accesses count against the LDS wait counter too, so every LDS wait would
+// also wait for them.  Front storage is global memory: this view makes the
+// accesses global ones.
+template <typename T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ inline GlobalPtr<T> globalView(void* p) {
+  return (GlobalPtr<T>)(p);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const int* tileStart, const int* tileTp,
                                                DevFronts fd) {
@@ -2343,7 +2354,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
   // metadata is read again only when the walk enters another front.
   struct Tile {
     int f, s, p, m, k0, k1, Tp, cut, lo, hi, ti, tj;
-    T* F;
+    GlobalPtr<T> F;
   };
   auto tileOf = [&](int t, const Tile* prev) -> Tile {
     Tile x;
@@ -2358,7 +2369,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
       x.m = fd.m[x.s];
       x.k1 = fd.kcur[x.s];
       x.k0 = x.k1 - x.p;
-      x.F = static_cast<T*>(fd.F[x.s]);
+      x.F = globalView<T>(fd.F[x.s]);
       x.Tp = tileTp[x.f];
       x.cut = fd.schurCut[x.s];
     }
