@@ -2411,8 +2411,24 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
         }
   };
 
-  int t = blockIdx.x;
-  if (t >= total) return;
+  // XCD-aware walk: workgroups go to the eight XCDs round-robin (blockIdx
+  // modulo 8), so each XCD's workgroups take a contiguous eighth of the tiles
+  // and neighbouring tiles, which share operand rows, meet in one L2.
+  // (Any other placement only costs L2 hits.)
+  constexpr int kXcd = 8;
+  int t, stride, tEnd;
+  if (static_cast<int>(gridDim.x) >= kXcd) {
+    const int xcd = blockIdx.x % kXcd;
+    const int chunk = (total + kXcd - 1) / kXcd;
+    t = xcd * chunk + static_cast<int>(blockIdx.x) / kXcd;
+    stride = (static_cast<int>(gridDim.x) - xcd + kXcd - 1) / kXcd;  // workgroups with this residue
+    tEnd = min(total, (xcd + 1) * chunk);
+  } else {
+    t = blockIdx.x;
+    stride = gridDim.x;
+    tEnd = total;
+  }
+  if (t >= tEnd) return;
   Tile cur = tileOf(t, nullptr);
   int pivFront = -1;  // front whose pivots are in pdd / pdo / pkd
   // running max |F| of the updated entries: per lane over the tiles of one
@@ -2425,8 +2441,8 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
     if (lane == 0) atomicMaxNonNeg(fd.runMaxBits + mxFront, wmx);
   };
   if (live(cur)) loadTile(cur);
-  while (t < total) {
-    const int tn = t + gridDim.x;
+  while (t < tEnd) {
+    const int tn = t + stride;
     if (live(cur)) {
       if (cur.s != pivFront) {
         for (int q = tid; q < cur.p; q += blockDim.x) {
@@ -2481,7 +2497,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) cn[bi][bj][rr] = cv[bi][bj][rr];
     Tile nxt;
-    if (tn < total) {
+    if (tn < tEnd) {
       nxt = tileOf(tn, &cur);
       if (live(nxt)) loadTile(nxt);
     }
@@ -2526,7 +2542,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, int count, const
       __syncthreads();  // the LDS tiles are restaged for the next tile
     }
     t = tn;
-    if (t < total) cur = nxt;
+    if (t < tEnd) cur = nxt;
   }
   flushMax();
 }
