@@ -1229,28 +1229,26 @@ __global__ __launch_bounds__(256) void k_residual_wide(int n, int nrhs, const in
     if (i >= n) break;
     double acc = 0.0;
     const int64_t t0 = rowPtr[i], t1 = rowPtr[i + 1];
-    // the row's entries are read once per 64 (a lane each) and broadcast by shuffles, so the gathers of
-    // X rows for consecutive entries have their addresses at hand and go out back to back
+    // the row's entries are read once per 64 (a lane each) and taken lane by lane into scalar registers;
+    // the gathers of X rows for 16 consecutive entries all go out before their products (entries past the
+    // row's end gather the row's last column again and are dropped), the products in entry order
     for (int64_t base = t0; base < t1; base += 64) {
       const int cnt = static_cast<int>(min<int64_t>(64, t1 - base));
-      const int64_t tl = base + lane;
-      const int colL = lane < cnt ? colIdx[tl] : 0;
-      const double aL = lane < cnt ? aSym[tl] : 0.0;
-      int j = 0;
-      for (; j + 4 <= cnt; j += 4) {
-        const int cA = __shfl(colL, j, 64), cB = __shfl(colL, j + 1, 64), cC = __shfl(colL, j + 2, 64),
-                  cD = __shfl(colL, j + 3, 64);
-        const double xA = Xt[static_cast<int64_t>(cA) * nrhs + cl];
-        const double xB = Xt[static_cast<int64_t>(cB) * nrhs + cl];
-        const double xC = Xt[static_cast<int64_t>(cC) * nrhs + cl];
-        const double xD = Xt[static_cast<int64_t>(cD) * nrhs + cl];
-        acc = fma(__shfl(aL, j, 64), xA, acc);
-        acc = fma(__shfl(aL, j + 1, 64), xB, acc);
-        acc = fma(__shfl(aL, j + 2, 64), xC, acc);
-        acc = fma(__shfl(aL, j + 3, 64), xD, acc);
+      const int64_t tl = base + min(lane, cnt - 1);
+      const int colL = colIdx[tl];
+      const double aL = aSym[tl];
+      for (int j0 = 0; j0 < cnt; j0 += 16) {
+        double xv[16], av[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = min(j0 + u, cnt - 1);
+          const int cj = __builtin_amdgcn_readlane(colL, j);
+          av[u] = readLane(aL, j);
+          xv[u] = Xt[static_cast<int64_t>(cj) * nrhs + cl];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = (j0 + u < cnt) ? fma(av[u], xv[u], acc) : acc;
       }
-      for (; j < cnt; ++j)
-        acc = fma(__shfl(aL, j, 64), Xt[static_cast<int64_t>(__shfl(colL, j, 64)) * nrhs + cl], acc);
     }
     tile[lane][rr] -= acc;
   }
