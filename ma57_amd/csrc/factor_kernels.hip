@@ -2810,9 +2810,22 @@ void launchPanelRows(const int* list, int count, const int* blockStart, int tota
 void launchSyrk(const int* list, int count, const int* tileStart, const int* tileTp, int totalTiles,
                 const DevFronts& fd, hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
-  // persistent workgroups: as many per CU as the operand tiles' LDS allows
-  // (70 KB in FP64: two; 35 KB in FP32: four)
-  const int blocks = std::min(totalTiles, (fd.fp32 ? 4 : 2) * 256);
+  // persistent workgroups: as many as are resident at once (LDS and
+  // registers allow two per CU in FP64, three in FP32), so that none starts
+  // after the others have taken their share of tiles
+  static int resident[2] = {0, 0};
+  int& res = resident[fd.fp32 ? 1 : 0];
+  if (res == 0) {
+    int dev = 0, cus = 0, per = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    hipCheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    hipCheck(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per, fd.fp32 ? reinterpret_cast<const void*>(k_schur<float>) : reinterpret_cast<const void*>(k_schur<double>),
+                 256, 0),
+             "k_schur occupancy");
+    res = std::max(1, per) * std::max(1, cus);
+  }
+  const int blocks = std::min(totalTiles, res);
   if (fd.fp32)
     k_schur<float><<<blocks, 256, 0, st>>>(list, count, tileStart, tileTp, fd);
   else
