@@ -2329,7 +2329,7 @@ __device__ inline GlobalPtr<T> globalView(void* p) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur(const int* list, int count, const int* tileStart, const int* tileTp,
+__global__ __launch_bounds__(256, 4) void k_schur(const int* list, int count, const int* tileStart, const int* tileTp,
                                                DevFronts fd) {
   using M = SchurMfma<T>;
   constexpr int TS = 64;
@@ -2811,8 +2811,9 @@ void launchSyrk(const int* list, int count, const int* tileStart, const int* til
                 const DevFronts& fd, hipStream_t st) {
   if (count <= 0 || totalTiles <= 0) return;
   // persistent workgroups: as many as are resident at once (LDS and
-  // registers allow two per CU in FP64, three in FP32), so that none starts
-  // after the others have taken their share of tiles
+  // registers allow two per CU in FP64, four in FP32 with the kernel's
+  // register bound), so that none starts after the others have taken their
+  // share of tiles
   static int resident[2] = {0, 0};
   int& res = resident[fd.fp32 ? 1 : 0];
   if (res == 0) {
