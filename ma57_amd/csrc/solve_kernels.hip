@@ -1020,9 +1020,10 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int* list, int count, De
     for (int c = 0; c < 8; ++c) v[c] = ldOr0(static_cast<const double*>(Dl), (gOff[s] + lane) * nrhs + cg + c, row && cg + c < nrhs);
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
-      if (j >= nf) break;
+      if (j < nf) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c) v[c] = fma(lr[j], readLane(v[c], j), v[c]);
+        for (int c = 0; c < 8; ++c) v[c] = fma(lr[j], readLane(v[c], j), v[c]);
+      }
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
