@@ -78,6 +78,8 @@ def test_stencil27_80_full_size():
     s.setOrdering(OrderingType.Metis)
     s.analyzePattern(A)
     nd_front = max(len(x.rowPattern) for x in s.symbolicFactor().supernodes)
+    # a front past 4,096 eligible rows: every panel front takes the speculative step by default
+    assert s._get_tuning().speculative_min_rows == 0
     for prec in (Precision.FP64, Precision.FP32):
         s.setPrecision(prec)
         s.factorize(A)

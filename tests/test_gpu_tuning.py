@@ -33,6 +33,18 @@ def test_every_tuning_is_correct(small, tile):
     assert P.backward_error(A, s.solve(B), B) <= 1e-12
 
 
+def test_speculative_default_for_mid_size_fronts():
+    """Plans whose fronts stay at or below 4,096 eligible rows keep the speculative step for fronts above
+    2,048 rows (the 7-point Laplacians' panels fail the first test too often for it to pay below that)."""
+    A = P.laplace3d(20, 20, 20, 0.5)
+    s = SymLDLT()
+    s.setOrdering(OrderingType.Metis)
+    s.analyzePattern(A)
+    assert max(x.ncols for x in s.symbolicFactor().supernodes) <= 4096
+    t = s._get_tuning()
+    assert t.speculative_panel == 1 and t.speculative_min_rows == 2048
+
+
 def test_bad_tuning_rejected():
     A = P.laplace2d(8, 8, 0.3)
     s = SymLDLT()
