@@ -126,7 +126,7 @@ typedef struct {
   int repaired;                 /* 1 when the delayed-pivot repair pass ran */
   /* SYMLA_PREC_* of the returned factorization.  Static-regularized mode
    * with FP32 fronts falls back to the reference's FP64 retry sequence when
-   * two FP32 attempts miss the expected inertia (no reference counterpart). */
+   * the FP32 attempt misses the expected inertia (no reference counterpart). */
   int precision_used;
 } symla_factor_info;
 

@@ -1012,10 +1012,12 @@ void Plan::factorize(const double* aHost, const double* aDev, const FactorOption
   const int maxAttempts = std::max(1, opts.maxRetries + 1);
   if (prec_ == 1) {
     // FP32 fronts with a perturbation sized for FP32 (eps of float); if they
-    // do not hold the expected inertia within two attempts, the element
-    // growth of the perturbed pivots is beyond FP32 and the reference's own
-    // FP64 sequence runs instead (reported as precision_used = FP64)
-    if (retryLoop(std::min(2, maxAttempts), static_cast<double>(std::numeric_limits<float>::epsilon()))) {
+    // do not hold the expected inertia, the element growth of the perturbed
+    // pivots is beyond FP32 and the reference's own FP64 sequence runs
+    // instead (reported as precision_used = FP64).  A second FP32 attempt
+    // with a grown perturbation did not hold it either on the KKT systems
+    // measured, and cost a whole factorization.
+    if (retryLoop(1, static_cast<double>(std::numeric_limits<float>::epsilon()))) {
       finish();
       return;
     }
