@@ -125,7 +125,7 @@ __host__ __device__ inline bool specFront(const FactorParams& p, int nE) { retur
 // values of the rows below the block (m x kPanel front elements, row i of
 // panel column q at i + q m), from which a failed test restores them.
 constexpr int kSpecRec = 8 * kPanel + 2 * kPanel * kPanel;
-constexpr int kSpecRows = 256;  // rows below the block per workgroup
+constexpr int kSpecRows = 256;  // rows below the block per workgroup (one per thread)
 __host__ __device__ inline int64_t panelWorkspace(int m, int nE, bool spec) {
   return 2 * static_cast<int64_t>(nE) + kSpecRec + (spec ? static_cast<int64_t>(m) * kPanel : 0);
 }

@@ -1924,14 +1924,16 @@ __global__ __launch_bounds__(256) void k_spec_block(const int* list, int count, 
 // to the record: over the eligible rows rounded up to float (colmax[parity],
 // the threshold test) and exact (emax), over the rows [nE, m) exact (nmax).
 template <typename T>
-__global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count, const int* wgStart, int parity,
+__global__ __launch_bounds__(kSpecRows) void k_spec_factor(const int* list, int count, const int* wgStart, int parity,
                                                      DevFronts fd, FactorParams prm, double* wsBase,
                                                      const int64_t* wsOff) {
   constexpr int P = kPanel;
+  constexpr int NT = kSpecRows;  // one row per thread
+  constexpr int NW = NT / 64;
   __shared__ __align__(16) double W[P][P];  // whole 16-byte rows: broadcast reads of two coefficients at once
   __shared__ double rdv[P];
   __shared__ double cm[16][kSpecRows + 1];  // |c| (eligible rows) or -|c| (rows [nE, m)) of 16 columns
-  __shared__ double partE[4][P], partN[4][P];
+  __shared__ double partE[NW][P], partN[NW][P];
   if (static_cast<int>(blockIdx.x) >= wgStart[count]) return;
   const int f = findSegment(wgStart, count, blockIdx.x);
   const int s = list[f];
@@ -1953,9 +1955,9 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
   // the block's substitution coefficients, the reciprocal pivots and the
   // row's raw values all in flight together (coefficients outside the
   // block's lower triangle and skipped row entries are read and dropped)
-  double wv[P * P / 256];
+  double wv[P * P / NT];
 #pragma unroll
-  for (int u = 0; u < P * P / 256; ++u) wv[u] = rec.wmat[tid + 256 * u];
+  for (int u = 0; u < P * P / NT; ++u) wv[u] = rec.wmat[tid + NT * u];
   const double rv = rec.rdiv[tid & (P - 1)];
   T raw[P];
   bool ok[P];
@@ -1965,8 +1967,8 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
     raw[q] = F[ok[q] ? i + static_cast<int64_t>(k0 + q) * m : 0];
   }
 #pragma unroll
-  for (int u = 0; u < P * P / 256; ++u) {
-    const int e = tid + 256 * u, q = e / P, j = e % P;
+  for (int u = 0; u < P * P / NT; ++u) {
+    const int e = tid + NT * u, q = e / P, j = e % P;
     W[q][j] = (j < q && q < pw) ? wv[u] : 0.0;
   }
   if (tid < P) rdv[tid] = rv;
@@ -2013,8 +2015,12 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
       if (q < pw) F[i + static_cast<int64_t>(k0 + q) * m] = static_cast<T>(x[q]);
   }
   if (tid < pw) {
-    const double ve = fmax(fmax(partE[0][tid], partE[1][tid]), fmax(partE[2][tid], partE[3][tid]));
-    const double vn = fmax(fmax(partN[0][tid], partN[1][tid]), fmax(partN[2][tid], partN[3][tid]));
+    double ve = partE[0][tid], vn = partN[0][tid];
+#pragma unroll
+    for (int k = 1; k < NW; ++k) {
+      ve = fmax(ve, partE[k][tid]);
+      vn = fmax(vn, partN[k][tid]);
+    }
     if (ve > 0.0) {
       atomicMaxNonNeg(rec.colmax + parity * P + tid, roundUpToFloat(ve));
       atomicMaxNonNeg(rec.emax + tid, ve);
@@ -2031,7 +2037,7 @@ __global__ __launch_bounds__(256) void k_spec_factor(const int* list, int count,
 // maximum, sets kcur = k0 + q*, pcount = q*, and clears the other colmax
 // buffer and the exact maxima for the next panel.
 template <typename T>
-__global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, const int* wgStart, int parity,
+__global__ __launch_bounds__(kSpecRows) void k_spec_apply(const int* list, int count, const int* wgStart, int parity,
                                                     DevFronts fd, FactorParams prm, double* wsBase,
                                                     const int64_t* wsOff) {
   constexpr int P = kPanel;
@@ -2087,7 +2093,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const int* list, int count, 
   }
   if (w != 0) return;
   double tmax = 0.0;
-  for (int e = tid; e < pw * pw; e += 256) {
+  for (int e = tid; e < pw * pw; e += kSpecRows) {
     const int q = e / pw, r = e % pw;  // column q, row r of the block
     if (q < qs && r >= q) Fat(k0 + r, k0 + q) = static_cast<T>(r == q ? rec.dval[q] : rec.lblk[r * P + q]);
   }
@@ -2786,15 +2792,15 @@ void launchSpecPanel(const int* list, int count, const int* wgStart, int blocks,
   if (fd.fp32) {
     k_spec_block<float><<<count, 256, 0, st>>>(list, count, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_block");
-    k_spec_factor<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    k_spec_factor<float><<<blocks, kSpecRows, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_factor");
-    k_spec_apply<float><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    k_spec_apply<float><<<blocks, kSpecRows, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
   } else {
     k_spec_block<double><<<count, 256, 0, st>>>(list, count, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_block");
-    k_spec_factor<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    k_spec_factor<double><<<blocks, kSpecRows, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
     hipCheck(hipGetLastError(), "k_spec_factor");
-    k_spec_apply<double><<<blocks, 256, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
+    k_spec_apply<double><<<blocks, kSpecRows, 0, st>>>(list, count, wgStart, parity, fd, prm, ws, wsOff);
   }
   hipCheck(hipGetLastError(), "launchSpecPanel");
 }
